@@ -1,0 +1,200 @@
+/*
+ * mcx_hip.h -- C ABI of the MI355X (gfx950) spin-lattice sampling library.
+ *
+ * The library runs the sampling hot path of MonteCarloX.jl + SpinSystems on
+ * one GPU: checkerboard Metropolis / heat-bath sweeps over Ising (2D/3D) and
+ * Blume-Capel lattices, replica exchange (parallel tempering) and
+ * multicanonical / Wang-Landau walkers.  Every entry point is plain C: opaque
+ * handles, plain pointers and sizes, no torch or HIP types.
+ *
+ * The reference has no FFI: its "plugin API" is Julia multiple dispatch on the
+ * exported names.  A per-attempt call across an FFI cannot run at ns cost, so
+ * each entry point below replaces a reference call at sweep / batch
+ * granularity; the comment on each names the reference interface it replaces
+ * (paths relative to the reference repository root).
+ *
+ * Conventions
+ *   - Every function returns mcx_status (0 = MCX_OK).  On error,
+ *     mcx_last_error() returns a thread-local message.
+ *   - Status codes map to the reference's exception types:
+ *       MCX_EINVAL -> ArgumentError   (e.g. replica_exchange.jl:54, binned_object.jl:50-53)
+ *       MCX_EBOUNDS-> BoundsError     (bin out of range, test_multicanonical.jl:39-43)
+ *       MCX_EDOMAIN-> AssertionError  (binned_object.jl:198-213)
+ *       MCX_ENOMEM, MCX_EHIP -> ErrorException carrying mcx_last_error()
+ *   - The library owns all device memory.  Host buffers are caller-owned and
+ *     copied synchronously before the call returns.
+ *   - A handle may be used from one host thread at a time; all device work of
+ *     a context is ordered on that context's stream.
+ *   - Spins are int8, sites numbered x-fastest: i = x + L*y (+ L*L*z), 0-based,
+ *     periodic boundaries (IsingLatticeOptim layout, SpinSystems/src/ising.jl:444-456).
+ */
+#ifndef MCX_HIP_H
+#define MCX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t mcx_status;
+
+#define MCX_OK 0
+#define MCX_EINVAL 1
+#define MCX_EBOUNDS 2
+#define MCX_EDOMAIN 3
+#define MCX_ENOMEM 4
+#define MCX_EHIP 5
+#define MCX_ESTATE 6
+
+/* models */
+#define MCX_ISING 0
+#define MCX_BLUME_CAPEL 1
+
+/* algorithms */
+#define MCX_METROPOLIS 0
+#define MCX_HEATBATH 1
+#define MCX_MULTICANONICAL 2
+#define MCX_WANGLANDAU 3
+
+/* initial states, init!(sys, type) */
+#define MCX_INIT_UP 0
+#define MCX_INIT_DOWN 1
+#define MCX_INIT_ZERO 2
+#define MCX_INIT_RANDOM 3
+
+/* run flags */
+#define MCX_RUN_RECORD 1u /* write per-sweep E/M(/Q) series and FP64 moments */
+#define MCX_RUN_PLAIN 2u  /* one-chain-per-launch path: a kernel per colour per chain per sweep */
+
+/* per-chain FP64 moments, in this order */
+#define MCX_MOM_COUNT 0
+#define MCX_MOM_E 1
+#define MCX_MOM_E2 2
+#define MCX_MOM_ABSM 3
+#define MCX_MOM_M2 4
+#define MCX_MOM_M4 5
+#define MCX_N_MOMENTS 6
+
+typedef struct mcx_ctx mcx_ctx;
+typedef struct mcx_batch mcx_batch;
+typedef struct mcx_pt mcx_pt;
+typedef struct mcx_gen mcx_gen;
+typedef struct mcx_server mcx_server;
+
+/* ------------------------------------------------------------------------ */
+/* Context                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* No reference counterpart (device selection).  Creates a context bound to
+ * HIP device `device` with its own stream. */
+mcx_status mcx_ctx_create(int32_t device, mcx_ctx** out);
+mcx_status mcx_ctx_destroy(mcx_ctx* ctx);
+/* Thread-local text of the last error (never NULL). */
+const char* mcx_last_error(void);
+mcx_status mcx_version(int32_t* major, int32_t* minor);
+
+/* ------------------------------------------------------------------------ */
+/* Lattice batches: N independent chains                                    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int32_t model;              /* MCX_ISING or MCX_BLUME_CAPEL                     */
+    int32_t dim;                /* 2 or 3                                           */
+    int32_t n_chains;           /* >= 1                                              */
+    int32_t L_uniform;          /* side length when L == NULL                       */
+    const int32_t* L;           /* per-chain even side length >= 4, or NULL         */
+    uint64_t seed;              /* Philox key word 0 = low 32 bits, counter word 3 = high 32 */
+    const uint32_t* chain_keys; /* per-chain Philox key word 1, or NULL = chain index */
+    double D_uniform;           /* Blume-Capel crystal field when D == NULL         */
+    const double* D;            /* per-chain crystal field, or NULL                  */
+    int64_t series_capacity;    /* sweeps of per-sweep series kept per chain (>= 0) */
+} mcx_batch_desc;
+
+/* Replaces: one system per chain, IsingLatticeOptim(L, L) / Ising([L,L(,L)]; J=1)
+ * (SpinSystems/src/ising.jl:413-424,440-461) or BlumeCapel([L,L]; J=1, D)
+ * (SpinSystems/src/blume_capel.jl:500-511), plus one algorithm per chain as in
+ * ParallelChains(ThreadsBackend(n), algs) (src/infrastructure/parallel_chains.jl:14-105).
+ * Spins start all up (the constructors' ones(Int8, N)). */
+mcx_status mcx_batch_create(mcx_ctx* ctx, const mcx_batch_desc* desc, mcx_batch** out);
+mcx_status mcx_batch_destroy(mcx_batch* b);
+mcx_status mcx_batch_num_sites(mcx_batch* b, int32_t chain, int64_t* n_sites);
+
+/* Replaces: Metropolis(rng; beta) / HeatBath(rng; beta) per chain
+ * (src/algorithms/metropolis.jl:205-206, heat_bath.jl:254): one beta >= 0 per
+ * chain; the library tabulates exp(-beta*dE) and logistic(beta*s*dE) exactly
+ * as logweight(BoltzmannEnsemble) (src/ensembles/boltzmann.jl:28) and
+ * logistic (src/infrastructure/utils.jl:82-88) evaluate them. */
+mcx_status mcx_batch_set_betas(mcx_batch* b, const double* beta_per_chain);
+
+/* Host-built acceptance probabilities (so a caller can supply its own exp()
+ * bits).  alg = MCX_METROPOLIS or MCX_HEATBATH; p has n_chains * n_per_chain
+ * entries in the class order documented by mcx_batch_table_layout. */
+mcx_status mcx_batch_set_tables(mcx_batch* b, int32_t alg, const double* p, int32_t n_per_chain);
+/* Number of probabilities per chain for (alg) and the class convention:
+ * Ising Metropolis: class k = number of anti-aligned neighbours, k < dim,
+ *   p = exp(-beta * dE) with dE = 2*(2*dim - 2k)
+ * Ising heat bath: class k = number of up neighbours, 0 <= k <= 2*dim,
+ *   p = logistic(beta * 2*(2k - 2*dim))
+ * Blume-Capel Metropolis (NS = 4*dim + 1 neighbour sums, 6*NS entries):
+ *   class ((s_old+1)*2 + alt)*NS + (sum_nbr + 2*dim), alt = proposal bit,
+ *   p = exp(-beta * dE) (p >= 1 accepts always)
+ * Blume-Capel heat bath (3*NS entries): per sum_nbr (w1, w1 + w2, z) of
+ *   SpinSystems/src/blume_capel.jl:66-76 */
+mcx_status mcx_batch_table_layout(mcx_batch* b, int32_t alg, int32_t* n_per_chain);
+
+/* Replaces init!(sys, :up/:down/:zero/:random; rng) (SpinSystems/src/ising.jl:74-78,
+ * blume_capel.jl:105-109).  :random draws from the Philox INIT stream at each
+ * chain's current sweep counter. */
+mcx_status mcx_batch_init_spins(mcx_batch* b, int32_t mode);
+/* Replaces direct access to sys.spins.  n must equal the chain's site count;
+ * values must be +-1 (Ising) or -1/0/+1 (Blume-Capel), else MCX_EDOMAIN. */
+mcx_status mcx_batch_set_spins(mcx_batch* b, int32_t chain, const int8_t* spins, size_t n);
+mcx_status mcx_batch_get_spins(mcx_batch* b, int32_t chain, int8_t* spins, size_t n);
+
+/* Replaces the user loop `for _ in 1:N*nsweeps; spin_flip!(sys, alg); end`
+ * over every chain (SpinSystems/src/ising.jl:35-58, blume_capel.jl:52-85),
+ * in checkerboard order.  alg = MCX_METROPOLIS or MCX_HEATBATH. */
+mcx_status mcx_batch_run(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t flags);
+/* Device time (ms) of the last run / pt run, measured with HIP events on the
+ * context stream around the kernels of that call. */
+mcx_status mcx_batch_last_run_ms(mcx_batch* b, double* ms);
+
+/* Replaces energy(sys) / magnetization(sys) (SpinSystems/src/ising.jl:17-18,
+ * blume_capel.jl:18-19): per chain E_pair = -sum_<ij> s_i s_j, M = sum s_i,
+ * Q = sum s_i^2 (any output may be NULL).  Energy = E_pair + D*Q. */
+mcx_status mcx_batch_get_observables(mcx_batch* b, int32_t* E_pair, int32_t* M, int32_t* Q);
+
+/* Replaces measure!(measurements, sys, i) of energy/magnetization after each
+ * sweep (src/measurements/measurements.jl:192-200).  Entries [first, first+count)
+ * of chain `chain`'s recorded series. */
+mcx_status mcx_batch_series_length(mcx_batch* b, int64_t* n);
+mcx_status mcx_batch_get_series(mcx_batch* b, int32_t chain, int64_t first, int64_t count,
+                                int32_t* E_pair, int32_t* M, int32_t* Q);
+/* Clears the series and the moments. */
+mcx_status mcx_batch_reset_series(mcx_batch* b);
+/* n_chains * MCX_N_MOMENTS doubles: count, sum E, sum E^2, sum |M|, sum M^2, sum M^4
+ * over recorded sweeps, accumulated in FP64 on the device. */
+mcx_status mcx_batch_get_moments(mcx_batch* b, double* out);
+
+/* Replaces alg.steps / alg.accepted and reset!(alg)
+ * (src/algorithms/importance_sampling.jl:80-110). */
+mcx_status mcx_batch_get_counters(mcx_batch* b, int64_t* steps, int64_t* accepted);
+mcx_status mcx_batch_reset_counters(mcx_batch* b);
+
+/* Replaces the alg.rng field (Metropolis(rng; beta), src/algorithms/metropolis.jl:173-206):
+ * rebinds the chains' Philox streams to key (seed & 0xffffffff, chain_keys[i])
+ * with counter word 3 = seed >> 32 (chain_keys NULL = chain index).  The
+ * stream position (sweep counter) is kept. */
+mcx_status mcx_batch_set_streams(mcx_batch* b, uint64_t seed, const uint32_t* chain_keys);
+
+/* Per-chain sweep counters (the Philox counter position); setting them is how
+ * a restored checkpoint resumes bit-exactly. */
+mcx_status mcx_batch_get_sweeps(mcx_batch* b, uint64_t* sweeps);
+mcx_status mcx_batch_set_sweeps(mcx_batch* b, const uint64_t* sweeps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,78 @@
+// mcx_api_impl.h -- host-side object layouts behind the opaque C-ABI handles.
+#pragma once
+
+#include "../../include/mcx_hip.h"
+#include "mcx_internal.h"
+
+#include <string>
+#include <vector>
+
+#define MCX_VERSION_MAJOR 0
+#define MCX_VERSION_MINOR 1
+
+#define HIPCHK(expr)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return ::mcx::fail(e_ == hipErrorOutOfMemory ? MCX_ENOMEM : MCX_EHIP,     \
+                               "%s failed: %s", #expr, hipGetErrorString(e_));        \
+    } while (0)
+
+struct mcx_ctx {
+    int device = 0;
+    int n_cu = 0;
+    size_t lds_max = 0;
+    char arch[64] = {0};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct mcx_batch {
+    struct Launch {
+        int first, count, block;
+        size_t lds;
+    };
+    mcx_ctx* ctx = nullptr;
+    int model = 0, dim = 2, n = 0;
+    uint64_t seed = 0;
+    std::vector<mcx::ChainRec> rec; // host mirror of the device chain records
+    std::vector<int64_t> nsites;
+    std::vector<int64_t> steps;
+    std::vector<double> betas;
+    std::vector<Launch> launches;
+    int64_t cap = 0, rec_len = 0, pool_bytes = 0, max_sites = 0;
+    bool tables_ready = false;
+    double last_ms = 0;
+    int8_t* d_spins = nullptr;
+    mcx::ChainRec* d_chains = nullptr;
+    mcx::IsingTable* d_itab = nullptr;
+    mcx::BCTable* d_btab = nullptr;
+    int32_t* d_E = nullptr;
+    int32_t* d_M = nullptr;
+    int32_t* d_Q = nullptr;
+    double* d_mom = nullptr;
+    unsigned long long* d_acc = nullptr;
+    int32_t* d_items = nullptr;
+    int32_t* d_obs = nullptr;
+};
+
+namespace mcx {
+
+mcx_status fail(mcx_status code, const char* fmt, ...);
+uint64_t threshold32(double p);
+double logistic(double x);
+void ising_probabilities(double beta, int dim, double* metro, double* hb);
+void fill_ising_metro(IsingTable& t, int dim, const double* p);
+void fill_ising_hb(IsingTable& t, int dim, const double* p);
+void fill_bc_table(BCTable& t, int dim, double beta, double D);
+void bc_probabilities(int dim, double beta, double D, double* metro, double* hb);
+void fill_bc_metro_from_p(BCTable& t, int dim, const double* p);
+void fill_bc_hb_from_p(BCTable& t, int dim, const double* p);
+size_t lattice_lds_bytes(int model, int dim, int L);
+int lattice_block(int dim, int L);
+mcx_status upload_chains(mcx_batch* b);
+mcx_status sync_check(mcx_batch* b, const char* what);
+mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t flags);
+mcx_status check_run(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t flags, const char* who);
+
+} // namespace mcx

@@ -1,0 +1,508 @@
+// mcx_bc.hip -- checkerboard Metropolis / heat-bath sweeps for Blume-Capel
+// lattices (spin-1, J = 1, crystal field D, h = 0; 2D square / 3D cubic,
+// periodic) on gfx950.
+//
+// Per-site rule:
+//   Metropolis (SpinSystems/src/blume_capel.jl:52-59, :21-30, :235-248):
+//     s_new = _propose_state(s_old, b) (b true selects the first alternative),
+//     dE = -(ds * sum_j s_j) + D * ds2, accept iff u < exp(-beta * dE)
+//   heat bath (blume_capel.jl:61-85):
+//     w1 = exp(-beta*e1), w2 = exp(-beta*0), w3 = exp(-beta*e3), z = w1+w2+w3,
+//     r = u * z; s_new = r < w1 ? -1 : (r < w1 + w2 ? 0 : 1)
+// Random words: site word w of (group, bit) = word (bit & 3) of Philox block
+// (8*group + bit/4, sweep, purpose).  Metropolis: b = w & 1, u = (w >> 1) / 2^31;
+// heat bath: u = w / 2^32.
+//
+// Batched kernel: one workgroup per chain; the lattice lives in LDS as four bit
+// planes (occupied, up) x (colour 0, colour 1) in the word layout of the Ising
+// kernels; one thread updates the 32 sites of a word, drawing one Philox block
+// per 4 sites.  The neighbour sum is formed bit-sliced for all 32 sites at once.
+#include "mcx_api_impl.h"
+#include "mcx_device.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace mcx {
+
+// ---------------------------------------------------------------------------
+// host: tables
+// ---------------------------------------------------------------------------
+
+static int8_t propose_host(int s_old, int alt)
+{
+    if (s_old == -1)
+        return alt ? 0 : 1;
+    if (s_old == 0)
+        return alt ? -1 : 1;
+    return alt ? -1 : 0;
+}
+
+// Probabilities in the layout of mcx_batch_table_layout:
+//   metro[((s_old+1)*2 + alt)*NS + (sum + 2*dim)] = exp(-beta * dE)
+//   hb[(sum + 2*dim)*3 + {0,1,2}] = (w1, w1 + w2, z)
+void bc_probabilities(int dim, double beta, double D, double* metro, double* hb)
+{
+    const int NS = 4 * dim + 1;
+    for (int so = -1; so <= 1; ++so)
+        for (int alt = 0; alt < 2; ++alt)
+            for (int sum = -2 * dim; sum <= 2 * dim; ++sum) {
+                const int sn = propose_host(so, alt);
+                const int ds = sn - so;
+                const int ds2 = sn * sn - so * so;
+                const double coupling = 1.0 * (double)sum; // local_coupling, J = 1
+                const double dpair = (double)ds * coupling;
+                const double dE = (-dpair - 0.0) + D * (double)ds2;
+                metro[((so + 1) * 2 + alt) * NS + (sum + 2 * dim)] = std::exp(-beta * dE);
+            }
+    for (int sum = -2 * dim; sum <= 2 * dim; ++sum) {
+        const double coupling = 1.0 * (double)sum;
+        const double h_i = 0.0;
+        const double e1 = 1.0 * coupling - h_i * (-1.0) + D;
+        const double e2 = 0.0;
+        const double e3 = -1.0 * coupling - h_i * 1.0 + D;
+        const double w1 = std::exp(-beta * e1);
+        const double w2 = std::exp(-beta * e2);
+        const double w3 = std::exp(-beta * e3);
+        const double z = w1 + w2 + w3;
+        double* o = hb + (sum + 2 * dim) * 3;
+        o[0] = w1;
+        o[1] = w1 + w2;
+        o[2] = z;
+    }
+}
+
+void fill_bc_metro_from_p(BCTable& t, int dim, const double* p)
+{
+    const int n = 6 * (4 * dim + 1);
+    t.metro_always[0] = t.metro_always[1] = t.metro_always[2] = 0;
+    for (int k = 0; k < 80; ++k)
+        t.metro_thr[k] = 0;
+    for (int k = 0; k < n; ++k) {
+        if (p[k] >= 1) {
+            t.metro_always[k >> 5] |= 1u << (k & 31);
+            t.metro_thr[k] = 0x80000000u;
+        } else {
+            const double v = std::ceil(std::ldexp(p[k] > 0 ? p[k] : 0.0, 31));
+            if (v >= 2147483648.0) {
+                t.metro_always[k >> 5] |= 1u << (k & 31);
+                t.metro_thr[k] = 0x80000000u;
+            } else {
+                t.metro_thr[k] = (uint32_t)v;
+            }
+        }
+    }
+}
+
+void fill_bc_hb_from_p(BCTable& t, int dim, const double* p)
+{
+    const int NS = 4 * dim + 1;
+    for (int k = 0; k < 13; ++k)
+        t.hb[k][0] = t.hb[k][1] = t.hb[k][2] = 0;
+    for (int k = 0; k < NS; ++k) {
+        t.hb[k][0] = p[3 * k];
+        t.hb[k][1] = p[3 * k + 1];
+        t.hb[k][2] = p[3 * k + 2];
+    }
+}
+
+void fill_bc_table(BCTable& t, int dim, double beta, double D)
+{
+    std::memset(&t, 0, sizeof t);
+    double metro[78], hb[39];
+    bc_probabilities(dim, beta, D, metro, hb);
+    fill_bc_metro_from_p(t, dim, metro);
+    fill_bc_hb_from_p(t, dim, hb);
+}
+
+// ---------------------------------------------------------------------------
+// device
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int bc_propose(int s_old, int alt)
+{
+    return s_old == -1 ? (alt ? 0 : 1) : (s_old == 0 ? (alt ? -1 : 1) : (alt ? -1 : 0));
+}
+
+struct BCNbr {
+    uint32_t occ[6], up[6]; // U, Dn, A, Bw, Zu, Zd
+};
+
+template <int DIM>
+__device__ __forceinline__ void bc_load_nbrs(const uint32_t* __restrict__ Oo,
+                                             const uint32_t* __restrict__ So, int L, int nw,
+                                             int half, int r, int q, int y, int z, int p, int W,
+                                             uint32_t valid, BCNbr& n)
+{
+    const int base = DIM == 3 ? z * L : 0;
+    const int yu = y == 0 ? L - 1 : y - 1;
+    const int yd = y == L - 1 ? 0 : y + 1;
+    const int iU = (base + yu) * nw + q, iD = (base + yd) * nw + q, iA = r * nw + q;
+    n.occ[0] = Oo[iU];
+    n.up[0] = So[iU];
+    n.occ[1] = Oo[iD];
+    n.up[1] = So[iD];
+    n.occ[2] = Oo[iA];
+    n.up[2] = So[iA];
+    if (DIM == 3) {
+        const int zu = z == 0 ? L - 1 : z - 1;
+        const int zd = z == L - 1 ? 0 : z + 1;
+        const int iZu = (zu * L + y) * nw + q, iZd = (zd * L + y) * nw + q;
+        n.occ[4] = Oo[iZu];
+        n.up[4] = So[iZu];
+        n.occ[5] = Oo[iZd];
+        n.up[5] = So[iZd];
+    } else {
+        n.occ[4] = n.up[4] = n.occ[5] = n.up[5] = 0;
+    }
+    const uint32_t A0 = n.occ[2], A1 = n.up[2];
+    if (nw == 1) {
+        if (p) {
+            n.occ[3] = ((A0 >> 1) | (A0 << (W - 1))) & valid;
+            n.up[3] = ((A1 >> 1) | (A1 << (W - 1))) & valid;
+        } else {
+            n.occ[3] = ((A0 << 1) | (A0 >> (W - 1))) & valid;
+            n.up[3] = ((A1 << 1) | (A1 >> (W - 1))) & valid;
+        }
+    } else if (p) {
+        const int qn = q + 1 == nw ? 0 : q + 1;
+        n.occ[3] = (A0 >> 1) | ((Oo[r * nw + qn] & 1u) << (W - 1));
+        n.up[3] = (A1 >> 1) | ((So[r * nw + qn] & 1u) << (W - 1));
+    } else {
+        const int qp = q == 0 ? nw - 1 : q - 1;
+        const int Wp = word_width(qp, half);
+        n.occ[3] = ((A0 << 1) | ((Oo[r * nw + qp] >> (Wp - 1)) & 1u)) & valid;
+        n.up[3] = ((A1 << 1) | ((So[r * nw + qp] >> (Wp - 1)) & 1u)) & valid;
+    }
+}
+
+// Bit-sliced (sum_j s_j + 2*dim) for 32 sites: each neighbour adds 2 if up,
+// 1 if empty, 0 if down.  Result bits v0..v3.
+template <int DIM>
+__device__ __forceinline__ void bc_shifted_sum(const BCNbr& n, uint32_t& v0, uint32_t& v1,
+                                               uint32_t& v2, uint32_t& v3)
+{
+    uint32_t a0, a1, a2, b0, b1, b2;
+    uint32_t a[6], e[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        a[k] = n.occ[k] & n.up[k];
+        e[k] = ~n.occ[k];
+    }
+    if (DIM == 2) {
+        count4(a[0], a[1], a[2], a[3], a0, a1, a2);
+        count4(e[0], e[1], e[2], e[3], b0, b1, b2);
+    } else {
+        count6(a[0], a[1], a[2], a[3], a[4], a[5], a0, a1, a2);
+        count6(e[0], e[1], e[2], e[3], e[4], e[5], b0, b1, b2);
+    }
+    // 2*A + B
+    v0 = b0;
+    v1 = a0 ^ b1;
+    const uint32_t c1 = a0 & b1;
+    v2 = a1 ^ b2 ^ c1;
+    const uint32_t c2 = (a1 & b2) | (c1 & (a1 | b2));
+    v3 = a2 ^ c2;
+}
+
+template <int DIM, int ALG>
+__global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
+{
+    extern __shared__ uint32_t lds[];
+    int32_t* part = reinterpret_cast<int32_t*>(lds); // [2][16][3]
+    uint32_t* tabw = lds + 96;                       // 80 thresholds + 3 masks
+    double* tabd = reinterpret_cast<double*>(lds + 184); // 39 doubles (8-byte aligned)
+    uint32_t* P = lds + 264;                          // [occ c0, occ c1, up c0, up c1]
+    constexpr int NS = 4 * DIM + 1;
+    const int tid = threadIdx.x;
+    const int B = blockDim.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int nwaves = (B + 63) >> 6;
+    const uint32_t k0 = (uint32_t)a.seed;
+    const uint32_t c3 = (uint32_t)(a.seed >> 32);
+    constexpr uint32_t purpose = ALG == 0 ? P_BC_METRO : P_BC_HB;
+
+    for (int item = blockIdx.x; item < a.n_items; item += gridDim.x) {
+        const int ch = a.items[item];
+        const ChainRec cr = a.chains[ch];
+        const int L = cr.L;
+        const int half = L >> 1;
+        const int nw = (half + 31) >> 5;
+        const int rows = DIM == 3 ? L * L : L;
+        const int words = rows * nw;
+        int8_t* __restrict__ sp = a.spins + cr.spin_off;
+        const BCTable* tb = a.btab + cr.table;
+        for (int k = tid; k < 83; k += B)
+            tabw[k] = k < 80 ? tb->metro_thr[k] : tb->metro_always[k - 80];
+        for (int k = tid; k < 39; k += B)
+            tabd[k] = tb->hb[k / 3][k % 3];
+
+        for (int w = tid; w < words; w += B) {
+            const int r = w / nw;
+            const int q = w - r * nw;
+            const int y = DIM == 3 ? r % L : r;
+            const int z = DIM == 3 ? r / L : 0;
+            const int par = (y + z) & 1;
+            const int W = word_width(q, half);
+            const int8_t* row = sp + (int64_t)r * L + 64 * q;
+            uint32_t eo = 0, eu = 0, oo = 0, ou = 0;
+            for (int j = 0; j < W; ++j) {
+                const int se = row[2 * j], so = row[2 * j + 1];
+                eo |= (uint32_t)(se != 0) << j;
+                eu |= (uint32_t)(se > 0) << j;
+                oo |= (uint32_t)(so != 0) << j;
+                ou |= (uint32_t)(so > 0) << j;
+            }
+            P[par * words + w] = eo;
+            P[(2 + par) * words + w] = eu;
+            P[(par ^ 1) * words + w] = oo;
+            P[(2 + (par ^ 1)) * words + w] = ou;
+        }
+        __syncthreads();
+
+        unsigned long long acc = 0;
+        double mE = 0, mE2 = 0, mAM = 0, mM2 = 0, mM4 = 0;
+        for (int64_t s = 0; s < a.nsweeps; ++s) {
+            const uint64_t sweep = cr.sweep + (uint64_t)s;
+            const uint32_t sw_lo = (uint32_t)sweep;
+            const uint32_t w2 = ctr_word2(sweep, 0, purpose);
+            int32_t pr = 0, mm = 0, qq = 0;
+            for (int c = 0; c < 2; ++c) {
+                uint32_t* __restrict__ Oc = P + c * words;
+                uint32_t* __restrict__ Sc = P + (2 + c) * words;
+                const uint32_t* __restrict__ Oo = P + (c ^ 1) * words;
+                const uint32_t* __restrict__ So = P + (2 + (c ^ 1)) * words;
+                for (int w = tid; w < words; w += B) {
+                    const int r = nw == 1 ? w : w / nw;
+                    const int q = w - r * nw;
+                    const int y = DIM == 3 ? r % L : r;
+                    const int z = DIM == 3 ? r / L : 0;
+                    const int p = (c + y + z) & 1;
+                    const int W = word_width(q, half);
+                    const uint32_t valid = width_mask(W);
+                    uint32_t occ = Oc[w], up = Sc[w];
+                    BCNbr n;
+                    bc_load_nbrs<DIM>(Oo, So, L, nw, half, r, q, y, z, p, W, valid, n);
+                    uint32_t v0, v1, v2, v3;
+                    bc_shifted_sum<DIM>(n, v0, v1, v2, v3);
+                    const uint32_t group = (uint32_t)((r * 2 + c) * nw + q);
+                    uint4 R = make_uint4(0, 0, 0, 0);
+                    uint32_t changed = 0;
+                    for (int j = 0; j < W; ++j) {
+                        if ((j & 3) == 0)
+                            R = philox4x32_10(8u * group + (uint32_t)(j >> 2), sw_lo, w2, c3, k0,
+                                              cr.key);
+                        const uint32_t rw = (j & 3) == 0 ? R.x : ((j & 3) == 1 ? R.y : ((j & 3) == 2 ? R.z : R.w));
+                        const int o = (occ >> j) & 1, u = (up >> j) & 1;
+                        const int s_old = o ? (u ? 1 : -1) : 0;
+                        const int sidx = (int)(((v0 >> j) & 1u) | (((v1 >> j) & 1u) << 1)
+                                               | (((v2 >> j) & 1u) << 2) | (((v3 >> j) & 1u) << 3));
+                        int s_new;
+                        if (ALG == 0) {
+                            const int alt = (int)(rw & 1u);
+                            const int k = ((s_old + 1) * 2 + alt) * NS + sidx;
+                            const bool ok = ((tabw[80 + (k >> 5)] >> (k & 31)) & 1u)
+                                            || (rw >> 1) < tabw[k];
+                            s_new = ok ? bc_propose(s_old, alt) : s_old;
+                        } else {
+                            const double rr = ((double)rw * 0x1.0p-32) * tabd[sidx * 3 + 2];
+                            s_new = rr < tabd[sidx * 3] ? -1 : (rr < tabd[sidx * 3 + 1] ? 0 : 1);
+                        }
+                        if (s_new != s_old) {
+                            changed |= 1u << j;
+                            occ = (occ & ~(1u << j)) | ((uint32_t)(s_new != 0) << j);
+                            up = (up & ~(1u << j)) | ((uint32_t)(s_new > 0) << j);
+                        }
+                    }
+                    acc += (unsigned long long)__popc(changed);
+                    Oc[w] = occ;
+                    Sc[w] = up;
+                    if (c == 1 && a.record) {
+                        const uint32_t dn = occ & ~up;
+                        const uint32_t upo = occ & up;
+#pragma unroll
+                        for (int k = 0; k < 2 * DIM; ++k) {
+                            const uint32_t both = occ & n.occ[k];
+                            const uint32_t anti = both & (up ^ n.up[k]);
+                            pr += __popc(both) - 2 * __popc(anti);
+                        }
+                        const uint32_t Ao = n.occ[2], Au = n.up[2];
+                        mm += __popc(upo) - __popc(dn) + __popc(Ao & Au) - __popc(Ao & ~Au);
+                        qq += __popc(occ) + __popc(Ao);
+                    }
+                }
+                if (c == 1 && a.record) {
+                    pr = wave_sum(pr);
+                    mm = wave_sum(mm);
+                    qq = wave_sum(qq);
+                    if (lane == 0) {
+                        part[(s & 1) * 48 + wave * 3] = pr;
+                        part[(s & 1) * 48 + wave * 3 + 1] = mm;
+                        part[(s & 1) * 48 + wave * 3 + 2] = qq;
+                    }
+                }
+                __syncthreads();
+            }
+            if (a.record && tid == 0) {
+                int64_t tp = 0, tm = 0, tq = 0;
+                for (int k = 0; k < nwaves; ++k) {
+                    tp += part[(s & 1) * 48 + k * 3];
+                    tm += part[(s & 1) * 48 + k * 3 + 1];
+                    tq += part[(s & 1) * 48 + k * 3 + 2];
+                }
+                const int64_t at = cr.series_off + a.rec_pos + s;
+                a.ser_E[at] = (int32_t)(-tp);
+                a.ser_M[at] = (int32_t)tm;
+                a.ser_Q[at] = (int32_t)tq;
+                const double e = (double)(int32_t)(-tp) + cr.D * (double)(int32_t)tq;
+                const double m = (double)tm;
+                mE += e;
+                mE2 += e * e;
+                mAM += fabs(m);
+                mM2 += m * m;
+                mM4 += (m * m) * (m * m);
+            }
+        }
+
+        for (int w = tid; w < words; w += B) {
+            const int r = w / nw;
+            const int q = w - r * nw;
+            const int y = DIM == 3 ? r % L : r;
+            const int z = DIM == 3 ? r / L : 0;
+            const int par = (y + z) & 1;
+            const int W = word_width(q, half);
+            int8_t* row = sp + (int64_t)r * L + 64 * q;
+            const uint32_t eo = P[par * words + w], eu = P[(2 + par) * words + w];
+            const uint32_t oo = P[(par ^ 1) * words + w], ou = P[(2 + (par ^ 1)) * words + w];
+            for (int j = 0; j < W; ++j) {
+                row[2 * j] = ((eo >> j) & 1u) ? (((eu >> j) & 1u) ? 1 : -1) : 0;
+                row[2 * j + 1] = ((oo >> j) & 1u) ? (((ou >> j) & 1u) ? 1 : -1) : 0;
+            }
+        }
+        acc = wave_sum(acc);
+        if (lane == 0 && acc)
+            atomicAdd(&a.accepted[ch], acc);
+        if (tid == 0) {
+            if (a.record) {
+                double* mo = a.moments + (int64_t)ch * 6;
+                mo[0] += (double)a.nsweeps;
+                mo[1] += mE;
+                mo[2] += mE2;
+                mo[3] += mAM;
+                mo[4] += mM2;
+                mo[5] += mM4;
+            }
+            a.chains[ch].sweep = cr.sweep + (uint64_t)a.nsweeps;
+        }
+        __syncthreads();
+    }
+}
+
+// Plain path: one thread per site of one colour of one chain, int8 in HBM.
+template <int DIM, int ALG>
+__global__ void __launch_bounds__(256) k_bc_plain(PlainArgs a, int n_col)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const ChainRec cr = a.chains[a.chain];
+    const int L = cr.L;
+    const int half = L >> 1;
+    const int nw = (half + 31) >> 5;
+    const int c = a.colour;
+    constexpr int NS = 4 * DIM + 1;
+    int e_part = 0, m_part = 0, q_part = 0;
+    unsigned long long flips = 0;
+    if (idx < n_col) {
+        const int r = idx / half;
+        const int j = idx - r * half;
+        const int y = DIM == 3 ? r % L : r;
+        const int z = DIM == 3 ? r / L : 0;
+        const int p = (c + y + z) & 1;
+        const int x = 2 * j + p;
+        int8_t* sp = a.spins + cr.spin_off;
+        const int64_t rowb = (int64_t)L * r;
+        const int xl = x == 0 ? L - 1 : x - 1;
+        const int xr = x == L - 1 ? 0 : x + 1;
+        const int yu = y == 0 ? L - 1 : y - 1;
+        const int yd = y == L - 1 ? 0 : y + 1;
+        const int64_t zb = DIM == 3 ? (int64_t)z * L * L : 0;
+        int nb = sp[rowb + xl] + sp[rowb + xr] + sp[zb + (int64_t)yu * L + x]
+               + sp[zb + (int64_t)yd * L + x];
+        if (DIM == 3) {
+            const int zu = z == 0 ? L - 1 : z - 1;
+            const int zd = z == L - 1 ? 0 : z + 1;
+            nb += sp[(int64_t)zu * L * L + (int64_t)y * L + x]
+                + sp[(int64_t)zd * L * L + (int64_t)y * L + x];
+        }
+        const int s_old = sp[rowb + x];
+        const BCTable& tb = a.btab[cr.table];
+        const uint32_t group = (uint32_t)((r * 2 + c) * nw + (j >> 5));
+        const uint32_t rw = site_word(a.seed, cr.key, group, j & 31, a.sweep,
+                                      ALG == 0 ? P_BC_METRO : P_BC_HB);
+        const int sidx = nb + 2 * DIM;
+        int s_new;
+        if (ALG == 0) {
+            const int alt = (int)(rw & 1u);
+            const int k = ((s_old + 1) * 2 + alt) * NS + sidx;
+            const bool ok = ((tb.metro_always[k >> 5] >> (k & 31)) & 1u) || (rw >> 1) < tb.metro_thr[k];
+            s_new = ok ? bc_propose(s_old, alt) : s_old;
+        } else {
+            const double rr = ((double)rw * 0x1.0p-32) * tb.hb[sidx][2];
+            s_new = rr < tb.hb[sidx][0] ? -1 : (rr < tb.hb[sidx][1] ? 0 : 1);
+        }
+        if (s_new != s_old) {
+            sp[rowb + x] = (int8_t)s_new;
+            flips = 1;
+        }
+        if (c == 1 && a.rec_index >= 0) {
+            const int partner = sp[rowb + (p ? xl : xr)];
+            e_part = -s_new * nb;
+            m_part = s_new + partner;
+            q_part = s_new * s_new + partner * partner;
+        }
+    }
+    flips = wave_sum(flips);
+    if ((threadIdx.x & 63) == 0 && flips)
+        atomicAdd(&a.accepted[a.chain], flips);
+    if (c == 1 && a.rec_index >= 0) {
+        e_part = wave_sum(e_part);
+        m_part = wave_sum(m_part);
+        q_part = wave_sum(q_part);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&a.ser_E[cr.series_off + a.rec_index], e_part);
+            atomicAdd(&a.ser_M[cr.series_off + a.rec_index], m_part);
+            atomicAdd(&a.ser_Q[cr.series_off + a.rec_index], q_part);
+        }
+    }
+}
+
+hipError_t launch_bc_sweeps(int dim, int alg, const SweepArgs& a, int block, int grid,
+                            size_t lds_bytes, hipStream_t st)
+{
+    if (dim == 2 && alg == 0)
+        hipLaunchKernelGGL((k_bc_sweeps<2, 0>), dim3(grid), dim3(block), lds_bytes, st, a);
+    else if (dim == 2)
+        hipLaunchKernelGGL((k_bc_sweeps<2, 1>), dim3(grid), dim3(block), lds_bytes, st, a);
+    else if (alg == 0)
+        hipLaunchKernelGGL((k_bc_sweeps<3, 0>), dim3(grid), dim3(block), lds_bytes, st, a);
+    else
+        hipLaunchKernelGGL((k_bc_sweeps<3, 1>), dim3(grid), dim3(block), lds_bytes, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bc_plain(int dim, int alg, const PlainArgs& a, int n_col, hipStream_t st)
+{
+    const int block = 256;
+    const int grid = (n_col + block - 1) / block;
+    if (dim == 2 && alg == 0)
+        hipLaunchKernelGGL((k_bc_plain<2, 0>), dim3(grid), dim3(block), 0, st, a, n_col);
+    else if (dim == 2)
+        hipLaunchKernelGGL((k_bc_plain<2, 1>), dim3(grid), dim3(block), 0, st, a, n_col);
+    else if (alg == 0)
+        hipLaunchKernelGGL((k_bc_plain<3, 0>), dim3(grid), dim3(block), 0, st, a, n_col);
+    else
+        hipLaunchKernelGGL((k_bc_plain<3, 1>), dim3(grid), dim3(block), 0, st, a, n_col);
+    return hipGetLastError();
+}
+
+} // namespace mcx

@@ -1,0 +1,107 @@
+// mcx_device.h -- device helpers shared by the sweep kernels: Philox4x32-10,
+// the checkerboard word geometry and wavefront reductions.
+#pragma once
+
+#include "mcx_internal.h"
+
+namespace mcx {
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
+// as 1, 2, 3", SC'11).  The 32x32->64 products map to v_mad_u64_u32; round
+// keys depend only on (k0, k1) and fold into scalar registers when the key is
+// uniform across the wavefront.
+__device__ __forceinline__ uint4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                               uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return make_uint4(c0, c1, c2, c3);
+}
+
+// Counter word 2 of a site stream: sweep bits 32..47, chunk, purpose.
+__device__ __forceinline__ uint32_t ctr_word2(uint64_t sweep, uint32_t chunk, uint32_t purpose)
+{
+    return (uint32_t)((sweep >> 32) & 0xffffu) | (chunk << 16) | (purpose << 24);
+}
+
+// Site word of site (group, bit): word (bit & 3) of the block at counter
+// (8*group + bit/4, sweep, purpose).
+__device__ __forceinline__ uint32_t site_word(uint64_t seed, uint32_t key, uint32_t group, int bit,
+                                              uint64_t sweep, uint32_t purpose)
+{
+    const uint4 r = philox4x32_10(8u * group + (uint32_t)(bit >> 2), (uint32_t)sweep,
+                                  ctr_word2(sweep, 0, purpose), (uint32_t)(seed >> 32),
+                                  (uint32_t)seed, key);
+    const int k = bit & 3;
+    return k == 0 ? r.x : (k == 1 ? r.y : (k == 2 ? r.z : r.w));
+}
+
+__device__ __forceinline__ int word_width(int q, int half)
+{
+    const int w = half - 32 * q;
+    return w > 32 ? 32 : w;
+}
+
+__device__ __forceinline__ uint32_t width_mask(int W)
+{
+    return W >= 32 ? 0xffffffffu : ((1u << W) - 1u);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Bit-sliced population count of 4 or 6 one-bit-per-site words:
+// count = b0 + 2*b1 + 4*b2 per bit position.
+__device__ __forceinline__ void count4(uint32_t x1, uint32_t x2, uint32_t x3, uint32_t x4,
+                                       uint32_t& b0, uint32_t& b1, uint32_t& b2)
+{
+    const uint32_t s = x1 ^ x2 ^ x3;
+    const uint32_t c = (x1 & x2) | (x3 & (x1 | x2));
+    b0 = s ^ x4;
+    const uint32_t c2 = s & x4;
+    b1 = c ^ c2;
+    b2 = c & c2;
+}
+
+__device__ __forceinline__ void count6(uint32_t x1, uint32_t x2, uint32_t x3, uint32_t x4,
+                                       uint32_t x5, uint32_t x6, uint32_t& b0, uint32_t& b1,
+                                       uint32_t& b2)
+{
+    const uint32_t s1 = x1 ^ x2 ^ x3;
+    const uint32_t c1 = (x1 & x2) | (x3 & (x1 | x2));
+    const uint32_t s2 = x4 ^ x5 ^ x6;
+    const uint32_t c2 = (x4 & x5) | (x6 & (x4 | x5));
+    b0 = s1 ^ s2;
+    const uint32_t k = s1 & s2;
+    b1 = c1 ^ c2 ^ k;
+    b2 = (c1 & c2) | (k & (c1 | c2));
+}
+
+// Mask of positions whose 3-bit count equals K.
+template <int K>
+__device__ __forceinline__ uint32_t count_eq(uint32_t b0, uint32_t b1, uint32_t b2)
+{
+    const uint32_t m0 = (K & 1) ? b0 : ~b0;
+    const uint32_t m1 = (K & 2) ? b1 : ~b1;
+    const uint32_t m2 = (K & 4) ? b2 : ~b2;
+    return m0 & m1 & m2;
+}
+
+} // namespace mcx

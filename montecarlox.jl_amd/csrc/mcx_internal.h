@@ -1,0 +1,113 @@
+// mcx_internal.h -- device-side records shared by the host library and the
+// HIP kernels of libmcx_hip.so.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mcx {
+
+// Random-stream purposes: top byte of Philox counter word 2.
+enum Purpose : uint32_t {
+    P_METRO = 1,    // Ising Metropolis acceptance (bit-sliced uniform)
+    P_HB = 2,       // Ising heat-bath draw (bit-sliced uniform)
+    P_INIT = 3,     // random initial state (site word)
+    P_BC_METRO = 4, // Blume-Capel Metropolis: bit 0 = proposal, bits 31..1 = uniform
+    P_BC_HB = 5,    // Blume-Capel heat bath (site word)
+    P_GEN = 6,      // multicanonical / Wang-Landau acceptance (site word)
+    P_PT = 8        // replica-exchange shared uniform
+};
+
+// One chain (lattice + its stream position) as the kernels see it.
+struct ChainRec {
+    int64_t spin_off;   // byte offset of the chain's int8 lattice in the spin pool
+    int64_t series_off; // element offset of the chain's series (chain * capacity)
+    uint64_t sweep;     // sweeps done so far: the Philox counter position
+    int32_t L;          // even side length >= 4
+    int32_t dim;        // 2 or 3
+    int32_t table;      // index into the acceptance tables
+    uint32_t key;       // Philox key word 1
+    double D;           // Blume-Capel crystal field
+};
+
+// Canonical Ising acceptance for one temperature.  thr = low 32 bits of
+// ceil(p * 2^32); a uniform word w accepts iff w < thr.  Bit k of `always`
+// marks classes with ceil(p*2^32) >= 2^32 (always accept / always up), bit k
+// of `never` classes with p == 0.
+struct IsingTable {
+    uint32_t metro_thr[4]; // class = number of anti-aligned neighbours (< dim)
+    uint32_t metro_always, metro_never;
+    uint32_t hb_thr[8];    // class = number of up neighbours (0..2*dim)
+    uint32_t hb_always, hb_never;
+    uint32_t pad[2];
+};
+
+// Blume-Capel acceptance for one (beta, D).  Metropolis: per (s_old, alt,
+// sum_nbr) the threshold ceil(p * 2^31) on the 31-bit uniform w >> 1, with
+// index ((s_old+1)*2 + alt) * NS + (sum_nbr + 2*dim), NS = 4*dim + 1; bit k of
+// metro_always marks p >= 1.  Heat bath: per sum_nbr the FP64 triple
+// (w1, w1 + w2, z) of SpinSystems/src/blume_capel.jl:66-76.
+struct BCTable {
+    uint32_t metro_thr[80];
+    uint32_t metro_always[3];
+    uint32_t pad;
+    double hb[13][3];
+};
+
+struct SweepArgs {
+    const int32_t* items; // chains to process, in launch order
+    int32_t n_items;
+    int32_t record;       // write series + moments
+    ChainRec* chains;
+    const IsingTable* itab;
+    const BCTable* btab;
+    int8_t* spins;
+    int32_t* ser_E;       // E_pair = -sum s_i s_j
+    int32_t* ser_M;
+    int32_t* ser_Q;       // sum s_i^2 (Blume-Capel)
+    double* moments;      // n_chains * 6
+    unsigned long long* accepted;
+    int64_t nsweeps;
+    int64_t rec_pos;      // series index of the first sweep of this call
+    uint64_t seed;
+};
+
+struct PlainArgs {
+    ChainRec* chains;
+    const IsingTable* itab;
+    const BCTable* btab;
+    int8_t* spins;
+    int32_t* ser_E;
+    int32_t* ser_M;
+    int32_t* ser_Q;
+    unsigned long long* accepted;
+    int64_t rec_index;  // series index of this sweep (< 0: not recording)
+    uint64_t sweep;     // sweep counter value for this sweep
+    uint64_t seed;
+    int32_t chain;
+    int32_t colour;
+};
+
+// Launchers (defined in the .hip translation units).
+hipError_t launch_ising_sweeps(int dim, int alg, const SweepArgs& a, int block, int grid,
+                               size_t lds_bytes, hipStream_t st);
+hipError_t launch_ising_plain(int dim, int alg, const PlainArgs& a, int n_sites_colour,
+                              hipStream_t st);
+hipError_t launch_bc_sweeps(int dim, int alg, const SweepArgs& a, int block, int grid,
+                            size_t lds_bytes, hipStream_t st);
+hipError_t launch_bc_plain(int dim, int alg, const PlainArgs& a, int n_sites_colour,
+                           hipStream_t st);
+hipError_t launch_init_spins(int model, int mode, ChainRec* chains, int32_t n_chains,
+                             int64_t max_sites, int8_t* spins, uint64_t seed, hipStream_t st);
+hipError_t launch_observables(ChainRec* chains, int32_t n_chains, int64_t max_sites,
+                              const int8_t* spins, int32_t* E, int32_t* M, int32_t* Q,
+                              hipStream_t st);
+hipError_t launch_series_moments(ChainRec* chains, int32_t n_chains, const int32_t* E,
+                                 const int32_t* M, const int32_t* Q, int64_t first,
+                                 int64_t count, double* moments, hipStream_t st);
+hipError_t launch_advance_sweeps(ChainRec* chains, int32_t n_chains, int64_t nsweeps,
+                                 hipStream_t st);
+hipError_t launch_zero_series(ChainRec* chains, int32_t n_chains, int32_t* E, int32_t* M,
+                              int32_t* Q, int64_t first, int64_t count, hipStream_t st);
+
+} // namespace mcx

@@ -1,0 +1,463 @@
+// mcx_ising.hip -- checkerboard Metropolis / heat-bath sweeps for Ising
+// lattices (2D square, 3D simple cubic, periodic, J = 1, h = 0) on gfx950.
+//
+// Per-site rule (SpinSystems/src/ising.jl:35-58, src/algorithms/metropolis.jl:14-17,
+// src/algorithms/importance_sampling.jl:80-85):
+//   Metropolis: dE = 2 s_i sum_j s_j; accept iff -beta*dE > 0 or u < exp(-beta*dE)
+//   heat bath:  s_new = +1 iff u < logistic(beta * s_i * dE); flip iff s_new != s_i
+// visited in checkerboard order (colour 0 then colour 1; one colour's sites
+// have disjoint neighbourhoods, so the parallel update of a colour equals the
+// sequential one).
+//
+// Batched kernel (k_ising_sweeps): one workgroup per chain, the lattice held in
+// LDS as bit planes for all the sweeps of the call.  Plane word (c, r, q) holds
+// bit j of the 32 sites x = 2*(32q + j) + p of colour c in row r = y + L*z, with
+// p = (c + y + z) & 1, so one 32-bit VALU op updates 32 sites.  Each site's
+// uniform is bit-sliced: bit b (MSB first) of its 32-bit word w is bit j of
+// word (b & 3) of Philox block (group, sweep, chunk = b >> 2, purpose), group =
+// (2r + c)*nw + q.  u < p  <=>  w < ceil(p * 2^32) is decided bit-serially
+// MSB first, and a wavefront draws the next Philox block only while one of its
+// sites is still tied with its threshold -- usually three blocks for 32 sites.
+//
+// Plain kernel (k_ising_plain): the one-chain-per-launch path -- one launch per
+// colour per chain per sweep, one thread per site, int8 spins in HBM.  It
+// evaluates the same bit-sliced uniform and therefore gives identical results.
+#include "mcx_device.h"
+
+namespace mcx {
+
+struct NbrWords {
+    uint32_t A;  // other colour, same row, same bits (x-neighbour on one side)
+    uint32_t Bw; // other colour, same row, shifted (x-neighbour on the other side)
+    uint32_t U, Dn, Zu, Zd;
+};
+
+template <int DIM>
+__device__ __forceinline__ void load_nbr_words(const uint32_t* __restrict__ Po, int L, int nw,
+                                               int half, int r, int q, int y, int z, int p,
+                                               int W, uint32_t valid, NbrWords& n)
+{
+    const int base = DIM == 3 ? z * L : 0;
+    const int yu = y == 0 ? L - 1 : y - 1;
+    const int yd = y == L - 1 ? 0 : y + 1;
+    n.A = Po[r * nw + q];
+    n.U = Po[(base + yu) * nw + q];
+    n.Dn = Po[(base + yd) * nw + q];
+    if (DIM == 3) {
+        const int zu = z == 0 ? L - 1 : z - 1;
+        const int zd = z == L - 1 ? 0 : z + 1;
+        n.Zu = Po[(zu * L + y) * nw + q];
+        n.Zd = Po[(zd * L + y) * nw + q];
+    } else {
+        n.Zu = 0;
+        n.Zd = 0;
+    }
+    if (nw == 1) {
+        // whole half-row in one word: rotate within W bits
+        n.Bw = p ? (((n.A >> 1) | (n.A << (W - 1))) & valid)
+                 : (((n.A << 1) | (n.A >> (W - 1))) & valid);
+    } else if (p) {
+        // x = 2j+1: right neighbour is other-colour bit j+1
+        const int qn = q + 1 == nw ? 0 : q + 1;
+        n.Bw = (n.A >> 1) | ((Po[r * nw + qn] & 1u) << (W - 1));
+    } else {
+        // x = 2j: left neighbour is other-colour bit j-1
+        const int qp = q == 0 ? nw - 1 : q - 1;
+        const int Wp = word_width(qp, half);
+        n.Bw = ((n.A << 1) | ((Po[r * nw + qp] >> (Wp - 1)) & 1u)) & valid;
+    }
+}
+
+// Bit-serial "w < thr[class]" for the 32 sites of one word.  eq: sites still
+// undecided (tied so far); m[k]: sites of class k.  Returns the "less" mask.
+template <int NC>
+__device__ __forceinline__ uint32_t bitserial_less(uint32_t eq, const uint32_t (&m)[NC],
+                                                   const uint32_t (&thr)[NC], uint32_t group,
+                                                   uint32_t sw_lo, uint32_t w2, uint32_t c3,
+                                                   uint32_t k0, uint32_t k1)
+{
+    uint32_t lt = 0;
+    for (uint32_t chunk = 0; chunk < 8; ++chunk) {
+        if (!__any(eq != 0u))
+            break;
+        const uint4 R = philox4x32_10(group, sw_lo, w2 | (chunk << 16), c3, k0, k1);
+        const uint32_t Rb[4] = {R.x, R.y, R.z, R.w};
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const uint32_t sh = 31u - (4u * chunk + (uint32_t)b);
+            uint32_t Tb = 0;
+#pragma unroll
+            for (int k = 0; k < NC; ++k)
+                Tb |= m[k] & (0u - ((thr[k] >> sh) & 1u));
+            const uint32_t x = Rb[b] ^ Tb;
+            lt |= eq & Tb & x;
+            eq &= ~x;
+        }
+    }
+    return lt;
+}
+
+template <int DIM, int ALG>
+__global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
+{
+    extern __shared__ uint32_t lds[];
+    uint32_t* part = lds;    // [2 parity][16 waves][2]
+    uint32_t* P = lds + 64;  // planes [2 colours][rows][nw]
+    const int tid = threadIdx.x;
+    const int B = blockDim.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int nwaves = (B + 63) >> 6;
+    const uint32_t k0 = (uint32_t)a.seed;
+    const uint32_t c3 = (uint32_t)(a.seed >> 32);
+    constexpr uint32_t purpose = ALG == 0 ? P_METRO : P_HB;
+    constexpr int NMC = DIM;         // Metropolis uphill classes
+    constexpr int NHC = 2 * DIM + 1; // heat-bath classes
+
+    for (int item = blockIdx.x; item < a.n_items; item += gridDim.x) {
+        const int ch = a.items[item];
+        const ChainRec cr = a.chains[ch];
+        const int L = cr.L;
+        const int half = L >> 1;
+        const int nw = (half + 31) >> 5;
+        const int rows = DIM == 3 ? L * L : L;
+        const int words = rows * nw;
+        const int64_t N = (int64_t)rows * L;
+        int8_t* __restrict__ sp = a.spins + cr.spin_off;
+        const IsingTable tb = a.itab[cr.table];
+
+        // ---- int8 lattice -> bit planes (up = 1) ----
+        for (int w = tid; w < words; w += B) {
+            const int r = w / nw;
+            const int q = w - r * nw;
+            const int y = DIM == 3 ? r % L : r;
+            const int z = DIM == 3 ? r / L : 0;
+            const int par = (y + z) & 1; // colour of the even-x sites of this row
+            const int W = word_width(q, half);
+            const int8_t* row = sp + (int64_t)r * L + 64 * q;
+            uint32_t ev = 0, od = 0;
+            if ((L & 3) == 0) {
+                const uint32_t* row4 = reinterpret_cast<const uint32_t*>(row);
+                for (int k = 0; k < (W >> 1); ++k) {
+                    const uint32_t v = ~row4[k];
+                    ev |= (((v >> 7) & 1u) << (2 * k)) | (((v >> 23) & 1u) << (2 * k + 1));
+                    od |= (((v >> 15) & 1u) << (2 * k)) | (((v >> 31) & 1u) << (2 * k + 1));
+                }
+            } else {
+                for (int j = 0; j < W; ++j) {
+                    ev |= (uint32_t)(row[2 * j] > 0) << j;
+                    od |= (uint32_t)(row[2 * j + 1] > 0) << j;
+                }
+            }
+            P[par * words + w] = ev;
+            P[(par ^ 1) * words + w] = od;
+        }
+        __syncthreads();
+
+        unsigned long long acc = 0;
+        double mE = 0, mE2 = 0, mAM = 0, mM2 = 0, mM4 = 0;
+
+        for (int64_t s = 0; s < a.nsweeps; ++s) {
+            const uint64_t sweep = cr.sweep + (uint64_t)s;
+            const uint32_t sw_lo = (uint32_t)sweep;
+            const uint32_t w2 = ctr_word2(sweep, 0, purpose);
+            uint32_t nu = 0, na = 0;
+            for (int c = 0; c < 2; ++c) {
+                uint32_t* __restrict__ Pc = P + c * words;
+                const uint32_t* __restrict__ Po = P + (c ^ 1) * words;
+                for (int w = tid; w < words; w += B) {
+                    const int r = nw == 1 ? w : w / nw;
+                    const int q = w - r * nw;
+                    const int y = DIM == 3 ? r % L : r;
+                    const int z = DIM == 3 ? r / L : 0;
+                    const int p = (c + y + z) & 1;
+                    const int W = word_width(q, half);
+                    const uint32_t valid = width_mask(W);
+                    const uint32_t s0 = Pc[w];
+                    NbrWords n;
+                    load_nbr_words<DIM>(Po, L, nw, half, r, q, y, z, p, W, valid, n);
+                    const uint32_t group = (uint32_t)((r * 2 + c) * nw + q);
+                    uint32_t snew, flip;
+                    if (ALG == 0) {
+                        uint32_t b0, b1, b2;
+                        if (DIM == 2)
+                            count4(s0 ^ n.U, s0 ^ n.Dn, s0 ^ n.A, s0 ^ n.Bw, b0, b1, b2);
+                        else
+                            count6(s0 ^ n.U, s0 ^ n.Dn, s0 ^ n.A, s0 ^ n.Bw, s0 ^ n.Zu, s0 ^ n.Zd,
+                                   b0, b1, b2);
+                        uint32_t m[NMC];
+                        m[0] = count_eq<0>(b0, b1, b2) & valid;
+                        m[1] = count_eq<1>(b0, b1, b2) & valid;
+                        if (DIM == 3)
+                            m[NMC - 1] = count_eq<2>(b0, b1, b2) & valid;
+                        const uint32_t freem = DIM == 2 ? (b1 | b2) : (b2 | (b1 & b0));
+                        uint32_t forced = 0, need = 0;
+                        uint32_t thr[NMC];
+#pragma unroll
+                        for (int k = 0; k < NMC; ++k) {
+                            thr[k] = tb.metro_thr[k];
+                            if ((tb.metro_always >> k) & 1u)
+                                forced |= m[k];
+                            else if (!((tb.metro_never >> k) & 1u))
+                                need |= m[k];
+                        }
+                        const uint32_t lt =
+                            bitserial_less<NMC>(need, m, thr, group, sw_lo, w2, c3, k0, cr.key);
+                        flip = (freem | forced | lt) & valid;
+                        snew = s0 ^ flip;
+                    } else {
+                        uint32_t b0, b1, b2;
+                        if (DIM == 2)
+                            count4(n.U, n.Dn, n.A, n.Bw, b0, b1, b2);
+                        else
+                            count6(n.U, n.Dn, n.A, n.Bw, n.Zu, n.Zd, b0, b1, b2);
+                        uint32_t m[NHC];
+                        m[0] = count_eq<0>(b0, b1, b2) & valid;
+                        m[1] = count_eq<1>(b0, b1, b2) & valid;
+                        m[2] = count_eq<2>(b0, b1, b2) & valid;
+                        m[3] = count_eq<3>(b0, b1, b2) & valid;
+                        m[4] = count_eq<4>(b0, b1, b2) & valid;
+                        if (DIM == 3) {
+                            m[NHC - 2] = count_eq<5>(b0, b1, b2) & valid;
+                            m[NHC - 1] = count_eq<6>(b0, b1, b2) & valid;
+                        }
+                        uint32_t up = 0, need = 0;
+                        uint32_t thr[NHC];
+#pragma unroll
+                        for (int k = 0; k < NHC; ++k) {
+                            thr[k] = tb.hb_thr[k];
+                            if ((tb.hb_always >> k) & 1u)
+                                up |= m[k];
+                            else if (!((tb.hb_never >> k) & 1u))
+                                need |= m[k];
+                        }
+                        const uint32_t lt =
+                            bitserial_less<NHC>(need, m, thr, group, sw_lo, w2, c3, k0, cr.key);
+                        snew = (up | lt) & valid;
+                        flip = snew ^ s0;
+                    }
+                    acc += (unsigned long long)__popc(flip);
+                    Pc[w] = snew;
+                    if (c == 1 && a.record) {
+                        // every bond joins colour 0 and colour 1: count each from its colour-1 end
+                        na += __popc(snew ^ n.U) + __popc(snew ^ n.Dn) + __popc(snew ^ n.A)
+                            + __popc(snew ^ n.Bw);
+                        if (DIM == 3)
+                            na += __popc(snew ^ n.Zu) + __popc(snew ^ n.Zd);
+                        nu += __popc(snew) + __popc(n.A);
+                    }
+                }
+                if (c == 1 && a.record) {
+                    nu = wave_sum(nu);
+                    na = wave_sum(na);
+                    if (lane == 0) {
+                        part[(s & 1) * 32 + wave * 2] = nu;
+                        part[(s & 1) * 32 + wave * 2 + 1] = na;
+                    }
+                }
+                __syncthreads();
+            }
+            if (a.record && tid == 0) {
+                uint32_t tu = 0, ta = 0;
+                for (int k = 0; k < nwaves; ++k) {
+                    tu += part[(s & 1) * 32 + k * 2];
+                    ta += part[(s & 1) * 32 + k * 2 + 1];
+                }
+                const int32_t E = (int32_t)(2 * (int64_t)ta - (int64_t)DIM * N);
+                const int32_t M = (int32_t)(2 * (int64_t)tu - N);
+                const int64_t at = cr.series_off + a.rec_pos + s;
+                a.ser_E[at] = E;
+                a.ser_M[at] = M;
+                const double e = (double)E, m = (double)M;
+                mE += e;
+                mE2 += e * e;
+                mAM += fabs(m);
+                mM2 += m * m;
+                mM4 += (m * m) * (m * m);
+            }
+        }
+
+        // ---- bit planes -> int8 lattice ----
+        for (int w = tid; w < words; w += B) {
+            const int r = w / nw;
+            const int q = w - r * nw;
+            const int y = DIM == 3 ? r % L : r;
+            const int z = DIM == 3 ? r / L : 0;
+            const int par = (y + z) & 1;
+            const int W = word_width(q, half);
+            int8_t* row = sp + (int64_t)r * L + 64 * q;
+            const uint32_t ev = P[par * words + w];
+            const uint32_t od = P[(par ^ 1) * words + w];
+            if ((L & 3) == 0) {
+                uint32_t* row4 = reinterpret_cast<uint32_t*>(row);
+                for (int k = 0; k < (W >> 1); ++k) {
+                    const uint32_t e0 = (ev >> (2 * k)) & 1u, e1 = (ev >> (2 * k + 1)) & 1u;
+                    const uint32_t o0 = (od >> (2 * k)) & 1u, o1 = (od >> (2 * k + 1)) & 1u;
+                    row4[k] = (0xFFu - 0xFEu * e0) | ((0xFFu - 0xFEu * o0) << 8)
+                            | ((0xFFu - 0xFEu * e1) << 16) | ((0xFFu - 0xFEu * o1) << 24);
+                }
+            } else {
+                for (int j = 0; j < W; ++j) {
+                    row[2 * j] = ((ev >> j) & 1u) ? 1 : -1;
+                    row[2 * j + 1] = ((od >> j) & 1u) ? 1 : -1;
+                }
+            }
+        }
+        acc = wave_sum(acc);
+        if (lane == 0 && acc)
+            atomicAdd(&a.accepted[ch], acc);
+        if (tid == 0) {
+            if (a.record) {
+                double* mo = a.moments + (int64_t)ch * 6;
+                mo[0] += (double)a.nsweeps;
+                mo[1] += mE;
+                mo[2] += mE2;
+                mo[3] += mAM;
+                mo[4] += mM2;
+                mo[5] += mM4;
+            }
+            a.chains[ch].sweep = cr.sweep + (uint64_t)a.nsweeps;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain path: one thread per site of one colour of one chain.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t bitserial_less_site(uint32_t thr, uint32_t group, int t,
+                                                        uint32_t sw_lo, uint32_t w2, uint32_t c3,
+                                                        uint32_t k0, uint32_t k1)
+{
+    for (uint32_t chunk = 0; chunk < 8; ++chunk) {
+        const uint4 R = philox4x32_10(group, sw_lo, w2 | (chunk << 16), c3, k0, k1);
+        const uint32_t Rb[4] = {R.x, R.y, R.z, R.w};
+        for (int b = 0; b < 4; ++b) {
+            const uint32_t bit = (Rb[b] >> t) & 1u;
+            const uint32_t tbit = (thr >> (31u - (4u * chunk + (uint32_t)b))) & 1u;
+            if (bit != tbit)
+                return tbit; // w < thr iff the first differing bit is 0 in w
+        }
+    }
+    return 0u; // w == thr
+}
+
+template <int DIM, int ALG>
+__global__ void __launch_bounds__(256) k_ising_plain(PlainArgs a, int n_col)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const ChainRec cr = a.chains[a.chain];
+    const int L = cr.L;
+    const int half = L >> 1;
+    const int nw = (half + 31) >> 5;
+    const int c = a.colour;
+    int e_part = 0, m_part = 0;
+    unsigned long long flips = 0;
+    if (idx < n_col) {
+        const int r = idx / half;
+        const int j = idx - r * half;
+        const int y = DIM == 3 ? r % L : r;
+        const int z = DIM == 3 ? r / L : 0;
+        const int p = (c + y + z) & 1;
+        const int x = 2 * j + p;
+        int8_t* sp = a.spins + cr.spin_off;
+        const int64_t rowb = (int64_t)L * r;
+        const int xl = x == 0 ? L - 1 : x - 1;
+        const int xr = x == L - 1 ? 0 : x + 1;
+        const int yu = y == 0 ? L - 1 : y - 1;
+        const int yd = y == L - 1 ? 0 : y + 1;
+        const int64_t zb = DIM == 3 ? (int64_t)z * L * L : 0;
+        int nb = sp[rowb + xl] + sp[rowb + xr] + sp[zb + (int64_t)yu * L + x]
+               + sp[zb + (int64_t)yd * L + x];
+        if (DIM == 3) {
+            const int zu = z == 0 ? L - 1 : z - 1;
+            const int zd = z == L - 1 ? 0 : z + 1;
+            nb += sp[(int64_t)zu * L * L + (int64_t)y * L + x]
+                + sp[(int64_t)zd * L * L + (int64_t)y * L + x];
+        }
+        const int8_t si = sp[rowb + x];
+        const IsingTable& tb = a.itab[cr.table];
+        const uint32_t group = (uint32_t)((r * 2 + c) * nw + (j >> 5));
+        const int t = j & 31;
+        const uint32_t sw_lo = (uint32_t)a.sweep;
+        const uint32_t k0 = (uint32_t)a.seed, c3 = (uint32_t)(a.seed >> 32);
+        int8_t snew = si;
+        if (ALG == 0) {
+            const int n_anti = (2 * DIM - si * nb) >> 1;
+            bool ok;
+            if (n_anti >= DIM)
+                ok = true;
+            else if ((tb.metro_always >> n_anti) & 1u)
+                ok = true;
+            else if ((tb.metro_never >> n_anti) & 1u)
+                ok = false;
+            else
+                ok = bitserial_less_site(tb.metro_thr[n_anti], group, t, sw_lo,
+                                         ctr_word2(a.sweep, 0, P_METRO), c3, k0, cr.key) != 0u;
+            if (ok)
+                snew = (int8_t)-si;
+        } else {
+            const int n_up = (nb + 2 * DIM) >> 1;
+            bool up;
+            if ((tb.hb_always >> n_up) & 1u)
+                up = true;
+            else if ((tb.hb_never >> n_up) & 1u)
+                up = false;
+            else
+                up = bitserial_less_site(tb.hb_thr[n_up], group, t, sw_lo,
+                                         ctr_word2(a.sweep, 0, P_HB), c3, k0, cr.key) != 0u;
+            snew = up ? 1 : -1;
+        }
+        if (snew != si) {
+            sp[rowb + x] = snew;
+            flips = 1;
+        }
+        if (c == 1 && a.rec_index >= 0) {
+            e_part = -(int)snew * nb;
+            m_part = snew + sp[rowb + (p ? xl : xr)];
+        }
+    }
+    flips = wave_sum(flips);
+    if ((threadIdx.x & 63) == 0 && flips)
+        atomicAdd(&a.accepted[a.chain], flips);
+    if (c == 1 && a.rec_index >= 0) {
+        e_part = wave_sum(e_part);
+        m_part = wave_sum(m_part);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&a.ser_E[cr.series_off + a.rec_index], e_part);
+            atomicAdd(&a.ser_M[cr.series_off + a.rec_index], m_part);
+        }
+    }
+}
+
+hipError_t launch_ising_sweeps(int dim, int alg, const SweepArgs& a, int block, int grid,
+                               size_t lds_bytes, hipStream_t st)
+{
+    if (dim == 2 && alg == 0)
+        hipLaunchKernelGGL((k_ising_sweeps<2, 0>), dim3(grid), dim3(block), lds_bytes, st, a);
+    else if (dim == 2)
+        hipLaunchKernelGGL((k_ising_sweeps<2, 1>), dim3(grid), dim3(block), lds_bytes, st, a);
+    else if (alg == 0)
+        hipLaunchKernelGGL((k_ising_sweeps<3, 0>), dim3(grid), dim3(block), lds_bytes, st, a);
+    else
+        hipLaunchKernelGGL((k_ising_sweeps<3, 1>), dim3(grid), dim3(block), lds_bytes, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_ising_plain(int dim, int alg, const PlainArgs& a, int n_col, hipStream_t st)
+{
+    const int block = 256;
+    const int grid = (n_col + block - 1) / block;
+    if (dim == 2 && alg == 0)
+        hipLaunchKernelGGL((k_ising_plain<2, 0>), dim3(grid), dim3(block), 0, st, a, n_col);
+    else if (dim == 2)
+        hipLaunchKernelGGL((k_ising_plain<2, 1>), dim3(grid), dim3(block), 0, st, a, n_col);
+    else if (alg == 0)
+        hipLaunchKernelGGL((k_ising_plain<3, 0>), dim3(grid), dim3(block), 0, st, a, n_col);
+    else
+        hipLaunchKernelGGL((k_ising_plain<3, 1>), dim3(grid), dim3(block), 0, st, a, n_col);
+    return hipGetLastError();
+}
+
+} // namespace mcx

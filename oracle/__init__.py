@@ -1,0 +1,208 @@
+"""CPU oracle for the MonteCarloX spin-lattice path -- TEST INFRASTRUCTURE ONLY.
+
+ctypes wrapper around ``oracle/liboracle.so`` (built from ``mcx_oracle.c`` by
+``oracle/Makefile``).  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this package; the product never
+does.  It is the checker, never the thing measured or shipped.
+
+Pinning status of each function (what checks it besides the GPU parity tests):
+  philox4x32_10        Random123 known-answer vectors + rocRAND's host engine
+  canonical_sweeps     reference bookkeeping KATs, cached == full invariants,
+                       brute-force 4x4 / 3x3 enumerations, Kaufman exact Z (L=16),
+                       Beale exact DOS (L=8), detailed balance on 2x... enumerations
+  generalized_sweeps   reference muca/WL KATs (flat weights accept all, histogram
+                       total == steps, WL decrement), flat-histogram convergence
+                       to the Beale DOS at L=8
+  pt_round             reference exchange KATs (lr = 2.5, u=0 accepts, u=1 rejects)
+  refpath_ising2d      timing only (CPU baseline); detailed-balance statistics
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+P_METRO, P_HB, P_INIT, P_BC_METRO, P_BC_HB, P_GEN, P_PT = 1, 2, 3, 4, 5, 6, 8
+
+_lib = None
+
+
+def build() -> None:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_i8p = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class _NullablePtr:
+    """ndpointer that also accepts None."""
+
+    def __init__(self, base):
+        self.base = base
+
+    def from_param(self, obj):
+        if obj is None:
+            return None
+        return self.base.from_param(obj)
+
+
+def _declare(L):
+    L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+    L.orc_site_group.argtypes = [C.c_int, C.c_int, C.c_int64, C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.orc_bitsliced_word.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_uint64, C.c_int]
+    L.orc_bitsliced_word.restype = C.c_uint32
+    L.orc_site_word.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_uint64, C.c_int]
+    L.orc_site_word.restype = C.c_uint32
+    L.orc_pt_word.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64]
+    L.orc_pt_word.restype = C.c_uint32
+    L.orc_logistic.argtypes = [C.c_double]
+    L.orc_logistic.restype = C.c_double
+    L.orc_threshold32.argtypes = [C.c_double]
+    L.orc_threshold32.restype = C.c_uint64
+    for f in ("orc_pair_sum", "orc_magnetization", "orc_sum_sq"):
+        getattr(L, f).argtypes = [C.c_int, C.c_int, _i8p]
+        getattr(L, f).restype = C.c_int64
+    L.orc_init_random.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint32, C.c_uint64, _i8p]
+    n32 = _NullablePtr(_i32p)
+    L.orc_canonical_sweeps.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                       C.c_uint64, C.c_uint32, C.c_uint64, C.c_int64, _i8p,
+                                       n32, n32, n32, C.POINTER(C.c_int64)]
+    L.orc_generalized_sweeps.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                         C.c_double, C.c_double, C.c_int64, _f64p, _i64p, C.c_double,
+                                         C.c_uint64, C.c_uint32, C.c_uint64, C.c_int64, _i8p,
+                                         C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                         n32, n32, n32, C.POINTER(C.c_int64)]
+    L.orc_generalized_sweeps.restype = C.c_int
+    L.orc_pt_round.argtypes = [C.c_int, _f64p, _f64p, _i32p, _i32p, _i32p, C.POINTER(C.c_int32),
+                               C.c_uint64, _u32p, C.c_uint64]
+    L.orc_muca_update.argtypes = [C.c_int64, _f64p, _f64p]
+    L.orc_refpath_ising2d.argtypes = [C.c_int, C.c_int, _f64p, C.c_uint64, C.c_int64, C.c_int,
+                                      _NullablePtr(_i8p)]
+    L.orc_refpath_ising2d.restype = C.c_int64
+
+
+# --------------------------------------------------------------------------
+# thin Python API
+# --------------------------------------------------------------------------
+
+def philox4x32_10(ctr, key):
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(np.asarray(ctr, np.uint32), np.asarray(key, np.uint32), out)
+    return out
+
+
+def site_group(dim, L, site):
+    g, b, c = C.c_uint32(), C.c_int(), C.c_int()
+    lib().orc_site_group(dim, L, site, C.byref(g), C.byref(b), C.byref(c))
+    return g.value, b.value, c.value
+
+
+def bitsliced_word(seed, chain_key, group, bit, sweep, purpose):
+    return lib().orc_bitsliced_word(seed, chain_key, group, bit, sweep, purpose)
+
+
+def site_word(seed, chain_key, group, bit, sweep, purpose):
+    return lib().orc_site_word(seed, chain_key, group, bit, sweep, purpose)
+
+
+def pt_word(seed, chain_key, rnd):
+    return lib().orc_pt_word(seed, chain_key, rnd)
+
+
+def logistic(x):
+    return lib().orc_logistic(x)
+
+
+def threshold32(p):
+    return lib().orc_threshold32(p)
+
+
+def pair_sum(dim, L, spins):
+    return lib().orc_pair_sum(dim, L, np.ascontiguousarray(spins, np.int8))
+
+
+def magnetization(dim, L, spins):
+    return lib().orc_magnetization(dim, L, np.ascontiguousarray(spins, np.int8))
+
+
+def sum_sq(dim, L, spins):
+    return lib().orc_sum_sq(dim, L, np.ascontiguousarray(spins, np.int8))
+
+
+def init_random(model, dim, L, seed, chain_key, sweep=0):
+    N = L ** dim
+    s = np.zeros(N, np.int8)
+    lib().orc_init_random(model, dim, L, seed, chain_key, sweep, s)
+    return s
+
+
+def canonical_sweeps(model, alg, dim, L, beta, spins, seed, chain_key, sweep0, nsweeps, D=0.0):
+    """Run nsweeps in place on `spins` (int8, modified).  Returns (E_pair, M, Q, accepted)."""
+    E = np.zeros(nsweeps, np.int32)
+    M = np.zeros(nsweeps, np.int32)
+    Q = np.zeros(nsweeps, np.int32)
+    acc = C.c_int64(0)
+    lib().orc_canonical_sweeps(model, alg, dim, L, beta, D, seed, chain_key, sweep0, nsweeps,
+                               spins, E, M, Q, C.byref(acc))
+    return E, M, Q, acc.value
+
+
+class BoundsError(IndexError):
+    pass
+
+
+def generalized_sweeps(model, wl, dim, L, lw, hist, spins, seed, chain_key, sweep0, nsweeps,
+                       bin_start, bin_step, logf=0.0, D=0.0):
+    """Sequential multicanonical (wl=0) / Wang-Landau (wl=1) sweeps, in place on
+    spins, lw (float64) and hist (int64).  Returns (E_pair, M, Q, steps, accepted)."""
+    E = np.zeros(nsweeps, np.int32)
+    M = np.zeros(nsweeps, np.int32)
+    Q = np.zeros(nsweeps, np.int32)
+    steps, acc, err = C.c_int64(0), C.c_int64(0), C.c_int64(-1)
+    rc = lib().orc_generalized_sweeps(model, wl, dim, L, D, float(bin_start), float(bin_step),
+                                      len(lw), lw, hist, logf, seed, chain_key, sweep0, nsweeps,
+                                      spins, C.byref(steps), C.byref(acc), E, M, Q, C.byref(err))
+    if rc == 2:
+        raise BoundsError(f"energy bin out of range at site {err.value}")
+    return E, M, Q, steps.value, acc.value
+
+
+def pt_round(betas_of_ladder, x, indices, steps, accepted, stage, seed, chain_keys, rnd):
+    """One exchange round; mutates indices/steps/accepted, returns new stage."""
+    st = C.c_int32(stage)
+    lib().orc_pt_round(len(x), np.ascontiguousarray(betas_of_ladder, np.float64),
+                       np.ascontiguousarray(x, np.float64), indices, steps, accepted,
+                       C.byref(st), seed, np.ascontiguousarray(chain_keys, np.uint32), rnd)
+    return st.value
+
+
+def muca_update(lw, h):
+    lib().orc_muca_update(len(lw), lw, np.ascontiguousarray(h, np.float64))
+
+
+def refpath_ising2d(L, betas, seed, nsweeps, nthreads, keep_spins=False):
+    betas = np.ascontiguousarray(betas, np.float64)
+    out = np.zeros(len(betas) * L * L, np.int8) if keep_spins else None
+    n = lib().orc_refpath_ising2d(L, len(betas), betas, seed, nsweeps, nthreads, out)
+    return n, out

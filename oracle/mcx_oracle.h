@@ -1,0 +1,108 @@
+/*
+ * mcx_oracle.h -- CPU oracle for the MonteCarloX spin-lattice sampling path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * library is compared against; only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product never links it.
+ *
+ * Everything here is a straight, sequential restatement of the reference's
+ * per-site update rules (zierenberg/MonteCarloX.jl + SpinSystems) under the
+ * arithmetic contract the HIP library implements (checkerboard site order and
+ * counter-based Philox4x32-10 streams instead of a mutable Xoshiro state).
+ * Each function cites the reference lines it follows.
+ */
+#ifndef MCX_ORACLE_H
+#define MCX_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Random-stream purposes (top byte of counter word 2). */
+enum {
+    ORC_P_METRO = 1,   /* Ising Metropolis acceptance, bit-sliced uniform   */
+    ORC_P_HB = 2,      /* Ising heat-bath draw, bit-sliced uniform          */
+    ORC_P_INIT = 3,    /* random initial state, site word                   */
+    ORC_P_BC_METRO = 4,/* Blume-Capel Metropolis: proposal bit + 31-bit u   */
+    ORC_P_BC_HB = 5,   /* Blume-Capel heat bath: 32-bit u                   */
+    ORC_P_GEN = 6,     /* multicanonical / Wang-Landau acceptance, site word */
+    ORC_P_PT = 8       /* replica-exchange shared uniform                   */
+};
+
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11). */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Site -> (group, bit) map of the checkerboard layout. */
+void orc_site_group(int dim, int L, int64_t site, uint32_t* group, int* bit, int* colour);
+
+/* 32-bit uniforms of the contract. */
+uint32_t orc_bitsliced_word(uint64_t seed, uint32_t chain_key, uint32_t group, int bit,
+                            uint64_t sweep, int purpose);
+uint32_t orc_site_word(uint64_t seed, uint32_t chain_key, uint32_t group, int bit,
+                       uint64_t sweep, int purpose);
+uint32_t orc_pt_word(uint64_t seed, uint32_t chain_key, uint64_t round);
+
+/* Helpers of the reference arithmetic. */
+double orc_logistic(double x);
+uint64_t orc_threshold32(double p);   /* ceil(p * 2^32), p in [0, 1] */
+
+/* Observables: E = -J sum_<ij> s_i s_j (+ D sum s^2), M = sum s_i. */
+int64_t orc_pair_sum(int dim, int L, const int8_t* spins);      /* sum_<ij> s_i s_j */
+int64_t orc_magnetization(int dim, int L, const int8_t* spins);
+int64_t orc_sum_sq(int dim, int L, const int8_t* spins);
+
+/* Random initial states (init!(sys, :random)). model: 0 Ising, 1 Blume-Capel */
+void orc_init_random(int model, int dim, int L, uint64_t seed, uint32_t chain_key,
+                     uint64_t sweep, int8_t* spins);
+
+/* Canonical sweeps on one chain.
+ *   model 0 (Ising): alg 0 Metropolis, 1 heat bath.
+ *   model 1 (Blume-Capel): alg 0 Metropolis, 1 heat bath; D = crystal field.
+ * Per sweep, after both colours: E_pair[s] = -sum s_i s_j, M[s] = sum s_i,
+ * Q[s] = sum s_i^2 (any of the three may be NULL).
+ * *accepted accumulates accepted moves (Metropolis) / changed sites (heat bath). */
+void orc_canonical_sweeps(int model, int alg, int dim, int L, double beta, double D,
+                          uint64_t seed, uint32_t chain_key, uint64_t sweep0, int64_t nsweeps,
+                          int8_t* spins, int32_t* E_pair, int32_t* M, int32_t* Q,
+                          int64_t* accepted);
+
+/* Generalised-ensemble (multicanonical / Wang-Landau) sweeps on one walker.
+ * Bins: DiscreteBinning(start, step, num).  Ising energies use integer div
+ * binning, Blume-Capel energies (Float64 in the reference) use round-half-even.
+ * wl = 0: multicanonical (hist[visited] += 1 per attempt)
+ * wl = 1: Wang-Landau (lw[visited] -= logf per attempt; hist still counted).
+ * Returns 0, or 2 (bounds error: *err_site set; state as at the failing attempt). */
+int orc_generalized_sweeps(int model, int wl, int dim, int L, double D,
+                           double bin_start, double bin_step, int64_t nbins,
+                           double* lw, int64_t* hist, double logf,
+                           uint64_t seed, uint32_t chain_key, uint64_t sweep0, int64_t nsweeps,
+                           int8_t* spins, int64_t* steps, int64_t* accepted,
+                           int32_t* E_pair, int32_t* M, int32_t* Q, int64_t* err_site);
+
+/* One replica-exchange round on one set of n replicas (1-based ladder indices
+ * in `indices`, as in ReplicaExchange.indices).  beta_of_ladder[k-1] is the
+ * beta of ladder index k; x[r] is replica r's energy.  steps/accepted have
+ * n-1 entries.  *stage flips; round is the set's exchange-round counter. */
+void orc_pt_round(int n, const double* beta_of_ladder, const double* x, int32_t* indices,
+                  int32_t* steps, int32_t* accepted, int32_t* stage, uint64_t seed,
+                  const uint32_t* chain_keys, uint64_t round);
+
+/* Multicanonical weight recursion: lw[i] -= log(h[i]) where h[i] > 0. */
+void orc_muca_update(int64_t nbins, double* lw, const double* h);
+
+/* ---- reference CPU path (timed as the CPU baseline) ----------------------
+ * Random-site single-spin-flip Metropolis exactly as spin_flip!(IsingLatticeOptim,
+ * Metropolis) does it: i = rand(UInt) % N, rand() < exp(-beta*dE), exp per
+ * uphill move; Xoshiro256++ per chain seeded by SplitMix64(seed + chain).
+ * Runs `nchains` independent L x L chains over `nthreads` POSIX threads.
+ * Returns total attempts made. */
+int64_t orc_refpath_ising2d(int L, int nchains, const double* betas, uint64_t seed,
+                            int64_t nsweeps, int nthreads, int8_t* spins_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

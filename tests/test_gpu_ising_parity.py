@@ -1,0 +1,132 @@
+"""Bit-exact parity of the HIP Ising kernels with the CPU oracle.
+
+Every case runs the batched (LDS, bit-plane) kernel and / or the plain
+one-chain-per-launch kernel through the C ABI and compares final spins, the
+per-sweep E/M series and the accepted-move counters with the oracle's
+sequential checkerboard restatement on the same seeded inputs.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ISING, METRO, HB = 0, 0, 1
+
+
+def _run_case(mcx, orc, dim, alg, Ls, betas, nsweeps, seed=7, plain=False, keys=None,
+              runs=(None,)):
+    n = len(Ls)
+    print(f"case dim={dim} alg={alg} L={list(Ls)} nsweeps={nsweeps} plain={plain}", flush=True)
+    b = mcx.LatticeBatch("ising", dim, list(Ls), seed=seed, keys=keys, series_capacity=nsweeps)
+    b.set_betas(betas)
+    b.init("random")
+    start = [b.get_spins(k) for k in range(n)]
+    parts = [nsweeps] if runs == (None,) else list(runs)
+    assert sum(parts) == nsweeps
+    for p in parts:
+        b.run("heatbath" if alg == HB else "metropolis", p, record=True, plain=plain)
+    steps, acc = b.counters()
+    for k in range(n):
+        key = k if keys is None else keys[k]
+        s0 = orc.init_random(0, dim, Ls[k], seed, key, 0)
+        assert np.array_equal(s0, start[k]), f"chain {k}: initial state differs"
+        s = s0.copy()
+        E, M, _, a = orc.canonical_sweeps(ISING, alg, dim, Ls[k], float(betas[k]), s, seed, key, 0,
+                                          nsweeps)
+        Eg, Mg, _ = b.series(k)
+        assert np.array_equal(s, b.get_spins(k)), f"chain {k} (L={Ls[k]}): spins differ"
+        assert np.array_equal(E, Eg), f"chain {k}: E series differs"
+        assert np.array_equal(M, Mg), f"chain {k}: M series differs"
+        assert acc[k] == a, f"chain {k}: accepted {acc[k]} != {a}"
+        assert steps[k] == Ls[k] ** dim * nsweeps
+    return b
+
+
+@pytest.mark.parametrize("L", [4, 6, 16, 18, 32, 64, 66, 130])
+def test_2d_metropolis_sizes(mcx, orc, L):
+    betas = [0.2, 0.4406868, 0.7, 1.0]
+    _run_case(mcx, orc, 2, METRO, [L] * 4, betas, 12)
+
+
+@pytest.mark.parametrize("L", [4, 10, 16, 66])
+def test_2d_heatbath_sizes(mcx, orc, L):
+    _run_case(mcx, orc, 2, HB, [L] * 3, [0.1, 0.4406868, 0.9], 10)
+
+
+@pytest.mark.parametrize("L", [4, 8, 12, 16])
+def test_3d_metropolis(mcx, orc, L):
+    _run_case(mcx, orc, 3, METRO, [L] * 3, [0.15, 0.2216544, 0.35], 6)
+
+
+@pytest.mark.parametrize("L", [4, 8, 14])
+def test_3d_heatbath(mcx, orc, L):
+    _run_case(mcx, orc, 3, HB, [L] * 2, [0.1, 0.2216544], 6)
+
+
+def test_extreme_betas(mcx, orc):
+    # beta = 0: every move accepted (threshold 2^32); beta = 100: exp underflows to 0
+    _run_case(mcx, orc, 2, METRO, [16] * 4, [0.0, 1e-12, 5.0, 100.0], 8)
+    _run_case(mcx, orc, 2, HB, [16] * 3, [0.0, 5.0, 100.0], 8)
+
+
+def test_uneven_sizes_one_launch(mcx, orc):
+    Ls = [64, 16, 32, 48, 8, 130, 64, 4]
+    betas = np.linspace(0.25, 0.8, len(Ls))
+    _run_case(mcx, orc, 2, METRO, Ls, betas, 9)
+
+
+def test_plain_equals_batched_equals_oracle(mcx, orc):
+    for alg in (METRO, HB):
+        _run_case(mcx, orc, 2, alg, [32, 18], [0.44, 0.3], 5, plain=True)
+        _run_case(mcx, orc, 3, alg, [8], [0.22], 3, plain=True)
+
+
+def test_split_runs_resume(mcx, orc):
+    # n + m sweeps in separate calls equal one call of n + m (counter-based streams)
+    _run_case(mcx, orc, 2, METRO, [32, 32], [0.44, 0.5], 17, runs=(5, 1, 11))
+    _run_case(mcx, orc, 2, HB, [16], [0.44], 9, runs=(4, 5), plain=False)
+
+
+def test_custom_keys_and_seed(mcx, orc):
+    _run_case(mcx, orc, 2, METRO, [16, 16, 16], [0.4, 0.4, 0.4], 6, seed=(1 << 40) + 3,
+              keys=[7, 123456, 2 ** 32 - 1])
+
+
+def test_record_flag_does_not_change_dynamics(mcx):
+    outs = []
+    for record in (False, True):
+        b = mcx.LatticeBatch("ising", 2, 64, n_chains=8, seed=11, series_capacity=20)
+        b.set_betas(np.linspace(0.3, 0.6, 8))
+        b.init("random")
+        b.run("metropolis", 20, record=record)
+        outs.append([b.get_spins(k) for k in range(8)])
+        assert b.counters()[1].sum() > 0
+    for a, c in zip(*outs):
+        assert np.array_equal(a, c)
+
+
+def test_observables_match_full_recompute(mcx, orc):
+    b = mcx.LatticeBatch("ising", 3, [8, 12], seed=3)
+    b.set_betas([0.2, 0.3])
+    b.init("random")
+    b.run("metropolis", 7)
+    E, M, Q = b.observables()
+    for k, L in enumerate([8, 12]):
+        s = b.get_spins(k)
+        assert E[k] == -orc.pair_sum(3, L, s)
+        assert M[k] == orc.magnetization(3, L, s)
+        assert Q[k] == L ** 3
+
+
+def test_moments_match_series(mcx):
+    b = mcx.LatticeBatch("ising", 2, 32, n_chains=3, seed=5, series_capacity=40)
+    b.set_betas([0.3, 0.44, 0.6])
+    b.init("random")
+    b.run("metropolis", 25, record=True)
+    b.run("metropolis", 15, record=True)
+    mom = b.moments()
+    for k in range(3):
+        E, M, _ = b.series(k)
+        e, m = E.astype(np.float64), M.astype(np.float64)
+        np.testing.assert_allclose(mom[k], [40, e.sum(), (e * e).sum(), np.abs(m).sum(),
+                                            (m * m).sum(), (m ** 4).sum()], rtol=1e-12)
