@@ -1,0 +1,185 @@
+"""Benchmark: spin-flip attempts/ns on one MI355X.
+
+Workload (the batched-chains configuration): 4096 independent 2D Ising chains,
+L = 64 periodic, J = 1, h = 0, T_k = 1.5 + 2.0*k/4095 (beta_k = 1/T_k), spins
+i.i.d. uniform from the Philox INIT stream with key (7, k), checkerboard
+Metropolis, per-sweep int32 E/M series and FP64 moments recorded on the device.
+One step = one mcx_batch_run of --sweeps-per-step sweeps over all 4096 chains
+(default 1000, so the default 10 timed steps are the full 10^4-sweep run).
+
+Printed on one JSON line: whole-job attempts/ns of the batched path, the
+roofline of its kernel (2 algorithmic bytes per attempt: one int8 read + one
+int8 write per site per sweep), the plain one-chain-per-launch GPU path on a
+sample, and the reference CPU path (random-site Metropolis with Xoshiro256++,
+restated in C under oracle/) timed on the host cores on a sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "montecarlox.jl_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "spin-flip attempts/ns, batched vs one-chain-per-launch; p99 job latency"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_ATTEMPT = 2.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--sweeps-per-step", type=int, default=1000)
+    p.add_argument("--chains", type=int, default=4096)
+    p.add_argument("--L", type=int, default=64)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--no-plain", action="store_true", help="skip the plain-path sample")
+    p.add_argument("--cpu-sweeps", type=int, default=100)
+    p.add_argument("--plain-chains", type=int, default=16)
+    p.add_argument("--plain-sweeps", type=int, default=20)
+    p.add_argument("--copy-probe", type=int, default=0,
+                   help="also run a device copy of this many bytes (HBM counter scaling)")
+    return p.parse_args()
+
+
+def read_traffic():
+    """HBM bytes per launch of the batched kernel from the committed PMC summary."""
+    path = os.path.join(ROOT, "profiles", "k_ising_sweeps_2d_L64_4096chains_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def stream_copy_gbs(torch, nbytes=1 << 30, reps=10):
+    x = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    del x, y
+    return 2.0 * nbytes * reps / dt / 1e9
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(chains are independent: N GPUs would each run their own share of chains)",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import montecarlox as mcx
+
+    n, L, S = a.chains, a.L, a.sweeps_per_step
+    temps = 1.5 + 2.0 * np.arange(n) / max(n - 1, 1)
+    betas = 1.0 / temps
+    seed = 7
+    cap = S * (a.steps + a.warmup)
+    b = mcx.LatticeBatch("ising", 2, L, n_chains=n, seed=seed, series_capacity=cap)
+    b.set_betas(betas)
+    b.init("random")
+    torch.cuda.synchronize()
+    for _ in range(a.warmup):
+        b.run("metropolis", S, record=True)
+    torch.cuda.synchronize()
+    dev_ms = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        dev_ms.append(b.run("metropolis", S, record=True))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    attempts_per_step = float(n) * L * L * S
+    value = attempts_per_step * a.steps / (wall * 1e9)
+    kern_ms = float(np.mean(dev_ms))
+    achieved = BYTES_PER_ATTEMPT * attempts_per_step / (kern_ms * 1e-3) / 1e9
+    # sanity: the recorded series ends at the live observables
+    E, M, _ = b.observables()
+    Es, Ms, _ = b.series(n - 1, b.series_length() - 1, 1)
+    assert Es[0] == E[n - 1] and Ms[0] == M[n - 1]
+    _, acc = b.counters()
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "attempts/ns",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall * 1e3 / a.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8",
+        "data": "synthetic",
+        "config": {"workload": f"{n} independent 2D Ising chains, L={L} periodic, J=1, h=0, "
+                               f"T uniform over [1.5, 3.5], Philox key (7, k), checkerboard "
+                               f"Metropolis, per-sweep int32 E/M series + FP64 moments on device",
+                   "chains": n, "L": L, "sweeps_per_step": S,
+                   "sweeps_timed": S * a.steps, "acceptance": round(float(acc.sum()) / (attempts_per_step * (a.steps + a.warmup)), 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": read_traffic(),
+                     "kernel": "k_ising_sweeps<2,0>", "kernel_ms": round(kern_ms, 3),
+                     "bytes_per_attempt": BYTES_PER_ATTEMPT},
+    }
+    b.close()
+
+    bw = stream_copy_gbs(torch)
+    out["roofline"]["stream_copy_GBs"] = round(bw, 1)
+    out["roofline"]["frac_of_stream_copy"] = round(achieved / bw, 4)
+
+    if not a.no_plain:
+        pn, ps = a.plain_chains, a.plain_sweeps
+        p = mcx.LatticeBatch("ising", 2, L, n_chains=pn, seed=seed, series_capacity=ps + 2)
+        p.set_betas(betas[:pn])
+        p.init("random")
+        p.run("metropolis", 2, record=True, plain=True)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        p.run("metropolis", ps, record=True, plain=True)
+        torch.cuda.synchronize()
+        pw = time.perf_counter() - t0
+        pv = float(pn) * L * L * ps / (pw * 1e9)
+        out["plain"] = {"value": round(pv, 4), "unit": "attempts/ns",
+                        "sample": f"{pn} chains x {ps} sweeps, one kernel per colour per chain per sweep",
+                        "batched_over_plain": round(value / pv, 1),
+                        "roofline_frac": round(BYTES_PER_ATTEMPT * pv / HBM_PEAK_GBS, 6)}
+        p.close()
+
+    if not a.no_cpu:
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        attempts, _ = oracle.refpath_ising2d(L, betas, seed, a.cpu_sweeps, threads)
+        cw = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(attempts / (cw * 1e9), 4), "unit": "attempts/ns",
+                               "cores": threads, "kind": "port",
+                               "sample": f"all {n} chains x {a.cpu_sweeps} sweeps of random-site "
+                                         "Metropolis (C restatement of spin_flip!(IsingLatticeOptim, "
+                                         "Metropolis) with Xoshiro256++), one thread per chain group",
+                               "seconds": round(cw, 2)}
+
+    if a.copy_probe:
+        x = torch.empty(a.copy_probe, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        out["copy_probe_bytes"] = a.copy_probe
+
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
