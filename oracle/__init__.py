@@ -97,6 +97,10 @@ def _declare(L):
     L.orc_pt_round.argtypes = [C.c_int, _f64p, _f64p, _i32p, _i32p, _i32p, C.POINTER(C.c_int32),
                                C.c_uint64, _u32p, C.c_uint64]
     L.orc_muca_update.argtypes = [C.c_int64, _f64p, _f64p]
+    L.orc_exchange_pair.argtypes = [C.c_double] * 5 + [C.POINTER(C.c_double)]
+    L.orc_exchange_pair.restype = C.c_int
+    L.orc_refpath_series_ising2d.argtypes = [C.c_int, C.c_double, C.c_uint64, C.c_int64, _i8p,
+                                             _i32p, _i32p, C.POINTER(C.c_int64)]
     L.orc_refpath_ising2d.argtypes = [C.c_int, C.c_int, _f64p, C.c_uint64, C.c_int64, C.c_int,
                                       _NullablePtr(_i8p)]
     L.orc_refpath_ising2d.restype = C.c_int64
@@ -195,6 +199,23 @@ def pt_round(betas_of_ladder, x, indices, steps, accepted, stage, seed, chain_ke
                        np.ascontiguousarray(x, np.float64), indices, steps, accepted,
                        C.byref(st), seed, np.ascontiguousarray(chain_keys, np.uint32), rnd)
     return st.value
+
+
+def exchange_pair(beta_i, beta_j, x_i, x_j, u):
+    """(accepted, log ratio) of one replica-exchange attempt with shared u."""
+    lr = C.c_double()
+    ok = lib().orc_exchange_pair(beta_i, beta_j, x_i, x_j, u, C.byref(lr))
+    return bool(ok), lr.value
+
+
+def refpath_series_ising2d(L, beta, seed, nsweeps, spins):
+    """Reference CPU path (random site, Xoshiro256++) on one chain; spins in place.
+    Returns (E_pair, M, accepted) per sweep."""
+    E = np.zeros(nsweeps, np.int32)
+    M = np.zeros(nsweeps, np.int32)
+    acc = C.c_int64(0)
+    lib().orc_refpath_series_ising2d(L, beta, seed, nsweeps, spins, E, M, C.byref(acc))
+    return E, M, acc.value
 
 
 def muca_update(lw, h):

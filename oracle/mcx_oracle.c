@@ -481,6 +481,14 @@ void orc_muca_update(int64_t nbins, double* lw, const double* h)
 /* Replica exchange (replica_exchange.jl:110-178)                            */
 /* ------------------------------------------------------------------------ */
 
+int orc_exchange_pair(double beta_i, double beta_j, double x_i, double x_j, double u, double* lr)
+{
+    double r = ((-beta_i * x_j) - (-beta_i * x_i)) + ((-beta_j * x_i) - (-beta_j * x_j));
+    if (lr)
+        *lr = r;
+    return (r > 0) || (u < exp(r));
+}
+
 void orc_pt_round(int n, const double* beta_of_ladder, const double* x, int32_t* indices,
                   int32_t* steps, int32_t* accepted, int32_t* stage, uint64_t seed,
                   const uint32_t* chain_keys, uint64_t round)
@@ -596,6 +604,38 @@ static void* ref_worker(void* arg)
     }
     free(s);
     return NULL;
+}
+
+void orc_refpath_series_ising2d(int L, double beta, uint64_t seed, int64_t nsweeps, int8_t* s,
+                                int32_t* E_pair, int32_t* M, int64_t* accepted)
+{
+    int64_t N = (int64_t)L * L;
+    uint64_t sm = seed;
+    xoshiro_t rng;
+    for (int k = 0; k < 4; ++k)
+        rng.s[k] = splitmix64(&sm);
+    int64_t sum_pair = orc_pair_sum(2, L, s), sum_spins = orc_magnetization(2, L, s), acc = 0;
+    for (int64_t sw = 0; sw < nsweeps; ++sw) {
+        for (int64_t k = 0; k < N; ++k) {
+            int64_t i = (int64_t)(xoshiro_next(&rng) % (uint64_t)N);
+            int x = (int)(i % L), y = (int)(i / L);
+            int nb = s[y * L + (x + L - 1) % L] + s[y * L + (x + 1) % L]
+                   + s[((y + L - 1) % L) * L + x] + s[((y + 1) % L) * L + x];
+            int dpair = -2 * s[i] * nb;
+            double lr = -beta * (double)(-dpair);
+            int ok = (lr > 0) || ((double)(xoshiro_next(&rng) >> 11) * 0x1.0p-53 < exp(lr));
+            if (ok) {
+                sum_spins += -2 * s[i];
+                s[i] = (int8_t)-s[i];
+                sum_pair += dpair;
+                ++acc;
+            }
+        }
+        E_pair[sw] = (int32_t)(-sum_pair);
+        M[sw] = (int32_t)sum_spins;
+    }
+    if (accepted)
+        *accepted += acc;
 }
 
 int64_t orc_refpath_ising2d(int L, int nchains, const double* betas, uint64_t seed,

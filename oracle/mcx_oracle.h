@@ -90,6 +90,15 @@ void orc_pt_round(int n, const double* beta_of_ladder, const double* x, int32_t*
                   int32_t* steps, int32_t* accepted, int32_t* stage, uint64_t seed,
                   const uint32_t* chain_keys, uint64_t round);
 
+/* exchange_log_ratio + _accept_exchange for Boltzmann ensembles
+ * (replica_exchange.jl:110-115); returns accepted, *lr = log ratio. */
+int orc_exchange_pair(double beta_i, double beta_j, double x_i, double x_j, double u, double* lr);
+
+/* Reference CPU path on one 2D chain recording the cached E / M after every
+ * sweep of N random-site attempts (start: spins, modified in place). */
+void orc_refpath_series_ising2d(int L, double beta, uint64_t seed, int64_t nsweeps, int8_t* spins,
+                                int32_t* E_pair, int32_t* M, int64_t* accepted);
+
 /* Multicanonical weight recursion: lw[i] -= log(h[i]) where h[i] > 0. */
 void orc_muca_update(int64_t nbins, double* lw, const double* h);
 
