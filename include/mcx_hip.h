@@ -67,6 +67,7 @@ typedef int32_t mcx_status;
 /* run flags */
 #define MCX_RUN_RECORD 1u /* write per-sweep E/M(/Q) series and FP64 moments */
 #define MCX_RUN_PLAIN 2u  /* one-chain-per-launch path: a kernel per colour per chain per sweep */
+#define MCX_RUN_GRAPH 4u  /* mcx_pt_run: capture every round in one hipGraph and launch it once */
 
 /* per-chain FP64 moments, in this order */
 #define MCX_MOM_COUNT 0
@@ -193,6 +194,43 @@ mcx_status mcx_batch_set_streams(mcx_batch* b, uint64_t seed, const uint32_t* ch
  * a restored checkpoint resumes bit-exactly. */
 mcx_status mcx_batch_get_sweeps(mcx_batch* b, uint64_t* sweeps);
 mcx_status mcx_batch_set_sweeps(mcx_batch* b, const uint64_t* sweeps);
+
+/* ------------------------------------------------------------------------ */
+/* Replica exchange / parallel tempering                                     */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int32_t n_sets;       /* independent PT ladders                               */
+    int32_t replicas;     /* R >= 2 per set; replica r of set s is chain s*R + r   */
+    const double* betas;  /* n_sets*R: beta of ladder index k of set s = betas[s*R + k-1] */
+} mcx_pt_desc;
+
+/* Replaces ReplicaExchange(ThreadsBackend(R), algs) / ParallelTempering(betas; seed, rng)
+ * (src/algorithms/replica_exchange.jl:52-58, parallel_tempering.jl:254-298) for
+ * n_sets independent ladders over the chains of a batch (n_sets*R == n_chains).
+ * Replica r starts on ladder index r+1 (indices = collect(1:n)), stage 0. */
+mcx_status mcx_pt_create(mcx_batch* b, const mcx_pt_desc* desc, mcx_pt** out);
+mcx_status mcx_pt_destroy(mcx_pt* pt);
+/* Replaces reset!(rx) (replica_exchange.jl:67-73). */
+mcx_status mcx_pt_reset(mcx_pt* pt);
+/* Replaces update!(rx::ReplicaExchange{ThreadsBackend}, xs) (replica_exchange.jl:158-178):
+ * one exchange round per set; xs = per-chain energies, or NULL for the chains'
+ * current energies (E_pair + D*Q) computed on the device. */
+mcx_status mcx_pt_exchange(mcx_pt* pt, const double* xs);
+/* The PT user loop (docs/src/examples/spin_systems/pt_Ising2D.jl:84-109) on the
+ * device: n_rounds x (sweeps_between sweeps of every replica at its current
+ * ladder beta, then one exchange round with the energies after the sweeps).
+ * flags: MCX_RUN_RECORD, MCX_RUN_GRAPH. */
+mcx_status mcx_pt_run(mcx_pt* pt, int32_t alg, int64_t n_rounds, int64_t sweeps_between,
+                      uint32_t flags);
+/* rx.indices (n_chains, 1-based ladder index held by each replica) and rx.stage
+ * per set (replica_exchange.jl:13-19). */
+mcx_status mcx_pt_get_ladder(mcx_pt* pt, int32_t* indices, int32_t* stages);
+/* rx.steps / rx.accepted per edge: n_sets*(R-1) each (acceptance_rates,
+ * replica_exchange.jl:75-77). */
+mcx_status mcx_pt_get_counts(mcx_pt* pt, int32_t* steps, int32_t* accepted);
+/* Exchange rounds done per set (position of the PT random stream). */
+mcx_status mcx_pt_get_rounds(mcx_pt* pt, uint64_t* rounds);
 
 #ifdef __cplusplus
 }

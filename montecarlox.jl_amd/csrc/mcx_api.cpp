@@ -204,6 +204,43 @@ void batch_free(mcx_batch* b)
     hipFree(b->d_acc);
     hipFree(b->d_items);
     hipFree(b->d_obs);
+    hipFree(b->d_last);
+}
+
+// The device copy of the chain records is authoritative (kernels advance the
+// sweep counters, replica exchange rewrites the table indices): refresh the
+// host mirror before changing a field and uploading it again.
+mcx_status pull_chains(mcx_batch* b)
+{
+    HIPCHK(hipMemcpyAsync(b->rec.data(), b->d_chains, sizeof(ChainRec) * b->n,
+                          hipMemcpyDeviceToHost, b->ctx->stream));
+    return sync_check(b, "pull_chains");
+}
+
+// Acceptance tables from one beta per table index (the chain's own D).
+mcx_status set_chain_tables(mcx_batch* b, const double* beta)
+{
+    if (b->model == MCX_ISING) {
+        std::vector<IsingTable> t(b->n);
+        for (int i = 0; i < b->n; ++i) {
+            std::memset(&t[i], 0, sizeof t[i]);
+            double pm[4], ph[8];
+            ising_probabilities(beta[i], b->dim, pm, ph);
+            fill_ising_metro(t[i], b->dim, pm);
+            fill_ising_hb(t[i], b->dim, ph);
+        }
+        HIPCHK(hipMemcpyAsync(b->d_itab, t.data(), sizeof(IsingTable) * b->n,
+                              hipMemcpyHostToDevice, b->ctx->stream));
+    } else {
+        std::vector<BCTable> t(b->n);
+        for (int i = 0; i < b->n; ++i)
+            fill_bc_table(t[i], b->dim, beta[i], b->rec[i].D);
+        HIPCHK(hipMemcpyAsync(b->d_btab, t.data(), sizeof(BCTable) * b->n, hipMemcpyHostToDevice,
+                              b->ctx->stream));
+    }
+    b->betas.assign(beta, beta + b->n);
+    b->tables_ready = true;
+    return sync_check(b, "set_chain_tables");
 }
 
 mcx_status upload_chains(mcx_batch* b)
@@ -318,6 +355,7 @@ extern "C" mcx_status mcx_batch_create(mcx_ctx* ctx, const mcx_batch_desc* d, mc
     alloc((void**)&b->d_acc, sizeof(unsigned long long) * b->n);
     alloc((void**)&b->d_items, sizeof(int32_t) * b->n);
     alloc((void**)&b->d_obs, sizeof(int32_t) * 3 * b->n);
+    alloc((void**)&b->d_last, sizeof(int32_t) * 3 * b->n);
     if (b->cap > 0) {
         const size_t ser = sizeof(int32_t) * (size_t)b->cap * (size_t)b->n;
         alloc((void**)&b->d_E, ser);
@@ -362,6 +400,8 @@ extern "C" mcx_status mcx_batch_destroy(mcx_batch* b)
 {
     if (!b)
         return MCX_OK;
+    if (b->pt)
+        return fail(MCX_ESTATE, "mcx_batch_destroy: destroy the batch's replica-exchange object first");
     hipSetDevice(b->ctx->device);
     hipStreamSynchronize(b->ctx->stream);
     batch_free(b);
@@ -405,38 +445,32 @@ extern "C" mcx_status mcx_batch_set_betas(mcx_batch* b, const double* beta)
 {
     if (!b || !beta)
         return fail(MCX_EINVAL, "mcx_batch_set_betas: NULL argument");
+    if (b->pt)
+        return fail(MCX_ESTATE, "mcx_batch_set_betas: the batch's temperatures belong to its "
+                                "replica-exchange object (destroy it first)");
     for (int i = 0; i < b->n; ++i)
         if (!(beta[i] >= 0) || !std::isfinite(beta[i]))
             return fail(MCX_EINVAL, "mcx_batch_set_betas: chain %d: beta = %g must be finite and >= 0",
                         i, beta[i]);
     HIPCHK(hipSetDevice(b->ctx->device));
-    b->betas.assign(beta, beta + b->n);
-    if (b->model == MCX_ISING) {
-        std::vector<IsingTable> t(b->n);
-        for (int i = 0; i < b->n; ++i) {
-            std::memset(&t[i], 0, sizeof t[i]);
-            double pm[4], ph[8];
-            ising_probabilities(beta[i], b->dim, pm, ph);
-            fill_ising_metro(t[i], b->dim, pm);
-            fill_ising_hb(t[i], b->dim, ph);
-        }
-        HIPCHK(hipMemcpyAsync(b->d_itab, t.data(), sizeof(IsingTable) * b->n,
-                              hipMemcpyHostToDevice, b->ctx->stream));
-    } else {
-        std::vector<BCTable> t(b->n);
-        for (int i = 0; i < b->n; ++i)
-            fill_bc_table(t[i], b->dim, beta[i], b->rec[i].D);
-        HIPCHK(hipMemcpyAsync(b->d_btab, t.data(), sizeof(BCTable) * b->n, hipMemcpyHostToDevice,
-                              b->ctx->stream));
-    }
-    b->tables_ready = true;
-    return sync_check(b, "mcx_batch_set_betas");
+    mcx_status s = pull_chains(b);
+    if (s)
+        return s;
+    for (int i = 0; i < b->n; ++i)
+        b->rec[i].table = i;
+    s = upload_chains(b);
+    if (s)
+        return s;
+    return set_chain_tables(b, beta);
 }
 
 extern "C" mcx_status mcx_batch_set_tables(mcx_batch* b, int32_t alg, const double* p, int32_t n)
 {
     if (!b || !p)
         return fail(MCX_EINVAL, "mcx_batch_set_tables: NULL argument");
+    if (b->pt)
+        return fail(MCX_ESTATE, "mcx_batch_set_tables: the batch's temperatures belong to its "
+                                "replica-exchange object (destroy it first)");
     int32_t want = 0;
     mcx_status st = mcx_batch_table_layout(b, alg, &want);
     if (st)
@@ -528,7 +562,8 @@ extern "C" mcx_status mcx_batch_get_spins(mcx_batch* b, int32_t chain, int8_t* s
 
 namespace mcx {
 
-mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t flags)
+mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t flags,
+                            bool want_last)
 {
     const bool record = (flags & MCX_RUN_RECORD) != 0;
     hipStream_t st = b->ctx->stream;
@@ -566,6 +601,9 @@ mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t
         if (record)
             HIPCHK(launch_series_moments(b->d_chains, b->n, b->d_E, b->d_M, b->d_Q, b->rec_len,
                                          nsweeps, b->d_mom, st));
+        if (want_last)
+            HIPCHK(launch_observables(b->d_chains, b->n, b->max_sites, b->d_spins, b->d_last,
+                                      b->d_last + b->n, b->d_last + 2 * b->n, st));
     } else {
         SweepArgs a{};
         a.record = record ? 1 : 0;
@@ -581,6 +619,11 @@ mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t
         a.nsweeps = nsweeps;
         a.rec_pos = b->rec_len;
         a.seed = b->seed;
+        if (want_last && nsweeps > 0) {
+            a.last_E = b->d_last;
+            a.last_M = b->d_last + b->n;
+            a.last_Q = b->d_last + 2 * b->n;
+        }
         for (const auto& ln : b->launches) {
             a.items = b->d_items + ln.first;
             a.n_items = ln.count;
@@ -771,9 +814,12 @@ extern "C" mcx_status mcx_batch_set_sweeps(mcx_batch* b, const uint64_t* sweeps)
             return fail(MCX_EINVAL, "mcx_batch_set_sweeps: chain %d: sweep %llu exceeds 2^48", i,
                         (unsigned long long)sweeps[i]);
     HIPCHK(hipSetDevice(b->ctx->device));
+    mcx_status s = pull_chains(b);
+    if (s)
+        return s;
     for (int i = 0; i < b->n; ++i)
         b->rec[i].sweep = sweeps[i];
-    mcx_status s = upload_chains(b);
+    s = upload_chains(b);
     if (s)
         return s;
     return sync_check(b, "mcx_batch_set_sweeps");
@@ -784,10 +830,13 @@ extern "C" mcx_status mcx_batch_set_streams(mcx_batch* b, uint64_t seed, const u
     if (!b)
         return fail(MCX_EINVAL, "mcx_batch_set_streams: NULL batch");
     HIPCHK(hipSetDevice(b->ctx->device));
+    mcx_status s = pull_chains(b);
+    if (s)
+        return s;
     b->seed = seed;
     for (int i = 0; i < b->n; ++i)
         b->rec[i].key = keys ? keys[i] : (uint32_t)i;
-    mcx_status s = upload_chains(b);
+    s = upload_chains(b);
     if (s)
         return s;
     return sync_check(b, "mcx_batch_set_streams");

@@ -54,6 +54,8 @@ struct mcx_batch {
     unsigned long long* d_acc = nullptr;
     int32_t* d_items = nullptr;
     int32_t* d_obs = nullptr;
+    int32_t* d_last = nullptr; // [3 * n] E_pair, M, Q after the last sweep of a call
+    mcx_pt* pt = nullptr;      // replica-exchange object driving the tables, if any
 };
 
 namespace mcx {
@@ -71,8 +73,11 @@ void fill_bc_hb_from_p(BCTable& t, int dim, const double* p);
 size_t lattice_lds_bytes(int model, int dim, int L);
 int lattice_block(int dim, int L);
 mcx_status upload_chains(mcx_batch* b);
+mcx_status pull_chains(mcx_batch* b);
+mcx_status set_chain_tables(mcx_batch* b, const double* beta_per_table);
 mcx_status sync_check(mcx_batch* b, const char* what);
-mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t flags);
+mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t flags,
+                            bool want_last = false);
 mcx_status check_run(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t flags, const char* who);
 
 } // namespace mcx

@@ -267,6 +267,7 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
             const uint64_t sweep = cr.sweep + (uint64_t)s;
             const uint32_t sw_lo = (uint32_t)sweep;
             const uint32_t w2 = ctr_word2(sweep, 0, purpose);
+            const bool obs = a.record || (a.last_E != nullptr && s == a.nsweeps - 1);
             int32_t pr = 0, mm = 0, qq = 0;
             for (int c = 0; c < 2; ++c) {
                 uint32_t* __restrict__ Oc = P + c * words;
@@ -318,7 +319,7 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
                     acc += (unsigned long long)__popc(changed);
                     Oc[w] = occ;
                     Sc[w] = up;
-                    if (c == 1 && a.record) {
+                    if (c == 1 && obs) {
                         const uint32_t dn = occ & ~up;
                         const uint32_t upo = occ & up;
 #pragma unroll
@@ -332,7 +333,7 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
                         qq += __popc(occ) + __popc(Ao);
                     }
                 }
-                if (c == 1 && a.record) {
+                if (c == 1 && obs) {
                     pr = wave_sum(pr);
                     mm = wave_sum(mm);
                     qq = wave_sum(qq);
@@ -344,24 +345,31 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
                 }
                 __syncthreads();
             }
-            if (a.record && tid == 0) {
+            if (obs && tid == 0) {
                 int64_t tp = 0, tm = 0, tq = 0;
                 for (int k = 0; k < nwaves; ++k) {
                     tp += part[(s & 1) * 48 + k * 3];
                     tm += part[(s & 1) * 48 + k * 3 + 1];
                     tq += part[(s & 1) * 48 + k * 3 + 2];
                 }
-                const int64_t at = cr.series_off + a.rec_pos + s;
-                a.ser_E[at] = (int32_t)(-tp);
-                a.ser_M[at] = (int32_t)tm;
-                a.ser_Q[at] = (int32_t)tq;
-                const double e = (double)(int32_t)(-tp) + cr.D * (double)(int32_t)tq;
-                const double m = (double)tm;
-                mE += e;
-                mE2 += e * e;
-                mAM += fabs(m);
-                mM2 += m * m;
-                mM4 += (m * m) * (m * m);
+                if (a.last_E && s == a.nsweeps - 1) {
+                    a.last_E[ch] = (int32_t)(-tp);
+                    a.last_M[ch] = (int32_t)tm;
+                    a.last_Q[ch] = (int32_t)tq;
+                }
+                if (a.record) {
+                    const int64_t at = cr.series_off + a.rec_pos + s;
+                    a.ser_E[at] = (int32_t)(-tp);
+                    a.ser_M[at] = (int32_t)tm;
+                    a.ser_Q[at] = (int32_t)tq;
+                    const double e = (double)(int32_t)(-tp) + cr.D * (double)(int32_t)tq;
+                    const double m = (double)tm;
+                    mE += e;
+                    mE2 += e * e;
+                    mAM += fabs(m);
+                    mM2 += m * m;
+                    mM4 += (m * m) * (m * m);
+                }
             }
         }
 

@@ -70,6 +70,9 @@ struct SweepArgs {
     int64_t nsweeps;
     int64_t rec_pos;      // series index of the first sweep of this call
     uint64_t seed;
+    int32_t* last_E;      // if set: E_pair / M / Q after the last sweep, per chain
+    int32_t* last_M;
+    int32_t* last_Q;
 };
 
 struct PlainArgs {

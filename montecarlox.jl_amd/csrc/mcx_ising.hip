@@ -161,6 +161,8 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
             const uint64_t sweep = cr.sweep + (uint64_t)s;
             const uint32_t sw_lo = (uint32_t)sweep;
             const uint32_t w2 = ctr_word2(sweep, 0, purpose);
+            // observables: every recorded sweep, and the last one when asked for
+            const bool obs = a.record || (a.last_E != nullptr && s == a.nsweeps - 1);
             uint32_t nu = 0, na = 0;
             for (int c = 0; c < 2; ++c) {
                 uint32_t* __restrict__ Pc = P + c * words;
@@ -238,7 +240,7 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                     }
                     acc += (unsigned long long)__popc(flip);
                     Pc[w] = snew;
-                    if (c == 1 && a.record) {
+                    if (c == 1 && obs) {
                         // every bond joins colour 0 and colour 1: count each from its colour-1 end
                         na += __popc(snew ^ n.U) + __popc(snew ^ n.Dn) + __popc(snew ^ n.A)
                             + __popc(snew ^ n.Bw);
@@ -247,7 +249,7 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                         nu += __popc(snew) + __popc(n.A);
                     }
                 }
-                if (c == 1 && a.record) {
+                if (c == 1 && obs) {
                     nu = wave_sum(nu);
                     na = wave_sum(na);
                     if (lane == 0) {
@@ -257,7 +259,7 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                 }
                 __syncthreads();
             }
-            if (a.record && tid == 0) {
+            if (obs && tid == 0) {
                 uint32_t tu = 0, ta = 0;
                 for (int k = 0; k < nwaves; ++k) {
                     tu += part[(s & 1) * 32 + k * 2];
@@ -265,15 +267,22 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                 }
                 const int32_t E = (int32_t)(2 * (int64_t)ta - (int64_t)DIM * N);
                 const int32_t M = (int32_t)(2 * (int64_t)tu - N);
-                const int64_t at = cr.series_off + a.rec_pos + s;
-                a.ser_E[at] = E;
-                a.ser_M[at] = M;
-                const double e = (double)E, m = (double)M;
-                mE += e;
-                mE2 += e * e;
-                mAM += fabs(m);
-                mM2 += m * m;
-                mM4 += (m * m) * (m * m);
+                if (a.last_E && s == a.nsweeps - 1) {
+                    a.last_E[ch] = E;
+                    a.last_M[ch] = M;
+                    a.last_Q[ch] = (int32_t)N;
+                }
+                if (a.record) {
+                    const int64_t at = cr.series_off + a.rec_pos + s;
+                    a.ser_E[at] = E;
+                    a.ser_M[at] = M;
+                    const double e = (double)E, m = (double)M;
+                    mE += e;
+                    mE2 += e * e;
+                    mAM += fabs(m);
+                    mM2 += m * m;
+                    mM4 += (m * m) * (m * m);
+                }
             }
         }
 

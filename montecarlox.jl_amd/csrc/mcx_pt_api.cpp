@@ -1,0 +1,289 @@
+// mcx_pt_api.cpp -- C ABI for replica exchange (parallel tempering).
+#include "mcx_api_impl.h"
+#include "mcx_pt.h"
+
+#include <cmath>
+#include <cstring>
+
+using namespace mcx;
+
+struct mcx_pt {
+    mcx_batch* b = nullptr;
+    int n_sets = 0, R = 0;
+    std::vector<double> ladder;
+    int32_t *d_idx = nullptr, *d_pos = nullptr, *d_steps = nullptr, *d_acc = nullptr,
+            *d_stage = nullptr;
+    uint64_t* d_round = nullptr;
+    double* d_ladder = nullptr;
+    double* d_x = nullptr;
+};
+
+namespace {
+
+void pt_free(mcx_pt* p)
+{
+    hipFree(p->d_idx);
+    hipFree(p->d_pos);
+    hipFree(p->d_steps);
+    hipFree(p->d_acc);
+    hipFree(p->d_stage);
+    hipFree(p->d_round);
+    hipFree(p->d_ladder);
+    hipFree(p->d_x);
+}
+
+PTArgs pt_args(mcx_pt* p)
+{
+    PTArgs a{};
+    a.n_sets = p->n_sets;
+    a.R = p->R;
+    a.chains = p->b->d_chains;
+    a.idx = p->d_idx;
+    a.pos = p->d_pos;
+    a.ladder = p->d_ladder;
+    a.steps = p->d_steps;
+    a.acc = p->d_acc;
+    a.stage = p->d_stage;
+    a.round = p->d_round;
+    a.E = p->b->d_last;
+    a.Q = p->b->d_last + 2 * p->b->n;
+    a.x = nullptr;
+    a.seed = p->b->seed;
+    return a;
+}
+
+// identity ladder: replica r holds ladder index r + 1, its table is its own
+mcx_status pt_reset_state(mcx_pt* p)
+{
+    mcx_batch* b = p->b;
+    const int n = p->n_sets * p->R;
+    std::vector<int32_t> idx(n), pos(n);
+    for (int s = 0; s < p->n_sets; ++s)
+        for (int r = 0; r < p->R; ++r) {
+            idx[s * p->R + r] = r + 1;
+            pos[s * p->R + r] = r;
+        }
+    hipStream_t st = b->ctx->stream;
+    HIPCHK(hipMemcpyAsync(p->d_idx, idx.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(p->d_pos, pos.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(p->d_steps, 0, sizeof(int32_t) * p->n_sets * (p->R - 1), st));
+    HIPCHK(hipMemsetAsync(p->d_acc, 0, sizeof(int32_t) * p->n_sets * (p->R - 1), st));
+    HIPCHK(hipMemsetAsync(p->d_stage, 0, sizeof(int32_t) * p->n_sets, st));
+    mcx_status s = pull_chains(b);
+    if (s)
+        return s;
+    for (int i = 0; i < n; ++i)
+        b->rec[i].table = i;
+    s = upload_chains(b);
+    if (s)
+        return s;
+    return sync_check(b, "mcx_pt_reset");
+}
+
+} // namespace
+
+extern "C" mcx_status mcx_pt_create(mcx_batch* b, const mcx_pt_desc* d, mcx_pt** out)
+{
+    if (!b || !d || !out || !d->betas)
+        return fail(MCX_EINVAL, "mcx_pt_create: NULL argument");
+    *out = nullptr;
+    if (d->replicas < 2)
+        return fail(MCX_EINVAL, "need at least 2 algorithms for replica exchange");
+    if (d->n_sets < 1 || (int64_t)d->n_sets * d->replicas != b->n)
+        return fail(MCX_EINVAL, "mcx_pt_create: n_sets * replicas (%d * %d) must equal the batch's %d chains",
+                    d->n_sets, d->replicas, b->n);
+    if (b->pt)
+        return fail(MCX_ESTATE, "mcx_pt_create: the batch already has a replica-exchange object");
+    const int n = b->n;
+    for (int i = 0; i < n; ++i)
+        if (!(d->betas[i] >= 0) || !std::isfinite(d->betas[i]))
+            return fail(MCX_EINVAL, "mcx_pt_create: ladder beta %d = %g must be finite and >= 0", i,
+                        d->betas[i]);
+    HIPCHK(hipSetDevice(b->ctx->device));
+    auto* p = new mcx_pt();
+    p->b = b;
+    p->n_sets = d->n_sets;
+    p->R = d->replicas;
+    p->ladder.assign(d->betas, d->betas + n);
+    hipError_t e = hipSuccess;
+    auto alloc = [&](void** ptr, size_t bytes) {
+        if (e == hipSuccess)
+            e = hipMalloc(ptr, bytes);
+    };
+    alloc((void**)&p->d_idx, sizeof(int32_t) * n);
+    alloc((void**)&p->d_pos, sizeof(int32_t) * n);
+    alloc((void**)&p->d_steps, sizeof(int32_t) * p->n_sets * (p->R - 1));
+    alloc((void**)&p->d_acc, sizeof(int32_t) * p->n_sets * (p->R - 1));
+    alloc((void**)&p->d_stage, sizeof(int32_t) * p->n_sets);
+    alloc((void**)&p->d_round, sizeof(uint64_t) * p->n_sets);
+    alloc((void**)&p->d_ladder, sizeof(double) * n);
+    alloc((void**)&p->d_x, sizeof(double) * n);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(p->d_ladder, p->ladder.data(), sizeof(double) * n, hipMemcpyHostToDevice,
+                           b->ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(p->d_round, 0, sizeof(uint64_t) * p->n_sets, b->ctx->stream);
+    if (e != hipSuccess) {
+        pt_free(p);
+        delete p;
+        return fail(e == hipErrorOutOfMemory ? MCX_ENOMEM : MCX_EHIP, "mcx_pt_create: %s",
+                    hipGetErrorString(e));
+    }
+    // acceptance tables indexed by (set, ladder index)
+    mcx_status s = set_chain_tables(b, p->ladder.data());
+    if (s == MCX_OK)
+        s = pt_reset_state(p);
+    if (s) {
+        pt_free(p);
+        delete p;
+        return s;
+    }
+    b->pt = p;
+    *out = p;
+    return MCX_OK;
+}
+
+extern "C" mcx_status mcx_pt_destroy(mcx_pt* p)
+{
+    if (!p)
+        return MCX_OK;
+    hipSetDevice(p->b->ctx->device);
+    hipStreamSynchronize(p->b->ctx->stream);
+    p->b->pt = nullptr;
+    pt_free(p);
+    delete p;
+    return MCX_OK;
+}
+
+extern "C" mcx_status mcx_pt_reset(mcx_pt* p)
+{
+    if (!p)
+        return fail(MCX_EINVAL, "mcx_pt_reset: NULL handle");
+    HIPCHK(hipSetDevice(p->b->ctx->device));
+    return pt_reset_state(p);
+}
+
+extern "C" mcx_status mcx_pt_exchange(mcx_pt* p, const double* x)
+{
+    if (!p)
+        return fail(MCX_EINVAL, "mcx_pt_exchange: NULL handle");
+    mcx_batch* b = p->b;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    PTArgs a = pt_args(p);
+    if (x) {
+        for (int i = 0; i < b->n; ++i)
+            if (!std::isfinite(x[i]))
+                return fail(MCX_EINVAL, "mcx_pt_exchange: energy %d is not finite", i);
+        HIPCHK(hipMemcpyAsync(p->d_x, x, sizeof(double) * b->n, hipMemcpyHostToDevice, st));
+        a.x = p->d_x;
+    } else {
+        HIPCHK(launch_observables(b->d_chains, b->n, b->max_sites, b->d_spins, b->d_last,
+                                  b->d_last + b->n, b->d_last + 2 * b->n, st));
+    }
+    HIPCHK(launch_pt_exchange(a, st));
+    return sync_check(b, "mcx_pt_exchange");
+}
+
+extern "C" mcx_status mcx_pt_run(mcx_pt* p, int32_t alg, int64_t n_rounds, int64_t sweeps_between,
+                                 uint32_t flags)
+{
+    if (!p)
+        return fail(MCX_EINVAL, "mcx_pt_run: NULL handle");
+    mcx_batch* b = p->b;
+    if (n_rounds < 0 || sweeps_between < 1)
+        return fail(MCX_EINVAL, "mcx_pt_run: need n_rounds >= 0 and sweeps_between >= 1");
+    if (flags & ~(MCX_RUN_RECORD | MCX_RUN_GRAPH))
+        return fail(MCX_EINVAL, "mcx_pt_run: flags must be a combination of MCX_RUN_RECORD, MCX_RUN_GRAPH");
+    mcx_status s = check_run(b, alg, n_rounds * sweeps_between, flags & MCX_RUN_RECORD, "mcx_pt_run");
+    if (s)
+        return s;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    const bool graph = (flags & MCX_RUN_GRAPH) != 0;
+    const uint32_t rflags = flags & MCX_RUN_RECORD;
+    PTArgs a = pt_args(p);
+    HIPCHK(hipEventRecord(b->ctx->ev0, st));
+    if (graph && n_rounds > 0) {
+        // one graph holding every round: sweep launch(es) then the exchange
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        mcx_status rs = MCX_OK;
+        for (int64_t k = 0; k < n_rounds && rs == MCX_OK; ++k) {
+            rs = run_sweeps_async(b, alg, sweeps_between, rflags, true);
+            if (rs == MCX_OK && launch_pt_exchange(a, st) != hipSuccess)
+                rs = fail(MCX_EHIP, "mcx_pt_run: exchange launch failed during capture");
+        }
+        hipError_t ce = hipStreamEndCapture(st, &g);
+        if (rs)
+            return rs;
+        if (ce != hipSuccess)
+            return fail(MCX_EHIP, "mcx_pt_run: graph capture failed: %s", hipGetErrorString(ce));
+        hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (ie == hipSuccess)
+            ie = hipGraphLaunch(ge, st);
+        if (ie == hipSuccess)
+            ie = hipStreamSynchronize(st);
+        if (ge)
+            hipGraphExecDestroy(ge);
+        hipGraphDestroy(g);
+        if (ie != hipSuccess)
+            return fail(MCX_EHIP, "mcx_pt_run: graph execution failed: %s", hipGetErrorString(ie));
+    } else {
+        for (int64_t k = 0; k < n_rounds; ++k) {
+            s = run_sweeps_async(b, alg, sweeps_between, rflags, true);
+            if (s)
+                return s;
+            HIPCHK(launch_pt_exchange(a, st));
+        }
+    }
+    HIPCHK(hipEventRecord(b->ctx->ev1, st));
+    s = sync_check(b, "mcx_pt_run");
+    if (s)
+        return s;
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1));
+    b->last_ms = ms;
+    return MCX_OK;
+}
+
+extern "C" mcx_status mcx_pt_get_ladder(mcx_pt* p, int32_t* indices, int32_t* stages)
+{
+    if (!p)
+        return fail(MCX_EINVAL, "mcx_pt_get_ladder: NULL handle");
+    mcx_batch* b = p->b;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    if (indices)
+        HIPCHK(hipMemcpyAsync(indices, p->d_idx, sizeof(int32_t) * b->n, hipMemcpyDeviceToHost, st));
+    if (stages)
+        HIPCHK(hipMemcpyAsync(stages, p->d_stage, sizeof(int32_t) * p->n_sets, hipMemcpyDeviceToHost, st));
+    return sync_check(b, "mcx_pt_get_ladder");
+}
+
+extern "C" mcx_status mcx_pt_get_counts(mcx_pt* p, int32_t* steps, int32_t* accepted)
+{
+    if (!p)
+        return fail(MCX_EINVAL, "mcx_pt_get_counts: NULL handle");
+    mcx_batch* b = p->b;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    const size_t n = sizeof(int32_t) * p->n_sets * (p->R - 1);
+    if (steps)
+        HIPCHK(hipMemcpyAsync(steps, p->d_steps, n, hipMemcpyDeviceToHost, st));
+    if (accepted)
+        HIPCHK(hipMemcpyAsync(accepted, p->d_acc, n, hipMemcpyDeviceToHost, st));
+    return sync_check(b, "mcx_pt_get_counts");
+}
+
+extern "C" mcx_status mcx_pt_get_rounds(mcx_pt* p, uint64_t* rounds)
+{
+    if (!p || !rounds)
+        return fail(MCX_EINVAL, "mcx_pt_get_rounds: NULL argument");
+    mcx_batch* b = p->b;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    HIPCHK(hipMemcpyAsync(rounds, p->d_round, sizeof(uint64_t) * p->n_sets, hipMemcpyDeviceToHost,
+                          b->ctx->stream));
+    return sync_check(b, "mcx_pt_get_rounds");
+}

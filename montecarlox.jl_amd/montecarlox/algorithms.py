@@ -105,8 +105,12 @@ def ensemble(alg):
 
 
 def acceptance_rate(alg) -> float:
-    """accepted / steps, 0.0 before the first step (importance_sampling.jl:95-96)."""
-    return alg.accepted / alg.steps if alg.steps > 0 else 0.0
+    """accepted / steps, 0.0 before the first step (importance_sampling.jl:95-96;
+    for a replica-exchange object the total over edges, replica_exchange.jl:88-91)."""
+    steps, acc = alg.steps, alg.accepted
+    if hasattr(steps, "sum"):
+        steps, acc = steps.sum(), acc.sum()
+    return float(acc / steps) if steps > 0 else 0.0
 
 
 def reset_(alg):

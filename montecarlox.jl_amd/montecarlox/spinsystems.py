@@ -48,7 +48,8 @@ class SpinSystemBatch:
                                   ctx=ctx)
         self.dim = dim
         self._bound = None  # (seed, keys) currently bound
-        self._tables = None  # (alg code, betas) currently uploaded
+        self._tables = None  # betas currently uploaded
+        self._pt = None  # ParallelTempering driving this batch, if any
 
     def __len__(self):
         return self.batch.n
@@ -73,6 +74,8 @@ class SpinSystemBatch:
         if self._bound is None or self._bound[0] != seed or not np.array_equal(self._bound[1], keys):
             self.batch.set_streams(seed, keys)
             self._bound = (seed, keys)
+        if self._pt is not None:
+            return  # a replica-exchange ladder owns the acceptance tables
         betas = np.array([algorithm_beta(a) for a in algs], np.float64)
         if self._tables is None or not np.array_equal(self._tables, betas):
             self.batch.set_betas(betas)
