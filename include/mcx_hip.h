@@ -232,6 +232,69 @@ mcx_status mcx_pt_get_counts(mcx_pt* pt, int32_t* steps, int32_t* accepted);
 /* Exchange rounds done per set (position of the PT random stream). */
 mcx_status mcx_pt_get_rounds(mcx_pt* pt, uint64_t* rounds);
 
+/* ------------------------------------------------------------------------ */
+/* Multicanonical / Wang-Landau walkers                                      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int32_t kind;                  /* MCX_MULTICANONICAL or MCX_WANGLANDAU                     */
+    int32_t n_groups;              /* walkers of one group share a binning (and lattice size) */
+    const int32_t* group_of_chain; /* n_chains entries, or NULL = all in group 0; the first  */
+                                   /* chain of a group is its root                            */
+    const double* bin_start;       /* per group: DiscreteBinning(start, step, num)            */
+    const double* bin_step;
+    const int64_t* n_bins;
+} mcx_gen_desc;
+
+/* Replaces Multicanonical(rng, bins) / WangLandau(rng, bins; logf) per chain
+ * (src/algorithms/multicanonical.jl:9-25, wang_landau.jl:44-51) and
+ * ParallelMulticanonical(ThreadsBackend(n), algs) (parallel_multicanonical.jl:91-99)
+ * over the chains of a batch.  Each walker owns FP64 log-weights (initially 0)
+ * and an int64 histogram.  Ising energies bin with integer div, Blume-Capel
+ * (Float64) energies with round-half-even, as _binindex (binned_object.jl:18-24). */
+mcx_status mcx_gen_create(mcx_batch* b, const mcx_gen_desc* desc, mcx_gen** out);
+mcx_status mcx_gen_destroy(mcx_gen* g);
+/* ensemble(alg).logweight.values / .histogram.values of one walker. */
+mcx_status mcx_gen_set_logweight(mcx_gen* g, int32_t chain, const double* lw, int64_t n);
+mcx_status mcx_gen_get_logweight(mcx_gen* g, int32_t chain, double* lw, int64_t n);
+mcx_status mcx_gen_get_histogram(mcx_gen* g, int32_t chain, int64_t* hist, int64_t n);
+/* WangLandauEnsemble.logf (chain < 0: every walker). */
+mcx_status mcx_gen_set_logf(mcx_gen* g, int32_t chain, double logf);
+mcx_status mcx_gen_get_logf(mcx_gen* g, double* logf);
+/* N*nsweeps spin_flip!(sys, alg::AbstractImportanceSampling) attempts per walker
+ * (SpinSystems/src/ising.jl:25-33, blume_capel.jl:41-50) in checkerboard order,
+ * with accept!(alg, E_new, E_old) (importance_sampling.jl:69-78) or the
+ * Wang-Landau accept! (algorithms/wang_landau.jl:29-37).  An energy outside the
+ * binned domain stops that walker at the attempt (counters unchanged for it)
+ * and the call returns MCX_EBOUNDS.  flags: MCX_RUN_RECORD. */
+mcx_status mcx_gen_run(mcx_gen* g, int64_t nsweeps, uint32_t flags);
+/* reset!(alg): histogram and steps/accepted to zero (algorithms/multicanonical.jl:27-33);
+ * chain < 0: every walker. */
+mcx_status mcx_gen_reset(mcx_gen* g, int32_t chain);
+/* update!(::MulticanonicalEnsemble): lw -= log(h) where h > 0
+ * (src/ensembles/multicanonical.jl:32-44); chain < 0: every walker. */
+mcx_status mcx_gen_update(mcx_gen* g, int32_t chain);
+/* merge_histograms! / distribute_logweight! within one group
+ * (src/algorithms/parallel_multicanonical.jl:38-73). */
+mcx_status mcx_gen_merge(mcx_gen* g, int32_t group);
+mcx_status mcx_gen_distribute(mcx_gen* g, int32_t group);
+/* No reference counterpart (parallel Wang-Landau): root log-weight = mean over
+ * the group's walkers. */
+mcx_status mcx_gen_average_logweight(mcx_gen* g, int32_t group);
+/* No reference counterpart (Wang-Landau flatness test): {min, mean, count} of
+ * the root histogram over bins reached (h > 0 or lw != 0). */
+mcx_status mcx_gen_flatness(mcx_gen* g, int32_t group, double* min_mean_count);
+/* alg.steps / alg.accepted of every walker. */
+mcx_status mcx_gen_get_counters(mcx_gen* g, int64_t* steps, int64_t* accepted);
+mcx_status mcx_gen_group_info(mcx_gen* g, int32_t group, int32_t* root, int32_t* count,
+                              int64_t* n_bins);
+/* Replaces direct writes to ensemble(alg).histogram.values (counts >= 0). */
+mcx_status mcx_gen_set_histogram(mcx_gen* g, int32_t chain, const int64_t* hist, int64_t n);
+/* update!(::MulticanonicalEnsemble) on caller-held arrays (the reference keeps a
+ * Float64 histogram): lw[i] -= log(h[i]) where h[i] > 0, evaluated on the
+ * device (src/ensembles/multicanonical.jl:32-44). */
+mcx_status mcx_muca_update_values(mcx_ctx* ctx, double* lw, const double* h, int64_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -151,6 +151,9 @@ extern "C" mcx_status mcx_ctx_destroy(mcx_ctx* c)
 {
     if (!c)
         return MCX_OK;
+    if (c->n_users)
+        return fail(MCX_ESTATE, "mcx_ctx_destroy: %d batch / server object(s) still use the context",
+                    c->n_users);
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     hipEventDestroy(c->ev0);
@@ -392,6 +395,7 @@ extern "C" mcx_status mcx_batch_create(mcx_ctx* ctx, const mcx_batch_desc* d, mc
         delete b;
         return fail(MCX_EHIP, "mcx_batch_create: %s", hipGetErrorString(e));
     }
+    ++ctx->n_users;
     *out = b;
     return MCX_OK;
 }
@@ -402,8 +406,12 @@ extern "C" mcx_status mcx_batch_destroy(mcx_batch* b)
         return MCX_OK;
     if (b->pt)
         return fail(MCX_ESTATE, "mcx_batch_destroy: destroy the batch's replica-exchange object first");
+    if (b->n_gen)
+        return fail(MCX_ESTATE, "mcx_batch_destroy: destroy the batch's %d walker object(s) first",
+                    b->n_gen);
     hipSetDevice(b->ctx->device);
     hipStreamSynchronize(b->ctx->stream);
+    --b->ctx->n_users;
     batch_free(b);
     delete b;
     return MCX_OK;

@@ -25,6 +25,7 @@ struct mcx_ctx {
     char arch[64] = {0};
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int n_users = 0; // batches and servers built on this context
 };
 
 struct mcx_batch {
@@ -56,6 +57,7 @@ struct mcx_batch {
     int32_t* d_obs = nullptr;
     int32_t* d_last = nullptr; // [3 * n] E_pair, M, Q after the last sweep of a call
     mcx_pt* pt = nullptr;      // replica-exchange object driving the tables, if any
+    int n_gen = 0;             // multicanonical / Wang-Landau objects built on this batch
 };
 
 namespace mcx {

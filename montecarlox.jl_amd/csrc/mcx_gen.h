@@ -1,0 +1,67 @@
+// mcx_gen.h -- device state of multicanonical / Wang-Landau walkers.
+#pragma once
+
+#include "mcx_internal.h"
+
+namespace mcx {
+
+// One staged attempt: what a lane precomputes for one site of a 64-site chunk.
+struct AttemptRec {
+    double dEf;     // Blume-Capel energy change (FP64)
+    uint32_t w;     // site word
+    int32_t site;
+    int32_t dE;     // Ising energy change
+    int32_t dpair;  // change of sum_<ij> s_i s_j
+    int16_t dm, dq; // change of M and of Q = sum s^2
+    int8_t snew;
+    int8_t pad[7];
+};
+
+struct GenArgs {
+    const int32_t* items;   // walkers of this launch (all from one binning group)
+    int32_t n_items;
+    int32_t kind;           // 2 = multicanonical, 3 = Wang-Landau
+    int32_t model;          // 0 Ising (integer energy, div binning), 1 Blume-Capel (FP64, round)
+    int32_t dim;
+    int32_t record;
+    int32_t hist_in_lds;    // 1: uint32 histogram staged in LDS, else int64 atomics in HBM
+    int32_t pad;
+    double bin_start, bin_step;
+    int64_t nbins;
+    ChainRec* chains;
+    int8_t* spins;
+    double* lw;             // per walker, at lw_off[w]
+    unsigned long long* hist; // per walker, at lw_off[w]
+    const int64_t* lw_off;
+    const double* logf;     // per walker (Wang-Landau)
+    unsigned long long* steps;
+    unsigned long long* accepted;
+    int64_t* err_site;      // per walker: -1, or the site whose attempt left the binned domain
+    int32_t* ser_E;
+    int32_t* ser_M;
+    int32_t* ser_Q;
+    int64_t nsweeps;
+    int64_t rec_pos;
+    uint64_t seed;
+};
+
+hipError_t launch_gen_sweeps(const GenArgs& a, size_t lds_bytes, hipStream_t st);
+hipError_t launch_gen_update(double* lw, const unsigned long long* hist, const int64_t* lw_off,
+                             const int32_t* walkers, int32_t n_walkers, int64_t nbins,
+                             hipStream_t st);
+hipError_t launch_muca_update_values(double* lw, const double* h, int64_t n, hipStream_t st);
+hipError_t launch_gen_merge(unsigned long long* hist, const int64_t* lw_off,
+                            const int32_t* walkers, int32_t n_walkers, int64_t nbins,
+                            hipStream_t st);
+hipError_t launch_gen_distribute(double* lw, const int64_t* lw_off, const int32_t* walkers,
+                                 int32_t n_walkers, int64_t nbins, hipStream_t st);
+hipError_t launch_gen_average(double* lw, const int64_t* lw_off, const int32_t* walkers,
+                              int32_t n_walkers, int64_t nbins, hipStream_t st);
+hipError_t launch_gen_zero_hist(unsigned long long* hist, const int64_t* lw_off,
+                                const int32_t* walkers, int32_t n_walkers, int64_t nbins,
+                                hipStream_t st);
+hipError_t launch_gen_flatness(const unsigned long long* hist, const double* lw,
+                               const int64_t* lw_off, int32_t root, int64_t nbins,
+                               double* out, hipStream_t st);
+
+} // namespace mcx

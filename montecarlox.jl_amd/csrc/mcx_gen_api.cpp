@@ -1,0 +1,498 @@
+// mcx_gen_api.cpp -- C ABI for multicanonical / Wang-Landau walkers.
+#include "mcx_api_impl.h"
+#include "mcx_gen.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+using namespace mcx;
+
+struct mcx_gen {
+    mcx_batch* b = nullptr;
+    int kind = MCX_MULTICANONICAL;
+    int n_groups = 0;
+    std::vector<int32_t> group_of;
+    std::vector<double> start, step;
+    std::vector<int64_t> nbins, off, member_off;
+    std::vector<std::vector<int32_t>> members; // per group; members[g][0] is the root
+    double* d_lw = nullptr;
+    unsigned long long* d_hist = nullptr;
+    int64_t* d_off = nullptr;
+    double* d_logf = nullptr;
+    unsigned long long* d_steps = nullptr;
+    unsigned long long* d_acc = nullptr;
+    int64_t* d_err = nullptr;
+    int32_t* d_members = nullptr;
+    double* d_flat = nullptr;
+    std::vector<double> logf;
+};
+
+namespace {
+
+void gen_free(mcx_gen* g)
+{
+    hipFree(g->d_lw);
+    hipFree(g->d_hist);
+    hipFree(g->d_off);
+    hipFree(g->d_logf);
+    hipFree(g->d_steps);
+    hipFree(g->d_acc);
+    hipFree(g->d_err);
+    hipFree(g->d_members);
+    hipFree(g->d_flat);
+}
+
+mcx_status check_chain(mcx_gen* g, int32_t chain, const char* who)
+{
+    if (!g)
+        return fail(MCX_EINVAL, "%s: NULL handle", who);
+    if (chain < 0 || chain >= g->b->n)
+        return fail(MCX_EBOUNDS, "%s: chain %d not in [0, %d)", who, chain, g->b->n);
+    return MCX_OK;
+}
+
+mcx_status check_group(mcx_gen* g, int32_t group, const char* who)
+{
+    if (!g)
+        return fail(MCX_EINVAL, "%s: NULL handle", who);
+    if (group < 0 || group >= g->n_groups)
+        return fail(MCX_EBOUNDS, "%s: group %d not in [0, %d)", who, group, g->n_groups);
+    return MCX_OK;
+}
+
+size_t gen_lds(int64_t nbins, int64_t nsites, bool hist_in_lds)
+{
+    return 64 * sizeof(AttemptRec) + 8 * (size_t)nbins + (hist_in_lds ? 4 * (size_t)nbins : 0)
+         + (size_t)((nsites + 15) / 16 * 16);
+}
+
+} // namespace
+
+extern "C" mcx_status mcx_gen_create(mcx_batch* b, const mcx_gen_desc* d, mcx_gen** out)
+{
+    if (!b || !d || !out)
+        return fail(MCX_EINVAL, "mcx_gen_create: NULL argument");
+    *out = nullptr;
+    if (d->kind != MCX_MULTICANONICAL && d->kind != MCX_WANGLANDAU)
+        return fail(MCX_EINVAL, "mcx_gen_create: kind must be MCX_MULTICANONICAL or MCX_WANGLANDAU");
+    if (d->n_groups < 1 || !d->bin_start || !d->bin_step || !d->n_bins)
+        return fail(MCX_EINVAL, "mcx_gen_create: need n_groups >= 1 and per-group binnings");
+    auto* g = new mcx_gen();
+    g->b = b;
+    g->kind = d->kind;
+    g->n_groups = d->n_groups;
+    g->start.assign(d->bin_start, d->bin_start + d->n_groups);
+    g->step.assign(d->bin_step, d->bin_step + d->n_groups);
+    g->nbins.assign(d->n_bins, d->n_bins + d->n_groups);
+    g->group_of.resize(b->n);
+    g->members.resize(d->n_groups);
+    for (int i = 0; i < b->n; ++i) {
+        const int grp = d->group_of_chain ? d->group_of_chain[i] : 0;
+        if (grp < 0 || grp >= d->n_groups) {
+            delete g;
+            return fail(MCX_EINVAL, "mcx_gen_create: chain %d in group %d of %d", i, grp, d->n_groups);
+        }
+        g->group_of[i] = grp;
+        g->members[grp].push_back(i);
+    }
+    for (int k = 0; k < d->n_groups; ++k) {
+        if (g->members[k].empty()) {
+            delete g;
+            return fail(MCX_EINVAL, "mcx_gen_create: group %d has no walkers", k);
+        }
+        if (g->nbins[k] < 1 || !(g->step[k] != 0) || !std::isfinite(g->start[k]) ||
+            !std::isfinite(g->step[k])) {
+            delete g;
+            return fail(MCX_EINVAL, "mcx_gen_create: group %d: invalid binning", k);
+        }
+        if (b->model == MCX_ISING &&
+            (g->start[k] != std::floor(g->start[k]) || g->step[k] != std::floor(g->step[k]))) {
+            delete g;
+            return fail(MCX_EINVAL, "mcx_gen_create: group %d: Ising energies are integers; "
+                                    "start and step must be integral", k);
+        }
+        const int L0 = b->rec[g->members[k][0]].L;
+        for (int c : g->members[k])
+            if (b->rec[c].L != L0) {
+                delete g;
+                return fail(MCX_EINVAL, "mcx_gen_create: group %d mixes lattice sizes", k);
+            }
+        if (gen_lds(g->nbins[k], b->nsites[g->members[k][0]], false) > b->ctx->lds_max) {
+            delete g;
+            return fail(MCX_EINVAL, "mcx_gen_create: group %d: %lld bins do not fit the device LDS",
+                        k, (long long)g->nbins[k]);
+        }
+    }
+    g->off.resize(b->n);
+    int64_t tot = 0;
+    for (int i = 0; i < b->n; ++i) {
+        g->off[i] = tot;
+        tot += (g->nbins[g->group_of[i]] + 31) / 32 * 32;
+    }
+    std::vector<int32_t> flat;
+    for (int k = 0; k < g->n_groups; ++k) {
+        g->member_off.push_back((int64_t)flat.size());
+        flat.insert(flat.end(), g->members[k].begin(), g->members[k].end());
+    }
+    g->logf.assign(b->n, 1.0);
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipError_t e = hipSuccess;
+    auto alloc = [&](void** p, size_t bytes) {
+        if (e == hipSuccess)
+            e = hipMalloc(p, bytes);
+    };
+    alloc((void**)&g->d_lw, sizeof(double) * tot);
+    alloc((void**)&g->d_hist, sizeof(unsigned long long) * tot);
+    alloc((void**)&g->d_off, sizeof(int64_t) * b->n);
+    alloc((void**)&g->d_logf, sizeof(double) * b->n);
+    alloc((void**)&g->d_steps, sizeof(unsigned long long) * b->n);
+    alloc((void**)&g->d_acc, sizeof(unsigned long long) * b->n);
+    alloc((void**)&g->d_err, sizeof(int64_t) * b->n);
+    alloc((void**)&g->d_members, sizeof(int32_t) * b->n);
+    alloc((void**)&g->d_flat, sizeof(double) * 3);
+    hipStream_t st = b->ctx->stream;
+    if (e == hipSuccess)
+        e = hipMemsetAsync(g->d_lw, 0, sizeof(double) * tot, st);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(g->d_hist, 0, sizeof(unsigned long long) * tot, st);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(g->d_steps, 0, sizeof(unsigned long long) * b->n, st);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(g->d_acc, 0, sizeof(unsigned long long) * b->n, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(g->d_off, g->off.data(), sizeof(int64_t) * b->n, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(g->d_logf, g->logf.data(), sizeof(double) * b->n, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(g->d_members, flat.data(), sizeof(int32_t) * b->n, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        gen_free(g);
+        delete g;
+        return fail(e == hipErrorOutOfMemory ? MCX_ENOMEM : MCX_EHIP, "mcx_gen_create: %s",
+                    hipGetErrorString(e));
+    }
+    ++b->n_gen;
+    *out = g;
+    return MCX_OK;
+}
+
+extern "C" mcx_status mcx_gen_destroy(mcx_gen* g)
+{
+    if (!g)
+        return MCX_OK;
+    hipSetDevice(g->b->ctx->device);
+    hipStreamSynchronize(g->b->ctx->stream);
+    --g->b->n_gen;
+    gen_free(g);
+    delete g;
+    return MCX_OK;
+}
+
+extern "C" mcx_status mcx_gen_set_logweight(mcx_gen* g, int32_t chain, const double* lw, int64_t n)
+{
+    mcx_status s = check_chain(g, chain, "mcx_gen_set_logweight");
+    if (s)
+        return s;
+    if (!lw || n != g->nbins[g->group_of[chain]])
+        return fail(MCX_EINVAL, "mcx_gen_set_logweight: chain %d has %lld bins", chain,
+                    (long long)g->nbins[g->group_of[chain]]);
+    HIPCHK(hipSetDevice(g->b->ctx->device));
+    HIPCHK(hipMemcpyAsync(g->d_lw + g->off[chain], lw, sizeof(double) * n, hipMemcpyHostToDevice,
+                          g->b->ctx->stream));
+    return sync_check(g->b, "mcx_gen_set_logweight");
+}
+
+extern "C" mcx_status mcx_gen_get_logweight(mcx_gen* g, int32_t chain, double* lw, int64_t n)
+{
+    mcx_status s = check_chain(g, chain, "mcx_gen_get_logweight");
+    if (s)
+        return s;
+    if (!lw || n != g->nbins[g->group_of[chain]])
+        return fail(MCX_EINVAL, "mcx_gen_get_logweight: chain %d has %lld bins", chain,
+                    (long long)g->nbins[g->group_of[chain]]);
+    HIPCHK(hipSetDevice(g->b->ctx->device));
+    HIPCHK(hipMemcpyAsync(lw, g->d_lw + g->off[chain], sizeof(double) * n, hipMemcpyDeviceToHost,
+                          g->b->ctx->stream));
+    return sync_check(g->b, "mcx_gen_get_logweight");
+}
+
+extern "C" mcx_status mcx_gen_get_histogram(mcx_gen* g, int32_t chain, int64_t* h, int64_t n)
+{
+    mcx_status s = check_chain(g, chain, "mcx_gen_get_histogram");
+    if (s)
+        return s;
+    if (!h || n != g->nbins[g->group_of[chain]])
+        return fail(MCX_EINVAL, "mcx_gen_get_histogram: chain %d has %lld bins", chain,
+                    (long long)g->nbins[g->group_of[chain]]);
+    HIPCHK(hipSetDevice(g->b->ctx->device));
+    HIPCHK(hipMemcpyAsync(h, g->d_hist + g->off[chain], sizeof(int64_t) * n, hipMemcpyDeviceToHost,
+                          g->b->ctx->stream));
+    return sync_check(g->b, "mcx_gen_get_histogram");
+}
+
+extern "C" mcx_status mcx_gen_set_logf(mcx_gen* g, int32_t chain, double logf)
+{
+    if (!g)
+        return fail(MCX_EINVAL, "mcx_gen_set_logf: NULL handle");
+    if (chain >= 0) {
+        mcx_status s = check_chain(g, chain, "mcx_gen_set_logf");
+        if (s)
+            return s;
+    }
+    if (!std::isfinite(logf))
+        return fail(MCX_EINVAL, "mcx_gen_set_logf: logf must be finite");
+    for (int i = 0; i < g->b->n; ++i)
+        if (chain < 0 || chain == i)
+            g->logf[i] = logf;
+    HIPCHK(hipSetDevice(g->b->ctx->device));
+    HIPCHK(hipMemcpyAsync(g->d_logf, g->logf.data(), sizeof(double) * g->b->n, hipMemcpyHostToDevice,
+                          g->b->ctx->stream));
+    return sync_check(g->b, "mcx_gen_set_logf");
+}
+
+extern "C" mcx_status mcx_gen_get_logf(mcx_gen* g, double* out)
+{
+    if (!g || !out)
+        return fail(MCX_EINVAL, "mcx_gen_get_logf: NULL argument");
+    std::memcpy(out, g->logf.data(), sizeof(double) * g->b->n);
+    return MCX_OK;
+}
+
+extern "C" mcx_status mcx_gen_run(mcx_gen* g, int64_t nsweeps, uint32_t flags)
+{
+    if (!g)
+        return fail(MCX_EINVAL, "mcx_gen_run: NULL handle");
+    mcx_batch* b = g->b;
+    if (nsweeps < 0)
+        return fail(MCX_EINVAL, "mcx_gen_run: nsweeps must be >= 0");
+    if (flags & ~MCX_RUN_RECORD)
+        return fail(MCX_EINVAL, "mcx_gen_run: only MCX_RUN_RECORD is supported");
+    const bool record = (flags & MCX_RUN_RECORD) != 0;
+    if (record && b->rec_len + nsweeps > b->cap)
+        return fail(MCX_EINVAL, "mcx_gen_run: recording %lld sweeps exceeds the series capacity",
+                    (long long)nsweeps);
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    HIPCHK(hipMemsetAsync(g->d_err, 0xff, sizeof(int64_t) * b->n, st));
+    HIPCHK(hipEventRecord(b->ctx->ev0, st));
+    for (int k = 0; k < g->n_groups; ++k) {
+        GenArgs a{};
+        a.items = g->d_members + g->member_off[k];
+        a.n_items = (int32_t)g->members[k].size();
+        a.kind = g->kind;
+        a.model = b->model;
+        a.dim = b->dim;
+        a.record = record ? 1 : 0;
+        const int64_t ns = b->nsites[g->members[k][0]];
+        a.hist_in_lds = gen_lds(g->nbins[k], ns, true) <= b->ctx->lds_max ? 1 : 0;
+        a.bin_start = g->start[k];
+        a.bin_step = g->step[k];
+        a.nbins = g->nbins[k];
+        a.chains = b->d_chains;
+        a.spins = b->d_spins;
+        a.lw = g->d_lw;
+        a.hist = g->d_hist;
+        a.lw_off = g->d_off;
+        a.logf = g->d_logf;
+        a.steps = g->d_steps;
+        a.accepted = g->d_acc;
+        a.err_site = g->d_err;
+        a.ser_E = b->d_E;
+        a.ser_M = b->d_M;
+        a.ser_Q = b->d_Q;
+        a.nsweeps = nsweeps;
+        a.rec_pos = b->rec_len;
+        a.seed = b->seed;
+        HIPCHK(launch_gen_sweeps(a, gen_lds(g->nbins[k], ns, a.hist_in_lds), st));
+    }
+    HIPCHK(hipEventRecord(b->ctx->ev1, st));
+    mcx_status s = sync_check(b, "mcx_gen_run");
+    if (s)
+        return s;
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1));
+    b->last_ms = ms;
+    s = pull_chains(b);
+    if (s)
+        return s;
+    if (record)
+        b->rec_len += nsweeps;
+    std::vector<int64_t> err(b->n);
+    HIPCHK(hipMemcpy(err.data(), g->d_err, sizeof(int64_t) * b->n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < b->n; ++i)
+        if (err[i] >= 0)
+            return fail(MCX_EBOUNDS, "mcx_gen_run: chain %d: the energy of the attempt at site %lld "
+                                     "lies outside the binned domain", i, (long long)err[i]);
+    return MCX_OK;
+}
+
+extern "C" mcx_status mcx_gen_reset(mcx_gen* g, int32_t chain)
+{
+    if (!g)
+        return fail(MCX_EINVAL, "mcx_gen_reset: NULL handle");
+    mcx_batch* b = g->b;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    for (int i = 0; i < b->n; ++i) {
+        if (chain >= 0 && chain != i)
+            continue;
+        const int64_t nb = g->nbins[g->group_of[i]];
+        HIPCHK(hipMemsetAsync(g->d_hist + g->off[i], 0, sizeof(unsigned long long) * nb, st));
+        HIPCHK(hipMemsetAsync(g->d_steps + i, 0, sizeof(unsigned long long), st));
+        HIPCHK(hipMemsetAsync(g->d_acc + i, 0, sizeof(unsigned long long), st));
+    }
+    return sync_check(b, "mcx_gen_reset");
+}
+
+extern "C" mcx_status mcx_gen_update(mcx_gen* g, int32_t chain)
+{
+    if (!g)
+        return fail(MCX_EINVAL, "mcx_gen_update: NULL handle");
+    if (chain >= 0) {
+        mcx_status s = check_chain(g, chain, "mcx_gen_update");
+        if (s)
+            return s;
+    }
+    mcx_batch* b = g->b;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    for (int k = 0; k < g->n_groups; ++k) {
+        if (chain >= 0) {
+            if (g->group_of[chain] != k)
+                continue;
+            const auto& m = g->members[k];
+            const int pos = (int)(std::find(m.begin(), m.end(), chain) - m.begin());
+            HIPCHK(launch_gen_update(g->d_lw, g->d_hist, g->d_off, g->d_members + g->member_off[k] + pos,
+                                     1, g->nbins[k], st));
+        } else {
+            HIPCHK(launch_gen_update(g->d_lw, g->d_hist, g->d_off, g->d_members + g->member_off[k],
+                                     (int32_t)g->members[k].size(), g->nbins[k], st));
+        }
+    }
+    return sync_check(b, "mcx_gen_update");
+}
+
+extern "C" mcx_status mcx_gen_merge(mcx_gen* g, int32_t group)
+{
+    mcx_status s = check_group(g, group, "mcx_gen_merge");
+    if (s)
+        return s;
+    HIPCHK(hipSetDevice(g->b->ctx->device));
+    HIPCHK(launch_gen_merge(g->d_hist, g->d_off, g->d_members + g->member_off[group],
+                            (int32_t)g->members[group].size(), g->nbins[group], g->b->ctx->stream));
+    return sync_check(g->b, "mcx_gen_merge");
+}
+
+extern "C" mcx_status mcx_gen_distribute(mcx_gen* g, int32_t group)
+{
+    mcx_status s = check_group(g, group, "mcx_gen_distribute");
+    if (s)
+        return s;
+    HIPCHK(hipSetDevice(g->b->ctx->device));
+    HIPCHK(launch_gen_distribute(g->d_lw, g->d_off, g->d_members + g->member_off[group],
+                                 (int32_t)g->members[group].size(), g->nbins[group],
+                                 g->b->ctx->stream));
+    return sync_check(g->b, "mcx_gen_distribute");
+}
+
+extern "C" mcx_status mcx_gen_average_logweight(mcx_gen* g, int32_t group)
+{
+    mcx_status s = check_group(g, group, "mcx_gen_average_logweight");
+    if (s)
+        return s;
+    HIPCHK(hipSetDevice(g->b->ctx->device));
+    HIPCHK(launch_gen_average(g->d_lw, g->d_off, g->d_members + g->member_off[group],
+                              (int32_t)g->members[group].size(), g->nbins[group], g->b->ctx->stream));
+    return sync_check(g->b, "mcx_gen_average_logweight");
+}
+
+extern "C" mcx_status mcx_gen_flatness(mcx_gen* g, int32_t group, double* out3)
+{
+    mcx_status s = check_group(g, group, "mcx_gen_flatness");
+    if (s)
+        return s;
+    if (!out3)
+        return fail(MCX_EINVAL, "mcx_gen_flatness: NULL output");
+    HIPCHK(hipSetDevice(g->b->ctx->device));
+    hipStream_t st = g->b->ctx->stream;
+    HIPCHK(launch_gen_flatness(g->d_hist, g->d_lw, g->d_off, g->members[group][0], g->nbins[group],
+                               g->d_flat, st));
+    HIPCHK(hipMemcpyAsync(out3, g->d_flat, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
+    return sync_check(g->b, "mcx_gen_flatness");
+}
+
+extern "C" mcx_status mcx_gen_get_counters(mcx_gen* g, int64_t* steps, int64_t* accepted)
+{
+    if (!g)
+        return fail(MCX_EINVAL, "mcx_gen_get_counters: NULL handle");
+    mcx_batch* b = g->b;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    if (steps)
+        HIPCHK(hipMemcpyAsync(steps, g->d_steps, sizeof(int64_t) * b->n, hipMemcpyDeviceToHost, st));
+    if (accepted)
+        HIPCHK(hipMemcpyAsync(accepted, g->d_acc, sizeof(int64_t) * b->n, hipMemcpyDeviceToHost, st));
+    return sync_check(b, "mcx_gen_get_counters");
+}
+
+extern "C" mcx_status mcx_gen_group_info(mcx_gen* g, int32_t group, int32_t* root, int32_t* count,
+                                         int64_t* nbins)
+{
+    mcx_status s = check_group(g, group, "mcx_gen_group_info");
+    if (s)
+        return s;
+    if (root)
+        *root = g->members[group][0];
+    if (count)
+        *count = (int32_t)g->members[group].size();
+    if (nbins)
+        *nbins = g->nbins[group];
+    return MCX_OK;
+}
+
+extern "C" mcx_status mcx_gen_set_histogram(mcx_gen* g, int32_t chain, const int64_t* h, int64_t n)
+{
+    mcx_status s = check_chain(g, chain, "mcx_gen_set_histogram");
+    if (s)
+        return s;
+    if (!h || n != g->nbins[g->group_of[chain]])
+        return fail(MCX_EINVAL, "mcx_gen_set_histogram: chain %d has %lld bins", chain,
+                    (long long)g->nbins[g->group_of[chain]]);
+    for (int64_t k = 0; k < n; ++k)
+        if (h[k] < 0)
+            return fail(MCX_EDOMAIN, "mcx_gen_set_histogram: bin %lld has a negative count",
+                        (long long)k);
+    HIPCHK(hipSetDevice(g->b->ctx->device));
+    HIPCHK(hipMemcpyAsync(g->d_hist + g->off[chain], h, sizeof(int64_t) * n, hipMemcpyHostToDevice,
+                          g->b->ctx->stream));
+    return sync_check(g->b, "mcx_gen_set_histogram");
+}
+
+extern "C" mcx_status mcx_muca_update_values(mcx_ctx* ctx, double* lw, const double* h, int64_t n)
+{
+    if (!ctx || !lw || !h || n < 1)
+        return fail(MCX_EINVAL, "mcx_muca_update_values: NULL argument or n < 1");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    double *dl = nullptr, *dh = nullptr;
+    HIPCHK(hipMalloc(&dl, sizeof(double) * n));
+    hipError_t e = hipMalloc(&dh, sizeof(double) * n);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dl, lw, sizeof(double) * n, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dh, h, sizeof(double) * n, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = launch_muca_update_values(dl, dh, n, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(lw, dl, sizeof(double) * n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    hipFree(dl);
+    hipFree(dh);
+    if (e != hipSuccess)
+        return fail(MCX_EHIP, "mcx_muca_update_values: %s", hipGetErrorString(e));
+    return MCX_OK;
+}

@@ -16,6 +16,8 @@ from .batch import LatticeBatch
 from .ensembles import (BinnedObject, BoltzmannEnsemble, MulticanonicalEnsemble,
                         WangLandauEnsemble, get_centers, get_edges, get_values, logweight, set_,
                         update_)
+from .muca import (GeneralizedWalkers, ParallelChains, ParallelMulticanonical, ThreadsBackend,
+                   distribute_logweight_, merge_histograms_)
 from .pt import ParallelTempering, ReplicaExchange, acceptance_rates, index, set_betas
 from .spinsystems import (BlumeCapel, Ising, IsingLatticeOptim, SpinSystemBatch, energy, init_,
                           magnetization, spin_flip_, sweep_)

@@ -118,6 +118,10 @@ def reset_(alg):
     histogram (importance_sampling.jl:106-110, algorithms/multicanonical.jl:27-33)."""
     if hasattr(alg, "reset_"):
         return alg.reset_()
+    dev = getattr(getattr(alg, "ensemble", None), "_dev", None)
+    if dev is not None:
+        dev[0].reset_(dev[1])
+        return None
     alg.steps = 0
     if hasattr(alg, "accepted"):
         alg.accepted = 0

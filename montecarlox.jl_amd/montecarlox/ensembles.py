@@ -235,12 +235,16 @@ class MulticanonicalEnsemble(AbstractEnsemble):
         return self.logweight(x)
 
     def update_(self, mode: str = "simple"):
-        """update!(e; mode=:simple): lw -= log(h) where h > 0 (multicanonical.jl:32-44)."""
+        """update!(e; mode=:simple): lw -= log(h) where h > 0 (multicanonical.jl:32-44),
+        evaluated on the GPU (on the bound walker, or on these arrays)."""
         if mode != "simple":
             raise MCXArgumentError(f"unsupported mode={mode}, currently only :simple")
-        h = self.histogram.values
-        mask = h > 0
-        self.logweight.values[mask] -= np.log(h[mask])
+        dev = getattr(self, "_dev", None)
+        if dev is not None:
+            dev[0].update_(dev[1])
+        else:
+            from .muca import muca_update_values
+            muca_update_values(self.logweight.values, self.histogram.values)
         return None
 
 

@@ -1,0 +1,277 @@
+"""Multicanonical / Wang-Landau walkers on the GPU.
+
+  Multicanonical(rng, bins), WangLandau(rng, bins; logf)   src/algorithms/multicanonical.jl,
+                                                           src/algorithms/wang_landau.jl
+  update!(::MulticanonicalEnsemble), update!(::WangLandauEnsemble)
+                                                           src/ensembles/{multicanonical,wang_landau}.jl
+  ParallelMulticanonical, merge_histograms!, distribute_logweight!
+                                                           src/algorithms/parallel_multicanonical.jl
+
+GeneralizedWalkers binds the ensembles of a list of ImportanceSampling
+algorithms to device walkers (mcx_gen).  While bound, the device holds the
+log-weights and histograms; the host BinnedObject arrays are refreshed after
+every device call and uploaded again when the caller changed them.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import MCXArgumentError, call
+from .algorithms import ImportanceSampling
+from .ensembles import BinnedObject, DiscreteBinning, MulticanonicalEnsemble, WangLandauEnsemble
+
+
+class GenDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("n_groups", C.c_int32),
+                ("group_of_chain", C.POINTER(C.c_int32)), ("bin_start", C.POINTER(C.c_double)),
+                ("bin_step", C.POINTER(C.c_double)), ("n_bins", C.POINTER(C.c_int64))]
+
+
+_P = C.c_void_p
+_pd, _pi64, _pi32 = C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_int32)
+for _n, _a in {
+    "mcx_gen_create": [_P, C.POINTER(GenDesc), C.POINTER(_P)],
+    "mcx_gen_destroy": [_P],
+    "mcx_gen_set_logweight": [_P, C.c_int32, _pd, C.c_int64],
+    "mcx_gen_get_logweight": [_P, C.c_int32, _pd, C.c_int64],
+    "mcx_gen_get_histogram": [_P, C.c_int32, _pi64, C.c_int64],
+    "mcx_gen_set_histogram": [_P, C.c_int32, _pi64, C.c_int64],
+    "mcx_gen_set_logf": [_P, C.c_int32, C.c_double],
+    "mcx_gen_get_logf": [_P, _pd],
+    "mcx_gen_run": [_P, C.c_int64, C.c_uint32],
+    "mcx_gen_reset": [_P, C.c_int32],
+    "mcx_gen_update": [_P, C.c_int32],
+    "mcx_gen_merge": [_P, C.c_int32],
+    "mcx_gen_distribute": [_P, C.c_int32],
+    "mcx_gen_average_logweight": [_P, C.c_int32],
+    "mcx_gen_flatness": [_P, C.c_int32, _pd],
+    "mcx_gen_get_counters": [_P, _pi64, _pi64],
+    "mcx_gen_group_info": [_P, C.c_int32, _pi32, _pi32, _pi64],
+    "mcx_muca_update_values": [_P, _pd, _pd, C.c_int64],
+}.items():
+    _lib.declare(_n, _a)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def muca_update_values(lw: np.ndarray, h: np.ndarray, ctx=None):
+    """lw[i] -= log(h[i]) where h[i] > 0, on the device, in place."""
+    ctx = ctx or _lib.default_context()
+    if lw.dtype != np.float64 or not lw.flags.c_contiguous:
+        raise MCXArgumentError("logweight values must be a contiguous float64 array")
+    hh = np.ascontiguousarray(h, np.float64)
+    call("mcx_muca_update_values", ctx.handle, _ptr(lw, C.c_double), _ptr(hh, C.c_double), lw.size)
+
+
+def _binning(ens):
+    b = ens.logweight.bins
+    if not isinstance(b, DiscreteBinning):
+        raise MCXArgumentError("device walkers need discrete (start, step, num) binning")
+    return (float(b.start), float(b.step), int(b.num))
+
+
+class GeneralizedWalkers:
+    """Device walkers for the ImportanceSampling algorithms `algs` driving the
+    systems of a SpinSystemBatch (one algorithm per system).  Walkers whose
+    ensembles share a binning form a group; the first of a group is its root."""
+
+    def __init__(self, systems, algs: Sequence[ImportanceSampling]):
+        algs = list(algs)
+        if len(algs) != len(systems):
+            raise MCXArgumentError("one algorithm per system")
+        kinds = {type(a.ensemble) for a in algs}
+        if kinds == {MulticanonicalEnsemble}:
+            self.kind = _lib.MULTICANONICAL
+        elif kinds == {WangLandauEnsemble}:
+            self.kind = _lib.WANGLANDAU
+        else:
+            raise MCXArgumentError("all walkers use MulticanonicalEnsemble or all WangLandauEnsemble")
+        self.systems, self.algs = systems, algs
+        keys, groups, self.group_of = [], [], []
+        for a in algs:
+            bk = _binning(a.ensemble)
+            if bk not in keys:
+                keys.append(bk)
+                groups.append([])
+            g = keys.index(bk)
+            groups[g].append(len(self.group_of))
+            self.group_of.append(g)
+        self.groups = groups
+        start = np.array([k[0] for k in keys])
+        step = np.array([k[1] for k in keys])
+        nb = np.array([k[2] for k in keys], np.int64)
+        gof = np.array(self.group_of, np.int32)
+        systems.bind_streams([a.rng for a in algs])
+        d = GenDesc(self.kind, len(keys), _ptr(gof, C.c_int32), _ptr(start, C.c_double),
+                    _ptr(step, C.c_double), _ptr(nb, C.c_int64))
+        self._h = C.c_void_p()
+        call("mcx_gen_create", systems.batch.handle, C.byref(d), C.byref(self._h))
+        self._last_lw = [None] * len(algs)
+        self._last_h = [None] * len(algs)
+        self._last_logf = [None] * len(algs)
+        for k, a in enumerate(algs):
+            a.ensemble._dev = (self, k)
+        self.sync_in()
+
+    # ---- host <-> device ---------------------------------------------------------
+    def sync_in(self):
+        for k, a in enumerate(self.algs):
+            e = a.ensemble
+            lw = np.ascontiguousarray(e.logweight.values, np.float64)
+            if self._last_lw[k] is None or not np.array_equal(lw, self._last_lw[k]):
+                call("mcx_gen_set_logweight", self._h, k, _ptr(lw, C.c_double), lw.size)
+                self._last_lw[k] = lw.copy()
+            if isinstance(e, MulticanonicalEnsemble):
+                h = e.histogram.values
+                if self._last_h[k] is None or not np.array_equal(h, self._last_h[k]):
+                    hi = np.ascontiguousarray(np.rint(h), np.int64)
+                    if not np.array_equal(hi, h):
+                        raise MCXArgumentError("device histograms hold integer counts")
+                    call("mcx_gen_set_histogram", self._h, k, _ptr(hi, C.c_int64), hi.size)
+                    self._last_h[k] = np.asarray(h, np.float64).copy()
+            else:
+                if self._last_logf[k] != e.logf:
+                    call("mcx_gen_set_logf", self._h, k, float(e.logf))
+                    self._last_logf[k] = e.logf
+
+    def sync_out(self):
+        n = len(self.algs)
+        st = np.empty(n, np.int64)
+        ac = np.empty(n, np.int64)
+        call("mcx_gen_get_counters", self._h, _ptr(st, C.c_int64), _ptr(ac, C.c_int64))
+        for k, a in enumerate(self.algs):
+            e = a.ensemble
+            lw = np.empty(len(e.logweight.values), np.float64)
+            call("mcx_gen_get_logweight", self._h, k, _ptr(lw, C.c_double), lw.size)
+            e.logweight.values[:] = lw
+            self._last_lw[k] = lw
+            h = np.empty(len(lw), np.int64)
+            call("mcx_gen_get_histogram", self._h, k, _ptr(h, C.c_int64), h.size)
+            if isinstance(e, MulticanonicalEnsemble):
+                e.histogram.values[:] = h
+                self._last_h[k] = h.astype(np.float64)
+            e.device_histogram = h
+            a.steps, a.accepted = int(st[k]), int(ac[k])
+
+    # ---- operations ----------------------------------------------------------------
+    def run_(self, nsweeps: int, record: bool = False) -> float:
+        self.sync_in()
+        try:
+            call("mcx_gen_run", self._h, int(nsweeps), _lib.RUN_RECORD if record else 0)
+        finally:
+            self.sync_out()
+        return self.systems.batch.last_run_ms()
+
+    def reset_(self, k: int = -1):
+        self.sync_in()
+        call("mcx_gen_reset", self._h, int(k))
+        self.sync_out()
+
+    def update_(self, k: int = -1):
+        self.sync_in()
+        call("mcx_gen_update", self._h, int(k))
+        self.sync_out()
+
+    def merge_(self, group: int = 0):
+        self.sync_in()
+        call("mcx_gen_merge", self._h, int(group))
+        self.sync_out()
+
+    def distribute_(self, group: int = 0):
+        self.sync_in()
+        call("mcx_gen_distribute", self._h, int(group))
+        self.sync_out()
+
+    def average_(self, group: int = 0):
+        self.sync_in()
+        call("mcx_gen_average_logweight", self._h, int(group))
+        self.sync_out()
+
+    def flatness(self, group: int = 0):
+        """(min, mean, bins) of the group root's histogram over reached bins."""
+        self.sync_in()
+        out = np.empty(3, np.float64)
+        call("mcx_gen_flatness", self._h, int(group), _ptr(out, C.c_double))
+        return float(out[0]), float(out[1]), int(out[2])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for a in self.algs:
+                a.ensemble._dev = None
+            call("mcx_gen_destroy", self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ThreadsBackend:
+    """ThreadsBackend(n) (src/infrastructure/parallel_backends.jl:25-33): the
+    chains of one device batch."""
+
+    def __init__(self, n: int):
+        self.n = int(n)
+
+    def size(self):
+        return self.n
+
+
+class ParallelChains:
+    """ParallelChains(backend, algs) (src/infrastructure/parallel_chains.jl:14-128)."""
+
+    def __init__(self, backend: ThreadsBackend, algs: Sequence):
+        if backend.n != len(algs):
+            raise MCXArgumentError("size(backend) must equal length(algs)")
+        self.backend, self.alg = backend, list(algs)
+
+    def size(self):
+        return len(self.alg)
+
+    def rank(self):
+        return 0
+
+    def is_root(self):
+        return True
+
+    def algorithm(self, i: int):
+        return self.alg[i - 1]
+
+
+def ParallelMulticanonical(backend: ThreadsBackend, algs: Sequence[ImportanceSampling]):
+    for a in algs:
+        if not isinstance(a.ensemble, MulticanonicalEnsemble):
+            raise MCXArgumentError("ParallelMulticanonical needs MulticanonicalEnsemble algorithms")
+    return ParallelChains(backend, algs)
+
+
+def _walkers_of(pc: ParallelChains):
+    devs = {getattr(a.ensemble, "_dev", None) and id(a.ensemble._dev[0]) for a in pc.alg}
+    if len(devs) == 1 and None not in devs:
+        w = pc.alg[0].ensemble._dev[0]
+        return w, w.group_of[pc.alg[0].ensemble._dev[1]]
+    return None, None
+
+
+def merge_histograms_(pc: ParallelChains):
+    """Sum every chain's histogram into the root chain's (parallel_multicanonical.jl:108-117)."""
+    w, g = _walkers_of(pc)
+    if w is None:
+        raise MCXArgumentError("merge_histograms! runs on device walkers: run the chains first")
+    w.merge_(g)
+
+
+def distribute_logweight_(pc: ParallelChains):
+    """Copy the root's log-weights to every chain (parallel_multicanonical.jl:129-138)."""
+    w, g = _walkers_of(pc)
+    if w is None:
+        raise MCXArgumentError("distribute_logweight! runs on device walkers: run the chains first")
+    w.distribute_(g)

@@ -18,7 +18,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import MCXArgumentError, MCXAssertionError
-from .algorithms import HeatBath, Metropolis, Philox, algorithm_beta, algorithm_code
+from .algorithms import (HeatBath, ImportanceSampling, Metropolis, Philox, algorithm_beta,
+                         algorithm_code)
 from .batch import LatticeBatch
 
 
@@ -50,6 +51,7 @@ class SpinSystemBatch:
         self._bound = None  # (seed, keys) currently bound
         self._tables = None  # betas currently uploaded
         self._pt = None  # ParallelTempering driving this batch, if any
+        self._walkers = None  # GeneralizedWalkers bound to this batch, if any
 
     def __len__(self):
         return self.batch.n
@@ -64,8 +66,7 @@ class SpinSystemBatch:
         return self.batch.D
 
     # stream / table binding -------------------------------------------------
-    def bind(self, algs: Sequence):
-        rngs = [a.rng for a in algs]
+    def bind_streams(self, rngs: Sequence[Philox]):
         seeds = {r.seed for r in rngs}
         if len(seeds) != 1:
             raise MCXArgumentError("all chains of one batch share the Philox seed; keys differ")
@@ -74,6 +75,9 @@ class SpinSystemBatch:
         if self._bound is None or self._bound[0] != seed or not np.array_equal(self._bound[1], keys):
             self.batch.set_streams(seed, keys)
             self._bound = (seed, keys)
+
+    def bind(self, algs: Sequence):
+        self.bind_streams([a.rng for a in algs])
         if self._pt is not None:
             return  # a replica-exchange ladder owns the acceptance tables
         betas = np.array([algorithm_beta(a) for a in algs], np.float64)
@@ -81,10 +85,24 @@ class SpinSystemBatch:
             self.batch.set_betas(betas)
             self._tables = betas
 
+    def walkers(self, algs: Sequence):
+        """Device multicanonical / Wang-Landau walkers bound to these algorithms."""
+        from .muca import GeneralizedWalkers
+        w = self._walkers
+        if w is None or len(w.algs) != len(algs) or any(x is not y for x, y in zip(w.algs, algs)):
+            if w is not None:
+                w.close()
+            w = self._walkers = GeneralizedWalkers(self, algs)
+        return w
+
     def sweep_(self, algs, nsweeps: int = 1, record: bool = False, plain: bool = False) -> float:
         algs = list(algs) if isinstance(algs, (list, tuple)) else [algs] * self.batch.n
         if len(algs) != self.batch.n:
             raise MCXArgumentError("one algorithm per chain")
+        if all(isinstance(a, ImportanceSampling) for a in algs):
+            if plain:
+                raise MCXArgumentError("multicanonical / Wang-Landau walkers have no plain path")
+            return self.walkers(algs).run_(nsweeps, record=record)
         codes = {algorithm_code(a) for a in algs}
         if len(codes) != 1:
             raise MCXArgumentError("all chains of one sweep_ call use the same algorithm type")

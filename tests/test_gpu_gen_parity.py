@@ -1,0 +1,166 @@
+"""Multicanonical / Wang-Landau walkers on the GPU vs the sequential oracle.
+
+Integer outputs (spins, histograms, E/M/Q series, counters) and Wang-Landau
+weights (exact FP64 subtractions) are compared bit for bit; the multicanonical
+recursion lw -= log(h) is compared to 4 ulp (device log vs libm log).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _walkers(mcx, model, dim, L, n, kind, start, step, nbins, seed, lw0=None, logf=1.0, D=0.0,
+             cap=0):
+    from montecarlox.spinsystems import SpinSystemBatch
+    from montecarlox.ensembles import DiscreteBinning
+    sys = SpinSystemBatch(model, [L] * dim, n=n, D=D, series_capacity=cap)
+    algs = []
+    for k in range(n):
+        # a range domain start:step:stop carries its step (binned_object.jl:44-46)
+        bo = mcx.BinnedObject(DiscreteBinning(start, step, nbins), 0.0)
+        if lw0 is not None:
+            bo.values[:] = lw0[k]
+        if kind == "muca":
+            algs.append(mcx.Multicanonical(mcx.Philox(seed, k), bo))
+        else:
+            algs.append(mcx.WangLandau(mcx.Philox(seed, k), bo, logf=logf))
+    sys.init_("random", [a.rng for a in algs])
+    return sys, algs
+
+
+def _check(mcx, orc, model, dim, L, n, kind, start, step, nbins, seed, nsweeps, D=0.0, logf=1.0,
+           lw_scale=0.3, runs=None):
+    rng = np.random.default_rng(seed)
+    lw0 = rng.normal(0, lw_scale, size=(n, nbins))
+    sys, algs = _walkers(mcx, model, dim, L, n, kind, start, step, nbins, seed, lw0, logf, D,
+                         cap=nsweeps)
+    init = [sys.batch.get_spins(k) for k in range(n)]
+    print(f"gen model={model} dim={dim} L={L} kind={kind} nbins={nbins}", flush=True)
+    for p in (runs or [nsweeps]):
+        mcx.sweep_(sys, algs, p, record=True)
+    code = {"ising": 0, "blume_capel": 1}[model]
+    for k in range(n):
+        s = init[k].copy()
+        lw = lw0[k].copy()
+        h = np.zeros(nbins, np.int64)
+        E, M, Q, st, ac = orc.generalized_sweeps(code, 1 if kind == "wl" else 0, dim, L, lw, h, s,
+                                                 seed, k, 0, nsweeps, bin_start=start,
+                                                 bin_step=step, logf=logf, D=D)
+        e = algs[k].ensemble
+        hg = e.histogram.values if kind == "muca" else e.device_histogram
+        assert np.array_equal(s, sys.batch.get_spins(k)), f"walker {k} spins"
+        assert np.array_equal(h, np.asarray(hg, np.int64)), f"walker {k} histogram"
+        assert np.array_equal(lw, e.logweight.values), f"walker {k} log-weights"
+        Eg, Mg, Qg = sys.batch.series(k)
+        assert np.array_equal(E, Eg) and np.array_equal(M, Mg)
+        if Qg is not None:
+            assert np.array_equal(Q, Qg)
+        assert algs[k].steps == st and algs[k].accepted == ac
+    return sys, algs
+
+
+@pytest.mark.parametrize("kind", ["muca", "wl"])
+def test_ising_2d_walkers(mcx, orc, kind):
+    N = 64
+    _check(mcx, orc, "ising", 2, 8, 3, kind, -2 * N, 4, N + 1, 21, 12)
+
+
+@pytest.mark.parametrize("kind", ["muca", "wl"])
+def test_ising_3d_walkers(mcx, orc, kind):
+    N = 64
+    _check(mcx, orc, "ising", 3, 4, 2, kind, -3 * N, 4, 3 * N // 2 + 1, 5, 6)
+
+
+@pytest.mark.parametrize("kind", ["muca", "wl"])
+@pytest.mark.parametrize("D", [1.0, 0.3])
+def test_blume_capel_walkers(mcx, orc, kind, D):
+    L, N = 6, 36
+    # domain wide enough for every state: E in [-2N, 2N + D N]
+    start, step = -2.0 * N, (1.0 if D == 1.0 else 0.1)
+    nbins = int(round((2 * N + D * N - start) / step)) + 2
+    _check(mcx, orc, "blume_capel", 2, L, 2, kind, start, step, nbins, 9, 5, D=D, logf=0.25)
+
+
+def test_histogram_in_hbm_path(mcx, orc):
+    # L = 64 Blume-Capel: 16385 bins -- the uint32 LDS histogram does not fit, int64 atomics in HBM
+    L, N = 64, 4096
+    _check(mcx, orc, "blume_capel", 2, L, 1, "muca", -N * 1.0, 1.0, 4 * N + 1, 3, 2, D=1.0)
+
+
+def test_split_runs(mcx, orc):
+    _check(mcx, orc, "ising", 2, 8, 2, "wl", -128, 4, 65, 4, 9, runs=[3, 6], logf=0.5)
+
+
+def test_bounds_error(mcx):
+    sys, algs = _walkers(mcx, "ising", 2, 4, 1, "muca", 0, 4, 3, 1)  # domain [0, 8]
+    sys.batch.set_spins(0, np.ones(16, np.int8))  # E = -32: outside
+    with pytest.raises(mcx.MCXBoundsError):
+        mcx.sweep_(sys, algs, 1)
+    assert algs[0].steps == 0 and algs[0].ensemble.histogram.values.sum() == 0
+
+
+def test_flat_weights_accept_everything(mcx):
+    sys, algs = _walkers(mcx, "ising", 2, 6, 2, "muca", -72, 4, 37, 8)
+    mcx.sweep_(sys, algs, 10)
+    for a in algs:
+        assert a.steps == 360 and mcx.acceptance_rate(a) == 1.0
+        assert a.ensemble.histogram.values.sum() == a.steps
+    mcx.reset_(algs[0])
+    assert algs[0].steps == 0 and algs[0].ensemble.histogram.values.sum() == 0
+
+
+def test_update_merge_distribute(mcx, orc):
+    # parallel_multicanonical KATs (test_parallel_ensembles.jl:95-113) on device walkers
+    sys, algs = _walkers(mcx, "ising", 2, 4, 2, "muca", -32, 4, 4, 2)
+    w = sys.walkers(algs)
+    algs[0].ensemble.histogram.values[:] = [1, 2, 3, 4]
+    algs[1].ensemble.histogram.values[:] = [4, 3, 2, 1]
+    pc = mcx.ParallelMulticanonical(mcx.ThreadsBackend(2), algs)
+    mcx.merge_histograms_(pc)
+    assert list(algs[0].ensemble.histogram.values) == [5, 5, 5, 5]
+    assert list(algs[1].ensemble.histogram.values) == [4, 3, 2, 1]
+    algs[0].ensemble.logweight.values[:] = [1.0, 2.0, 3.0, 4.0]
+    mcx.distribute_logweight_(pc)
+    assert list(algs[1].ensemble.logweight.values) == [1.0, 2.0, 3.0, 4.0]
+    # update!: lw -= log(h) where h > 0, on the device
+    algs[1].ensemble.histogram.values[:] = [0, 7, 1, 3]
+    mcx.update_(algs[1].ensemble)
+    ref = np.array([1.0, 2.0, 3.0, 4.0])
+    orc.muca_update(ref, np.array([0.0, 7.0, 1.0, 3.0]))
+    got = algs[1].ensemble.logweight.values
+    assert np.all(np.abs(got - ref) <= 4 * np.spacing(np.abs(ref)))
+    # unbound ensemble with a Float64 histogram (test_multicanonical.jl:53-73)
+    e = mcx.MulticanonicalEnsemble(mcx.BinnedObject(np.arange(0.0, 5.0).tolist(), 0.0))  # 0.0:1.0:4.0
+    e.histogram.values[:] = [0.2, 0.8, 1.1, 2.5]
+    assert e.update_() is None
+    exp = -np.log([0.2, 0.8, 1.1, 2.5])
+    assert np.all(np.abs(e.logweight.values - exp) <= 4 * np.spacing(np.abs(exp)))
+
+
+def test_wang_landau_converges_on_gpu(mcx):
+    raw = np.loadtxt(os.path.join(GOLD, "ising2D_8x8.csv"), delimiter=",", skiprows=1)
+    L, N, n = 8, 64, 8
+    nb = N + 1
+    exact = np.full(nb, np.nan)
+    exact[((raw[:, 0] + 2 * N) // 4).astype(int)] = raw[:, 1]
+    sys, algs = _walkers(mcx, "ising", 2, L, n, "wl", -2 * N, 4, nb, 13, logf=1.0)
+    w = sys.walkers(algs)
+    logf = 1.0
+    while logf > 1e-4:
+        for a in algs:
+            a.ensemble.logf = logf
+        mcx.sweep_(sys, algs, 2000)
+        w.average_(0)
+        logf *= 0.5
+    lw = algs[0].ensemble.logweight.values
+    ok = np.isfinite(exact)
+    est = -(lw - lw[N // 2])
+    ref = exact - exact[N // 2]
+    rmse = math.sqrt(np.mean((est[ok] - ref[ok]) ** 2))
+    print("rmse", rmse)
+    assert rmse < 0.1
