@@ -11,7 +11,9 @@ Printed on one JSON line: whole-job attempts/ns of the batched path, the
 roofline of its kernel (2 algorithmic bytes per attempt: one int8 read + one
 int8 write per site per sweep), the plain one-chain-per-launch GPU path on a
 sample, and the reference CPU path (random-site Metropolis with Xoshiro256++,
-restated in C under oracle/) timed on the host cores on a sample.
+restated in C under oracle/) timed on the host cores on a sample, and the
+serving stream: 200 Poisson-arriving jobs of mixed model, L, T and sweep count
+replayed in real time against the job server, with p50/p99 job latency.
 """
 import argparse
 import json
@@ -38,9 +40,14 @@ def parse():
     p.add_argument("--L", type=int, default=64)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-plain", action="store_true", help="skip the plain-path sample")
-    p.add_argument("--cpu-sweeps", type=int, default=100)
+    p.add_argument("--cpu-sweeps", type=int, default=1000)
     p.add_argument("--plain-chains", type=int, default=16)
     p.add_argument("--plain-sweeps", type=int, default=20)
+    p.add_argument("--no-serving", action="store_true", help="skip the serving stream")
+    p.add_argument("--serving-jobs", type=int, default=200,
+                   help="jobs of the Poisson serving stream (200/s, seed 19)")
+    p.add_argument("--serving-budget", type=int, default=1 << 28,
+                   help="HBM budget of the job server in bytes")
     p.add_argument("--copy-probe", type=int, default=0,
                    help="also run a device copy of this many bytes (HBM counter scaling)")
     return p.parse_args()
@@ -69,6 +76,47 @@ def stream_copy_gbs(torch, nbytes=1 << 30, reps=10):
     dt = time.perf_counter() - t0
     del x, y
     return 2.0 * nbytes * reps / dt / 1e9
+
+
+def serving_stream(mcx, n_jobs, budget):
+    """Replays the Poisson serving stream in real time against one job server:
+    job k is submitted at its arrival time; latency runs from submission to the
+    result being ready.  Returns the server's latency / throughput summary."""
+    jobs = mcx.poisson_workload(n_jobs)
+    # warm the kernels of every class once (not counted)
+    with mcx.Server(hbm_budget=budget, max_jobs=64) as w:
+        w.run([mcx.Job(m, L, 0.5, 2, alg, D=0.5) for m in ("ising", "blume_capel")
+               for L in (16, 64, 256, 512) for alg in ("metropolis",)])
+    srv = mcx.Server(hbm_budget=budget, max_jobs=1024)
+    t0 = time.perf_counter()
+    results = []
+    for j in jobs:
+        dt = t0 + j.arrival_s - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        srv.submit(j)
+        results.extend(srv.poll(1024))
+    srv.drain()
+    results.extend(srv.poll(1024))
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    assert len(results) == n_jobs and all(r["status"] == 0 for r in results)
+    work = float(sum(j.attempts for j in jobs))
+    return {
+        "workload": f"{n_jobs} jobs, Poisson arrivals at 200/s (seed 19): 70% 2D Ising / 30% "
+                    "Blume-Capel (D uniform [0, 1.9]), L log-uniform [16, 512] rounded to even, "
+                    "T uniform [1.5, 3.5], sweeps log-uniform [1e3, 1e5], checkerboard "
+                    "Metropolis from a random start, FP64 moments of every sweep",
+        "p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2),
+        "max_ms": round(st["max_ms"], 2),
+        "mean_queue_ms": round(float(sum(r["queue_ms"] for r in results)) / n_jobs, 3),
+        "attempts": work,
+        "attempts_per_ns": round(work / (wall * 1e9), 3),
+        "busy_attempts_per_ns": round(st["busy_attempts_per_ns"], 3),
+        "offered_attempts_per_ns": round(work / (jobs[-1].arrival_s * 1e9), 3),
+        "wall_s": round(wall, 3), "quanta": st["quanta"], "hbm_budget": budget,
+    }
 
 
 def main():
@@ -170,6 +218,9 @@ def main():
                                          "Metropolis (C restatement of spin_flip!(IsingLatticeOptim, "
                                          "Metropolis) with Xoshiro256++), one thread per chain group",
                                "seconds": round(cw, 2)}
+
+    if not a.no_serving and a.serving_jobs > 0:
+        out["serving"] = serving_stream(mcx, a.serving_jobs, a.serving_budget)
 
     if a.copy_probe:
         x = torch.empty(a.copy_probe, dtype=torch.uint8, device="cuda")

@@ -295,6 +295,70 @@ mcx_status mcx_gen_set_histogram(mcx_gen* g, int32_t chain, const int64_t* hist,
  * device (src/ensembles/multicanonical.jl:32-44). */
 mcx_status mcx_muca_update_values(mcx_ctx* ctx, double* lw, const double* h, int64_t n);
 
+/* ------------------------------------------------------------------------ */
+/* Serving: continuous batching of independent simulation jobs              */
+/* ------------------------------------------------------------------------ */
+/* No reference counterpart: the reference runs one chain per thread to
+ * completion (ParallelChains(ThreadsBackend(n), ...), parallel_chains.jl:99-105).
+ * A server owns a device pool of hbm_budget bytes for int8 lattices and
+ * max_jobs job slots.  A scheduler thread admits queued jobs as pool space and
+ * slots free up (arrival order; a job that does not fit yet does not block
+ * smaller ones behind it) and advances all resident jobs in quanta, each job
+ * running min(remaining, max(1, quantum_attempts / max(N, min_sites))) sweeps
+ * per quantum.  A job's result depends only on its descriptor: it equals a
+ * one-chain mcx_batch with the same model, dim, L, beta, D, seed, chain key,
+ * MCX_INIT_RANDOM start and nsweeps sweeps of mcx_batch_run(alg, RECORD), for
+ * spins, counters, last-sweep observables and moments alike. */
+typedef struct {
+    int64_t hbm_budget;       /* bytes for resident lattices (N bytes each, 256-aligned) */
+    int32_t max_jobs;         /* resident job slots                                       */
+    int32_t min_sites;        /* cost floor per sweep in the quantum rule (0 = 4096)      */
+    int64_t quantum_attempts; /* target attempts per job per quantum (0 = 2^24)           */
+} mcx_server_desc;
+
+typedef struct {
+    int32_t model, dim, L, alg; /* L even >= 4; alg MCX_METROPOLIS or MCX_HEATBATH      */
+    double beta, D;             /* D: Blume-Capel crystal field (ignored for Ising)     */
+    uint64_t seed;              /* Philox key word 0 = low 32 bits, counter word 3 = high */
+    uint32_t key;               /* Philox key word 1 (the chain key)                    */
+    int64_t nsweeps;            /* >= 1                                                 */
+    uint64_t tag;               /* returned unchanged in the result                     */
+} mcx_job_desc;
+
+typedef struct {
+    uint64_t job_id, tag;
+    int32_t status; /* MCX_OK */
+    int32_t L;
+    int64_t nsweeps, accepted;
+    int32_t E_pair, M, Q, pad; /* observables after the last sweep */
+    double moments[MCX_N_MOMENTS]; /* as mcx_batch_get_moments, over every sweep */
+    double latency_ms;  /* submit -> result ready                        */
+    double queue_ms;    /* submit -> admission                           */
+    double device_ms;   /* device time of the quanta the job took part in */
+} mcx_job_result;
+
+typedef struct {
+    int64_t submitted, completed, pending, active, quanta;
+    double attempts;             /* site visits done                                 */
+    double busy_ms;              /* device time of all quanta                        */
+    double wall_ms;              /* first admission -> last completion               */
+    double p50_ms, p99_ms, max_ms; /* job latency (submit -> result), interpolated   */
+    double attempts_per_ns;      /* attempts / wall_ms                               */
+    double busy_attempts_per_ns; /* attempts / busy_ms                               */
+} mcx_server_stats_t;
+
+mcx_status mcx_server_create(mcx_ctx* ctx, const mcx_server_desc* desc, mcx_server** out);
+/* Destroys after the current quantum; jobs not finished are dropped. */
+mcx_status mcx_server_destroy(mcx_server* s);
+/* Queues a job (thread-safe); MCX_EINVAL for a job that can never fit. */
+mcx_status mcx_server_submit(mcx_server* s, const mcx_job_desc* job, uint64_t* job_id);
+/* Moves up to max finished results, oldest first, into out; *n = number moved. */
+mcx_status mcx_server_poll(mcx_server* s, mcx_job_result* out, int32_t max, int32_t* n);
+/* Waits until every submitted job has finished (timeout_ms < 0: no limit);
+ * MCX_ESTATE on timeout. */
+mcx_status mcx_server_drain(mcx_server* s, double timeout_ms);
+mcx_status mcx_server_stats(mcx_server* s, mcx_server_stats_t* out);
+
 #ifdef __cplusplus
 }
 #endif

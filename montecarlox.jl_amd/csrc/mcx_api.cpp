@@ -316,6 +316,7 @@ extern "C" mcx_status mcx_batch_create(mcx_ctx* ctx, const mcx_batch_desc* d, mc
         r.table = i;
         r.key = d->chain_keys ? d->chain_keys[i] : (uint32_t)i;
         r.D = d->D ? d->D[i] : d->D_uniform;
+        r.seed = d->seed;
         if (d->model == MCX_ISING)
             r.D = 0.0;
         b->nsites[i] = N;
@@ -527,7 +528,7 @@ extern "C" mcx_status mcx_batch_init_spins(mcx_batch* b, int32_t mode)
     if (mode == MCX_INIT_ZERO && b->model == MCX_ISING)
         return fail(MCX_EINVAL, "Unknown initialization type: zero (Ising spins are +-1)");
     HIPCHK(hipSetDevice(b->ctx->device));
-    HIPCHK(launch_init_spins(b->model, mode, b->d_chains, b->n, b->max_sites, b->d_spins, b->seed,
+    HIPCHK(launch_init_spins(b->model, mode, b->d_chains, b->n, b->max_sites, b->d_spins,
                              b->ctx->stream));
     return sync_check(b, "mcx_batch_init_spins");
 }
@@ -588,7 +589,6 @@ mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t
         a.ser_M = b->d_M;
         a.ser_Q = b->d_Q;
         a.accepted = b->d_acc;
-        a.seed = b->seed;
         for (int64_t s = 0; s < nsweeps; ++s) {
             a.rec_index = record ? b->rec_len + s : -1;
             for (int ch = 0; ch < b->n; ++ch) {
@@ -626,7 +626,6 @@ mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t
         a.accepted = b->d_acc;
         a.nsweeps = nsweeps;
         a.rec_pos = b->rec_len;
-        a.seed = b->seed;
         if (want_last && nsweeps > 0) {
             a.last_E = b->d_last;
             a.last_M = b->d_last + b->n;
@@ -842,8 +841,10 @@ extern "C" mcx_status mcx_batch_set_streams(mcx_batch* b, uint64_t seed, const u
     if (s)
         return s;
     b->seed = seed;
-    for (int i = 0; i < b->n; ++i)
+    for (int i = 0; i < b->n; ++i) {
         b->rec[i].key = keys ? keys[i] : (uint32_t)i;
+        b->rec[i].seed = seed;
+    }
     s = upload_chains(b);
     if (s)
         return s;

@@ -219,13 +219,13 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int nwaves = (B + 63) >> 6;
-    const uint32_t k0 = (uint32_t)a.seed;
-    const uint32_t c3 = (uint32_t)(a.seed >> 32);
     constexpr uint32_t purpose = ALG == 0 ? P_BC_METRO : P_BC_HB;
 
     for (int item = blockIdx.x; item < a.n_items; item += gridDim.x) {
         const int ch = a.items[item];
         const ChainRec cr = a.chains[ch];
+        const uint32_t k0 = (uint32_t)cr.seed;
+        const uint32_t c3 = (uint32_t)(cr.seed >> 32);
         const int L = cr.L;
         const int half = L >> 1;
         const int nw = (half + 31) >> 5;
@@ -263,11 +263,12 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
 
         unsigned long long acc = 0;
         double mE = 0, mE2 = 0, mAM = 0, mM2 = 0, mM4 = 0;
-        for (int64_t s = 0; s < a.nsweeps; ++s) {
+        const int64_t nsw = a.item_nsweeps ? a.item_nsweeps[item] : a.nsweeps;
+        for (int64_t s = 0; s < nsw; ++s) {
             const uint64_t sweep = cr.sweep + (uint64_t)s;
             const uint32_t sw_lo = (uint32_t)sweep;
             const uint32_t w2 = ctr_word2(sweep, 0, purpose);
-            const bool obs = a.record || (a.last_E != nullptr && s == a.nsweeps - 1);
+            const bool obs = a.record || (a.last_E != nullptr && s == nsw - 1);
             int32_t pr = 0, mm = 0, qq = 0;
             for (int c = 0; c < 2; ++c) {
                 uint32_t* __restrict__ Oc = P + c * words;
@@ -352,16 +353,18 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
                     tm += part[(s & 1) * 48 + k * 3 + 1];
                     tq += part[(s & 1) * 48 + k * 3 + 2];
                 }
-                if (a.last_E && s == a.nsweeps - 1) {
+                if (a.last_E && s == nsw - 1) {
                     a.last_E[ch] = (int32_t)(-tp);
                     a.last_M[ch] = (int32_t)tm;
                     a.last_Q[ch] = (int32_t)tq;
                 }
                 if (a.record) {
-                    const int64_t at = cr.series_off + a.rec_pos + s;
-                    a.ser_E[at] = (int32_t)(-tp);
-                    a.ser_M[at] = (int32_t)tm;
-                    a.ser_Q[at] = (int32_t)tq;
+                    if (a.ser_E) {
+                        const int64_t at = cr.series_off + a.rec_pos + s;
+                        a.ser_E[at] = (int32_t)(-tp);
+                        a.ser_M[at] = (int32_t)tm;
+                        a.ser_Q[at] = (int32_t)tq;
+                    }
                     const double e = (double)(int32_t)(-tp) + cr.D * (double)(int32_t)tq;
                     const double m = (double)tm;
                     mE += e;
@@ -394,14 +397,14 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
         if (tid == 0) {
             if (a.record) {
                 double* mo = a.moments + (int64_t)ch * 6;
-                mo[0] += (double)a.nsweeps;
+                mo[0] += (double)nsw;
                 mo[1] += mE;
                 mo[2] += mE2;
                 mo[3] += mAM;
                 mo[4] += mM2;
                 mo[5] += mM4;
             }
-            a.chains[ch].sweep = cr.sweep + (uint64_t)a.nsweeps;
+            a.chains[ch].sweep = cr.sweep + (uint64_t)nsw;
         }
         __syncthreads();
     }
@@ -445,7 +448,7 @@ __global__ void __launch_bounds__(256) k_bc_plain(PlainArgs a, int n_col)
         const int s_old = sp[rowb + x];
         const BCTable& tb = a.btab[cr.table];
         const uint32_t group = (uint32_t)((r * 2 + c) * nw + (j >> 5));
-        const uint32_t rw = site_word(a.seed, cr.key, group, j & 31, a.sweep,
+        const uint32_t rw = site_word(cr.seed, cr.key, group, j & 31, a.sweep,
                                       ALG == 0 ? P_BC_METRO : P_BC_HB);
         const int sidx = nb + 2 * DIM;
         int s_new;

@@ -42,7 +42,6 @@ struct GenArgs {
     int32_t* ser_Q;
     int64_t nsweeps;
     int64_t rec_pos;
-    uint64_t seed;
 };
 
 hipError_t launch_gen_sweeps(const GenArgs& a, size_t lds_bytes, hipStream_t st);

@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                         const int si = sp[rowb + x];
                         const uint32_t g = (uint32_t)((r * 2 + c) * nw + (j >> 5));
                         AttemptRec t;
-                        t.w = site_word(a.seed, cr.key, g, j & 31, sweep, P_GEN);
+                        t.w = site_word(cr.seed, cr.key, g, j & 31, sweep, P_GEN);
                         t.site = (int32_t)(rowb + x);
                         if (MODEL == 0) {
                             t.dE = 2 * si * nbs;

@@ -305,7 +305,6 @@ extern "C" mcx_status mcx_gen_run(mcx_gen* g, int64_t nsweeps, uint32_t flags)
         a.ser_Q = b->d_Q;
         a.nsweeps = nsweeps;
         a.rec_pos = b->rec_len;
-        a.seed = b->seed;
         HIPCHK(launch_gen_sweeps(a, gen_lds(g->nbins[k], ns, a.hist_in_lds), st));
     }
     HIPCHK(hipEventRecord(b->ctx->ev1, st));

@@ -23,6 +23,7 @@ struct ChainRec {
     int64_t spin_off;   // byte offset of the chain's int8 lattice in the spin pool
     int64_t series_off; // element offset of the chain's series (chain * capacity)
     uint64_t sweep;     // sweeps done so far: the Philox counter position
+    uint64_t seed;      // Philox key word 0 = seed mod 2^32, counter word 3 = seed >> 32
     int32_t L;          // even side length >= 4
     int32_t dim;        // 2 or 3
     int32_t table;      // index into the acceptance tables
@@ -57,7 +58,7 @@ struct BCTable {
 struct SweepArgs {
     const int32_t* items; // chains to process, in launch order
     int32_t n_items;
-    int32_t record;       // write series + moments
+    int32_t record;       // accumulate moments (and write the series when ser_E is set)
     ChainRec* chains;
     const IsingTable* itab;
     const BCTable* btab;
@@ -69,10 +70,10 @@ struct SweepArgs {
     unsigned long long* accepted;
     int64_t nsweeps;
     int64_t rec_pos;      // series index of the first sweep of this call
-    uint64_t seed;
     int32_t* last_E;      // if set: E_pair / M / Q after the last sweep, per chain
     int32_t* last_M;
     int32_t* last_Q;
+    const int64_t* item_nsweeps; // if set: sweeps of item k (overrides nsweeps)
 };
 
 struct PlainArgs {
@@ -86,7 +87,6 @@ struct PlainArgs {
     unsigned long long* accepted;
     int64_t rec_index;  // series index of this sweep (< 0: not recording)
     uint64_t sweep;     // sweep counter value for this sweep
-    uint64_t seed;
     int32_t chain;
     int32_t colour;
 };
@@ -101,7 +101,7 @@ hipError_t launch_bc_sweeps(int dim, int alg, const SweepArgs& a, int block, int
 hipError_t launch_bc_plain(int dim, int alg, const PlainArgs& a, int n_sites_colour,
                            hipStream_t st);
 hipError_t launch_init_spins(int model, int mode, ChainRec* chains, int32_t n_chains,
-                             int64_t max_sites, int8_t* spins, uint64_t seed, hipStream_t st);
+                             int64_t max_sites, int8_t* spins, hipStream_t st);
 hipError_t launch_observables(ChainRec* chains, int32_t n_chains, int64_t max_sites,
                               const int8_t* spins, int32_t* E, int32_t* M, int32_t* Q,
                               hipStream_t st);

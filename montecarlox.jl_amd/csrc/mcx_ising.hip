@@ -108,8 +108,6 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int nwaves = (B + 63) >> 6;
-    const uint32_t k0 = (uint32_t)a.seed;
-    const uint32_t c3 = (uint32_t)(a.seed >> 32);
     constexpr uint32_t purpose = ALG == 0 ? P_METRO : P_HB;
     constexpr int NMC = DIM;         // Metropolis uphill classes
     constexpr int NHC = 2 * DIM + 1; // heat-bath classes
@@ -117,6 +115,8 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
     for (int item = blockIdx.x; item < a.n_items; item += gridDim.x) {
         const int ch = a.items[item];
         const ChainRec cr = a.chains[ch];
+        const uint32_t k0 = (uint32_t)cr.seed;
+        const uint32_t c3 = (uint32_t)(cr.seed >> 32);
         const int L = cr.L;
         const int half = L >> 1;
         const int nw = (half + 31) >> 5;
@@ -157,12 +157,13 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
         unsigned long long acc = 0;
         double mE = 0, mE2 = 0, mAM = 0, mM2 = 0, mM4 = 0;
 
-        for (int64_t s = 0; s < a.nsweeps; ++s) {
+        const int64_t nsw = a.item_nsweeps ? a.item_nsweeps[item] : a.nsweeps;
+        for (int64_t s = 0; s < nsw; ++s) {
             const uint64_t sweep = cr.sweep + (uint64_t)s;
             const uint32_t sw_lo = (uint32_t)sweep;
             const uint32_t w2 = ctr_word2(sweep, 0, purpose);
             // observables: every recorded sweep, and the last one when asked for
-            const bool obs = a.record || (a.last_E != nullptr && s == a.nsweeps - 1);
+            const bool obs = a.record || (a.last_E != nullptr && s == nsw - 1);
             uint32_t nu = 0, na = 0;
             for (int c = 0; c < 2; ++c) {
                 uint32_t* __restrict__ Pc = P + c * words;
@@ -267,15 +268,17 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                 }
                 const int32_t E = (int32_t)(2 * (int64_t)ta - (int64_t)DIM * N);
                 const int32_t M = (int32_t)(2 * (int64_t)tu - N);
-                if (a.last_E && s == a.nsweeps - 1) {
+                if (a.last_E && s == nsw - 1) {
                     a.last_E[ch] = E;
                     a.last_M[ch] = M;
                     a.last_Q[ch] = (int32_t)N;
                 }
                 if (a.record) {
-                    const int64_t at = cr.series_off + a.rec_pos + s;
-                    a.ser_E[at] = E;
-                    a.ser_M[at] = M;
+                    if (a.ser_E) {
+                        const int64_t at = cr.series_off + a.rec_pos + s;
+                        a.ser_E[at] = E;
+                        a.ser_M[at] = M;
+                    }
                     const double e = (double)E, m = (double)M;
                     mE += e;
                     mE2 += e * e;
@@ -318,14 +321,14 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
         if (tid == 0) {
             if (a.record) {
                 double* mo = a.moments + (int64_t)ch * 6;
-                mo[0] += (double)a.nsweeps;
+                mo[0] += (double)nsw;
                 mo[1] += mE;
                 mo[2] += mE2;
                 mo[3] += mAM;
                 mo[4] += mM2;
                 mo[5] += mM4;
             }
-            a.chains[ch].sweep = cr.sweep + (uint64_t)a.nsweeps;
+            a.chains[ch].sweep = cr.sweep + (uint64_t)nsw;
         }
         __syncthreads();
     }
@@ -390,7 +393,7 @@ __global__ void __launch_bounds__(256) k_ising_plain(PlainArgs a, int n_col)
         const uint32_t group = (uint32_t)((r * 2 + c) * nw + (j >> 5));
         const int t = j & 31;
         const uint32_t sw_lo = (uint32_t)a.sweep;
-        const uint32_t k0 = (uint32_t)a.seed, c3 = (uint32_t)(a.seed >> 32);
+        const uint32_t k0 = (uint32_t)cr.seed, c3 = (uint32_t)(cr.seed >> 32);
         int8_t snew = si;
         if (ALG == 0) {
             const int n_anti = (2 * DIM - si * nb) >> 1;

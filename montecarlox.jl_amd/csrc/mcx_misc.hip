@@ -15,7 +15,7 @@ __device__ __forceinline__ void site_coords(int64_t i, int L, int dim, int& x, i
 // :random draws site word (P_INIT) of each site at the chain's sweep counter:
 // Ising +1 iff its top bit is set; Blume-Capel state floor(3w / 2^32) - 1.
 __global__ void k_init_spins(int model, int mode, ChainRec* chains, int32_t n_chains,
-                             int8_t* spins, uint64_t seed)
+                             int8_t* spins)
 {
     for (int ch = blockIdx.y; ch < n_chains; ch += gridDim.y) {
         const ChainRec cr = chains[ch];
@@ -39,7 +39,7 @@ __global__ void k_init_spins(int model, int mode, ChainRec* chains, int32_t n_ch
                 const int j = x >> 1;
                 const int nw = ((L >> 1) + 31) >> 5;
                 const uint32_t g = (uint32_t)((r * 2 + c) * nw + (j >> 5));
-                const uint32_t w = site_word(seed, cr.key, g, j & 31, cr.sweep, P_INIT);
+                const uint32_t w = site_word(cr.seed, cr.key, g, j & 31, cr.sweep, P_INIT);
                 if (model == 0)
                     v = (w >> 31) ? 1 : -1;
                 else
@@ -155,13 +155,13 @@ __global__ void k_zero_series(ChainRec* chains, int32_t n_chains, int32_t* E, in
 }
 
 hipError_t launch_init_spins(int model, int mode, ChainRec* chains, int32_t n_chains,
-                             int64_t max_sites, int8_t* spins, uint64_t seed, hipStream_t st)
+                             int64_t max_sites, int8_t* spins, hipStream_t st)
 {
     const int block = 256;
     const int64_t gx = (max_sites + block - 1) / block;
     dim3 grid((unsigned)(gx > 4096 ? 4096 : gx), (unsigned)(n_chains > 65535 ? 65535 : n_chains));
     hipLaunchKernelGGL(k_init_spins, grid, dim3(block), 0, st, model, mode, chains, n_chains,
-                       spins, seed);
+                       spins);
     return hipGetLastError();
 }
 

@@ -19,7 +19,6 @@ struct PTArgs {
     const int32_t* E;    // energies: E_pair + D * Q per chain ...
     const int32_t* Q;
     const double* x;     // ... or caller-supplied energies (non-null overrides)
-    uint64_t seed;
 };
 
 hipError_t launch_pt_exchange(const PTArgs& a, hipStream_t st);

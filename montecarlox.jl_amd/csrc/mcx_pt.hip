@@ -40,8 +40,9 @@ __global__ void k_pt_exchange(PTArgs a)
         const double lr = ((-bi * xj) - (-bi * xi)) + ((-bj * xi) - (-bj * xj));
         bool ok = lr > 0;
         if (!ok) {
+            const uint64_t seed = a.chains[ci].seed;
             const uint4 w = philox4x32_10((uint32_t)round, (uint32_t)(round >> 32), P_PT << 24,
-                                          (uint32_t)(a.seed >> 32), (uint32_t)a.seed,
+                                          (uint32_t)(seed >> 32), (uint32_t)seed,
                                           a.chains[ci].key);
             ok = (double)w.x * 0x1.0p-32 < exp(lr);
         }

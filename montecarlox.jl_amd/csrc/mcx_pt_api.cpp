@@ -48,7 +48,6 @@ PTArgs pt_args(mcx_pt* p)
     a.E = p->b->d_last;
     a.Q = p->b->d_last + 2 * p->b->n;
     a.x = nullptr;
-    a.seed = p->b->seed;
     return a;
 }
 

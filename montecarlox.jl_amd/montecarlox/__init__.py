@@ -19,6 +19,7 @@ from .ensembles import (BinnedObject, BoltzmannEnsemble, MulticanonicalEnsemble,
 from .muca import (GeneralizedWalkers, ParallelChains, ParallelMulticanonical, ThreadsBackend,
                    distribute_logweight_, merge_histograms_)
 from .pt import ParallelTempering, ReplicaExchange, acceptance_rates, index, set_betas
+from .serving import Job, Server, poisson_workload
 from .spinsystems import (BlumeCapel, Ising, IsingLatticeOptim, SpinSystemBatch, energy, init_,
                           magnetization, spin_flip_, sweep_)
 

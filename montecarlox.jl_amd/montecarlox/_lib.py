@@ -56,8 +56,12 @@ class BatchDesc(C.Structure):
 class GenDesc(C.Structure):
     _fields_ = [("kind", C.c_int32), ("n_groups", C.c_int32),
                 ("group_of_chain", C.POINTER(C.c_int32)), ("bin_start", C.POINTER(C.c_double)),
-                ("bin_step", C.POINTER(C.c_double)), ("n_bins", C.POINTER(C.c_int64)),
-                ("float_bins", C.c_int32)]
+                ("bin_step", C.POINTER(C.c_double)), ("n_bins", C.POINTER(C.c_int64))]
+
+
+class ServerDesc(C.Structure):
+    _fields_ = [("hbm_budget", C.c_int64), ("max_jobs", C.c_int32), ("min_sites", C.c_int32),
+                ("quantum_attempts", C.c_int64)]
 
 
 class JobDesc(C.Structure):
@@ -70,9 +74,16 @@ class JobResult(C.Structure):
     _fields_ = [("job_id", C.c_uint64), ("tag", C.c_uint64), ("status", C.c_int32),
                 ("L", C.c_int32), ("nsweeps", C.c_int64), ("accepted", C.c_int64),
                 ("E_pair", C.c_int32), ("M", C.c_int32), ("Q", C.c_int32), ("pad", C.c_int32),
-                ("mean_E", C.c_double), ("mean_E2", C.c_double), ("mean_absM", C.c_double),
-                ("mean_M2", C.c_double), ("mean_M4", C.c_double), ("latency_ms", C.c_double),
-                ("device_ms", C.c_double)]
+                ("moments", C.c_double * N_MOMENTS), ("latency_ms", C.c_double),
+                ("queue_ms", C.c_double), ("device_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("pending", C.c_int64),
+                ("active", C.c_int64), ("quanta", C.c_int64), ("attempts", C.c_double),
+                ("busy_ms", C.c_double), ("wall_ms", C.c_double), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("max_ms", C.c_double), ("attempts_per_ns", C.c_double),
+                ("busy_attempts_per_ns", C.c_double)]
 
 
 _lib = None
@@ -143,7 +154,7 @@ def call(name, *args):
     st = getattr(lib, name)(*args)
     if st != MCX_OK:
         msg = lib.mcx_last_error().decode(errors="replace")
-        raise _EXC.get(st, MCXError)(f"{name}: {msg}")
+        raise _EXC.get(st, MCXError)(msg if msg.startswith(name) else f"{name}: {msg}")
 
 
 def exported_symbols():

@@ -1,0 +1,133 @@
+"""The continuous-batching server: every job's result equals the CPU oracle's
+run of the same (model, L, beta, D, seed, key) from the same random start, is
+bit-identical alone and in a mix that forces many quanta and admissions under
+a tight HBM budget, and equals a one-chain LatticeBatch run."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MODEL = {"ising": 0, "blume_capel": 1}
+ALG = {"metropolis": 0, "heatbath": 1}
+
+
+def _jobs(mcx):
+    J = mcx.Job
+    return [
+        J("ising", 16, 0.4406868, 200, "metropolis", seed=5, key=0),
+        J("ising", 32, 0.3, 150, "heatbath", seed=5, key=1),
+        J("blume_capel", 20, 0.7, 120, "metropolis", D=1.2, seed=5, key=2),
+        J("blume_capel", 64, 0.9, 60, "heatbath", D=0.5, seed=5, key=3),
+        J("ising", 8, 0.22, 100, "metropolis", dim=3, seed=5, key=4),
+        J("ising", 6, 1.0, 300, "metropolis", seed=9, key=5),
+        J("blume_capel", 6, 0.5, 90, "heatbath", dim=3, D=0.3, seed=9, key=6),
+        J("ising", 128, 0.4406868, 30, "metropolis", seed=2**33 + 7, key=7),
+        J("ising", 66, 0.5, 40, "heatbath", seed=5, key=8),
+        J("blume_capel", 4, 2.0, 500, "metropolis", D=1.9, seed=5, key=9),
+    ]
+
+
+def _oracle(orc, job):
+    m, a = MODEL[job.model], ALG[job.alg]
+    s = orc.init_random(m, job.dim, job.L, job.seed, job.key, 0)
+    E, M, Q, acc = orc.canonical_sweeps(m, a, job.dim, job.L, job.beta, s, job.seed, job.key, 0,
+                                        job.nsweeps, D=job.D)
+    e = E.astype(np.float64) + (job.D * Q if m == 1 else 0.0)
+    mf = M.astype(np.float64)
+    mom = np.array([job.nsweeps, e.sum(), (e * e).sum(), np.abs(mf).sum(), (mf * mf).sum(),
+                    ((mf * mf) ** 2).sum()])
+    return int(E[-1]), int(M[-1]), int(Q[-1]), acc, mom
+
+
+def _same(r1, r2):
+    for f in ("E_pair", "M", "Q", "accepted", "nsweeps", "L"):
+        assert r1[f] == r2[f], f
+    assert np.array_equal(r1["moments"], r2["moments"])
+
+
+# small quanta and a budget of three 128^2 lattices: many quanta, admissions wait
+SERVER = dict(hbm_budget=3 * 128 * 128, max_jobs=4, quantum_attempts=20000, min_sites=256)
+
+
+def test_server_matches_oracle_alone_and_in_mix(mcx, orc):
+    jobs = _jobs(mcx)
+    print("serving: mix of", len(jobs), "jobs", flush=True)
+    with mcx.Server(**SERVER) as srv:
+        mix = srv.run(jobs, timeout_ms=120000)
+        st = srv.stats()
+    assert st["completed"] == len(jobs) and st["quanta"] > 20
+    for job, r in zip(jobs, mix):
+        assert r["tag"] == job.tag and r["status"] == 0
+        E, M, Q, acc, mom = _oracle(orc, job)
+        assert (r["E_pair"], r["M"], r["Q"], r["accepted"]) == (E, M, Q, acc), job
+        assert np.allclose(r["moments"], mom, rtol=1e-12, atol=0), job
+    print("serving: each job alone", flush=True)
+    with mcx.Server(**SERVER) as srv:
+        for job, r in zip(jobs, mix):
+            alone = srv.run([job], timeout_ms=60000)[0]
+            _same(alone, r)
+
+
+def test_server_quantum_size_does_not_change_the_chain(mcx, orc):
+    jobs = _jobs(mcx)[:6]
+    print("serving: quantum sizes", flush=True)
+    with mcx.Server(hbm_budget=1 << 20, max_jobs=16) as srv:
+        a = srv.run(jobs, timeout_ms=60000)
+    with mcx.Server(hbm_budget=1 << 20, max_jobs=2, quantum_attempts=3000, min_sites=64) as srv:
+        b = srv.run(jobs, timeout_ms=60000)
+    for r1, r2 in zip(a, b):
+        for f in ("E_pair", "M", "Q", "accepted"):
+            assert r1[f] == r2[f]
+        # integer-valued sums below 2^53: exact whatever the split
+        assert np.allclose(r1["moments"], r2["moments"], rtol=1e-12, atol=0)
+
+
+def test_server_matches_lattice_batch(mcx):
+    print("serving: vs LatticeBatch", flush=True)
+    job = mcx.Job("blume_capel", 24, 0.8, 77, "heatbath", D=0.9, seed=123, key=31)
+    with mcx.Server(hbm_budget=1 << 16, max_jobs=2, quantum_attempts=5000) as srv:
+        r = srv.run([job])[0]
+    b = mcx.LatticeBatch("blume_capel", 2, [24], seed=123, keys=[31], D=[0.9], series_capacity=77)
+    b.set_betas([0.8])
+    b.init("random")
+    b.run("heatbath", 77, record=True)
+    E, M, Q = b.observables()
+    _, acc = b.counters()
+    assert (r["E_pair"], r["M"], r["Q"], r["accepted"]) == (E[0], M[0], Q[0], acc[0])
+    assert np.allclose(r["moments"], b.moments()[0], rtol=1e-12, atol=0)
+    b.close()
+
+
+def test_server_poisson_stream_small(mcx, orc):
+    """A scaled-down serving stream (sweeps / 100): every result checks against
+    the oracle's last-sweep observables and acceptance count."""
+    jobs = mcx.poisson_workload(24, sweep_scale=0.01, L_range=(16, 96))
+    print("serving: poisson stream of", len(jobs), flush=True)
+    with mcx.Server(hbm_budget=64 * 96 * 96, max_jobs=64) as srv:
+        res = srv.run(jobs, timeout_ms=120000)
+        st = srv.stats()
+    assert st["completed"] == len(jobs)
+    assert 0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
+    for job, r in zip(jobs, res):
+        E, M, Q, acc, _ = _oracle(orc, job)
+        assert (r["E_pair"], r["M"], r["Q"], r["accepted"]) == (E, M, Q, acc), job
+
+
+def test_server_rejects_bad_jobs(mcx):
+    with mcx.Server(hbm_budget=4096, max_jobs=2) as srv:
+        with pytest.raises(mcx.MCXArgumentError):
+            srv.submit(mcx.Job("ising", 17, 0.4, 10))      # odd L
+        with pytest.raises(mcx.MCXArgumentError):
+            srv.submit(mcx.Job("ising", 66, 0.4, 10))      # over the budget
+        with pytest.raises(mcx.MCXArgumentError):
+            srv.submit(mcx.Job("ising", 16, 0.4, 0))       # no sweeps
+        with pytest.raises(mcx.MCXArgumentError):
+            srv.submit(mcx.Job("potts", 16, 0.4, 10))
+        assert srv.stats()["submitted"] == 0
+
+
+def test_server_destroy_with_work_in_flight(mcx):
+    srv = mcx.Server(hbm_budget=1 << 16, max_jobs=2, quantum_attempts=4096)
+    for k in range(6):
+        srv.submit(mcx.Job("ising", 32, 0.44, 400, seed=1, key=k))
+    srv.close()  # stops after the current quantum; must not hang

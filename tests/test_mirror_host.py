@@ -118,3 +118,36 @@ def test_binning_error_on_ar1(mcx):
         x[t] = phi * x[t - 1] + eps[t]
     expect = math.sqrt(2 * 4.5 / n)
     assert abs(binning_error(x, 40) / expect - 1) < 0.35
+
+
+def test_poisson_workload_matches_the_stream_definition(mcx):
+    """The serving stream: deterministic from its seed, even L in [16, 512],
+    about 70 % Ising, mean offered work near 8.1e8 attempts per job and mean
+    inter-arrival 1/200 s."""
+    a = mcx.poisson_workload(2000)
+    b = mcx.poisson_workload(2000)
+    assert [(j.L, j.nsweeps, j.beta, j.D, j.model) for j in a] == \
+           [(j.L, j.nsweeps, j.beta, j.D, j.model) for j in b]
+    Ls = np.array([j.L for j in a])
+    assert Ls.min() >= 16 and Ls.max() <= 512 and np.all(Ls % 2 == 0)
+    ising = np.array([j.model == "ising" for j in a])
+    assert abs(ising.mean() - 0.7) < 0.04
+    assert all(j.D == 0.0 for j in a if j.model == "ising")
+    assert all(0.0 <= j.D <= 1.9 for j in a if j.model != "ising")
+    assert all(1.5 <= 1.0 / j.beta <= 3.5 for j in a)
+    sw = np.array([j.nsweeps for j in a])
+    assert sw.min() >= 1000 and sw.max() <= 100000
+    work = np.mean([j.attempts for j in a])
+    assert 6.5e8 < work < 9.8e8
+    t = np.array([j.arrival_s for j in a])
+    assert np.all(np.diff(t) > 0) and abs(t[-1] / len(a) - 1 / 200) < 0.0005
+    assert [j.key for j in a] == list(range(2000))
+
+
+def test_job_descriptor_validation_on_host(mcx):
+    with pytest.raises(mcx.MCXArgumentError):
+        mcx.Job("potts", 16, 0.4, 10).desc()
+    with pytest.raises(mcx.MCXArgumentError):
+        mcx.Job("ising", 16, 0.4, 10, alg="glauber").desc()
+    d = mcx.Job("blume_capel", 16, 0.4, 10, alg="heatbath", D=1.5, seed=2**40 + 3, key=7).desc()
+    assert (d.model, d.alg, d.seed, d.key, d.D) == (1, 1, 2**40 + 3, 7, 1.5)
