@@ -303,17 +303,25 @@ mcx_status mcx_muca_update_values(mcx_ctx* ctx, double* lw, const double* h, int
  * A server owns a device pool of hbm_budget bytes for int8 lattices and
  * max_jobs job slots.  A scheduler thread admits queued jobs as pool space and
  * slots free up (arrival order; a job that does not fit yet does not block
- * smaller ones behind it) and advances all resident jobs in quanta, each job
- * running min(remaining, max(1, quantum_attempts / max(N, min_sites))) sweeps
- * per quantum.  A job's result depends only on its descriptor: it equals a
- * one-chain mcx_batch with the same model, dim, L, beta, D, seed, chain key,
+ * smaller ones behind it) and advances all resident jobs in quanta of about
+ * quantum_ms: one launch per kernel class over n_streams streams, each job
+ * running as many sweeps as its measured per-sweep time fits in its share
+ * (big lattices: up to 256 sweeps of per-colour launches over many workgroups).
+ * A job's result depends only on its descriptor: it equals a one-chain
+ * mcx_batch with the same model, dim, L, beta, D, seed, chain key,
  * MCX_INIT_RANDOM start and nsweeps sweeps of mcx_batch_run(alg, RECORD), for
- * spins, counters, last-sweep observables and moments alike. */
+ * spins, counters, last-sweep observables and moments alike, bit for bit. */
 typedef struct {
-    int64_t hbm_budget;       /* bytes for resident lattices (N bytes each, 256-aligned) */
-    int32_t max_jobs;         /* resident job slots                                       */
-    int32_t min_sites;        /* cost floor per sweep in the quantum rule (0 = 4096)      */
-    int64_t quantum_attempts; /* target attempts per job per quantum (0 = 2^24)           */
+    int64_t hbm_budget; /* bytes for resident jobs (see below), 256-aligned          */
+    int32_t max_jobs;   /* resident job slots                                        */
+    int32_t n_streams;  /* streams the kernel classes are spread over (0 = 4, <= 8)  */
+    double quantum_ms;  /* target device time of one quantum (0 = 2 ms)             */
+    int64_t tile_sites; /* lattices with at least this many sites run spread over    */
+                        /* many workgroups (bit planes in the pool, one launch per    */
+                        /* colour and sweep); 0 = 2^17 Ising / 2^14 Blume-Capel,      */
+                        /* < 0 = never.  Other lattices run one workgroup each, from  */
+                        /* LDS.  A job takes N bytes, plus N/4 (Ising) or N/2         */
+                        /* (Blume-Capel) bytes of planes on the tiled path.           */
 } mcx_server_desc;
 
 typedef struct {

@@ -205,6 +205,117 @@ __device__ __forceinline__ void bc_shifted_sum(const BCNbr& n, uint32_t& v0, uin
     v3 = a2 ^ c2;
 }
 
+// int8 lattice -> bit planes [occ c0, occ c1, up c0, up c1] for plane word w.
+template <int DIM>
+__device__ __forceinline__ void bc_pack_word(const int8_t* __restrict__ sp,
+                                             uint32_t* __restrict__ P, int w, int L, int nw,
+                                             int half, int words)
+{
+    const int r = w / nw;
+    const int q = w - r * nw;
+    const int y = DIM == 3 ? r % L : r;
+    const int z = DIM == 3 ? r / L : 0;
+    const int par = (y + z) & 1;
+    const int W = word_width(q, half);
+    const int8_t* row = sp + (int64_t)r * L + 64 * q;
+    uint32_t eo = 0, eu = 0, oo = 0, ou = 0;
+    for (int j = 0; j < W; ++j) {
+        const int se = row[2 * j], so = row[2 * j + 1];
+        eo |= (uint32_t)(se != 0) << j;
+        eu |= (uint32_t)(se > 0) << j;
+        oo |= (uint32_t)(so != 0) << j;
+        ou |= (uint32_t)(so > 0) << j;
+    }
+    P[par * words + w] = eo;
+    P[(2 + par) * words + w] = eu;
+    P[(par ^ 1) * words + w] = oo;
+    P[(2 + (par ^ 1)) * words + w] = ou;
+}
+
+// Bit planes -> int8 lattice for plane word w.
+template <int DIM>
+__device__ __forceinline__ void bc_unpack_word(int8_t* __restrict__ sp,
+                                               const uint32_t* __restrict__ P, int w, int L,
+                                               int nw, int half, int words)
+{
+    const int r = w / nw;
+    const int q = w - r * nw;
+    const int y = DIM == 3 ? r % L : r;
+    const int z = DIM == 3 ? r / L : 0;
+    const int par = (y + z) & 1;
+    const int W = word_width(q, half);
+    int8_t* row = sp + (int64_t)r * L + 64 * q;
+    const uint32_t eo = P[par * words + w], eu = P[(2 + par) * words + w];
+    const uint32_t oo = P[(par ^ 1) * words + w], ou = P[(2 + (par ^ 1)) * words + w];
+    for (int j = 0; j < W; ++j) {
+        row[2 * j] = ((eo >> j) & 1u) ? (((eu >> j) & 1u) ? 1 : -1) : 0;
+        row[2 * j + 1] = ((oo >> j) & 1u) ? (((ou >> j) & 1u) ? 1 : -1) : 0;
+    }
+}
+
+// One 32-site word of colour c: updates its occupancy / up bits in place from
+// the bit-sliced shifted neighbour sums v0..v3 (the per-site rule above, one
+// Philox block per 4 sites) and returns the number of changed sites this lane
+// decided.  The sites of a word are independent (one colour), so a word may
+// be shared by lpw adjacent lanes (lpw = 1, 2, 4 or 8, lane part = lane % lpw):
+// each takes the Philox blocks part, part + lpw, ... and the lanes then OR
+// their bit changes together, so every lane of the group ends with the whole
+// word.  tabw: 80 Metropolis thresholds + 3 always-masks, tabd: 13 x (w1,
+// w1 + w2, z).  Shared by the LDS-resident batched kernel and the
+// multi-workgroup tile kernel.
+template <int DIM, int ALG>
+__device__ __forceinline__ uint32_t bc_word(uint32_t& occ, uint32_t& up, uint32_t v0, uint32_t v1,
+                                            uint32_t v2, uint32_t v3, int W,
+                                            const uint32_t* __restrict__ tabw,
+                                            const double* __restrict__ tabd, uint32_t group,
+                                            uint32_t sw_lo, uint32_t w2, uint32_t c3, uint32_t k0,
+                                            uint32_t k1, int part, int lpw)
+{
+    constexpr int NS = 4 * DIM + 1;
+    const uint32_t occ0 = occ, up0 = up;
+    uint32_t changed = 0;
+    for (int j0 = 4 * part; j0 < W; j0 += 4 * lpw) {
+        const uint4 R = philox4x32_10(8u * group + (uint32_t)(j0 >> 2), sw_lo, w2, c3, k0, k1);
+        const uint32_t Rb[4] = {R.x, R.y, R.z, R.w};
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int j = j0 + b;
+            if (j >= W)
+                break;
+            const uint32_t rw = Rb[b];
+            const int o = (occ >> j) & 1, u = (up >> j) & 1;
+            const int s_old = o ? (u ? 1 : -1) : 0;
+            const int sidx = (int)(((v0 >> j) & 1u) | (((v1 >> j) & 1u) << 1)
+                                   | (((v2 >> j) & 1u) << 2) | (((v3 >> j) & 1u) << 3));
+            int s_new;
+            if (ALG == 0) {
+                const int alt = (int)(rw & 1u);
+                const int k = ((s_old + 1) * 2 + alt) * NS + sidx;
+                const bool ok = ((tabw[80 + (k >> 5)] >> (k & 31)) & 1u) || (rw >> 1) < tabw[k];
+                s_new = ok ? bc_propose(s_old, alt) : s_old;
+            } else {
+                const double rr = ((double)rw * 0x1.0p-32) * tabd[sidx * 3 + 2];
+                s_new = rr < tabd[sidx * 3] ? -1 : (rr < tabd[sidx * 3 + 1] ? 0 : 1);
+            }
+            if (s_new != s_old) {
+                changed |= 1u << j;
+                occ = (occ & ~(1u << j)) | ((uint32_t)(s_new != 0) << j);
+                up = (up & ~(1u << j)) | ((uint32_t)(s_new > 0) << j);
+            }
+        }
+    }
+    if (lpw > 1) {
+        uint32_t d_occ = occ ^ occ0, d_up = up ^ up0;
+        for (int off = 1; off < lpw; off <<= 1) {
+            d_occ |= __shfl_xor(d_occ, off, 64);
+            d_up |= __shfl_xor(d_up, off, 64);
+        }
+        occ = occ0 ^ d_occ;
+        up = up0 ^ d_up;
+    }
+    return (uint32_t)__popc(changed);
+}
+
 template <int DIM, int ALG>
 __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
 {
@@ -213,17 +324,19 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
     uint32_t* tabw = lds + 96;                       // 80 thresholds + 3 masks
     double* tabd = reinterpret_cast<double*>(lds + 184); // 39 doubles (8-byte aligned)
     uint32_t* P = lds + 264;                          // [occ c0, occ c1, up c0, up c1]
-    constexpr int NS = 4 * DIM + 1;
     const int tid = threadIdx.x;
     const int B = blockDim.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int nwaves = (B + 63) >> 6;
     constexpr uint32_t purpose = ALG == 0 ? P_BC_METRO : P_BC_HB;
+    const int lpw = a.lanes_per_word > 1 ? a.lanes_per_word : 1;
+    const int lpart = tid & (lpw - 1);
 
     for (int item = blockIdx.x; item < a.n_items; item += gridDim.x) {
         const int ch = a.items[item];
         const ChainRec cr = a.chains[ch];
+        const long long t_item = a.item_ticks && tid == 0 ? wall_clock64() : 0;
         const uint32_t k0 = (uint32_t)cr.seed;
         const uint32_t c3 = (uint32_t)(cr.seed >> 32);
         const int L = cr.L;
@@ -238,31 +351,22 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
         for (int k = tid; k < 39; k += B)
             tabd[k] = tb->hb[k / 3][k % 3];
 
-        for (int w = tid; w < words; w += B) {
-            const int r = w / nw;
-            const int q = w - r * nw;
-            const int y = DIM == 3 ? r % L : r;
-            const int z = DIM == 3 ? r / L : 0;
-            const int par = (y + z) & 1;
-            const int W = word_width(q, half);
-            const int8_t* row = sp + (int64_t)r * L + 64 * q;
-            uint32_t eo = 0, eu = 0, oo = 0, ou = 0;
-            for (int j = 0; j < W; ++j) {
-                const int se = row[2 * j], so = row[2 * j + 1];
-                eo |= (uint32_t)(se != 0) << j;
-                eu |= (uint32_t)(se > 0) << j;
-                oo |= (uint32_t)(so != 0) << j;
-                ou |= (uint32_t)(so > 0) << j;
-            }
-            P[par * words + w] = eo;
-            P[(2 + par) * words + w] = eu;
-            P[(par ^ 1) * words + w] = oo;
-            P[(2 + (par ^ 1)) * words + w] = ou;
-        }
+        for (int w = tid; w < words; w += B)
+            bc_pack_word<DIM>(sp, P, w, L, nw, half, words);
         __syncthreads();
 
         unsigned long long acc = 0;
+        // running sums continue from the stored ones, sweep by sweep, so the
+        // FP64 moments do not depend on how a run is split into launches
         double mE = 0, mE2 = 0, mAM = 0, mM2 = 0, mM4 = 0;
+        if (a.record && tid == 0) {
+            const double* mo = a.moments + (int64_t)ch * 6;
+            mE = mo[1];
+            mE2 = mo[2];
+            mAM = mo[3];
+            mM2 = mo[4];
+            mM4 = mo[5];
+        }
         const int64_t nsw = a.item_nsweeps ? a.item_nsweeps[item] : a.nsweeps;
         for (int64_t s = 0; s < nsw; ++s) {
             const uint64_t sweep = cr.sweep + (uint64_t)s;
@@ -275,7 +379,7 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
                 uint32_t* __restrict__ Sc = P + (2 + c) * words;
                 const uint32_t* __restrict__ Oo = P + (c ^ 1) * words;
                 const uint32_t* __restrict__ So = P + (2 + (c ^ 1)) * words;
-                for (int w = tid; w < words; w += B) {
+                for (int w = tid / lpw; w < words; w += B / lpw) {
                     const int r = nw == 1 ? w : w / nw;
                     const int q = w - r * nw;
                     const int y = DIM == 3 ? r % L : r;
@@ -289,38 +393,13 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
                     uint32_t v0, v1, v2, v3;
                     bc_shifted_sum<DIM>(n, v0, v1, v2, v3);
                     const uint32_t group = (uint32_t)((r * 2 + c) * nw + q);
-                    uint4 R = make_uint4(0, 0, 0, 0);
-                    uint32_t changed = 0;
-                    for (int j = 0; j < W; ++j) {
-                        if ((j & 3) == 0)
-                            R = philox4x32_10(8u * group + (uint32_t)(j >> 2), sw_lo, w2, c3, k0,
-                                              cr.key);
-                        const uint32_t rw = (j & 3) == 0 ? R.x : ((j & 3) == 1 ? R.y : ((j & 3) == 2 ? R.z : R.w));
-                        const int o = (occ >> j) & 1, u = (up >> j) & 1;
-                        const int s_old = o ? (u ? 1 : -1) : 0;
-                        const int sidx = (int)(((v0 >> j) & 1u) | (((v1 >> j) & 1u) << 1)
-                                               | (((v2 >> j) & 1u) << 2) | (((v3 >> j) & 1u) << 3));
-                        int s_new;
-                        if (ALG == 0) {
-                            const int alt = (int)(rw & 1u);
-                            const int k = ((s_old + 1) * 2 + alt) * NS + sidx;
-                            const bool ok = ((tabw[80 + (k >> 5)] >> (k & 31)) & 1u)
-                                            || (rw >> 1) < tabw[k];
-                            s_new = ok ? bc_propose(s_old, alt) : s_old;
-                        } else {
-                            const double rr = ((double)rw * 0x1.0p-32) * tabd[sidx * 3 + 2];
-                            s_new = rr < tabd[sidx * 3] ? -1 : (rr < tabd[sidx * 3 + 1] ? 0 : 1);
-                        }
-                        if (s_new != s_old) {
-                            changed |= 1u << j;
-                            occ = (occ & ~(1u << j)) | ((uint32_t)(s_new != 0) << j);
-                            up = (up & ~(1u << j)) | ((uint32_t)(s_new > 0) << j);
-                        }
+                    acc += bc_word<DIM, ALG>(occ, up, v0, v1, v2, v3, W, tabw, tabd, group,
+                                             sw_lo, w2, c3, k0, cr.key, lpart, lpw);
+                    if (lpart == 0) {
+                        Oc[w] = occ;
+                        Sc[w] = up;
                     }
-                    acc += (unsigned long long)__popc(changed);
-                    Oc[w] = occ;
-                    Sc[w] = up;
-                    if (c == 1 && obs) {
+                    if (c == 1 && obs && lpart == 0) {
                         const uint32_t dn = occ & ~up;
                         const uint32_t upo = occ & up;
 #pragma unroll
@@ -376,21 +455,8 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
             }
         }
 
-        for (int w = tid; w < words; w += B) {
-            const int r = w / nw;
-            const int q = w - r * nw;
-            const int y = DIM == 3 ? r % L : r;
-            const int z = DIM == 3 ? r / L : 0;
-            const int par = (y + z) & 1;
-            const int W = word_width(q, half);
-            int8_t* row = sp + (int64_t)r * L + 64 * q;
-            const uint32_t eo = P[par * words + w], eu = P[(2 + par) * words + w];
-            const uint32_t oo = P[(par ^ 1) * words + w], ou = P[(2 + (par ^ 1)) * words + w];
-            for (int j = 0; j < W; ++j) {
-                row[2 * j] = ((eo >> j) & 1u) ? (((eu >> j) & 1u) ? 1 : -1) : 0;
-                row[2 * j + 1] = ((oo >> j) & 1u) ? (((ou >> j) & 1u) ? 1 : -1) : 0;
-            }
-        }
+        for (int w = tid; w < words; w += B)
+            bc_unpack_word<DIM>(sp, P, w, L, nw, half, words);
         acc = wave_sum(acc);
         if (lane == 0 && acc)
             atomicAdd(&a.accepted[ch], acc);
@@ -398,15 +464,17 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
             if (a.record) {
                 double* mo = a.moments + (int64_t)ch * 6;
                 mo[0] += (double)nsw;
-                mo[1] += mE;
-                mo[2] += mE2;
-                mo[3] += mAM;
-                mo[4] += mM2;
-                mo[5] += mM4;
+                mo[1] = mE;
+                mo[2] = mE2;
+                mo[3] = mAM;
+                mo[4] = mM2;
+                mo[5] = mM4;
             }
             a.chains[ch].sweep = cr.sweep + (uint64_t)nsw;
         }
         __syncthreads();
+        if (a.item_ticks && tid == 0)
+            a.item_ticks[item] = wall_clock64() - t_item;
     }
 }
 
@@ -485,6 +553,130 @@ __global__ void __launch_bounds__(256) k_bc_plain(PlainArgs a, int n_col)
             atomicAdd(&a.ser_Q[cr.series_off + a.rec_index], q_part);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Tile kernel: one colour of one sweep of every work item, one thread per plane
+// word, the planes in device memory.  Grid (word blocks, items).  Word update
+// and observables as in k_bc_sweeps.
+// ---------------------------------------------------------------------------
+
+template <int DIM, int ALG>
+__global__ void __launch_bounds__(256) k_bc_tiles(TileArgs a)
+{
+    __shared__ uint32_t tabw[84];
+    __shared__ double tabd[39];
+    const int item = blockIdx.y;
+    if (a.s >= a.item_nsw[item])
+        return;
+    constexpr uint32_t purpose = ALG == 0 ? P_BC_METRO : P_BC_HB;
+    const int tid = threadIdx.x;
+    const int ch = a.items[item];
+    const ChainRec cr = a.chains[ch];
+    const BCTable* tb = a.btab + cr.table;
+    for (int k = tid; k < 83; k += blockDim.x)
+        tabw[k] = k < 80 ? tb->metro_thr[k] : tb->metro_always[k - 80];
+    for (int k = tid; k < 39; k += blockDim.x)
+        tabd[k] = tb->hb[k / 3][k % 3];
+    __syncthreads();
+    const int L = cr.L;
+    const int half = L >> 1;
+    const int nw = (half + 31) >> 5;
+    const int rows = DIM == 3 ? L * L : L;
+    const int words = rows * nw;
+    const int c = a.colour;
+    const int lpw = a.lpw > 1 ? a.lpw : 1;
+    const int part = tid & (lpw - 1);
+    const int w = (blockIdx.x * blockDim.x + tid) / lpw;
+    uint32_t flips = 0;
+    int32_t pr = 0, mm = 0, qq = 0;
+    if (w < words) {
+        uint32_t* P = reinterpret_cast<uint32_t*>(a.base + a.item_plane[item]);
+        uint32_t* Oc = P + c * words;
+        uint32_t* Sc = P + (2 + c) * words;
+        const uint32_t* Oo = P + (c ^ 1) * words;
+        const uint32_t* So = P + (2 + (c ^ 1)) * words;
+        const uint64_t sweep = cr.sweep + (uint64_t)a.s;
+        const int r = nw == 1 ? w : w / nw;
+        const int q = w - r * nw;
+        const int y = DIM == 3 ? r % L : r;
+        const int z = DIM == 3 ? r / L : 0;
+        const int p = (c + y + z) & 1;
+        const int W = word_width(q, half);
+        const uint32_t valid = width_mask(W);
+        uint32_t occ = Oc[w], up = Sc[w];
+        BCNbr n;
+        bc_load_nbrs<DIM>(Oo, So, L, nw, half, r, q, y, z, p, W, valid, n);
+        uint32_t v0, v1, v2, v3;
+        bc_shifted_sum<DIM>(n, v0, v1, v2, v3);
+        const uint32_t group = (uint32_t)((r * 2 + c) * nw + q);
+        flips = bc_word<DIM, ALG>(occ, up, v0, v1, v2, v3, W, tabw, tabd, group, (uint32_t)sweep,
+                                  ctr_word2(sweep, 0, purpose), (uint32_t)(cr.seed >> 32),
+                                  (uint32_t)cr.seed, cr.key, part, lpw);
+        if (part == 0) {
+            Oc[w] = occ;
+            Sc[w] = up;
+        }
+        if (c == 1 && part == 0) {
+            const uint32_t dn = occ & ~up;
+            const uint32_t upo = occ & up;
+#pragma unroll
+            for (int k = 0; k < 2 * DIM; ++k) {
+                const uint32_t both = occ & n.occ[k];
+                const uint32_t anti = both & (up ^ n.up[k]);
+                pr += __popc(both) - 2 * __popc(anti);
+            }
+            const uint32_t Ao = n.occ[2], Au = n.up[2];
+            mm += __popc(upo) - __popc(dn) + __popc(Ao & Au) - __popc(Ao & ~Au);
+            qq += __popc(occ) + __popc(Ao);
+        }
+    }
+    tile_block_add(a, item, pr, mm, qq, (int32_t)flips);
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(256) k_bc_pack(TileArgs a, int unpack)
+{
+    const int item = blockIdx.y;
+    const ChainRec cr = a.chains[a.items[item]];
+    const int L = cr.L;
+    const int half = L >> 1;
+    const int nw = (half + 31) >> 5;
+    const int words = (DIM == 3 ? L * L : L) * nw;
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= words)
+        return;
+    uint32_t* P = reinterpret_cast<uint32_t*>(a.base + a.item_plane[item]);
+    int8_t* sp = reinterpret_cast<int8_t*>(a.base) + cr.spin_off;
+    if (unpack)
+        bc_unpack_word<DIM>(sp, P, w, L, nw, half, words);
+    else
+        bc_pack_word<DIM>(sp, P, w, L, nw, half, words);
+}
+
+hipError_t launch_bc_tiles(int dim, int alg, const TileArgs& a, int max_words, hipStream_t st)
+{
+    const int lpw = a.lpw > 1 ? a.lpw : 1;
+    const dim3 grid(((int64_t)max_words * lpw + 255) / 256, a.n_items);
+    if (dim == 2 && alg == 0)
+        hipLaunchKernelGGL((k_bc_tiles<2, 0>), grid, dim3(256), 0, st, a);
+    else if (dim == 2)
+        hipLaunchKernelGGL((k_bc_tiles<2, 1>), grid, dim3(256), 0, st, a);
+    else if (alg == 0)
+        hipLaunchKernelGGL((k_bc_tiles<3, 0>), grid, dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_bc_tiles<3, 1>), grid, dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bc_pack(int dim, int unpack, const TileArgs& a, int max_words, hipStream_t st)
+{
+    const dim3 grid((max_words + 255) / 256, a.n_items);
+    if (dim == 2)
+        hipLaunchKernelGGL((k_bc_pack<2>), grid, dim3(256), 0, st, a, unpack);
+    else
+        hipLaunchKernelGGL((k_bc_pack<3>), grid, dim3(256), 0, st, a, unpack);
+    return hipGetLastError();
 }
 
 hipError_t launch_bc_sweeps(int dim, int alg, const SweepArgs& a, int block, int grid,

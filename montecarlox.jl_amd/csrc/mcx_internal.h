@@ -74,6 +74,8 @@ struct SweepArgs {
     int32_t* last_M;
     int32_t* last_Q;
     const int64_t* item_nsweeps; // if set: sweeps of item k (overrides nsweeps)
+    int64_t* item_ticks;         // if set: wall-clock ticks item k took
+    int32_t lanes_per_word;      // Blume-Capel: lanes sharing one plane word (1, 2, 4, 8; 0 = 1)
 };
 
 struct PlainArgs {
@@ -91,6 +93,32 @@ struct PlainArgs {
     int32_t colour;
 };
 
+constexpr int kTileObsReplicas = 16;
+
+// Multi-workgroup path for big lattices: the lattice lives in bit planes in
+// device memory (layout of the LDS planes) and one launch updates one colour
+// of one sweep for every work item, one thread per 32-site plane word.
+struct TileArgs {
+    const int32_t* items;      // chain of work item k
+    const int64_t* item_nsw;   // sweeps of item k in this series of launches
+    const int64_t* item_plane; // byte offset of item k's planes in `base`
+    int32_t n_items;
+    int32_t colour;
+    int64_t s;                 // sweep index within the series (items with nsw <= s skip)
+    ChainRec* chains;
+    const IsingTable* itab;
+    const BCTable* btab;
+    uint8_t* base;
+    const int64_t* item_obs;   // byte offset of item k's per-sweep sums in `base`:
+                               // int32 [max_s][4 fields][kTileObsReplicas], fields
+                               // Ising (up, anti-aligned bonds, -, flips),
+                               // Blume-Capel (pair sum, M, Q, flips); each block adds
+                               // into replica blockIdx.x % kTileObsReplicas
+    int32_t max_s;
+    int32_t lpw;               // Blume-Capel: lanes sharing one plane word (1, 2, 4, 8; 0 = 1)
+    unsigned long long* accepted;
+};
+
 // Launchers (defined in the .hip translation units).
 hipError_t launch_ising_sweeps(int dim, int alg, const SweepArgs& a, int block, int grid,
                                size_t lds_bytes, hipStream_t st);
@@ -100,6 +128,16 @@ hipError_t launch_bc_sweeps(int dim, int alg, const SweepArgs& a, int block, int
                             size_t lds_bytes, hipStream_t st);
 hipError_t launch_bc_plain(int dim, int alg, const PlainArgs& a, int n_sites_colour,
                            hipStream_t st);
+hipError_t launch_ising_tiles(int dim, int alg, const TileArgs& a, int max_words, hipStream_t st);
+hipError_t launch_bc_tiles(int dim, int alg, const TileArgs& a, int max_words, hipStream_t st);
+// int8 lattice <-> planes of every work item (unpack = 1: planes -> int8)
+hipError_t launch_ising_pack(int dim, int unpack, const TileArgs& a, int max_words, hipStream_t st);
+hipError_t launch_bc_pack(int dim, int unpack, const TileArgs& a, int max_words, hipStream_t st);
+// Folds the per-sweep sums of a series into each item's FP64 moments (sweep by
+// sweep, as the batched kernels do), last-sweep E/M/Q and sweep counter, and
+// clears the sums.
+hipError_t launch_tiles_fold(int model, const TileArgs& a, double* moments, int32_t* last_E,
+                             int32_t* last_M, int32_t* last_Q, hipStream_t st);
 hipError_t launch_init_spins(int model, int mode, ChainRec* chains, int32_t n_chains,
                              int64_t max_sites, int8_t* spins, hipStream_t st);
 hipError_t launch_observables(ChainRec* chains, int32_t n_chains, int64_t max_sites,

@@ -97,6 +97,134 @@ __device__ __forceinline__ uint32_t bitserial_less(uint32_t eq, const uint32_t (
     return lt;
 }
 
+// int8 lattice row word -> bit planes (up = 1) for plane word w.
+template <int DIM>
+__device__ __forceinline__ void ising_pack_word(const int8_t* __restrict__ sp,
+                                                uint32_t* __restrict__ P, int w, int L, int nw,
+                                                int half, int words)
+{
+    const int r = w / nw;
+    const int q = w - r * nw;
+    const int y = DIM == 3 ? r % L : r;
+    const int z = DIM == 3 ? r / L : 0;
+    const int par = (y + z) & 1; // colour of the even-x sites of this row
+    const int W = word_width(q, half);
+    const int8_t* row = sp + (int64_t)r * L + 64 * q;
+    uint32_t ev = 0, od = 0;
+    if ((L & 3) == 0) {
+        const uint32_t* row4 = reinterpret_cast<const uint32_t*>(row);
+        for (int k = 0; k < (W >> 1); ++k) {
+            const uint32_t v = ~row4[k];
+            ev |= (((v >> 7) & 1u) << (2 * k)) | (((v >> 23) & 1u) << (2 * k + 1));
+            od |= (((v >> 15) & 1u) << (2 * k)) | (((v >> 31) & 1u) << (2 * k + 1));
+        }
+    } else {
+        for (int j = 0; j < W; ++j) {
+            ev |= (uint32_t)(row[2 * j] > 0) << j;
+            od |= (uint32_t)(row[2 * j + 1] > 0) << j;
+        }
+    }
+    P[par * words + w] = ev;
+    P[(par ^ 1) * words + w] = od;
+}
+
+// Bit planes -> int8 lattice for plane word w.
+template <int DIM>
+__device__ __forceinline__ void ising_unpack_word(int8_t* __restrict__ sp,
+                                                  const uint32_t* __restrict__ P, int w, int L,
+                                                  int nw, int half, int words)
+{
+    const int r = w / nw;
+    const int q = w - r * nw;
+    const int y = DIM == 3 ? r % L : r;
+    const int z = DIM == 3 ? r / L : 0;
+    const int par = (y + z) & 1;
+    const int W = word_width(q, half);
+    int8_t* row = sp + (int64_t)r * L + 64 * q;
+    const uint32_t ev = P[par * words + w];
+    const uint32_t od = P[(par ^ 1) * words + w];
+    if ((L & 3) == 0) {
+        uint32_t* row4 = reinterpret_cast<uint32_t*>(row);
+        for (int k = 0; k < (W >> 1); ++k) {
+            const uint32_t e0 = (ev >> (2 * k)) & 1u, e1 = (ev >> (2 * k + 1)) & 1u;
+            const uint32_t o0 = (od >> (2 * k)) & 1u, o1 = (od >> (2 * k + 1)) & 1u;
+            row4[k] = (0xFFu - 0xFEu * e0) | ((0xFFu - 0xFEu * o0) << 8)
+                    | ((0xFFu - 0xFEu * e1) << 16) | ((0xFFu - 0xFEu * o1) << 24);
+        }
+    } else {
+        for (int j = 0; j < W; ++j) {
+            row[2 * j] = ((ev >> j) & 1u) ? 1 : -1;
+            row[2 * j + 1] = ((od >> j) & 1u) ? 1 : -1;
+        }
+    }
+}
+
+// One 32-site word of colour c: the new spins of the word from its old spins
+// s0 and its neighbour words (the per-site rule above, bit-sliced).  Shared by
+// the LDS-resident batched kernel and the multi-workgroup tile kernel.
+template <int DIM, int ALG>
+__device__ __forceinline__ uint32_t ising_word(uint32_t s0, const NbrWords& n, uint32_t valid,
+                                               const IsingTable& tb, uint32_t group,
+                                               uint32_t sw_lo, uint32_t w2, uint32_t c3,
+                                               uint32_t k0, uint32_t k1)
+{
+    constexpr int NMC = DIM;         // Metropolis uphill classes
+    constexpr int NHC = 2 * DIM + 1; // heat-bath classes
+    if (ALG == 0) {
+        uint32_t b0, b1, b2;
+        if (DIM == 2)
+            count4(s0 ^ n.U, s0 ^ n.Dn, s0 ^ n.A, s0 ^ n.Bw, b0, b1, b2);
+        else
+            count6(s0 ^ n.U, s0 ^ n.Dn, s0 ^ n.A, s0 ^ n.Bw, s0 ^ n.Zu, s0 ^ n.Zd, b0, b1, b2);
+        uint32_t m[NMC];
+        m[0] = count_eq<0>(b0, b1, b2) & valid;
+        m[1] = count_eq<1>(b0, b1, b2) & valid;
+        if (DIM == 3)
+            m[NMC - 1] = count_eq<2>(b0, b1, b2) & valid;
+        const uint32_t freem = DIM == 2 ? (b1 | b2) : (b2 | (b1 & b0));
+        uint32_t forced = 0, need = 0;
+        uint32_t thr[NMC];
+#pragma unroll
+        for (int k = 0; k < NMC; ++k) {
+            thr[k] = tb.metro_thr[k];
+            if ((tb.metro_always >> k) & 1u)
+                forced |= m[k];
+            else if (!((tb.metro_never >> k) & 1u))
+                need |= m[k];
+        }
+        const uint32_t lt = bitserial_less<NMC>(need, m, thr, group, sw_lo, w2, c3, k0, k1);
+        return s0 ^ ((freem | forced | lt) & valid);
+    } else {
+        uint32_t b0, b1, b2;
+        if (DIM == 2)
+            count4(n.U, n.Dn, n.A, n.Bw, b0, b1, b2);
+        else
+            count6(n.U, n.Dn, n.A, n.Bw, n.Zu, n.Zd, b0, b1, b2);
+        uint32_t m[NHC];
+        m[0] = count_eq<0>(b0, b1, b2) & valid;
+        m[1] = count_eq<1>(b0, b1, b2) & valid;
+        m[2] = count_eq<2>(b0, b1, b2) & valid;
+        m[3] = count_eq<3>(b0, b1, b2) & valid;
+        m[4] = count_eq<4>(b0, b1, b2) & valid;
+        if (DIM == 3) {
+            m[NHC - 2] = count_eq<5>(b0, b1, b2) & valid;
+            m[NHC - 1] = count_eq<6>(b0, b1, b2) & valid;
+        }
+        uint32_t up = 0, need = 0;
+        uint32_t thr[NHC];
+#pragma unroll
+        for (int k = 0; k < NHC; ++k) {
+            thr[k] = tb.hb_thr[k];
+            if ((tb.hb_always >> k) & 1u)
+                up |= m[k];
+            else if (!((tb.hb_never >> k) & 1u))
+                need |= m[k];
+        }
+        const uint32_t lt = bitserial_less<NHC>(need, m, thr, group, sw_lo, w2, c3, k0, k1);
+        return (up | lt) & valid;
+    }
+}
+
 template <int DIM, int ALG>
 __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
 {
@@ -109,12 +237,11 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
     const int wave = tid >> 6;
     const int nwaves = (B + 63) >> 6;
     constexpr uint32_t purpose = ALG == 0 ? P_METRO : P_HB;
-    constexpr int NMC = DIM;         // Metropolis uphill classes
-    constexpr int NHC = 2 * DIM + 1; // heat-bath classes
 
     for (int item = blockIdx.x; item < a.n_items; item += gridDim.x) {
         const int ch = a.items[item];
         const ChainRec cr = a.chains[ch];
+        const long long t_item = a.item_ticks && tid == 0 ? wall_clock64() : 0;
         const uint32_t k0 = (uint32_t)cr.seed;
         const uint32_t c3 = (uint32_t)(cr.seed >> 32);
         const int L = cr.L;
@@ -127,35 +254,22 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
         const IsingTable tb = a.itab[cr.table];
 
         // ---- int8 lattice -> bit planes (up = 1) ----
-        for (int w = tid; w < words; w += B) {
-            const int r = w / nw;
-            const int q = w - r * nw;
-            const int y = DIM == 3 ? r % L : r;
-            const int z = DIM == 3 ? r / L : 0;
-            const int par = (y + z) & 1; // colour of the even-x sites of this row
-            const int W = word_width(q, half);
-            const int8_t* row = sp + (int64_t)r * L + 64 * q;
-            uint32_t ev = 0, od = 0;
-            if ((L & 3) == 0) {
-                const uint32_t* row4 = reinterpret_cast<const uint32_t*>(row);
-                for (int k = 0; k < (W >> 1); ++k) {
-                    const uint32_t v = ~row4[k];
-                    ev |= (((v >> 7) & 1u) << (2 * k)) | (((v >> 23) & 1u) << (2 * k + 1));
-                    od |= (((v >> 15) & 1u) << (2 * k)) | (((v >> 31) & 1u) << (2 * k + 1));
-                }
-            } else {
-                for (int j = 0; j < W; ++j) {
-                    ev |= (uint32_t)(row[2 * j] > 0) << j;
-                    od |= (uint32_t)(row[2 * j + 1] > 0) << j;
-                }
-            }
-            P[par * words + w] = ev;
-            P[(par ^ 1) * words + w] = od;
-        }
+        for (int w = tid; w < words; w += B)
+            ising_pack_word<DIM>(sp, P, w, L, nw, half, words);
         __syncthreads();
 
         unsigned long long acc = 0;
+        // running sums continue from the stored ones, sweep by sweep, so the
+        // FP64 moments do not depend on how a run is split into launches
         double mE = 0, mE2 = 0, mAM = 0, mM2 = 0, mM4 = 0;
+        if (a.record && tid == 0) {
+            const double* mo = a.moments + (int64_t)ch * 6;
+            mE = mo[1];
+            mE2 = mo[2];
+            mAM = mo[3];
+            mM2 = mo[4];
+            mM4 = mo[5];
+        }
 
         const int64_t nsw = a.item_nsweeps ? a.item_nsweeps[item] : a.nsweeps;
         for (int64_t s = 0; s < nsw; ++s) {
@@ -180,65 +294,9 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                     NbrWords n;
                     load_nbr_words<DIM>(Po, L, nw, half, r, q, y, z, p, W, valid, n);
                     const uint32_t group = (uint32_t)((r * 2 + c) * nw + q);
-                    uint32_t snew, flip;
-                    if (ALG == 0) {
-                        uint32_t b0, b1, b2;
-                        if (DIM == 2)
-                            count4(s0 ^ n.U, s0 ^ n.Dn, s0 ^ n.A, s0 ^ n.Bw, b0, b1, b2);
-                        else
-                            count6(s0 ^ n.U, s0 ^ n.Dn, s0 ^ n.A, s0 ^ n.Bw, s0 ^ n.Zu, s0 ^ n.Zd,
-                                   b0, b1, b2);
-                        uint32_t m[NMC];
-                        m[0] = count_eq<0>(b0, b1, b2) & valid;
-                        m[1] = count_eq<1>(b0, b1, b2) & valid;
-                        if (DIM == 3)
-                            m[NMC - 1] = count_eq<2>(b0, b1, b2) & valid;
-                        const uint32_t freem = DIM == 2 ? (b1 | b2) : (b2 | (b1 & b0));
-                        uint32_t forced = 0, need = 0;
-                        uint32_t thr[NMC];
-#pragma unroll
-                        for (int k = 0; k < NMC; ++k) {
-                            thr[k] = tb.metro_thr[k];
-                            if ((tb.metro_always >> k) & 1u)
-                                forced |= m[k];
-                            else if (!((tb.metro_never >> k) & 1u))
-                                need |= m[k];
-                        }
-                        const uint32_t lt =
-                            bitserial_less<NMC>(need, m, thr, group, sw_lo, w2, c3, k0, cr.key);
-                        flip = (freem | forced | lt) & valid;
-                        snew = s0 ^ flip;
-                    } else {
-                        uint32_t b0, b1, b2;
-                        if (DIM == 2)
-                            count4(n.U, n.Dn, n.A, n.Bw, b0, b1, b2);
-                        else
-                            count6(n.U, n.Dn, n.A, n.Bw, n.Zu, n.Zd, b0, b1, b2);
-                        uint32_t m[NHC];
-                        m[0] = count_eq<0>(b0, b1, b2) & valid;
-                        m[1] = count_eq<1>(b0, b1, b2) & valid;
-                        m[2] = count_eq<2>(b0, b1, b2) & valid;
-                        m[3] = count_eq<3>(b0, b1, b2) & valid;
-                        m[4] = count_eq<4>(b0, b1, b2) & valid;
-                        if (DIM == 3) {
-                            m[NHC - 2] = count_eq<5>(b0, b1, b2) & valid;
-                            m[NHC - 1] = count_eq<6>(b0, b1, b2) & valid;
-                        }
-                        uint32_t up = 0, need = 0;
-                        uint32_t thr[NHC];
-#pragma unroll
-                        for (int k = 0; k < NHC; ++k) {
-                            thr[k] = tb.hb_thr[k];
-                            if ((tb.hb_always >> k) & 1u)
-                                up |= m[k];
-                            else if (!((tb.hb_never >> k) & 1u))
-                                need |= m[k];
-                        }
-                        const uint32_t lt =
-                            bitserial_less<NHC>(need, m, thr, group, sw_lo, w2, c3, k0, cr.key);
-                        snew = (up | lt) & valid;
-                        flip = snew ^ s0;
-                    }
+                    const uint32_t snew = ising_word<DIM, ALG>(s0, n, valid, tb, group, sw_lo,
+                                                               w2, c3, k0, cr.key);
+                    const uint32_t flip = snew ^ s0;
                     acc += (unsigned long long)__popc(flip);
                     Pc[w] = snew;
                     if (c == 1 && obs) {
@@ -290,31 +348,8 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
         }
 
         // ---- bit planes -> int8 lattice ----
-        for (int w = tid; w < words; w += B) {
-            const int r = w / nw;
-            const int q = w - r * nw;
-            const int y = DIM == 3 ? r % L : r;
-            const int z = DIM == 3 ? r / L : 0;
-            const int par = (y + z) & 1;
-            const int W = word_width(q, half);
-            int8_t* row = sp + (int64_t)r * L + 64 * q;
-            const uint32_t ev = P[par * words + w];
-            const uint32_t od = P[(par ^ 1) * words + w];
-            if ((L & 3) == 0) {
-                uint32_t* row4 = reinterpret_cast<uint32_t*>(row);
-                for (int k = 0; k < (W >> 1); ++k) {
-                    const uint32_t e0 = (ev >> (2 * k)) & 1u, e1 = (ev >> (2 * k + 1)) & 1u;
-                    const uint32_t o0 = (od >> (2 * k)) & 1u, o1 = (od >> (2 * k + 1)) & 1u;
-                    row4[k] = (0xFFu - 0xFEu * e0) | ((0xFFu - 0xFEu * o0) << 8)
-                            | ((0xFFu - 0xFEu * e1) << 16) | ((0xFFu - 0xFEu * o1) << 24);
-                }
-            } else {
-                for (int j = 0; j < W; ++j) {
-                    row[2 * j] = ((ev >> j) & 1u) ? 1 : -1;
-                    row[2 * j + 1] = ((od >> j) & 1u) ? 1 : -1;
-                }
-            }
-        }
+        for (int w = tid; w < words; w += B)
+            ising_unpack_word<DIM>(sp, P, w, L, nw, half, words);
         acc = wave_sum(acc);
         if (lane == 0 && acc)
             atomicAdd(&a.accepted[ch], acc);
@@ -322,15 +357,17 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
             if (a.record) {
                 double* mo = a.moments + (int64_t)ch * 6;
                 mo[0] += (double)nsw;
-                mo[1] += mE;
-                mo[2] += mE2;
-                mo[3] += mAM;
-                mo[4] += mM2;
-                mo[5] += mM4;
+                mo[1] = mE;
+                mo[2] = mE2;
+                mo[3] = mAM;
+                mo[4] = mM2;
+                mo[5] = mM4;
             }
             a.chains[ch].sweep = cr.sweep + (uint64_t)nsw;
         }
         __syncthreads();
+        if (a.item_ticks && tid == 0)
+            a.item_ticks[item] = wall_clock64() - t_item;
     }
 }
 
@@ -441,6 +478,106 @@ __global__ void __launch_bounds__(256) k_ising_plain(PlainArgs a, int n_col)
             atomicAdd(&a.ser_M[cr.series_off + a.rec_index], m_part);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Tile kernel: one colour of one sweep of every work item, one thread per plane
+// word, the planes in device memory.  Grid (word blocks, items).  The word
+// update and the observables are those of k_ising_sweeps, so a lattice gives
+// the same chain on either path.
+// ---------------------------------------------------------------------------
+
+template <int DIM, int ALG>
+__global__ void __launch_bounds__(256) k_ising_tiles(TileArgs a)
+{
+    const int item = blockIdx.y;
+    if (a.s >= a.item_nsw[item])
+        return;
+    constexpr uint32_t purpose = ALG == 0 ? P_METRO : P_HB;
+    const int ch = a.items[item];
+    const ChainRec cr = a.chains[ch];
+    const int L = cr.L;
+    const int half = L >> 1;
+    const int nw = (half + 31) >> 5;
+    const int rows = DIM == 3 ? L * L : L;
+    const int words = rows * nw;
+    const int c = a.colour;
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t flips = 0, nu = 0, na = 0;
+    if (w < words) {
+        uint32_t* P = reinterpret_cast<uint32_t*>(a.base + a.item_plane[item]);
+        const IsingTable tb = a.itab[cr.table];
+        const uint64_t sweep = cr.sweep + (uint64_t)a.s;
+        const int r = nw == 1 ? w : w / nw;
+        const int q = w - r * nw;
+        const int y = DIM == 3 ? r % L : r;
+        const int z = DIM == 3 ? r / L : 0;
+        const int p = (c + y + z) & 1;
+        const int W = word_width(q, half);
+        const uint32_t valid = width_mask(W);
+        const uint32_t s0 = P[c * words + w];
+        NbrWords n;
+        load_nbr_words<DIM>(P + (c ^ 1) * words, L, nw, half, r, q, y, z, p, W, valid, n);
+        const uint32_t group = (uint32_t)((r * 2 + c) * nw + q);
+        const uint32_t snew =
+            ising_word<DIM, ALG>(s0, n, valid, tb, group, (uint32_t)sweep,
+                                 ctr_word2(sweep, 0, purpose), (uint32_t)(cr.seed >> 32),
+                                 (uint32_t)cr.seed, cr.key);
+        P[c * words + w] = snew;
+        flips = (uint32_t)__popc(snew ^ s0);
+        if (c == 1) {
+            na = __popc(snew ^ n.U) + __popc(snew ^ n.Dn) + __popc(snew ^ n.A)
+               + __popc(snew ^ n.Bw);
+            if (DIM == 3)
+                na += __popc(snew ^ n.Zu) + __popc(snew ^ n.Zd);
+            nu = __popc(snew) + __popc(n.A);
+        }
+    }
+    tile_block_add(a, item, (int32_t)nu, (int32_t)na, 0, (int32_t)flips);
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(256) k_ising_pack(TileArgs a, int unpack)
+{
+    const int item = blockIdx.y;
+    const ChainRec cr = a.chains[a.items[item]];
+    const int L = cr.L;
+    const int half = L >> 1;
+    const int nw = (half + 31) >> 5;
+    const int words = (DIM == 3 ? L * L : L) * nw;
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= words)
+        return;
+    uint32_t* P = reinterpret_cast<uint32_t*>(a.base + a.item_plane[item]);
+    int8_t* sp = reinterpret_cast<int8_t*>(a.base) + cr.spin_off;
+    if (unpack)
+        ising_unpack_word<DIM>(sp, P, w, L, nw, half, words);
+    else
+        ising_pack_word<DIM>(sp, P, w, L, nw, half, words);
+}
+
+hipError_t launch_ising_tiles(int dim, int alg, const TileArgs& a, int max_words, hipStream_t st)
+{
+    const dim3 grid((max_words + 255) / 256, a.n_items);
+    if (dim == 2 && alg == 0)
+        hipLaunchKernelGGL((k_ising_tiles<2, 0>), grid, dim3(256), 0, st, a);
+    else if (dim == 2)
+        hipLaunchKernelGGL((k_ising_tiles<2, 1>), grid, dim3(256), 0, st, a);
+    else if (alg == 0)
+        hipLaunchKernelGGL((k_ising_tiles<3, 0>), grid, dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_ising_tiles<3, 1>), grid, dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_ising_pack(int dim, int unpack, const TileArgs& a, int max_words, hipStream_t st)
+{
+    const dim3 grid((max_words + 255) / 256, a.n_items);
+    if (dim == 2)
+        hipLaunchKernelGGL((k_ising_pack<2>), grid, dim3(256), 0, st, a, unpack);
+    else
+        hipLaunchKernelGGL((k_ising_pack<3>), grid, dim3(256), 0, st, a, unpack);
+    return hipGetLastError();
 }
 
 hipError_t launch_ising_sweeps(int dim, int alg, const SweepArgs& a, int block, int grid,

@@ -1,6 +1,7 @@
 // mcx_misc.hip -- small kernels around the sweeps: initial states, full
 // observables, series moments, sweep-counter advance, series clearing.
 #include "mcx_device.h"
+#include "../../include/mcx_hip.h"
 
 namespace mcx {
 
@@ -112,7 +113,9 @@ __global__ void k_series_moments(ChainRec* chains, int32_t n_chains, const int32
     if (ch >= n_chains)
         return;
     const ChainRec cr = chains[ch];
-    double mE = 0, mE2 = 0, mAM = 0, mM2 = 0, mM4 = 0;
+    // running sums continue from the stored ones (split-independent, as in the sweep kernels)
+    double* mo = moments + (int64_t)ch * 6;
+    double mE = mo[1], mE2 = mo[2], mAM = mo[3], mM2 = mo[4], mM4 = mo[5];
     for (int64_t k = 0; k < count; ++k) {
         const int64_t at = cr.series_off + first + k;
         const double e = Q ? (double)E[at] + cr.D * (double)Q[at] : (double)E[at];
@@ -123,13 +126,12 @@ __global__ void k_series_moments(ChainRec* chains, int32_t n_chains, const int32
         mM2 += m * m;
         mM4 += (m * m) * (m * m);
     }
-    double* mo = moments + (int64_t)ch * 6;
     mo[0] += (double)count;
-    mo[1] += mE;
-    mo[2] += mE2;
-    mo[3] += mAM;
-    mo[4] += mM2;
-    mo[5] += mM4;
+    mo[1] = mE;
+    mo[2] = mE2;
+    mo[3] = mAM;
+    mo[4] = mM2;
+    mo[5] = mM4;
 }
 
 __global__ void k_advance_sweeps(ChainRec* chains, int32_t n_chains, int64_t nsweeps)
@@ -169,6 +171,78 @@ hipError_t launch_observables(ChainRec* chains, int32_t n_chains, int64_t, const
                               int32_t* E, int32_t* M, int32_t* Q, hipStream_t st)
 {
     hipLaunchKernelGGL(k_observables, dim3(n_chains), dim3(256), 0, st, chains, spins, E, M, Q);
+    return hipGetLastError();
+}
+
+// One thread per work item of a tile series: per-sweep sums -> E/M/Q, FP64
+// moments continued sweep by sweep (the same operations, in the same order, as
+// the batched sweep kernels), last-sweep observables, sweep counter.
+__global__ void k_tiles_fold(TileArgs a, int model, double* moments, int32_t* last_E,
+                             int32_t* last_M, int32_t* last_Q)
+{
+    const int item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= a.n_items)
+        return;
+    const int ch = a.items[item];
+    const int64_t nsw = a.item_nsw[item];
+    const ChainRec cr = a.chains[ch];
+    const int64_t N = (int64_t)cr.L * cr.L * (cr.dim == 3 ? cr.L : 1);
+    double* mo = moments + (int64_t)ch * 6;
+    double mE = mo[1], mE2 = mo[2], mAM = mo[3], mM2 = mo[4], mM4 = mo[5];
+    int32_t E = 0, M = 0, Q = 0;
+    int32_t* obs = reinterpret_cast<int32_t*>(a.base + a.item_obs[item]);
+    unsigned long long flips = 0;
+    for (int64_t s = 0; s < nsw; ++s) {
+        int32_t f[4];
+        for (int k = 0; k < 4; ++k) {
+            int32_t* o = obs + (s * 4 + k) * kTileObsReplicas;
+            int32_t v = 0;
+            for (int r = 0; r < kTileObsReplicas; ++r) {
+                v += o[r];
+                o[r] = 0;
+            }
+            f[k] = v;
+        }
+        flips += (unsigned long long)(uint32_t)f[3];
+        double e;
+        if (model == MCX_ISING) {
+            E = (int32_t)(2 * (int64_t)(uint32_t)f[1] - (int64_t)cr.dim * N);
+            M = (int32_t)(2 * (int64_t)(uint32_t)f[0] - N);
+            Q = (int32_t)N;
+            e = (double)E;
+        } else {
+            E = -f[0];
+            M = f[1];
+            Q = f[2];
+            e = (double)E + cr.D * (double)Q;
+        }
+        const double m = (double)M;
+        mE += e;
+        mE2 += e * e;
+        mAM += fabs(m);
+        mM2 += m * m;
+        mM4 += (m * m) * (m * m);
+    }
+    if (nsw > 0) {
+        a.accepted[ch] += flips;
+        last_E[ch] = E;
+        last_M[ch] = M;
+        last_Q[ch] = Q;
+        mo[0] += (double)nsw;
+        mo[1] = mE;
+        mo[2] = mE2;
+        mo[3] = mAM;
+        mo[4] = mM2;
+        mo[5] = mM4;
+        a.chains[ch].sweep = cr.sweep + (uint64_t)nsw;
+    }
+}
+
+hipError_t launch_tiles_fold(int model, const TileArgs& a, double* moments, int32_t* last_E,
+                             int32_t* last_M, int32_t* last_Q, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_tiles_fold, dim3((a.n_items + 63) / 64), dim3(64), 0, st, a, model,
+                       moments, last_E, last_M, last_Q);
     return hipGetLastError();
 }
 

@@ -4,16 +4,20 @@
 // completion.  Jobs (model, dim, L, algorithm, beta, D, seed, key, sweeps)
 // arrive at any time.  A scheduler thread keeps their lattices resident in a
 // device pool bounded by an HBM budget and advances every resident job in
-// quanta: one launch per kernel class (model, dim, algorithm, block size), in
-// which each job runs its own fixed number of sweeps per quantum, chosen so
-// that every job's share of a quantum costs about the same.  Finished jobs
-// leave at the quantum boundary; their pool space and slot are freed and
+// quanta.  A quantum is one launch per kernel class (model, dim, algorithm,
+// block size), the classes spread over a few streams so that different kernels
+// run side by side; within a launch every job runs the number of sweeps that
+// fills its share of the quantum's target time, from the per-sweep time the
+// kernel measured for it (device wall clock) in its previous quanta.  Finished
+// jobs leave at the quantum boundary; their pool space and slot are freed and
 // waiting jobs are admitted before the next quantum.
 //
 // Invariance: a job's Philox stream is keyed by its own (seed, key) and
-// counted by its own sweep index, and its quantum lengths depend only on the
-// job, so its spins, counters and FP64 moments are the same alone, in any mix
-// and as a one-chain mcx_batch run with the same seed, key and random start.
+// counted by its own sweep index, and the kernels continue the FP64 moment
+// sums sweep by sweep from the stored ones, so nothing in a result depends on
+// how its sweeps were split into quanta: spins, counters, last-sweep
+// observables and moments are bit-identical alone, in any mix and as a
+// one-chain mcx_batch run with the same seed, key and random start.
 #include "mcx_api_impl.h"
 
 #include <algorithm>
@@ -38,14 +42,37 @@ double now_ms()
 }
 
 constexpr int64_t kAlign = 256;
+constexpr int kMaxStreams = 8;
+constexpr int kTileMaxSweeps = 256; // sweeps of a tiled job per quantum (per-sweep sum buffer)
+// Lanes per plane word of a tiled Blume-Capel launch: up to 8 (one Philox
+// block each) while the launch stays near 16k lanes; more lanes only repeat
+// the neighbour gathers once the chip is covered.
+int tile_lanes(int max_words)
+{
+    int l = 8;
+    while (l > 1 && (int64_t)max_words * l > 16384)
+        l /= 2;
+    return l;
+}
+
+int64_t align_up(int64_t n)
+{
+    return (n + kAlign - 1) / kAlign * kAlign;
+}
 
 struct Job {
     uint64_t id = 0;
     mcx_job_desc d{};
     int64_t remaining = 0; // sweeps still to run
     int64_t nsites = 0;
-    int64_t per_quantum = 0;
+    int block = 0;         // threads per workgroup (LDS-resident path)
+    int lpw = 1;           // lanes per plane word (Blume-Capel)
+    bool tiled = false;    // multi-workgroup path (bit planes in the pool)
+    int words = 0;         // plane words per colour (tiled path)
+    double ticks_per_sweep = 0; // latest device measurement (0 = none yet)
     int64_t off = -1;      // byte offset of the lattice in the pool
+    int64_t plane_off = -1; // byte offset of its bit planes (tiled path)
+    int64_t obs_off = -1;   // byte offset of its per-sweep sums (tiled path)
     int64_t bytes = 0;
     int slot = -1;         // index into the per-job device arrays
     double t_submit = 0, t_admit = 0, device_ms = 0;
@@ -88,16 +115,31 @@ private:
     std::map<int64_t, int64_t> free_; // offset -> length
 };
 
+struct ClassKey {
+    int model, dim, alg, block, lpw;
+    bool operator<(const ClassKey& o) const
+    {
+        return std::tie(model, dim, alg, block, lpw) <
+               std::tie(o.model, o.dim, o.alg, o.block, o.lpw);
+    }
+    bool operator==(const ClassKey& o) const { return !(*this < o) && !(o < *this); }
+};
+
+using ShapeKey = std::tuple<int, int, int, int>; // model, dim, alg, L
+
 } // namespace
 
 struct mcx_server {
     mcx_ctx* ctx = nullptr;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int n_streams = 0;
+    hipStream_t streams[kMaxStreams] = {};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_in = nullptr, ev_done[kMaxStreams] = {};
     int64_t budget = 0;
     int max_slots = 0;
-    int64_t quantum_attempts = 0;
-    int64_t min_sites = 0;
+    double quantum_ms = 0;
+    double ticks_per_ms = 0; // device wall clock
+    int64_t tile_sites[2] = {0, 0}; // per model: lattices this big take the tiled path
+    double tile_ms_per_sweep = 0;   // latest measurement of one tiled sweep (0 = none yet)
     // device arrays, one entry per slot
     int8_t* d_pool = nullptr;
     ChainRec* d_chains = nullptr;
@@ -108,10 +150,17 @@ struct mcx_server {
     int32_t* d_last = nullptr;           // [3 * slots]: E, M, Q after the last sweep run
     int32_t* d_items = nullptr;          // [slots] work list of one quantum
     int64_t* d_nsw = nullptr;            // [slots] sweeps per work item
+    int64_t* d_ticks = nullptr;          // [slots] wall-clock ticks per work item
+    int32_t* d_titems = nullptr;         // [slots] tiled work list
+    int64_t* d_tnsw = nullptr;           // [slots]
+    int64_t* d_tplane = nullptr;         // [slots]
+    int64_t* d_tobs = nullptr;           // [slots]
+    hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
     // worker-only state
     std::vector<Job> active;
     PoolAlloc* pool = nullptr;
     std::vector<int> free_slots;
+    std::map<ShapeKey, double> learned; // last ticks/sweep seen per shape
     // shared state, guarded by mu
     std::mutex mu;
     std::condition_variable cv, cv_done;
@@ -133,9 +182,47 @@ int64_t job_sites(const mcx_job_desc& d)
     return (int64_t)d.L * d.L * (d.dim == 3 ? d.L : 1);
 }
 
-int64_t job_bytes(const mcx_job_desc& d)
+int job_words(const mcx_job_desc& d)
 {
-    return (job_sites(d) + kAlign - 1) / kAlign * kAlign;
+    const int nw = (d.L / 2 + 31) / 32;
+    return (d.dim == 3 ? d.L * d.L : d.L) * nw;
+}
+
+bool job_tiled(const mcx_server* s, const mcx_job_desc& d)
+{
+    const int64_t t = s->tile_sites[d.model == MCX_ISING ? 0 : 1];
+    return t > 0 && job_sites(d) >= t;
+}
+
+constexpr int64_t kTileObsBytes = (int64_t)kTileMaxSweeps * 4 * kTileObsReplicas * 4;
+
+// Pool bytes of a job: the int8 lattice, plus on the tiled path its bit planes
+// (2 planes for Ising, 4 for Blume-Capel, of 32-bit words) and its per-sweep
+// sums.
+int64_t job_bytes(const mcx_server* s, const mcx_job_desc& d)
+{
+    int64_t b = align_up(job_sites(d));
+    if (job_tiled(s, d))
+        b += align_up((int64_t)(d.model == MCX_ISING ? 2 : 4) * job_words(d) * 4) + kTileObsBytes;
+    return b;
+}
+
+// Workgroup shape for one LDS-resident lattice: lanes per 32-site plane word
+// (Blume-Capel up to 8, one Philox block each, while the workgroup stays
+// within 1024 threads; Ising 1, its bit-serial draws serve all 32 sites at
+// once) and threads = words * lanes, at most 1024 (a whole CU's worth of waves
+// for a big lattice on its own), rounded up to a power of two so that few
+// kernel classes exist.
+void server_shape(int model, int dim, int L, int& block, int& lpw)
+{
+    const int64_t nw = (L / 2 + 31) / 32;
+    const int64_t words = (dim == 3 ? (int64_t)L * L : L) * nw;
+    lpw = model == MCX_ISING ? 1 : 8;
+    while (lpw > 1 && words * lpw > 1024)
+        lpw /= 2;
+    block = 64;
+    while (block < words * lpw && block < 1024)
+        block *= 2;
 }
 
 mcx_status validate(const mcx_server* s, const mcx_job_desc* d)
@@ -154,12 +241,17 @@ mcx_status validate(const mcx_server* s, const mcx_job_desc* d)
         return fail(MCX_EINVAL, "mcx_server_submit: beta must be finite and >= 0, D finite");
     if (d->nsweeps < 1)
         return fail(MCX_EINVAL, "mcx_server_submit: nsweeps must be >= 1");
-    if (lattice_lds_bytes(d->model, d->dim, d->L) > s->ctx->lds_max)
-        return fail(MCX_EINVAL, "mcx_server_submit: L = %d does not fit the device LDS", d->L);
-    if (job_bytes(*d) > s->budget)
+    if (!job_tiled(s, *d) && lattice_lds_bytes(d->model, d->dim, d->L) > s->ctx->lds_max)
         return fail(MCX_EINVAL,
-                    "mcx_server_submit: a %lld-byte lattice exceeds the %lld-byte HBM budget",
-                    (long long)job_bytes(*d), (long long)s->budget);
+                    "mcx_server_submit: L = %d does not fit the device LDS and the server's "
+                    "tile_sites keeps it off the multi-workgroup path",
+                    d->L);
+    if ((int64_t)job_words(*d) * 4 > INT32_MAX)
+        return fail(MCX_EINVAL, "mcx_server_submit: L = %d is too large", d->L);
+    if (job_bytes(s, *d) > s->budget)
+        return fail(MCX_EINVAL,
+                    "mcx_server_submit: a %lld-byte job exceeds the %lld-byte HBM budget",
+                    (long long)job_bytes(s, *d), (long long)s->budget);
     return MCX_OK;
 }
 
@@ -175,6 +267,8 @@ void take_admissions(mcx_server* s, std::vector<Job>& fresh)
         }
         Job j = *it;
         j.off = off;
+        j.plane_off = j.tiled ? off + align_up(j.nsites) : -1;
+        j.obs_off = j.tiled ? off + j.bytes - kTileObsBytes : -1;
         j.slot = s->free_slots.back();
         s->free_slots.pop_back();
         j.t_admit = now_ms();
@@ -184,14 +278,17 @@ void take_admissions(mcx_server* s, std::vector<Job>& fresh)
 }
 
 // Chain record, acceptance table, counters and random start of new jobs.
-hipError_t setup_jobs(mcx_server* s, const std::vector<Job>& fresh)
+hipError_t setup_jobs(mcx_server* s, std::vector<Job>& fresh)
 {
+    hipStream_t st = s->streams[0];
     std::vector<ChainRec> rec(fresh.size());
     std::vector<IsingTable> it(fresh.size());
     std::vector<BCTable> bt(fresh.size());
     hipError_t e = hipSuccess;
     for (size_t i = 0; i < fresh.size() && e == hipSuccess; ++i) {
-        const Job& j = fresh[i];
+        Job& j = fresh[i];
+        auto lr = s->learned.find(ShapeKey{j.d.model, j.d.dim, j.d.alg, j.d.L});
+        j.ticks_per_sweep = lr != s->learned.end() ? lr->second : 0.0;
         ChainRec& r = rec[i];
         std::memset(&r, 0, sizeof r);
         r.spin_off = j.off;
@@ -202,7 +299,7 @@ hipError_t setup_jobs(mcx_server* s, const std::vector<Job>& fresh)
         r.table = j.slot;
         r.key = j.d.key;
         r.D = j.d.model == MCX_ISING ? 0.0 : j.d.D;
-        e = hipMemcpyAsync(s->d_chains + j.slot, &r, sizeof r, hipMemcpyHostToDevice, s->stream);
+        e = hipMemcpyAsync(s->d_chains + j.slot, &r, sizeof r, hipMemcpyHostToDevice, st);
         if (e == hipSuccess && j.d.model == MCX_ISING) {
             std::memset(&it[i], 0, sizeof it[i]);
             double pm[4], ph[8];
@@ -210,76 +307,234 @@ hipError_t setup_jobs(mcx_server* s, const std::vector<Job>& fresh)
             fill_ising_metro(it[i], j.d.dim, pm);
             fill_ising_hb(it[i], j.d.dim, ph);
             e = hipMemcpyAsync(s->d_itab + j.slot, &it[i], sizeof it[i], hipMemcpyHostToDevice,
-                               s->stream);
+                               st);
         } else if (e == hipSuccess) {
             fill_bc_table(bt[i], j.d.dim, j.d.beta, j.d.D);
             e = hipMemcpyAsync(s->d_btab + j.slot, &bt[i], sizeof bt[i], hipMemcpyHostToDevice,
-                               s->stream);
+                               st);
         }
         if (e == hipSuccess)
-            e = hipMemsetAsync(s->d_mom + 6 * (size_t)j.slot, 0, sizeof(double) * 6, s->stream);
+            e = hipMemsetAsync(s->d_mom + 6 * (size_t)j.slot, 0, sizeof(double) * 6, st);
         if (e == hipSuccess)
-            e = hipMemsetAsync(s->d_acc + j.slot, 0, sizeof(unsigned long long), s->stream);
+            e = hipMemsetAsync(s->d_acc + j.slot, 0, sizeof(unsigned long long), st);
         if (e == hipSuccess)
             e = launch_init_spins(j.d.model, MCX_INIT_RANDOM, s->d_chains + j.slot, 1, j.nsites,
-                                  s->d_pool, s->stream);
+                                  s->d_pool, st);
+        if (e == hipSuccess && j.tiled)
+            e = hipMemsetAsync(s->d_pool + j.obs_off, 0, kTileObsBytes, st);
     }
+    // tiled jobs: int8 lattice -> bit planes, per (model, dim)
+    std::vector<int32_t> titems;
+    std::vector<int64_t> tplane;
+    for (int model = 0; model < 2 && e == hipSuccess; ++model)
+        for (int dim = 2; dim <= 3 && e == hipSuccess; ++dim) {
+            titems.clear();
+            tplane.clear();
+            int max_words = 0;
+            for (const Job& j : fresh)
+                if (j.tiled && j.d.model == model && j.d.dim == dim) {
+                    titems.push_back(j.slot);
+                    tplane.push_back(j.plane_off);
+                    max_words = std::max(max_words, j.words);
+                }
+            if (titems.empty())
+                continue;
+            e = hipMemcpyAsync(s->d_titems, titems.data(), sizeof(int32_t) * titems.size(),
+                               hipMemcpyHostToDevice, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(s->d_tplane, tplane.data(), sizeof(int64_t) * tplane.size(),
+                                   hipMemcpyHostToDevice, st);
+            TileArgs a{};
+            a.items = s->d_titems;
+            a.item_plane = s->d_tplane;
+            a.n_items = (int32_t)titems.size();
+            a.chains = s->d_chains;
+            a.base = reinterpret_cast<uint8_t*>(s->d_pool);
+            if (e == hipSuccess)
+                e = model == MCX_ISING ? launch_ising_pack(dim, 0, a, max_words, st)
+                                       : launch_bc_pack(dim, 0, a, max_words, st);
+            // the staging vectors are refilled by the next (model, dim)
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(st);
+        }
     // the host staging vectors must outlive the copies
-    const hipError_t e2 = hipStreamSynchronize(s->stream);
+    const hipError_t e2 = hipStreamSynchronize(st);
     return e != hipSuccess ? e : e2;
 }
 
-struct ClassKey {
-    int model, dim, alg, block;
-    bool operator<(const ClassKey& o) const
-    {
-        return std::tie(model, dim, alg, block) < std::tie(o.model, o.dim, o.alg, o.block);
-    }
-    bool operator==(const ClassKey& o) const { return !(*this < o) && !(o < *this); }
-};
+// Sweeps a job runs in a quantum of `target` ticks.  Without a measurement
+// yet, a conservative model (2 us + 1 ns per site and sweep) sets a short
+// first quantum.
+int64_t quantum_sweeps(const mcx_server* s, const Job& j, double target)
+{
+    const double per = j.ticks_per_sweep > 0
+                           ? j.ticks_per_sweep
+                           : s->ticks_per_ms * (2e-3 + 1e-6 * (double)j.nsites);
+    const double n = std::floor(target / per);
+    return n >= (double)j.remaining ? j.remaining : std::max<int64_t>(1, (int64_t)n);
+}
 
-// One quantum over every active job: ran[i] = sweeps run by active[i], ms =
-// device time between the first and the last launch.
-hipError_t run_quantum(mcx_server* s, std::vector<int64_t>& ran, float& ms)
+// One quantum over every active job: ran[i] = sweeps run by active[i],
+// ticks[i] = the wall-clock ticks its LDS-resident work item took (0 on the
+// tiled path), ms = device time of the quantum; tile_ms / tile_sweeps = time
+// and sweep count of the tiled launches (0 when there were none).
+hipError_t run_quantum(mcx_server* s, std::vector<int64_t>& ran, std::vector<int64_t>& ticks,
+                       float& ms, float& tile_ms, int64_t& tile_sweeps)
 {
     const int n = (int)s->active.size();
-    std::vector<int> order(n);
-    std::vector<ClassKey> key(n);
-    ran.assign(n, 0);
+    using TileKey = std::tuple<int, int, int>; // model, dim, alg
+    std::map<ClassKey, std::vector<int>> classes;
+    std::map<TileKey, std::vector<int>> tclasses;
     for (int i = 0; i < n; ++i) {
         const Job& j = s->active[i];
-        order[i] = i;
-        key[i] = {j.d.model, j.d.dim, j.d.alg, lattice_block(j.d.dim, j.d.L)};
-        ran[i] = std::min(j.per_quantum, j.remaining);
+        if (j.tiled)
+            tclasses[TileKey{j.d.model, j.d.dim, j.d.alg}].push_back(i);
+        else
+            classes[{j.d.model, j.d.dim, j.d.alg, j.block, j.lpw}].push_back(i);
     }
-    // class by class, the most expensive items first
-    std::sort(order.begin(), order.end(), [&](int a, int b) {
-        if (!(key[a] == key[b]))
-            return key[a] < key[b];
-        return ran[a] * s->active[a].nsites > ran[b] * s->active[b].nsites;
-    });
-    std::vector<int32_t> items(n);
-    std::vector<int64_t> nsw(n);
-    for (int q = 0; q < n; ++q) {
-        items[q] = s->active[order[q]].slot;
-        nsw[q] = ran[order[q]];
+    // units of work dealt round-robin over the streams: the tiled launches as
+    // one unit first, then the LDS-resident classes in decreasing size
+    std::vector<std::pair<ClassKey, std::vector<int>>> cl(classes.begin(), classes.end());
+    std::stable_sort(cl.begin(), cl.end(),
+                     [](const auto& a, const auto& b) { return a.second.size() > b.second.size(); });
+    const bool has_tiles = !tclasses.empty();
+    const int n_units = (int)cl.size() + (has_tiles ? 1 : 0);
+    const int ns = std::max(1, std::min(s->n_streams, n_units));
+    std::vector<int> per_stream(ns, 0);
+    for (int u = 0; u < n_units; ++u)
+        ++per_stream[u % ns];
+    auto unit_stream = [&](int u) { return u % ns; };
+    const int unit0 = has_tiles ? 1 : 0; // unit of LDS class c is unit0 + c
+
+    ran.assign(n, 0);
+    // LDS-resident work list: class by class, the longest items first
+    std::vector<int32_t> items;
+    std::vector<int64_t> nsw;
+    std::vector<int> item_job;
+    std::vector<std::pair<int, int>> span(cl.size());
+    for (size_t c = 0; c < cl.size(); ++c) {
+        const double target =
+            s->quantum_ms * s->ticks_per_ms / per_stream[unit_stream(unit0 + (int)c)];
+        auto& v = cl[c].second;
+        for (int i : v)
+            ran[i] = quantum_sweeps(s, s->active[i], target);
+        std::stable_sort(v.begin(), v.end(), [&](int a, int b) {
+            return ran[a] * s->active[a].nsites > ran[b] * s->active[b].nsites;
+        });
+        span[c].first = (int)items.size();
+        for (int i : v) {
+            items.push_back(s->active[i].slot);
+            nsw.push_back(ran[i]);
+            item_job.push_back(i);
+        }
+        span[c].second = (int)v.size();
     }
-    hipError_t e = hipMemcpyAsync(s->d_items, items.data(), sizeof(int32_t) * n,
-                                  hipMemcpyHostToDevice, s->stream);
+    // tiled work list: every tiled job runs the same number of sweeps (or what
+    // it has left), sized from the last measured tiled sweep time
+    std::vector<int32_t> titems;
+    std::vector<int64_t> tnsw, tplane, tobs;
+    std::vector<std::tuple<TileKey, int, int, int>> tspan; // key, first, count, max words
+    tile_sweeps = 0;
+    if (has_tiles) {
+        const double target_ms = s->quantum_ms / per_stream[unit_stream(0)];
+        const double per = s->tile_ms_per_sweep > 0 ? s->tile_ms_per_sweep : 0.02;
+        const int64_t nt =
+            std::max<int64_t>(1, std::min<int64_t>(kTileMaxSweeps, (int64_t)std::floor(target_ms / per)));
+        for (auto& kv : tclasses) {
+            const int first = (int)titems.size();
+            int max_words = 0;
+            for (int i : kv.second) {
+                const Job& j = s->active[i];
+                ran[i] = std::min(nt, j.remaining);
+                tile_sweeps = std::max(tile_sweeps, ran[i]);
+                titems.push_back(j.slot);
+                tnsw.push_back(ran[i]);
+                tplane.push_back(j.plane_off);
+                tobs.push_back(j.obs_off);
+                max_words = std::max(max_words, j.words);
+            }
+            tspan.emplace_back(kv.first, first, (int)kv.second.size(), max_words);
+        }
+    }
+
+    hipStream_t s0 = s->streams[0];
+    const int nl = (int)items.size(), nt_items = (int)titems.size();
+    hipError_t e = hipEventRecord(s->ev0, s0);
+    if (e == hipSuccess && nl)
+        e = hipMemcpyAsync(s->d_items, items.data(), sizeof(int32_t) * nl, hipMemcpyHostToDevice, s0);
+    if (e == hipSuccess && nl)
+        e = hipMemcpyAsync(s->d_nsw, nsw.data(), sizeof(int64_t) * nl, hipMemcpyHostToDevice, s0);
+    if (e == hipSuccess && nt_items)
+        e = hipMemcpyAsync(s->d_titems, titems.data(), sizeof(int32_t) * nt_items,
+                           hipMemcpyHostToDevice, s0);
+    if (e == hipSuccess && nt_items)
+        e = hipMemcpyAsync(s->d_tnsw, tnsw.data(), sizeof(int64_t) * nt_items,
+                           hipMemcpyHostToDevice, s0);
+    if (e == hipSuccess && nt_items)
+        e = hipMemcpyAsync(s->d_tplane, tplane.data(), sizeof(int64_t) * nt_items,
+                           hipMemcpyHostToDevice, s0);
+    if (e == hipSuccess && nt_items)
+        e = hipMemcpyAsync(s->d_tobs, tobs.data(), sizeof(int64_t) * nt_items,
+                           hipMemcpyHostToDevice, s0);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(s->d_nsw, nsw.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
-                           s->stream);
-    if (e == hipSuccess)
-        e = hipEventRecord(s->ev0, s->stream);
-    for (int q = 0; q < n && e == hipSuccess;) {
-        const ClassKey k = key[order[q]];
-        const int first = q;
+        e = hipEventRecord(s->ev_in, s0);
+    for (int k = 1; k < ns && e == hipSuccess; ++k)
+        e = hipStreamWaitEvent(s->streams[k], s->ev_in, 0);
+    // tiled unit: per sweep, colour 0 then colour 1 of every tiled class, then the fold
+    if (has_tiles && e == hipSuccess) {
+        hipStream_t st = s->streams[unit_stream(0)];
+        e = hipEventRecord(s->ev_t0, st);
+        for (int64_t sw = 0; sw < tile_sweeps && e == hipSuccess; ++sw)
+            for (const auto& ts : tspan) {
+                const TileKey& k = std::get<0>(ts);
+                TileArgs a{};
+                a.items = s->d_titems + std::get<1>(ts);
+                a.item_nsw = s->d_tnsw + std::get<1>(ts);
+                a.item_plane = s->d_tplane + std::get<1>(ts);
+                a.n_items = std::get<2>(ts);
+                a.s = sw;
+                a.chains = s->d_chains;
+                a.itab = s->d_itab;
+                a.btab = s->d_btab;
+                a.base = reinterpret_cast<uint8_t*>(s->d_pool);
+                a.item_obs = s->d_tobs + std::get<1>(ts);
+                a.max_s = kTileMaxSweeps;
+                a.lpw = tile_lanes(std::get<3>(ts));
+                a.accepted = s->d_acc;
+                for (int c = 0; c < 2 && e == hipSuccess; ++c) {
+                    a.colour = c;
+                    e = std::get<0>(k) == MCX_ISING
+                            ? launch_ising_tiles(std::get<1>(k), std::get<2>(k), a, std::get<3>(ts), st)
+                            : launch_bc_tiles(std::get<1>(k), std::get<2>(k), a, std::get<3>(ts), st);
+                }
+                if (e != hipSuccess)
+                    break;
+            }
+        for (size_t t = 0; t < tspan.size() && e == hipSuccess; ++t) {
+            const auto& ts = tspan[t];
+            TileArgs a{};
+            a.items = s->d_titems + std::get<1>(ts);
+            a.item_nsw = s->d_tnsw + std::get<1>(ts);
+            a.n_items = std::get<2>(ts);
+            a.chains = s->d_chains;
+            a.base = reinterpret_cast<uint8_t*>(s->d_pool);
+            a.item_obs = s->d_tobs + std::get<1>(ts);
+            a.max_s = kTileMaxSweeps;
+            a.accepted = s->d_acc;
+            e = launch_tiles_fold(std::get<0>(std::get<0>(ts)), a, s->d_mom, s->d_last,
+                                  s->d_last + s->max_slots, s->d_last + 2 * s->max_slots, st);
+        }
+        if (e == hipSuccess)
+            e = hipEventRecord(s->ev_t1, st);
+    }
+    for (size_t c = 0; c < cl.size() && e == hipSuccess; ++c) {
+        const ClassKey& k = cl[c].first;
         size_t lds = 0;
-        for (; q < n && key[order[q]] == k; ++q)
-            lds = std::max(lds, lattice_lds_bytes(k.model, k.dim, s->active[order[q]].d.L));
+        for (int i : cl[c].second)
+            lds = std::max(lds, lattice_lds_bytes(k.model, k.dim, s->active[i].d.L));
         SweepArgs a{};
-        a.items = s->d_items + first;
-        a.n_items = q - first;
+        a.items = s->d_items + span[c].first;
+        a.n_items = span[c].second;
         a.record = 1;
         a.chains = s->d_chains;
         a.itab = s->d_itab;
@@ -290,19 +545,36 @@ hipError_t run_quantum(mcx_server* s, std::vector<int64_t>& ran, float& ms)
         a.last_E = s->d_last;
         a.last_M = s->d_last + s->max_slots;
         a.last_Q = s->d_last + 2 * s->max_slots;
-        a.item_nsweeps = s->d_nsw + first;
+        a.item_nsweeps = s->d_nsw + span[c].first;
+        a.item_ticks = s->d_ticks + span[c].first;
+        a.lanes_per_word = k.lpw;
+        hipStream_t st = s->streams[unit_stream(unit0 + (int)c)];
         e = k.model == MCX_ISING
-                ? launch_ising_sweeps(k.dim, k.alg, a, k.block, a.n_items, lds, s->stream)
-                : launch_bc_sweeps(k.dim, k.alg, a, k.block, a.n_items, lds, s->stream);
+                ? launch_ising_sweeps(k.dim, k.alg, a, k.block, a.n_items, lds, st)
+                : launch_bc_sweeps(k.dim, k.alg, a, k.block, a.n_items, lds, st);
     }
+    for (int k = 1; k < ns && e == hipSuccess; ++k) {
+        e = hipEventRecord(s->ev_done[k], s->streams[k]);
+        if (e == hipSuccess)
+            e = hipStreamWaitEvent(s0, s->ev_done[k], 0);
+    }
+    std::vector<int64_t> tk(std::max(nl, 1));
     if (e == hipSuccess)
-        e = hipEventRecord(s->ev1, s->stream);
-    const hipError_t e2 = hipStreamSynchronize(s->stream);
+        e = hipEventRecord(s->ev1, s0);
+    if (e == hipSuccess && nl)
+        e = hipMemcpyAsync(tk.data(), s->d_ticks, sizeof(int64_t) * nl, hipMemcpyDeviceToHost, s0);
+    const hipError_t e2 = hipStreamSynchronize(s0);
     if (e == hipSuccess)
         e = e2;
     ms = 0;
+    tile_ms = 0;
     if (e == hipSuccess)
         e = hipEventElapsedTime(&ms, s->ev0, s->ev1);
+    if (e == hipSuccess && has_tiles)
+        e = hipEventElapsedTime(&tile_ms, s->ev_t0, s->ev_t1);
+    ticks.assign(n, 0);
+    for (int q = 0; q < nl; ++q)
+        ticks[item_job[q]] = tk[q];
     return e;
 }
 
@@ -312,7 +584,7 @@ void worker_main(mcx_server* s)
     std::vector<double> mom(6 * n);
     std::vector<unsigned long long> acc(n);
     std::vector<int32_t> last(3 * n);
-    std::vector<int64_t> ran;
+    std::vector<int64_t> ran, ticks;
     auto die = [&](const char* what, hipError_t e) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->error = std::string(what) + ": " + hipGetErrorString(e);
@@ -340,14 +612,21 @@ void worker_main(mcx_server* s)
                 return die("job set-up", e);
             s->active.insert(s->active.end(), fresh.begin(), fresh.end());
         }
-        float ms = 0;
-        if ((e = run_quantum(s, ran, ms)) != hipSuccess)
+        float ms = 0, tile_ms = 0;
+        int64_t tile_sweeps = 0;
+        if ((e = run_quantum(s, ran, ticks, ms, tile_ms, tile_sweeps)) != hipSuccess)
             return die("quantum", e);
+        if (tile_sweeps > 0 && tile_ms > 0)
+            s->tile_ms_per_sweep = (double)tile_ms / (double)tile_sweeps;
         bool any_done = false;
         for (size_t i = 0; i < s->active.size(); ++i) {
             Job& j = s->active[i];
             j.remaining -= ran[i];
             j.device_ms += ms;
+            if (ticks[i] > 0) {
+                j.ticks_per_sweep = (double)ticks[i] / (double)ran[i];
+                s->learned[ShapeKey{j.d.model, j.d.dim, j.d.alg, j.d.L}] = j.ticks_per_sweep;
+            }
             any_done |= j.remaining == 0;
         }
         if (any_done) {
@@ -413,12 +692,20 @@ void free_device(mcx_server* s)
     hipFree(s->d_last);
     hipFree(s->d_items);
     hipFree(s->d_nsw);
-    if (s->ev0)
-        hipEventDestroy(s->ev0);
-    if (s->ev1)
-        hipEventDestroy(s->ev1);
-    if (s->stream)
-        hipStreamDestroy(s->stream);
+    hipFree(s->d_ticks);
+    hipFree(s->d_titems);
+    hipFree(s->d_tnsw);
+    hipFree(s->d_tplane);
+    hipFree(s->d_tobs);
+    for (hipEvent_t ev : {s->ev0, s->ev1, s->ev_in, s->ev_t0, s->ev_t1})
+        if (ev)
+            hipEventDestroy(ev);
+    for (int k = 0; k < kMaxStreams; ++k) {
+        if (s->ev_done[k])
+            hipEventDestroy(s->ev_done[k]);
+        if (s->streams[k])
+            hipStreamDestroy(s->streams[k]);
+    }
 }
 
 } // namespace
@@ -428,20 +715,35 @@ extern "C" mcx_status mcx_server_create(mcx_ctx* ctx, const mcx_server_desc* d, 
     if (!ctx || !d || !out)
         return fail(MCX_EINVAL, "mcx_server_create: NULL argument");
     *out = nullptr;
-    if (d->hbm_budget < kAlign || d->max_jobs < 1 || d->quantum_attempts < 0 || d->min_sites < 0)
+    if (d->hbm_budget < kAlign || d->max_jobs < 1 || d->n_streams < 0 ||
+        d->n_streams > kMaxStreams || !(d->quantum_ms >= 0) || !std::isfinite(d->quantum_ms))
         return fail(MCX_EINVAL,
-                    "mcx_server_create: need hbm_budget >= %lld bytes, max_jobs >= 1 and "
-                    "non-negative quantum_attempts and min_sites",
-                    (long long)kAlign);
+                    "mcx_server_create: need hbm_budget >= %lld bytes, max_jobs >= 1, "
+                    "0 <= n_streams <= %d and a finite quantum_ms >= 0",
+                    (long long)kAlign, kMaxStreams);
     HIPCHK(hipSetDevice(ctx->device));
+    int khz = 0;
+    HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+    if (khz <= 0)
+        return fail(MCX_EHIP, "mcx_server_create: the device reports no wall-clock rate");
     auto* s = new mcx_server();
     s->ctx = ctx;
     s->budget = d->hbm_budget / kAlign * kAlign;
     s->max_slots = d->max_jobs;
-    s->quantum_attempts = d->quantum_attempts > 0 ? d->quantum_attempts : ((int64_t)1 << 24);
-    s->min_sites = d->min_sites > 0 ? d->min_sites : 4096;
+    s->n_streams = d->n_streams > 0 ? d->n_streams : 4;
+    s->quantum_ms = d->quantum_ms > 0 ? d->quantum_ms : 2.0;
+    s->ticks_per_ms = (double)khz;
+    // tiled path from 2^17 Ising / 2^14 Blume-Capel sites on, where one
+    // workgroup per lattice gets slower per sweep than the tiled launches
+    s->tile_sites[0] = d->tile_sites > 0 ? d->tile_sites : (d->tile_sites < 0 ? 0 : 1 << 17);
+    s->tile_sites[1] = d->tile_sites > 0 ? d->tile_sites : (d->tile_sites < 0 ? 0 : 1 << 14);
     const size_t n = (size_t)s->max_slots;
-    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < s->n_streams && e == hipSuccess; ++k) {
+        e = hipStreamCreateWithFlags(&s->streams[k], hipStreamNonBlocking);
+        if (e == hipSuccess)
+            e = hipEventCreateWithFlags(&s->ev_done[k], hipEventDisableTiming);
+    }
     auto alloc = [&](void** p, size_t bytes) {
         if (e == hipSuccess)
             e = hipMalloc(p, bytes);
@@ -455,14 +757,26 @@ extern "C" mcx_status mcx_server_create(mcx_ctx* ctx, const mcx_server_desc* d, 
     alloc((void**)&s->d_last, sizeof(int32_t) * 3 * n);
     alloc((void**)&s->d_items, sizeof(int32_t) * n);
     alloc((void**)&s->d_nsw, sizeof(int64_t) * n);
+    alloc((void**)&s->d_ticks, sizeof(int64_t) * n);
+    alloc((void**)&s->d_titems, sizeof(int32_t) * n);
+    alloc((void**)&s->d_tnsw, sizeof(int64_t) * n);
+    alloc((void**)&s->d_tplane, sizeof(int64_t) * n);
+    alloc((void**)&s->d_tobs, sizeof(int64_t) * n);
     if (e == hipSuccess)
         e = hipEventCreate(&s->ev0);
     if (e == hipSuccess)
         e = hipEventCreate(&s->ev1);
     if (e == hipSuccess)
-        e = hipMemsetAsync(s->d_last, 0, sizeof(int32_t) * 3 * n, s->stream);
+        e = hipEventCreateWithFlags(&s->ev_in, hipEventDisableTiming);
     if (e == hipSuccess)
-        e = hipStreamSynchronize(s->stream);
+        e = hipEventCreate(&s->ev_t0);
+    if (e == hipSuccess)
+        e = hipEventCreate(&s->ev_t1);
+
+    if (e == hipSuccess)
+        e = hipMemsetAsync(s->d_last, 0, sizeof(int32_t) * 3 * n, s->streams[0]);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s->streams[0]);
     if (e != hipSuccess) {
         free_device(s);
         delete s;
@@ -489,8 +803,10 @@ extern "C" mcx_status mcx_server_submit(mcx_server* s, const mcx_job_desc* d, ui
     j.d = *d;
     j.remaining = d->nsweeps;
     j.nsites = job_sites(*d);
-    j.bytes = job_bytes(*d);
-    j.per_quantum = std::max<int64_t>(1, s->quantum_attempts / std::max(j.nsites, s->min_sites));
+    j.tiled = job_tiled(s, *d);
+    j.words = job_words(*d);
+    j.bytes = job_bytes(s, *d);
+    server_shape(d->model, d->dim, d->L, j.block, j.lpw);
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (!s->error.empty())
@@ -583,7 +899,8 @@ extern "C" mcx_status mcx_server_destroy(mcx_server* s)
     if (s->worker.joinable())
         s->worker.join();
     hipSetDevice(s->ctx->device);
-    hipStreamSynchronize(s->stream);
+    for (int k = 0; k < s->n_streams; ++k)
+        hipStreamSynchronize(s->streams[k]);
     free_device(s);
     --s->ctx->n_users;
     delete s->pool;

@@ -60,8 +60,8 @@ class GenDesc(C.Structure):
 
 
 class ServerDesc(C.Structure):
-    _fields_ = [("hbm_budget", C.c_int64), ("max_jobs", C.c_int32), ("min_sites", C.c_int32),
-                ("quantum_attempts", C.c_int64)]
+    _fields_ = [("hbm_budget", C.c_int64), ("max_jobs", C.c_int32), ("n_streams", C.c_int32),
+                ("quantum_ms", C.c_double), ("tile_sites", C.c_int64)]
 
 
 class JobDesc(C.Structure):

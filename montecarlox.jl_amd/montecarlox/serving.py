@@ -77,15 +77,17 @@ class Server:
     """A job server on one device.
 
     hbm_budget: bytes for resident lattices (L^dim int8 each, 256-aligned);
-    max_jobs: resident slots; quantum_attempts / min_sites: each job runs
-    max(1, quantum_attempts // max(L^dim, min_sites)) sweeps per quantum
-    (0 = library defaults 2^24 and 4096)."""
+    max_jobs: resident slots; quantum_ms: target device time of one quantum
+    (0 = 2 ms); n_streams: streams the kernel classes run on (0 = 4);
+    tile_sites: lattices at least this big run spread over many workgroups
+    (0 = 2^17 Ising / 2^14 Blume-Capel sites, negative = never)."""
 
-    def __init__(self, hbm_budget: int, max_jobs: int = 1024, quantum_attempts: int = 0,
-                 min_sites: int = 0, ctx: Optional[_lib.Context] = None):
+    def __init__(self, hbm_budget: int, max_jobs: int = 1024, quantum_ms: float = 0.0,
+                 n_streams: int = 0, tile_sites: int = 0, ctx: Optional[_lib.Context] = None):
         self.ctx = ctx or _lib.default_context()
         d = ServerDesc(hbm_budget=int(hbm_budget), max_jobs=int(max_jobs),
-                       min_sites=int(min_sites), quantum_attempts=int(quantum_attempts))
+                       n_streams=int(n_streams), quantum_ms=float(quantum_ms),
+                       tile_sites=int(tile_sites))
         self._h = _P()
         call("mcx_server_create", self.ctx.handle, C.byref(d), C.byref(self._h))
 

@@ -24,6 +24,8 @@ def _jobs(mcx):
         J("ising", 128, 0.4406868, 30, "metropolis", seed=2**33 + 7, key=7),
         J("ising", 66, 0.5, 40, "heatbath", seed=5, key=8),
         J("blume_capel", 4, 2.0, 500, "metropolis", D=1.9, seed=5, key=9),
+        J("ising", 256, 0.45, 6, "metropolis", seed=5, key=10),        # 1024-thread class
+        J("blume_capel", 130, 0.6, 8, "heatbath", D=0.7, seed=5, key=11),
     ]
 
 
@@ -45,8 +47,8 @@ def _same(r1, r2):
     assert np.array_equal(r1["moments"], r2["moments"])
 
 
-# small quanta and a budget of three 128^2 lattices: many quanta, admissions wait
-SERVER = dict(hbm_budget=3 * 128 * 128, max_jobs=4, quantum_attempts=20000, min_sites=256)
+# short quanta and a budget of four 256^2 lattices: many quanta, admissions wait
+SERVER = dict(hbm_budget=4 * 256 * 256, max_jobs=4, quantum_ms=0.05)
 
 
 def test_server_matches_oracle_alone_and_in_mix(mcx, orc):
@@ -68,24 +70,54 @@ def test_server_matches_oracle_alone_and_in_mix(mcx, orc):
             _same(alone, r)
 
 
+def test_tiled_path_equals_lds_path_and_oracle(mcx, orc):
+    """Every lattice on the multi-workgroup path (tile_sites=16) against every
+    lattice on the LDS-resident path (tile_sites=-1): bit-identical results,
+    and both equal to the oracle."""
+    jobs = _jobs(mcx)
+    print("serving: tiled vs LDS path", flush=True)
+    with mcx.Server(hbm_budget=1 << 22, max_jobs=6, quantum_ms=0.2, tile_sites=16) as srv:
+        tiled = srv.run(jobs, timeout_ms=120000)
+    with mcx.Server(hbm_budget=1 << 22, max_jobs=6, quantum_ms=0.2, tile_sites=-1) as srv:
+        lds = srv.run(jobs, timeout_ms=120000)
+    for job, a, b in zip(jobs, tiled, lds):
+        _same(a, b)
+        E, M, Q, acc, mom = _oracle(orc, job)
+        assert (a["E_pair"], a["M"], a["Q"], a["accepted"]) == (E, M, Q, acc), job
+        assert np.allclose(a["moments"], mom, rtol=1e-12, atol=0), job
+
+
+def test_tiled_path_beyond_lds(mcx, orc):
+    """A 3D lattice too big for one CU's LDS runs on the tiled path."""
+    job = mcx.Job("blume_capel", 96, 0.5, 3, "heatbath", dim=3, D=0.4, seed=8, key=1)
+    print("serving: 3D L=96 Blume-Capel on the tiled path", flush=True)
+    with mcx.Server(hbm_budget=1 << 22, max_jobs=2, tile_sites=1 << 14) as srv:
+        r = srv.run([job], timeout_ms=60000)[0]
+    with pytest.raises(mcx.MCXArgumentError):
+        with mcx.Server(hbm_budget=1 << 22, max_jobs=2, tile_sites=-1) as srv:
+            srv.submit(job)
+    E, M, Q, acc, mom = _oracle(orc, job)
+    assert (r["E_pair"], r["M"], r["Q"], r["accepted"]) == (E, M, Q, acc)
+    assert np.allclose(r["moments"], mom, rtol=1e-12, atol=0)
+
+
 def test_server_quantum_size_does_not_change_the_chain(mcx, orc):
     jobs = _jobs(mcx)[:6]
     print("serving: quantum sizes", flush=True)
-    with mcx.Server(hbm_budget=1 << 20, max_jobs=16) as srv:
+    with mcx.Server(hbm_budget=1 << 20, max_jobs=16, quantum_ms=50.0) as srv:
         a = srv.run(jobs, timeout_ms=60000)
-    with mcx.Server(hbm_budget=1 << 20, max_jobs=2, quantum_attempts=3000, min_sites=64) as srv:
+        assert srv.stats()["quanta"] < 10
+    with mcx.Server(hbm_budget=1 << 20, max_jobs=2, quantum_ms=0.02, n_streams=1) as srv:
         b = srv.run(jobs, timeout_ms=60000)
+        assert srv.stats()["quanta"] > 20
     for r1, r2 in zip(a, b):
-        for f in ("E_pair", "M", "Q", "accepted"):
-            assert r1[f] == r2[f]
-        # integer-valued sums below 2^53: exact whatever the split
-        assert np.allclose(r1["moments"], r2["moments"], rtol=1e-12, atol=0)
+        _same(r1, r2)
 
 
 def test_server_matches_lattice_batch(mcx):
     print("serving: vs LatticeBatch", flush=True)
     job = mcx.Job("blume_capel", 24, 0.8, 77, "heatbath", D=0.9, seed=123, key=31)
-    with mcx.Server(hbm_budget=1 << 16, max_jobs=2, quantum_attempts=5000) as srv:
+    with mcx.Server(hbm_budget=1 << 16, max_jobs=2, quantum_ms=0.05) as srv:
         r = srv.run([job])[0]
     b = mcx.LatticeBatch("blume_capel", 2, [24], seed=123, keys=[31], D=[0.9], series_capacity=77)
     b.set_betas([0.8])
@@ -94,7 +126,8 @@ def test_server_matches_lattice_batch(mcx):
     E, M, Q = b.observables()
     _, acc = b.counters()
     assert (r["E_pair"], r["M"], r["Q"], r["accepted"]) == (E[0], M[0], Q[0], acc[0])
-    assert np.allclose(r["moments"], b.moments()[0], rtol=1e-12, atol=0)
+    # the kernels continue the moment sums sweep by sweep: bit-identical
+    assert np.array_equal(r["moments"], b.moments()[0])
     b.close()
 
 
@@ -127,7 +160,7 @@ def test_server_rejects_bad_jobs(mcx):
 
 
 def test_server_destroy_with_work_in_flight(mcx):
-    srv = mcx.Server(hbm_budget=1 << 16, max_jobs=2, quantum_attempts=4096)
+    srv = mcx.Server(hbm_budget=1 << 16, max_jobs=2, quantum_ms=0.05)
     for k in range(6):
         srv.submit(mcx.Job("ising", 32, 0.44, 400, seed=1, key=k))
     srv.close()  # stops after the current quantum; must not hang

@@ -1,0 +1,98 @@
+"""Statistical agreement of the GPU chains with exact results and with the
+reference CPU path.
+
+* 2D Ising L=16: <E> and C = beta^2 var(E) from 64 independent GPU chains
+  against Kaufman's exact finite-lattice partition function (FP64 from a
+  40-digit evaluation), within 2 sigma, Metropolis and heat bath, three betas.
+* 2D Ising L=16 at beta_c: <E>, <|m|>, C and the Binder cumulant U4 of the GPU
+  chains against the reference CPU path (random-site Metropolis with
+  Xoshiro256++, restated in C), within 2 sigma of the combined error.
+
+Errors come from the scatter of independent chains (each chain's mean is one
+sample), so the correlation time does not enter them.  Seeds are fixed.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BETA_C = 0.4406868
+THERM, MEAS, CHAINS, L = 2000, 20000, 64, 16
+
+
+def _gpu_chain_moments(mcx, alg, beta, seed):
+    b = mcx.LatticeBatch("ising", 2, L, n_chains=CHAINS, seed=seed, series_capacity=MEAS)
+    b.set_betas(np.full(CHAINS, beta))
+    b.init("random")
+    b.run(alg, THERM)
+    b.run(alg, MEAS, record=True)
+    mom = b.moments()
+    b.close()
+    assert np.all(mom[:, 0] == MEAS)
+    return mom[:, 1:] / MEAS  # per chain <E>, <E^2>, <|M|>, <M^2>, <M^4>
+
+
+def _mean_err(x):
+    x = np.asarray(x, np.float64)
+    return float(x.mean()), float(x.std(ddof=1) / math.sqrt(len(x)))
+
+
+def _jackknife(f, cols):
+    """Jackknife over chains of f(column means)."""
+    cols = [np.asarray(c, np.float64) for c in cols]
+    n = len(cols[0])
+    tot = [c.sum() for c in cols]
+    est = f(*[t / n for t in tot])
+    jk = np.array([f(*[(t - c[i]) / (n - 1) for t, c in zip(tot, cols)]) for i in range(n)])
+    return float(est), float(math.sqrt((n - 1) / n * ((jk - jk.mean()) ** 2).sum()))
+
+
+def _observables(m, beta):
+    """<E>, <|m|>, C, U4 with errors from per-chain moments m[:, 0..4]."""
+    N = L * L
+    E, E2, AM, M2, M4 = (m[:, k] for k in range(5))
+    return {
+        "E": _mean_err(E),
+        "absm": _mean_err(AM / N),
+        "C": _jackknife(lambda e, e2: beta ** 2 * (e2 - e * e), [E, E2]),
+        "U4": _jackknife(lambda m2, m4: 1 - m4 / (3 * m2 * m2), [M2, M4]),
+    }
+
+
+@pytest.mark.parametrize("alg", ["metropolis", "heatbath"])
+@pytest.mark.parametrize("beta", [0.3, BETA_C, 0.55])
+def test_l16_matches_kaufman(mcx, alg, beta):
+    from oracle import exact
+    Ek, _, Ck = exact.kaufman_observables(L, beta)
+    seed = 1000 + int(beta * 1e4) + (7 if alg == "heatbath" else 0)
+    print(f"statistics: L=16 {alg} beta={beta} vs Kaufman", flush=True)
+    m = _gpu_chain_moments(mcx, alg, beta, seed)
+    obs = _observables(m, beta)
+    (e, de), (c, dc) = obs["E"], obs["C"]
+    ze, zc = (e - Ek) / de, (c - Ck) / dc
+    print(f"  <E> {e:.4f} +- {de:.4f} (exact {Ek:.4f}, z={ze:+.2f});"
+          f" C {c:.3f} +- {dc:.3f} (exact {Ck:.3f}, z={zc:+.2f})", flush=True)
+    assert abs(ze) < 2 and abs(zc) < 2
+
+
+def test_l16_matches_reference_cpu_path(mcx, orc):
+    beta = BETA_C
+    print("statistics: L=16 GPU checkerboard vs reference CPU path", flush=True)
+    gpu = _observables(_gpu_chain_moments(mcx, "metropolis", beta, 4242), beta)
+    n_cpu = 16
+    rows = []
+    for k in range(n_cpu):
+        s = np.ones(L * L, np.int8)
+        orc.refpath_series_ising2d(L, beta, 500 + k, THERM, s)
+        E, M, _ = orc.refpath_series_ising2d(L, beta, 900 + k, MEAS, s)
+        e, mm = E.astype(np.float64), M.astype(np.float64)
+        rows.append([e.mean(), (e * e).mean(), np.abs(mm).mean(), (mm * mm).mean(),
+                     ((mm * mm) ** 2).mean()])
+    cpu = _observables(np.array(rows), beta)
+    for k in ("E", "absm", "C", "U4"):
+        (a, da), (b, db) = gpu[k], cpu[k]
+        z = (a - b) / math.hypot(da, db)
+        print(f"  {k}: gpu {a:.5f} +- {da:.5f}  cpu {b:.5f} +- {db:.5f}  z={z:+.2f}", flush=True)
+        assert abs(z) < 2, k
