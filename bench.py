@@ -44,6 +44,10 @@ def parse():
     p.add_argument("--plain-chains", type=int, default=16)
     p.add_argument("--plain-sweeps", type=int, default=20)
     p.add_argument("--no-serving", action="store_true", help="skip the serving stream")
+    p.add_argument("--no-kernels", action="store_true",
+                   help="skip the per-family lines (replica exchange, Blume-Capel, walkers)")
+    p.add_argument("--pt-rounds", type=int, default=2000,
+                   help="exchange rounds of the 256-set replica-exchange line (10 sweeps each)")
     p.add_argument("--serving-jobs", type=int, default=200,
                    help="jobs of the Poisson serving stream (200/s, seed 19)")
     p.add_argument("--serving-budget", type=int, default=1 << 28,
@@ -117,6 +121,118 @@ def serving_stream(mcx, n_jobs, budget):
         "offered_attempts_per_ns": round(work / (jobs[-1].arrival_s * 1e9), 3),
         "wall_s": round(wall, 3), "quanta": st["quanta"], "hbm_budget": budget,
     }
+
+
+def _roof(attempts, ms, bytes_per_attempt=BYTES_PER_ATTEMPT):
+    ach = bytes_per_attempt * attempts / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_attempt": bytes_per_attempt}
+
+
+def _cpu(oracle, threads, sample, **kw):
+    t0 = time.perf_counter()
+    n = oracle.refpath_run(**kw, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": round(n / (dt * 1e9), 4), "unit": "attempts/ns", "cores": threads,
+            "kind": "port", "sample": sample, "seconds": round(dt, 2)}
+
+
+def kernel_lines(mcx, np, oracle, pt_rounds, threads, cpu):
+    """One line per kernel family beyond the headline: device time from HIP
+    events around the library call, 2 algorithmic bytes per attempt, and the
+    reference CPU path (C restatement, random-site, Xoshiro256++) on a bounded
+    sample of the same systems."""
+    from montecarlox.ensembles import DiscreteBinning
+    from montecarlox.spinsystems import SpinSystemBatch
+    out = {}
+
+    # replica exchange: 256 sets x 64 replicas of 3D Ising, L per set from
+    # {16, 24, 32, 48} (numpy PCG64(11)), ladder set_betas(64, 0.20, 0.24,
+    # geometric), 10 sweeps per exchange round
+    R, S = 64, 256
+    set_L = np.random.Generator(np.random.PCG64(11)).choice(np.array([16, 24, 32, 48]), size=S)
+    Ls = np.repeat(set_L, R)
+    sys_ = SpinSystemBatch("ising", [16] * 3, n=len(Ls), L=Ls)
+    pt = mcx.ParallelTempering(sys_, mcx.set_betas(R, 0.20, 0.24, "geometric"), n_sets=S, seed=11)
+    sys_.init_("random", [a.rng for a in pt.algs])
+    pt.run_(2, 10)
+    ms = pt.run_(pt_rounds, 10)
+    att = float((Ls.astype(np.int64) ** 3).sum()) * pt_rounds * 10
+    acc = float(np.mean(mcx.acceptance_rates(pt)))
+    line = {"workload": f"{S} replica-exchange sets x {R} replicas, 3D Ising, L per set from "
+                        "{16,24,32,48} (PCG64 seed 11: " +
+                        ",".join(f"{L}:{int((set_L == L).sum())}" for L in (16, 24, 32, 48)) +
+                        "), set_betas(64, 0.20, 0.24, geometric), checkerboard Metropolis, "
+                        f"{pt_rounds} rounds of 10 sweeps + one exchange",
+            "value": round(att / (ms * 1e6), 3), "unit": "attempts/ns", "ms": round(ms, 2),
+            "exchange_acceptance": round(acc, 4), "roofline": _roof(att, ms)}
+    pt.close()
+    sys_.batch.close()
+    if cpu:
+        line["cpu_baseline"] = _cpu(oracle, threads,
+                                    "one replica per set (256 chains, the same L mix) x 40 sweeps of "
+                                    "random-site Metropolis on Ising([L,L,L])",
+                                    model=0, dim=3, Ls=set_L, betas=0.22, seed=11, nsweeps=40)
+    out["replica_exchange"] = line
+
+    # Blume-Capel canonical: 4096 chains L=64, D=1, T uniform [1.5, 3.5]
+    n, L, sw = 4096, 64, 1000
+    b = mcx.LatticeBatch("blume_capel", 2, L, n_chains=n, seed=5, D=1.0, series_capacity=2 * sw)
+    b.set_betas(1.0 / (1.5 + 2.0 * np.arange(n) / (n - 1)))
+    b.init("random")
+    b.run("metropolis", sw, record=True)
+    ms = b.run("metropolis", sw, record=True)
+    att = float(n) * L * L * sw
+    line = {"workload": f"{n} Blume-Capel chains, L={L}, D=1, T uniform [1.5, 3.5], checkerboard "
+                        f"Metropolis, {sw} sweeps with per-sweep E/M/Q series + FP64 moments",
+            "value": round(att / (ms * 1e6), 3), "unit": "attempts/ns", "ms": round(ms, 2),
+            "roofline": _roof(att, ms)}
+    b.close()
+    if cpu:
+        line["cpu_baseline"] = _cpu(oracle, threads, "256 chains L=64 x 300 sweeps of random-site "
+                                    "Metropolis on BlumeCapel([L,L]; D=1)",
+                                    model=1, dim=2, Ls=[64] * 256, betas=0.4, D=1.0, seed=5,
+                                    nsweeps=300)
+    out["blume_capel"] = line
+
+    # multicanonical / Wang-Landau walkers: 1024 Blume-Capel walkers, 256 each
+    # at L = 16/32/48/64, D = 1, bins -N:1:3N (4N+1), one iteration of sweeps
+    for kind, sw in (("multicanonical", 2000), ("wang_landau", 2000)):
+        Lw = np.repeat(np.array([16, 32, 48, 64]), 256)
+        sys_ = SpinSystemBatch("blume_capel", [16, 16], n=len(Lw), L=Lw, D=1.0)
+        algs = []
+        for k, L in enumerate(Lw):
+            N = int(L) * int(L)
+            bo = mcx.BinnedObject(DiscreteBinning(-N, 1, 4 * N + 1), 0.0)
+            # canonical-like weights lw = -0.5 E, so moves are not all accepted
+            bo.values[:] = -0.5 * (np.arange(4 * N + 1) - N)
+            if kind == "multicanonical":
+                algs.append(mcx.Multicanonical(mcx.Philox(3, k), bo))
+            else:
+                algs.append(mcx.WangLandau(mcx.Philox(3, k), bo, logf=1e-3))
+        sys_.init_("random", [a.rng for a in algs])
+        w = mcx.GeneralizedWalkers(sys_, algs)
+        w.run_(10)
+        ms = w.run_(sw)
+        att = float((Lw.astype(np.int64) ** 2).sum()) * sw
+        line = {"workload": f"1024 Blume-Capel {kind.replace('_', '-')} walkers (256 each at "
+                            "L=16/32/48/64), D=1, 4N+1 bins from -N, weights -0.5 E, one wave "
+                            f"per walker in site order, {sw} sweeps",
+                "value": round(att / (ms * 1e6), 3), "unit": "attempts/ns", "ms": round(ms, 2),
+                "acceptance": round(sum(a.accepted for a in algs) / max(1, sum(a.steps for a in algs)), 4),
+                "roofline": _roof(att, ms)}
+        w.close()
+        sys_.batch.close()
+        if cpu and kind == "multicanonical":
+            N = 64 * 64
+            line["cpu_baseline"] = _cpu(oracle, threads, "64 walkers L=64 x 100 sweeps of random-site "
+                                        "multicanonical Blume-Capel (D=1, 4N+1 bins)",
+                                        model=1, dim=2, Ls=[64] * 64, betas=0.0, D=1.0, seed=3,
+                                        nsweeps=100, lw=-0.5 * (np.arange(4 * N + 1) - N),
+                                        bin_start=-N, bin_step=1)
+        out[kind] = line
+    return out
 
 
 def main():
@@ -218,6 +334,11 @@ def main():
                                          "Metropolis (C restatement of spin_flip!(IsingLatticeOptim, "
                                          "Metropolis) with Xoshiro256++), one thread per chain group",
                                "seconds": round(cw, 2)}
+
+    if not a.no_kernels:
+        import oracle
+        out["kernels"] = kernel_lines(mcx, np, oracle, a.pt_rounds, min(16, os.cpu_count() or 1),
+                                      not a.no_cpu)
 
     if not a.no_serving and a.serving_jobs > 0:
         out["serving"] = serving_stream(mcx, a.serving_jobs, a.serving_budget)

@@ -5,18 +5,6 @@
 
 namespace mcx {
 
-// One staged attempt: what a lane precomputes for one site of a 64-site chunk.
-struct AttemptRec {
-    double dEf;     // Blume-Capel energy change (FP64)
-    uint32_t w;     // site word
-    int32_t site;
-    int32_t dE;     // Ising energy change
-    int32_t dpair;  // change of sum_<ij> s_i s_j
-    int16_t dm, dq; // change of M and of Q = sum s^2
-    int8_t snew;
-    int8_t pad[7];
-};
-
 struct GenArgs {
     const int32_t* items;   // walkers of this launch (all from one binning group)
     int32_t n_items;
@@ -27,6 +15,9 @@ struct GenArgs {
     int32_t hist_in_lds;    // 1: uint32 histogram staged in LDS, else int64 atomics in HBM
     int32_t pad;
     double bin_start, bin_step;
+    double inv_step;        // 1 / bin_step when step_pow2
+    int32_t step_pow2;      // bin_step is a power of two (multiply / shift instead of divide)
+    int32_t step_shift;     // log2(bin_step) when step_pow2 and bin_step >= 1 is an integer
     int64_t nbins;
     ChainRec* chains;
     int8_t* spins;

@@ -11,32 +11,50 @@
 // increasing site index, then colour 1).
 //
 // One wavefront per walker.  Sites of one colour never neighbour each other,
-// so for 64 consecutive sites of the colour the lanes compute dE, the proposal
-// and the uniform in parallel and stage them in LDS; the wave then walks the
-// 64 attempts in order.  The walker's FP64 log-weights live in LDS; the weight
-// of the current bin is kept in a register, so an attempt costs one LDS load.
+// so for 64 consecutive sites of the colour the lanes compute dE, the proposal,
+// the uniform and its log in parallel and keep them in registers; the wave then
+// walks the 64 attempts in order, reading each from its lane (v_readlane).  The
+// walker's FP64 log-weights live in LDS; the weight and bin of the current
+// energy are kept in registers, so an attempt costs one LDS load and no exp
+// unless lr falls within 1e-9 of log(u).  Spin writes and visit counts (order
+// does not matter for either) are applied by the lanes after the walk.
 // Histograms are uint32 in LDS when they fit, otherwise int64 atomics in HBM.
 #include "mcx_device.h"
 #include "mcx_gen.h"
 
 namespace mcx {
 
+__device__ __forceinline__ int readlane_i32(int v, int k)
+{
+    return __builtin_amdgcn_readlane(v, k);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int k)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, k);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), k);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 __device__ __forceinline__ int gen_propose(int s_old, int alt)
 {
     return s_old == -1 ? (alt ? 0 : 1) : (s_old == 0 ? (alt ? -1 : 1) : (alt ? -1 : 0));
 }
 
-template <int MODEL, int DIM>
+template <int MODEL, int DIM, bool WL>
 __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    AttemptRec* rec = reinterpret_cast<AttemptRec*>(lds_raw);        // 64 records
-    double* lw = reinterpret_cast<double*>(lds_raw + 64 * sizeof(AttemptRec));
+    double* lw = reinterpret_cast<double*>(lds_raw);
+    // one chunk's staged attempts and the lane-0 walk's results
+    __shared__ int st_dE[64], st_dp[64], st_mq[64], st_site[64], st_res[64];
+    __shared__ double st_luh[64], st_lul[64], st_u[64];
     uint32_t* hl = reinterpret_cast<uint32_t*>(lw + a.nbins);
     int8_t* sp = reinterpret_cast<int8_t*>(hl + (a.hist_in_lds ? a.nbins : 0));
     const int lane = threadIdx.x;
     const int64_t nb = a.nbins;
-    const bool wl = a.kind == 3;
+    constexpr bool wl = WL;
 
     for (int item = blockIdx.x; item < a.n_items; item += gridDim.x) {
         const int wk = a.items[item];
@@ -61,7 +79,7 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
         __syncthreads();
 
         // full recompute of the cached sums (_recompute_cached!)
-        long long pair = 0, mm = 0, qq = 0;
+        long long pair0 = 0, mm0 = 0, qq0 = 0;
         for (int64_t i = lane; i < N; i += 64) {
             const int x = (int)(i % L);
             const int y = (int)((i / L) % L);
@@ -72,40 +90,91 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
             nbs += sp[zb + (int64_t)(y + 1 == L ? 0 : y + 1) * L + x];
             if (DIM == 3)
                 nbs += sp[(int64_t)(z + 1 == L ? 0 : z + 1) * L * L + (int64_t)y * L + x];
-            pair += s * nbs;
-            mm += s;
-            qq += s * s;
+            pair0 += s * nbs;
+            mm0 += s;
+            qq0 += s * s;
         }
-        pair = wave_sum(pair);
-        mm = wave_sum(mm);
-        qq = wave_sum(qq);
+        // the running sums fit 32 bits (|pair| <= 3N, N < 2^31 / 3); after the
+        // reduction they are wave-uniform and live in scalar registers
+        int pair = __builtin_amdgcn_readfirstlane((int)wave_sum(pair0));
+        int mm = __builtin_amdgcn_readfirstlane((int)wave_sum(mm0));
+        int qq = __builtin_amdgcn_readfirstlane((int)wave_sum(qq0));
         const double D = cr.D;
 
-        auto bin_of = [&](long long Ei, double Ef, int64_t& b) -> bool {
-            if (MODEL == 0) {
-                const long long d = Ei - (long long)a.bin_start;
-                const long long bb = d / (long long)a.bin_step; // truncating div
-                b = bb;
-                return bb >= 0 && bb < nb;
-            }
-            const double v = rint((Ef - a.bin_start) / a.bin_step);
-            if (!(v >= 0.0 && v < (double)nb))
-                return false;
-            b = (int64_t)v;
-            return true;
+        // bin index as _binindex evaluates it (binned_object.jl:18-24): integer
+        // energies (Ising) truncating div, FP64 energies round-half-even.  A
+        // power-of-two step divides exactly as a multiply / shift.
+        const bool step_pow2 = a.step_pow2 != 0;
+        // (the walk's state is wave-uniform: bins and decisions are moved to
+        // scalar registers so that its branches are scalar branches)
+        auto bin_int = [&](long long Ei, int& b) -> bool {
+            const long long d = Ei - (long long)a.bin_start;
+            long long bb;
+            if (step_pow2)
+                bb = d >= 0 ? (d >> a.step_shift) : -((-d) >> a.step_shift);
+            else
+                bb = d / (long long)a.bin_step; // truncating div
+            const bool in = bb >= 0 && bb < nb;
+            b = __builtin_amdgcn_readfirstlane(in ? (int)bb : 0);
+            return __builtin_amdgcn_readfirstlane(in ? 1 : 0) != 0;
+        };
+        auto bin_fp = [&](double Ef, int& b) -> bool {
+            const double q = step_pow2 ? (Ef - a.bin_start) * a.inv_step
+                                       : (Ef - a.bin_start) / a.bin_step;
+            const double v = rint(q);
+            const bool in = v >= 0.0 && v < (double)nb;
+            b = __builtin_amdgcn_readfirstlane(in ? (int)v : 0);
+            return __builtin_amdgcn_readfirstlane(in ? 1 : 0) != 0;
+        };
+
+        // Integer-energy walk: Ising always; Blume-Capel when D and the bin start
+        // are integers and the step is 1, so that every FP64 energy is an exact
+        // integer and round((E - start) / 1) = E - start -- the same bins and
+        // the same chain as the FP64 walk, from scalar integer arithmetic.
+        const double Dr = rint(D);
+        // (32-bit: energies, offsets from the bin start and D * Q stay below 2^29)
+        const double lim = 536870912.0;
+        const bool int_e = fabs(a.bin_start) < lim && 3.0 * (double)N < lim &&
+                           (MODEL == 0 ? step_pow2
+                                       : (Dr == D && fabs(D) * (double)N < lim && a.bin_step == 1.0 &&
+                                          rint(a.bin_start) == a.bin_start));
+        const long long Di = (long long)Dr;
+        const long long start_i = (long long)a.bin_start;
+        const int start32 = (int)start_i;
+        const int smask = step_pow2 ? (1 << a.step_shift) - 1 : 0;
+        const unsigned nbu = (unsigned)nb;
+        auto ibin = [&](long long E, int& b) -> bool {
+            if (MODEL == 0)
+                return bin_int(E, b);
+            const long long d = E - start_i;
+            const bool in = d >= 0 && d < nb;
+            b = in ? (int)d : 0;
+            return in;
         };
 
         int64_t err = -1;
-        int64_t b_cur = -1;  // bin whose weight lw_cur holds (none yet)
+        int b_cur = -1;  // bin whose weight lw_cur holds (none yet)
         double lw_cur = 0.0;
         unsigned long long steps = 0, nacc = 0;
+        // energy of the cached sums (_cached_energy) and its bin; recomputed
+        // only when a move is accepted
+        double Eo = -(double)pair + D * (double)qq;
+        long long Ei = -(long long)pair + (MODEL == 0 ? 0 : Di * (long long)qq);
+        int b_old = 0;
+        bool old_ok = int_e ? ibin(Ei, b_old) : bin_fp(Eo, b_old);
         int64_t s = 0;
         for (; s < a.nsweeps && err < 0; ++s) {
             const uint64_t sweep = cr.sweep + (uint64_t)s;
             for (int c = 0; c < 2 && err < 0; ++c) {
                 const int n_col = rows * half;
                 for (int base = 0; base < n_col && err < 0; base += 64) {
+                    // ---- stage: lane k prepares attempt base + k (the sites of one
+                    // colour are independent given the other colour) ----
                     const int o = base + lane;
+                    int t_site = 0, t_snew = 0, t_dE = 0, t_dpair = 0, t_dm = 0, t_dq = 0;
+                    double t_dEf = 0.0, t_u = 0.0, t_lu = 0.0;
+                    int t_mq = 8 | (8 << 16); // packed dm+8 | (dq+8) << 16
+                    double t_luh = INFINITY, t_lul = -INFINITY;
                     if (o < n_col) {
                         const int r = o / half;
                         const int j = o - r * half;
@@ -122,56 +191,212 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                                  + sp[(int64_t)(z == L - 1 ? 0 : z + 1) * L * L + (int64_t)y * L + x];
                         const int si = sp[rowb + x];
                         const uint32_t g = (uint32_t)((r * 2 + c) * nw + (j >> 5));
-                        AttemptRec t;
-                        t.w = site_word(cr.seed, cr.key, g, j & 31, sweep, P_GEN);
-                        t.site = (int32_t)(rowb + x);
+                        const uint32_t w = site_word(cr.seed, cr.key, g, j & 31, sweep, P_GEN);
+                        t_site = (int)(rowb + x);
                         if (MODEL == 0) {
-                            t.dE = 2 * si * nbs;
-                            t.dpair = -2 * si * nbs;
-                            t.dm = (int16_t)(-2 * si);
-                            t.dq = 0;
-                            t.snew = (int8_t)-si;
-                            t.dEf = 0.0;
+                            t_dE = 2 * si * nbs;
+                            t_dpair = -2 * si * nbs;
+                            t_dm = -2 * si;
+                            t_snew = -si;
+                            t_u = (double)w * 0x1.0p-32;
                         } else {
-                            const int sn = gen_propose(si, (int)(t.w & 1u));
+                            const int sn = gen_propose(si, (int)(w & 1u));
                             const int ds = sn - si;
                             const int ds2 = sn * sn - si * si;
-                            t.dEf = (-((double)ds * (1.0 * (double)nbs)) - 0.0) + D * (double)ds2;
-                            t.dE = 0;
-                            t.dpair = ds * nbs;
-                            t.dm = (int16_t)ds;
-                            t.dq = (int16_t)ds2;
-                            t.snew = (int8_t)sn;
+                            t_dEf = (-((double)ds * (1.0 * (double)nbs)) - 0.0) + D * (double)ds2;
+                            t_dE = int_e ? -(ds * nbs) + (int)Di * ds2 : 0;
+                            t_dpair = ds * nbs;
+                            t_dm = ds;
+                            t_dq = ds2;
+                            t_snew = sn;
+                            t_u = (double)(w >> 1) * 0x1.0p-31;
                         }
-                        rec[lane] = t;
+                        t_lu = log(t_u);
+                        // u < exp(lr) is certain for lr > log(u) + 1e-9 and false for
+                        // lr < log(u) - 1e-9 (log and exp err by ~1e-16 relative)
+                        t_luh = t_lu > -700.0 ? t_lu + 1e-9 : INFINITY;
+                        t_lul = t_lu - 1e-9;
+                        t_mq = (t_dm + 8) | ((t_dq + 8) << 16);
                     }
-                    __syncthreads();
                     const int cnt = n_col - base < 64 ? n_col - base : 64;
+                    // ---- walk the attempts in site order (wave-uniform state) ----
+                    int my_vis = -1, my_ok = 0, n_done = cnt;
+                    if (int_e) {
+                        // Integer walk on lane 0 alone: its state stays in vector
+                        // registers and every step is a VALU or LDS operation (no
+                        // scalar round trips, no per-attempt branches).  The
+                        // walker always sits in the bin it last visited, so
+                        // lw_cur holds that bin's weight; attempt k+1's staged
+                        // values and the weights of its proposed bin under both
+                        // outcomes of attempt k are loaded while k decides.  An
+                        // attempt out of the binned domain, or one whose lr lies
+                        // in the 1e-9 band around log(u), stops the fast loop; it
+                        // is settled exactly (with exp) and the loop resumes.
+                        st_dE[lane] = t_dE;
+                        st_dp[lane] = t_dpair;
+                        st_mq[lane] = t_mq;
+                        st_luh[lane] = t_luh;
+                        st_lul[lane] = t_lul;
+                        st_u[lane] = t_u;
+                        st_site[lane] = t_site;
+                        __syncthreads();
+                        int err_v = -1;
+                        if (lane == 0) {
+                            // bin of an energy (32-bit): Ising floor-free div by 2^s
+                            // rounding toward zero as div does; Blume-Capel E - start
+                            auto bin32 = [&](int E) -> int {
+                                const int d = E - start32;
+                                return MODEL == 0 ? (d + ((d >> 31) & smask)) >> a.step_shift : d;
+                            };
+                            auto binc = [&](int E) -> int { // clamped into the domain
+                                const int b = bin32(E);
+                                return (unsigned)b < nbu ? b : 0;
+                            };
+                            int E32 = (int)Ei;
+                            if (b_cur != b_old && old_ok) {
+                                lw_cur = lw[b_old];
+                                b_cur = b_old;
+                            }
+                            int kk = 0;
+                            if (!old_ok) {
+                                // the walker's current energy lies outside the binned
+                                // domain: its first attempt fails
+                                n_done = 0;
+                                err_v = st_site[0];
+                                kk = cnt;
+                            }
+                            while (kk < cnt) {
+                                int stop = cnt;
+                                bool alive = true;
+                                int dE_c = st_dE[kk], dp_c = st_dp[kk], mq_c = st_mq[kk];
+                                double luh_c = st_luh[kk], lul_c = st_lul[kk];
+                                double lw_pre = lw[binc(E32 + dE_c)];
+                                int pair_d = 0, mq_d = 0, n_go = 0, n_ok = 0;
+                                for (int k = kk; k < cnt; ++k) {
+                                    const int kn = k + 1 < cnt ? k + 1 : k;
+                                    const int dE_n = st_dE[kn], dp_n = st_dp[kn], mq_n = st_mq[kn];
+                                    const double luh_n = st_luh[kn], lul_n = st_lul[kn];
+                                    const int En = E32 + dE_c;
+                                    const int bb = bin32(En);
+                                    const bool okn = (unsigned)bb < nbu;
+                                    const int b_new = okn ? bb : 0;
+                                    const double lw_a = lw[binc(En + dE_n)];
+                                    const double lw_r = lw[binc(E32 + dE_n)];
+                                    const double lw_new = b_new == b_old ? lw_cur : lw_pre;
+                                    const double lr = lw_new - lw_cur;
+                                    const bool acc_c = lr > 0 || lr > luh_c;
+                                    const bool go = alive && okn && (acc_c || lr < lul_c);
+                                    const bool ok = go && acc_c;
+                                    stop = alive && !go ? k : stop;
+                                    alive = go;
+                                    n_go += go ? 1 : 0;
+                                    n_ok += ok ? 1 : 0;
+                                    E32 = ok ? En : E32;
+                                    b_old = ok ? b_new : b_old;
+                                    lw_cur = ok ? lw_new : lw_cur;
+                                    pair_d += ok ? dp_c : 0;
+                                    mq_d += ok ? mq_c : (8 | (8 << 16));
+                                    st_res[k] = b_old | (ok ? (int)0x80000000 : 0);
+                                    if (WL) {
+                                        // unchanged value when the attempt did not run
+                                        lw_cur = go ? lw_cur - logf : lw_cur;
+                                        lw[b_old] = lw_cur;
+                                    }
+                                    lw_pre = ok ? lw_a : lw_r;
+                                    dE_c = dE_n;
+                                    dp_c = dp_n;
+                                    mq_c = mq_n;
+                                    luh_c = luh_n;
+                                    lul_c = lul_n;
+                                }
+                                const int iters = cnt - kk;
+                                steps += (unsigned)n_go;
+                                nacc += (unsigned)n_ok;
+                                pair += pair_d;
+                                mm += (mq_d & 0xffff) - 8 * iters;
+                                qq += (mq_d >> 16) - 8 * iters;
+                                b_cur = b_old;
+                                if (stop == cnt)
+                                    break;
+                                // the attempt that stopped the fast loop, exactly
+                                const int En = E32 + st_dE[stop];
+                                const int bb = bin32(En);
+                                if (!((unsigned)bb < nbu)) {
+                                    n_done = stop;
+                                    err_v = st_site[stop];
+                                    break;
+                                }
+                                const double lw_new = bb == b_old ? lw_cur : lw[bb];
+                                const double lr = lw_new - lw_cur;
+                                const bool ok = lr > 0 || st_u[stop] < exp(lr);
+                                ++steps;
+                                nacc += ok ? 1u : 0u;
+                                if (ok) {
+                                    const int mq = st_mq[stop];
+                                    E32 = En;
+                                    b_old = bb;
+                                    lw_cur = lw_new;
+                                    pair += st_dp[stop];
+                                    mm += (mq & 0xffff) - 8;
+                                    qq += (mq >> 16) - 8;
+                                }
+                                st_res[stop] = b_old | (ok ? (int)0x80000000 : 0);
+                                if (WL) {
+                                    lw_cur = lw_cur - logf;
+                                    lw[b_old] = lw_cur;
+                                }
+                                b_cur = b_old;
+                                kk = stop + 1;
+                            }
+                            Ei = E32;
+                        }
+                        n_done = __builtin_amdgcn_readfirstlane(n_done);
+                        err_v = __builtin_amdgcn_readfirstlane(err_v);
+                        if (err_v >= 0)
+                            err = err_v;
+                        __syncthreads();
+                        if (lane < n_done) {
+                            const int r = st_res[lane];
+                            my_vis = r & 0x7fffffff;
+                            my_ok = (r >> 31) & 1;
+                        }
+                    } else
                     for (int k = 0; k < cnt; ++k) {
-                        const AttemptRec t = rec[k];
-                        // E_old from the cached sums, as _cached_energy evaluates it
-                        const double Eo = -(double)pair + D * (double)qq;
-                        int64_t b_new, b_old;
-                        const bool okn = bin_of(-pair + t.dE, Eo + t.dEf, b_new);
-                        const bool oko = bin_of(-pair, Eo, b_old);
-                        if (!okn || !oko) {
-                            err = t.site;
+                        int b_new;
+                        bool okn;
+                        if (MODEL == 0)
+                            okn = bin_int(-(long long)pair + readlane_i32(t_dE, k), b_new);
+                        else
+                            okn = bin_fp(Eo + readlane_f64(t_dEf, k), b_new);
+                        if (!okn || !old_ok) {
+                            err = readlane_i32(t_site, k);
+                            n_done = k;
                             break;
                         }
                         const double lw_old = b_old == b_cur ? lw_cur : lw[b_old];
                         const double lw_new = b_new == b_old ? lw_old : lw[b_new];
                         const double lr = lw_new - lw_old;
-                        const double u = MODEL == 0 ? (double)t.w * 0x1.0p-32
-                                                    : (double)(t.w >> 1) * 0x1.0p-31;
-                        const bool ok = lr > 0 || u < exp(lr);
+                        // lr > 0 || u < exp(lr); log(u) from the staging settles
+                        // all but a 1e-9-wide band, where exp decides exactly
+                        bool ok = __builtin_amdgcn_readfirstlane(lr > 0 ? 1 : 0) != 0;
+                        if (!ok) {
+                            const double lu = readlane_f64(t_lu, k);
+                            // 1 = certain accept, 0 = certain reject, 2 = evaluate exp
+                            const int cls = lr > lu + 1e-9 && lu > -700.0 ? 1
+                                          : (lr < lu - 1e-9 ? 0 : 2);
+                            const int clsu = __builtin_amdgcn_readfirstlane(cls);
+                            if (clsu == 2)
+                                ok = __builtin_amdgcn_readfirstlane(
+                                         readlane_f64(t_u, k) < exp(lr) ? 1 : 0) != 0;
+                            else
+                                ok = clsu == 1;
+                        }
                         ++steps;
                         nacc += ok ? 1u : 0u;
-                        const int64_t vis = ok ? b_new : b_old;
-                        if (lane == 0) {
-                            if (a.hist_in_lds)
-                                hl[vis] += 1u;
-                            else
-                                atomicAdd(&hist_g[vis], 1ull);
+                        const int vis = ok ? b_new : b_old;
+                        if (lane == k) {
+                            my_vis = vis;
+                            my_ok = ok ? 1 : 0;
                         }
                         double lw_vis = ok ? lw_new : lw_old;
                         if (wl) {
@@ -180,14 +405,27 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                                 lw[vis] = lw_vis;
                         }
                         if (ok) {
-                            if (lane == 0)
-                                sp[t.site] = t.snew;
-                            pair += t.dpair;
-                            mm += t.dm;
-                            qq += t.dq;
+                            pair += readlane_i32(t_dpair, k);
+                            mm += readlane_i32(t_dm, k);
+                            qq += readlane_i32(t_dq, k);
+                            if (MODEL == 0) {
+                                b_old = b_new;
+                            } else {
+                                Eo = -(double)pair + D * (double)qq;
+                                old_ok = bin_fp(Eo, b_old);
+                            }
                         }
                         b_cur = vis;
                         lw_cur = lw_vis;
+                    }
+                    // ---- deferred, in parallel: spins and visit counts (order-free) ----
+                    if (lane < n_done) {
+                        if (my_ok)
+                            sp[t_site] = (int8_t)t_snew;
+                        if (a.hist_in_lds)
+                            atomicAdd(&hl[my_vis], 1u);
+                        else
+                            atomicAdd(&hist_g[my_vis], 1ull);
                     }
                     __syncthreads();
                 }
@@ -220,17 +458,26 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
     }
 }
 
-hipError_t launch_gen_sweeps(const GenArgs& a, size_t lds_bytes, hipStream_t st)
+template <int M, int D>
+static void gen_launch(const GenArgs& a, size_t lds_bytes, hipStream_t st)
 {
     const dim3 grid(a.n_items), block(64);
-    if (a.model == 0 && a.dim == 2)
-        hipLaunchKernelGGL((k_gen_sweeps<0, 2>), grid, block, lds_bytes, st, a);
-    else if (a.model == 0)
-        hipLaunchKernelGGL((k_gen_sweeps<0, 3>), grid, block, lds_bytes, st, a);
-    else if (a.dim == 2)
-        hipLaunchKernelGGL((k_gen_sweeps<1, 2>), grid, block, lds_bytes, st, a);
+    if (a.kind == 3)
+        hipLaunchKernelGGL((k_gen_sweeps<M, D, true>), grid, block, lds_bytes, st, a);
     else
-        hipLaunchKernelGGL((k_gen_sweeps<1, 3>), grid, block, lds_bytes, st, a);
+        hipLaunchKernelGGL((k_gen_sweeps<M, D, false>), grid, block, lds_bytes, st, a);
+}
+
+hipError_t launch_gen_sweeps(const GenArgs& a, size_t lds_bytes, hipStream_t st)
+{
+    if (a.model == 0 && a.dim == 2)
+        gen_launch<0, 2>(a, lds_bytes, st);
+    else if (a.model == 0)
+        gen_launch<0, 3>(a, lds_bytes, st);
+    else if (a.dim == 2)
+        gen_launch<1, 2>(a, lds_bytes, st);
+    else
+        gen_launch<1, 3>(a, lds_bytes, st);
     return hipGetLastError();
 }
 

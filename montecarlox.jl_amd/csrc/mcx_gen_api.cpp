@@ -26,12 +26,25 @@ struct mcx_gen {
     int32_t* d_members = nullptr;
     double* d_flat = nullptr;
     std::vector<double> logf;
+    // groups run side by side on their own streams (forked from / joined to
+    // the context stream)
+    static constexpr int kStreams = 4;
+    hipStream_t gs[kStreams] = {};
+    hipEvent_t fork = nullptr, join[kStreams] = {};
 };
 
 namespace {
 
 void gen_free(mcx_gen* g)
 {
+    for (int k = 0; k < mcx_gen::kStreams; ++k) {
+        if (g->gs[k])
+            hipStreamDestroy(g->gs[k]);
+        if (g->join[k])
+            hipEventDestroy(g->join[k]);
+    }
+    if (g->fork)
+        hipEventDestroy(g->fork);
     hipFree(g->d_lw);
     hipFree(g->d_hist);
     hipFree(g->d_off);
@@ -61,9 +74,13 @@ mcx_status check_group(mcx_gen* g, int32_t group, const char* who)
     return MCX_OK;
 }
 
+// static LDS of the walker kernel: one chunk's staged attempts (64 x 5 ints + 3 doubles)
+constexpr size_t kGenStaticLds = 64 * (5 * 4 + 3 * 8);
+
+// dynamic LDS of one walker: weights, optional histogram, spins
 size_t gen_lds(int64_t nbins, int64_t nsites, bool hist_in_lds)
 {
-    return 64 * sizeof(AttemptRec) + 8 * (size_t)nbins + (hist_in_lds ? 4 * (size_t)nbins : 0)
+    return 8 * (size_t)nbins + (hist_in_lds ? 4 * (size_t)nbins : 0)
          + (size_t)((nsites + 15) / 16 * 16);
 }
 
@@ -118,7 +135,7 @@ extern "C" mcx_status mcx_gen_create(mcx_batch* b, const mcx_gen_desc* d, mcx_ge
                 delete g;
                 return fail(MCX_EINVAL, "mcx_gen_create: group %d mixes lattice sizes", k);
             }
-        if (gen_lds(g->nbins[k], b->nsites[g->members[k][0]], false) > b->ctx->lds_max) {
+        if (gen_lds(g->nbins[k], b->nsites[g->members[k][0]], false) + kGenStaticLds > b->ctx->lds_max) {
             delete g;
             return fail(MCX_EINVAL, "mcx_gen_create: group %d: %lld bins do not fit the device LDS",
                         k, (long long)g->nbins[k]);
@@ -151,6 +168,13 @@ extern "C" mcx_status mcx_gen_create(mcx_batch* b, const mcx_gen_desc* d, mcx_ge
     alloc((void**)&g->d_err, sizeof(int64_t) * b->n);
     alloc((void**)&g->d_members, sizeof(int32_t) * b->n);
     alloc((void**)&g->d_flat, sizeof(double) * 3);
+    for (int k = 0; k < mcx_gen::kStreams && e == hipSuccess; ++k) {
+        e = hipStreamCreateWithFlags(&g->gs[k], hipStreamNonBlocking);
+        if (e == hipSuccess)
+            e = hipEventCreateWithFlags(&g->join[k], hipEventDisableTiming);
+    }
+    if (e == hipSuccess)
+        e = hipEventCreateWithFlags(&g->fork, hipEventDisableTiming);
     hipStream_t st = b->ctx->stream;
     if (e == hipSuccess)
         e = hipMemsetAsync(g->d_lw, 0, sizeof(double) * tot, st);
@@ -278,7 +302,19 @@ extern "C" mcx_status mcx_gen_run(mcx_gen* g, int64_t nsweeps, uint32_t flags)
     hipStream_t st = b->ctx->stream;
     HIPCHK(hipMemsetAsync(g->d_err, 0xff, sizeof(int64_t) * b->n, st));
     HIPCHK(hipEventRecord(b->ctx->ev0, st));
-    for (int k = 0; k < g->n_groups; ++k) {
+    // groups by decreasing lattice size: the longest walks start first
+    std::vector<int> order(g->n_groups);
+    for (int k = 0; k < g->n_groups; ++k)
+        order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return b->nsites[g->members[x][0]] > b->nsites[g->members[y][0]];
+    });
+    const int nstr = std::min(g->n_groups, (int)mcx_gen::kStreams);
+    HIPCHK(hipEventRecord(g->fork, st));
+    for (int q = 0; q < nstr; ++q)
+        HIPCHK(hipStreamWaitEvent(g->gs[q], g->fork, 0));
+    for (int oi = 0; oi < g->n_groups; ++oi) {
+        const int k = order[oi];
         GenArgs a{};
         a.items = g->d_members + g->member_off[k];
         a.n_items = (int32_t)g->members[k].size();
@@ -287,9 +323,19 @@ extern "C" mcx_status mcx_gen_run(mcx_gen* g, int64_t nsweeps, uint32_t flags)
         a.dim = b->dim;
         a.record = record ? 1 : 0;
         const int64_t ns = b->nsites[g->members[k][0]];
-        a.hist_in_lds = gen_lds(g->nbins[k], ns, true) <= b->ctx->lds_max ? 1 : 0;
+        a.hist_in_lds = gen_lds(g->nbins[k], ns, true) + kGenStaticLds <= b->ctx->lds_max ? 1 : 0;
         a.bin_start = g->start[k];
         a.bin_step = g->step[k];
+        {
+            // a power-of-two step divides exactly by a multiply (FP64) or a
+            // shift (integer energies, which need an integral step)
+            int e = 0;
+            const double m = std::frexp(a.bin_step, &e);
+            const bool pow2 = a.bin_step > 0 && m == 0.5;
+            a.step_pow2 = pow2 && (b->model != MCX_ISING || e - 1 >= 0) ? 1 : 0;
+            a.step_shift = pow2 && e - 1 >= 0 ? e - 1 : 0;
+            a.inv_step = pow2 ? std::ldexp(1.0, 1 - e) : 0.0;
+        }
         a.nbins = g->nbins[k];
         a.chains = b->d_chains;
         a.spins = b->d_spins;
@@ -305,7 +351,11 @@ extern "C" mcx_status mcx_gen_run(mcx_gen* g, int64_t nsweeps, uint32_t flags)
         a.ser_Q = b->d_Q;
         a.nsweeps = nsweeps;
         a.rec_pos = b->rec_len;
-        HIPCHK(launch_gen_sweeps(a, gen_lds(g->nbins[k], ns, a.hist_in_lds), st));
+        HIPCHK(launch_gen_sweeps(a, gen_lds(g->nbins[k], ns, a.hist_in_lds), g->gs[oi % nstr]));
+    }
+    for (int q = 0; q < nstr; ++q) {
+        HIPCHK(hipEventRecord(g->join[q], g->gs[q]));
+        HIPCHK(hipStreamWaitEvent(st, g->join[q], 0));
     }
     HIPCHK(hipEventRecord(b->ctx->ev1, st));
     mcx_status s = sync_check(b, "mcx_gen_run");

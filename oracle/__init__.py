@@ -104,6 +104,10 @@ def _declare(L):
     L.orc_refpath_ising2d.argtypes = [C.c_int, C.c_int, _f64p, C.c_uint64, C.c_int64, C.c_int,
                                       _NullablePtr(_i8p)]
     L.orc_refpath_ising2d.restype = C.c_int64
+    nf64 = _NullablePtr(_f64p)
+    L.orc_refpath_run.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _i32p, _f64p, nf64, nf64,
+                                  C.c_int64, C.c_double, C.c_double, C.c_uint64, C.c_int64, C.c_int]
+    L.orc_refpath_run.restype = C.c_int64
 
 
 # --------------------------------------------------------------------------
@@ -227,3 +231,19 @@ def refpath_ising2d(L, betas, seed, nsweeps, nthreads, keep_spins=False):
     out = np.zeros(len(betas) * L * L, np.int8) if keep_spins else None
     n = lib().orc_refpath_ising2d(L, len(betas), betas, seed, nsweeps, nthreads, out)
     return n, out
+
+
+def refpath_run(model, dim, Ls, betas, seed, nsweeps, nthreads, D=None, lw=None,
+                bin_start=0.0, bin_step=1.0):
+    """General reference CPU path (random site, Xoshiro256++): Ising or
+    Blume-Capel (model 0 / 1), 2D or 3D, per-chain L / beta / D; canonical
+    Metropolis, or multicanonical with weights lw over DiscreteBinning(bin_start,
+    bin_step, len(lw)).  Returns the attempts made (timing baseline only)."""
+    Ls = np.ascontiguousarray(Ls, np.int32)
+    betas = np.ascontiguousarray(np.broadcast_to(np.asarray(betas, np.float64), Ls.shape))
+    Dv = None if D is None else np.ascontiguousarray(np.broadcast_to(np.asarray(D, np.float64),
+                                                                      Ls.shape))
+    lwv = None if lw is None else np.ascontiguousarray(lw, np.float64)
+    return lib().orc_refpath_run(model, dim, 0 if lwv is None else 1, len(Ls), Ls, betas, Dv, lwv,
+                                 0 if lwv is None else len(lwv), float(bin_start), float(bin_step),
+                                 seed, nsweeps, nthreads)

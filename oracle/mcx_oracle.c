@@ -679,3 +679,158 @@ int64_t orc_refpath_ising2d(int L, int nchains, const double* betas, uint64_t se
     free(nbr4);
     return total;
 }
+
+/* ---- reference CPU path, general form -------------------------------------
+ * One chain per thread slot, as ParallelChains(ThreadsBackend(n), algs) runs
+ * the reference (parallel_chains.jl:99-105): random site i = rand(UInt) % N
+ * (abstractions.jl:19) over a precomputed neighbour table (the Graphs.grid
+ * adjacency of Ising([L,L(,L)]) / BlumeCapel, ising.jl:413-424,
+ * blume_capel.jl:500-511), Xoshiro256++ per chain seeded by
+ * SplitMix64(seed + chain), and
+ *   Ising Metropolis       spin_flip! (ising.jl:35-41), accept! (metropolis.jl:14-17)
+ *   Blume-Capel Metropolis spin_flip! (blume_capel.jl:52-59) with _propose_state
+ *                          (:21-30, rand(rng, Bool)) and dE (:235-248)
+ *   multicanonical         spin_flip!(sys, ::AbstractImportanceSampling)
+ *                          (ising.jl:25-33, blume_capel.jl:41-50): lr = lw[bin(E_new)]
+ *                          - lw[bin(E_old)], accept! (importance_sampling.jl:69-78),
+ *                          then record_visit! (ensembles/multicanonical.jl:25-30)
+ *                          into a per-chain histogram; E outside the bins counts
+ *                          as a rejected move (the caller sizes the bins).
+ * Used only as the timed CPU baseline. */
+typedef struct {
+    int model, dim, muca, c0, c1;
+    const int32_t* L;
+    const double* betas;
+    const double* D;
+    const double* lw;
+    int64_t nbins;
+    double bin_start, bin_step;
+    uint64_t seed;
+    int64_t nsweeps;
+    int64_t attempts;
+} refrun_t;
+
+static void* refrun_worker(void* arg)
+{
+    refrun_t* job = (refrun_t*)arg;
+    const int Z = 2 * job->dim;
+    for (int ch = job->c0; ch < job->c1; ++ch) {
+        const int L = job->L[ch];
+        const int64_t N = job->dim == 3 ? (int64_t)L * L * L : (int64_t)L * L;
+        int8_t* s = (int8_t*)malloc((size_t)N);
+        int32_t* nbr = (int32_t*)malloc(sizeof(int32_t) * (size_t)(N * Z));
+        double* hist = job->muca ? (double*)calloc((size_t)job->nbins, sizeof(double)) : NULL;
+        for (int64_t i = 0; i < N; ++i) {
+            const int x = (int)(i % L), y = (int)((i / L) % L), z = (int)(i / ((int64_t)L * L));
+            int32_t* nb = nbr + i * Z;
+            const int64_t pl = (int64_t)L * L * z;
+            nb[0] = (int32_t)(pl + (int64_t)y * L + (x + L - 1) % L);
+            nb[1] = (int32_t)(pl + (int64_t)y * L + (x + 1) % L);
+            nb[2] = (int32_t)(pl + (int64_t)((y + L - 1) % L) * L + x);
+            nb[3] = (int32_t)(pl + (int64_t)((y + 1) % L) * L + x);
+            if (job->dim == 3) {
+                nb[4] = (int32_t)((int64_t)L * L * ((z + L - 1) % L) + (int64_t)y * L + x);
+                nb[5] = (int32_t)((int64_t)L * L * ((z + 1) % L) + (int64_t)y * L + x);
+            }
+        }
+        uint64_t sm = job->seed + (uint64_t)ch;
+        xoshiro_t rng;
+        for (int k = 0; k < 4; ++k)
+            rng.s[k] = splitmix64(&sm);
+        for (int64_t i = 0; i < N; ++i) {
+            const uint64_t r = xoshiro_next(&rng);
+            s[i] = job->model == 0 ? ((r >> 63) ? 1 : -1) : (int8_t)((int)(r % 3) - 1);
+        }
+        const double beta = job->betas[ch], D = job->D ? job->D[ch] : 0.0;
+        int64_t sum_pair = 0, sum_sq = 0;
+        for (int64_t i = 0; i < N; ++i) {
+            sum_sq += s[i] * s[i];
+            for (int q = 1; q < Z; q += 2) /* each bond once, from its lower end */
+                sum_pair += s[i] * s[nbr[i * Z + q]];
+        }
+        double E = -(double)sum_pair + D * (double)sum_sq;
+        int64_t steps = 0, accepted = 0;
+        for (int64_t k = 0; k < job->nsweeps * N; ++k) {
+            const int64_t i = (int64_t)(xoshiro_next(&rng) % (uint64_t)N);
+            const int32_t* nb = nbr + i * Z;
+            int sum = 0;
+            for (int q = 0; q < Z; ++q)
+                sum += s[nb[q]];
+            const int so = s[i];
+            int sn;
+            if (job->model == 0)
+                sn = -so;
+            else {
+                const int alt = (int)(xoshiro_next(&rng) >> 63);
+                sn = so == -1 ? (alt ? 0 : 1) : (so == 0 ? (alt ? -1 : 1) : (alt ? -1 : 0));
+            }
+            const double dE = -(double)((sn - so) * sum) + D * (double)(sn * sn - so * so);
+            double lr;
+            int64_t b_old = 0, b_new = 0;
+            if (job->muca) {
+                const int in_old = bin_float(E, job->bin_start, job->bin_step, job->nbins, &b_old);
+                const int in_new = bin_float(E + dE, job->bin_start, job->bin_step, job->nbins, &b_new);
+                lr = in_old && in_new ? job->lw[b_new] - job->lw[b_old] : -INFINITY;
+            } else {
+                lr = -beta * dE;
+            }
+            ++steps;
+            const int ok = (lr > 0) || ((double)(xoshiro_next(&rng) >> 11) * 0x1.0p-53 < exp(lr));
+            if (ok) {
+                s[i] = (int8_t)sn;
+                E += dE;
+                ++accepted;
+            }
+            if (job->muca) {
+                int64_t b;
+                if (bin_float(E, job->bin_start, job->bin_step, job->nbins, &b))
+                    hist[b] += 1.0;
+            }
+        }
+        job->attempts += steps;
+        (void)accepted;
+        free(hist);
+        free(nbr);
+        free(s);
+    }
+    return NULL;
+}
+
+int64_t orc_refpath_run(int model, int dim, int muca, int nchains, const int32_t* L,
+                        const double* betas, const double* D, const double* lw, int64_t nbins,
+                        double bin_start, double bin_step, uint64_t seed, int64_t nsweeps,
+                        int nthreads)
+{
+    if (nthreads < 1)
+        nthreads = 1;
+    if (nthreads > nchains)
+        nthreads = nchains;
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    refrun_t* jobs = (refrun_t*)calloc((size_t)nthreads, sizeof(refrun_t));
+    for (int t = 0; t < nthreads; ++t) {
+        refrun_t* j = &jobs[t];
+        j->model = model;
+        j->dim = dim;
+        j->muca = muca;
+        j->c0 = (int)((int64_t)nchains * t / nthreads);
+        j->c1 = (int)((int64_t)nchains * (t + 1) / nthreads);
+        j->L = L;
+        j->betas = betas;
+        j->D = D;
+        j->lw = lw;
+        j->nbins = nbins;
+        j->bin_start = bin_start;
+        j->bin_step = bin_step;
+        j->seed = seed;
+        j->nsweeps = nsweeps;
+        pthread_create(&th[t], NULL, refrun_worker, j);
+    }
+    int64_t total = 0;
+    for (int t = 0; t < nthreads; ++t) {
+        pthread_join(th[t], NULL);
+        total += jobs[t].attempts;
+    }
+    free(jobs);
+    free(th);
+    return total;
+}

@@ -111,6 +111,16 @@ void orc_muca_update(int64_t nbins, double* lw, const double* h);
 int64_t orc_refpath_ising2d(int L, int nchains, const double* betas, uint64_t seed,
                             int64_t nsweeps, int nthreads, int8_t* spins_out);
 
+/* General reference CPU path (timed baseline only): nchains independent chains
+ * of model (0 Ising, 1 Blume-Capel) in dim 2 or 3, per-chain L, beta and D
+ * (D may be NULL), random-site Metropolis, or multicanonical weights lw over
+ * DiscreteBinning(bin_start, bin_step, nbins) when muca != 0.  Returns the
+ * attempts made. */
+int64_t orc_refpath_run(int model, int dim, int muca, int nchains, const int32_t* L,
+                        const double* betas, const double* D, const double* lw, int64_t nbins,
+                        double bin_start, double bin_step, uint64_t seed, int64_t nsweeps,
+                        int nthreads);
+
 #ifdef __cplusplus
 }
 #endif
