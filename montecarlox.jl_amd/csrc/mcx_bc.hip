@@ -323,7 +323,8 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
     int32_t* part = reinterpret_cast<int32_t*>(lds); // [2][16][3]
     uint32_t* tabw = lds + 96;                       // 80 thresholds + 3 masks
     double* tabd = reinterpret_cast<double*>(lds + 184); // 39 doubles (8-byte aligned)
-    uint32_t* P = lds + 264;                          // [occ c0, occ c1, up c0, up c1]
+    int32_t* ring = reinterpret_cast<int32_t*>(lds + 264); // E[64], M[64], Q[64] of the last sweeps
+    uint32_t* P = lds + 456;                          // [occ c0, occ c1, up c0, up c1]
     const int tid = threadIdx.x;
     const int B = blockDim.x;
     const int lane = tid & 63;
@@ -368,6 +369,7 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
             mM4 = mo[5];
         }
         const int64_t nsw = a.item_nsweeps ? a.item_nsweeps[item] : a.nsweeps;
+        const bool series = a.record && a.ser_E != nullptr;
         for (int64_t s = 0; s < nsw; ++s) {
             const uint64_t sweep = cr.sweep + (uint64_t)s;
             const uint32_t sw_lo = (uint32_t)sweep;
@@ -432,18 +434,10 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
                     tm += part[(s & 1) * 48 + k * 3 + 1];
                     tq += part[(s & 1) * 48 + k * 3 + 2];
                 }
-                if (a.last_E && s == nsw - 1) {
-                    a.last_E[ch] = (int32_t)(-tp);
-                    a.last_M[ch] = (int32_t)tm;
-                    a.last_Q[ch] = (int32_t)tq;
-                }
+                ring[s & 63] = (int32_t)(-tp);
+                ring[64 + (s & 63)] = (int32_t)tm;
+                ring[128 + (s & 63)] = (int32_t)tq;
                 if (a.record) {
-                    if (a.ser_E) {
-                        const int64_t at = cr.series_off + a.rec_pos + s;
-                        a.ser_E[at] = (int32_t)(-tp);
-                        a.ser_M[at] = (int32_t)tm;
-                        a.ser_Q[at] = (int32_t)tq;
-                    }
                     const double e = (double)(int32_t)(-tp) + cr.D * (double)(int32_t)tq;
                     const double m = (double)tm;
                     mE += e;
@@ -453,6 +447,22 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
                     mM4 += (m * m) * (m * m);
                 }
             }
+            // the series leaves the ring every 64 sweeps, coalesced
+            if (series && ((s & 63) == 63 || s == nsw - 1)) {
+                __syncthreads();
+                const int64_t s0 = s & ~(int64_t)63;
+                const int64_t at0 = cr.series_off + a.rec_pos + s0;
+                for (int t = tid; t <= (int)(s - s0); t += B) {
+                    a.ser_E[at0 + t] = ring[t];
+                    a.ser_M[at0 + t] = ring[64 + t];
+                    a.ser_Q[at0 + t] = ring[128 + t];
+                }
+            }
+        }
+        if (a.last_E && nsw > 0 && tid == 0) {
+            a.last_E[ch] = ring[(nsw - 1) & 63];
+            a.last_M[ch] = ring[64 + ((nsw - 1) & 63)];
+            a.last_Q[ch] = ring[128 + ((nsw - 1) & 63)];
         }
 
         for (int w = tid; w < words; w += B)

@@ -229,8 +229,9 @@ template <int DIM, int ALG>
 __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
 {
     extern __shared__ uint32_t lds[];
-    uint32_t* part = lds;    // [2 parity][16 waves][2]
-    uint32_t* P = lds + 64;  // planes [2 colours][rows][nw]
+    uint32_t* part = lds;                                // [2 parity][16 waves][2]
+    int32_t* ring = reinterpret_cast<int32_t*>(lds + 64); // E[64], M[64] of the last sweeps
+    uint32_t* P = lds + 192;                             // planes [2 colours][rows][nw]
     const int tid = threadIdx.x;
     const int B = blockDim.x;
     const int lane = tid & 63;
@@ -272,6 +273,7 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
         }
 
         const int64_t nsw = a.item_nsweeps ? a.item_nsweeps[item] : a.nsweeps;
+        const bool series = a.record && a.ser_E != nullptr;
         for (int64_t s = 0; s < nsw; ++s) {
             const uint64_t sweep = cr.sweep + (uint64_t)s;
             const uint32_t sw_lo = (uint32_t)sweep;
@@ -326,17 +328,9 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                 }
                 const int32_t E = (int32_t)(2 * (int64_t)ta - (int64_t)DIM * N);
                 const int32_t M = (int32_t)(2 * (int64_t)tu - N);
-                if (a.last_E && s == nsw - 1) {
-                    a.last_E[ch] = E;
-                    a.last_M[ch] = M;
-                    a.last_Q[ch] = (int32_t)N;
-                }
+                ring[s & 63] = E;
+                ring[64 + (s & 63)] = M;
                 if (a.record) {
-                    if (a.ser_E) {
-                        const int64_t at = cr.series_off + a.rec_pos + s;
-                        a.ser_E[at] = E;
-                        a.ser_M[at] = M;
-                    }
                     const double e = (double)E, m = (double)M;
                     mE += e;
                     mE2 += e * e;
@@ -345,6 +339,21 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                     mM4 += (m * m) * (m * m);
                 }
             }
+            // the series leaves the ring every 64 sweeps, coalesced
+            if (series && ((s & 63) == 63 || s == nsw - 1)) {
+                __syncthreads();
+                const int64_t s0 = s & ~(int64_t)63;
+                const int64_t at0 = cr.series_off + a.rec_pos + s0;
+                for (int t = tid; t <= (int)(s - s0); t += B) {
+                    a.ser_E[at0 + t] = ring[t];
+                    a.ser_M[at0 + t] = ring[64 + t];
+                }
+            }
+        }
+        if (a.last_E && nsw > 0 && tid == 0) {
+            a.last_E[ch] = ring[(nsw - 1) & 63];
+            a.last_M[ch] = ring[64 + ((nsw - 1) & 63)];
+            a.last_Q[ch] = (int32_t)N;
         }
 
         // ---- bit planes -> int8 lattice ----
