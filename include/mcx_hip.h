@@ -296,6 +296,32 @@ mcx_status mcx_gen_set_histogram(mcx_gen* g, int32_t chain, const int64_t* hist,
 mcx_status mcx_muca_update_values(mcx_ctx* ctx, double* lw, const double* h, int64_t n);
 
 /* ------------------------------------------------------------------------ */
+/* Device-state snapshots (checkpoint / resume)                             */
+/* ------------------------------------------------------------------------ */
+/* Replace the state that checkpoint! / restore_checkpoint serialize
+ * (src/infrastructure/checkpointing.jl:48-111): the objects' device arrays are
+ * written to / read from a caller buffer of _state_size bytes (a versioned
+ * header + the arrays; the caller stores it).  A snapshot restores into an
+ * object of the same shape only (MCX_EINVAL otherwise).  The Philox streams are
+ * counter-based, so n sweeps, a snapshot, a restore and m more sweeps give the
+ * run of n + m sweeps (the deterministic restart of test_checkpointing.jl:100-147).
+ * Batch: chain records (sweep counters, streams, tables), spins, counters,
+ * moments, last observables and, with MCX_STATE_SERIES, the recorded series
+ * (restored without it, the series restarts empty). */
+#define MCX_STATE_SERIES 1u
+mcx_status mcx_batch_state_size(mcx_batch* b, uint32_t flags, size_t* bytes);
+mcx_status mcx_batch_save_state(mcx_batch* b, uint32_t flags, void* buf, size_t bytes);
+mcx_status mcx_batch_load_state(mcx_batch* b, const void* buf, size_t bytes);
+/* Replica exchange: ladder positions, per-edge counters, stages, round counters. */
+mcx_status mcx_pt_state_size(mcx_pt* pt, size_t* bytes);
+mcx_status mcx_pt_save_state(mcx_pt* pt, void* buf, size_t bytes);
+mcx_status mcx_pt_load_state(mcx_pt* pt, const void* buf, size_t bytes);
+/* Walkers: log-weights, histograms, ln f, counters. */
+mcx_status mcx_gen_state_size(mcx_gen* g, size_t* bytes);
+mcx_status mcx_gen_save_state(mcx_gen* g, void* buf, size_t bytes);
+mcx_status mcx_gen_load_state(mcx_gen* g, const void* buf, size_t bytes);
+
+/* ------------------------------------------------------------------------ */
 /* Serving: continuous batching of independent simulation jobs              */
 /* ------------------------------------------------------------------------ */
 /* No reference counterpart: the reference runs one chain per thread to

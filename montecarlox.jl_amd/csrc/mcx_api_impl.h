@@ -4,6 +4,7 @@
 #include "../../include/mcx_hip.h"
 #include "mcx_internal.h"
 
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -61,6 +62,82 @@ struct mcx_batch {
 };
 
 namespace mcx {
+
+// Device-state snapshots (mcx_*_save_state / _load_state): a fixed header and
+// the object's arrays in a fixed order, written to / read from a caller buffer.
+struct StateHeader {
+    char magic[4];    // "MCXS"
+    uint32_t version; // 1
+    uint32_t kind;    // 1 batch, 2 replica exchange, 3 walkers
+    uint32_t flags;
+    int64_t shape[8]; // kind-specific shape fields, checked on load
+};
+
+class BlobWriter {
+public:
+    BlobWriter(void* buf, size_t bytes) : p_(static_cast<uint8_t*>(buf)), left_(bytes) {}
+    bool put(const void* src, size_t n)
+    {
+        if (n > left_)
+            return false;
+        if (p_)
+            std::memcpy(p_, src, n);
+        p_ = p_ ? p_ + n : nullptr;
+        left_ -= n;
+        return true;
+    }
+    uint8_t* reserve(size_t n) // n bytes to be filled by the caller (e.g. a device copy)
+    {
+        if (n > left_)
+            return nullptr;
+        uint8_t* r = p_;
+        p_ += n;
+        left_ -= n;
+        return r;
+    }
+
+private:
+    uint8_t* p_;
+    size_t left_;
+};
+
+class BlobReader {
+public:
+    BlobReader(const void* buf, size_t bytes) : p_(static_cast<const uint8_t*>(buf)), left_(bytes) {}
+    const uint8_t* take(size_t n)
+    {
+        if (n > left_)
+            return nullptr;
+        const uint8_t* r = p_;
+        p_ += n;
+        left_ -= n;
+        return r;
+    }
+    bool get(void* dst, size_t n)
+    {
+        const uint8_t* r = take(n);
+        if (!r)
+            return false;
+        std::memcpy(dst, r, n);
+        return true;
+    }
+    size_t left() const { return left_; }
+
+private:
+    const uint8_t* p_;
+    size_t left_;
+};
+
+inline StateHeader make_state_header(uint32_t kind, uint32_t flags)
+{
+    StateHeader h;
+    std::memset(&h, 0, sizeof h);
+    std::memcpy(h.magic, "MCXS", 4);
+    h.version = 1;
+    h.kind = kind;
+    h.flags = flags;
+    return h;
+}
 
 mcx_status fail(mcx_status code, const char* fmt, ...);
 uint64_t threshold32(double p);

@@ -12,6 +12,7 @@ struct mcx_gen {
     mcx_batch* b = nullptr;
     int kind = MCX_MULTICANONICAL;
     int n_groups = 0;
+    int64_t tot = 0; // weights / histogram entries over all walkers (bins padded to 32)
     std::vector<int32_t> group_of;
     std::vector<double> start, step;
     std::vector<int64_t> nbins, off, member_off;
@@ -147,6 +148,7 @@ extern "C" mcx_status mcx_gen_create(mcx_batch* b, const mcx_gen_desc* d, mcx_ge
         g->off[i] = tot;
         tot += (g->nbins[g->group_of[i]] + 31) / 32 * 32;
     }
+    g->tot = tot;
     std::vector<int32_t> flat;
     for (int k = 0; k < g->n_groups; ++k) {
         g->member_off.push_back((int64_t)flat.size());
@@ -544,4 +546,90 @@ extern "C" mcx_status mcx_muca_update_values(mcx_ctx* ctx, double* lw, const dou
     if (e != hipSuccess)
         return fail(MCX_EHIP, "mcx_muca_update_values: %s", hipGetErrorString(e));
     return MCX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device-state snapshot (checkpoint / resume): log-weights, histograms, ln f
+// and counters of every walker; restores into walkers of the same layout.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+size_t gen_state_bytes(const mcx_gen* g)
+{
+    const size_t n = (size_t)g->b->n, t = (size_t)g->tot;
+    return sizeof(StateHeader) + sizeof(int64_t) * n + sizeof(double) * t +
+           sizeof(unsigned long long) * t + sizeof(double) * n + sizeof(unsigned long long) * 2 * n;
+}
+
+} // namespace
+
+extern "C" mcx_status mcx_gen_state_size(mcx_gen* g, size_t* bytes)
+{
+    if (!g || !bytes)
+        return fail(MCX_EINVAL, "mcx_gen_state_size: NULL argument");
+    *bytes = gen_state_bytes(g);
+    return MCX_OK;
+}
+
+extern "C" mcx_status mcx_gen_save_state(mcx_gen* g, void* buf, size_t bytes)
+{
+    if (!g || !buf)
+        return fail(MCX_EINVAL, "mcx_gen_save_state: NULL argument");
+    if (bytes < gen_state_bytes(g))
+        return fail(MCX_EINVAL, "mcx_gen_save_state: buffer of %zu bytes, %zu needed", bytes,
+                    gen_state_bytes(g));
+    mcx_batch* b = g->b;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    const size_t n = (size_t)b->n, t = (size_t)g->tot;
+    StateHeader h = make_state_header(3, 0);
+    h.shape[0] = g->kind;
+    h.shape[1] = b->n;
+    h.shape[2] = g->n_groups;
+    h.shape[3] = g->tot;
+    BlobWriter w(buf, bytes);
+    w.put(&h, sizeof h);
+    w.put(g->off.data(), sizeof(int64_t) * n);
+    hipStream_t st = b->ctx->stream;
+    auto d2h = [&](const void* src, size_t len) -> hipError_t {
+        return hipMemcpyAsync(w.reserve(len), src, len, hipMemcpyDeviceToHost, st);
+    };
+    HIPCHK(d2h(g->d_lw, sizeof(double) * t));
+    HIPCHK(d2h(g->d_hist, sizeof(unsigned long long) * t));
+    HIPCHK(d2h(g->d_logf, sizeof(double) * n));
+    HIPCHK(d2h(g->d_steps, sizeof(unsigned long long) * n));
+    HIPCHK(d2h(g->d_acc, sizeof(unsigned long long) * n));
+    return sync_check(b, "mcx_gen_save_state");
+}
+
+extern "C" mcx_status mcx_gen_load_state(mcx_gen* g, const void* buf, size_t bytes)
+{
+    if (!g || !buf)
+        return fail(MCX_EINVAL, "mcx_gen_load_state: NULL argument");
+    BlobReader r(buf, bytes);
+    StateHeader h;
+    if (!r.get(&h, sizeof h) || std::memcmp(h.magic, "MCXS", 4) != 0 || h.version != 1 || h.kind != 3)
+        return fail(MCX_EINVAL, "mcx_gen_load_state: not a walker snapshot of this library");
+    mcx_batch* b = g->b;
+    if (h.shape[0] != g->kind || h.shape[1] != b->n || h.shape[2] != g->n_groups ||
+        h.shape[3] != g->tot || bytes < gen_state_bytes(g))
+        return fail(MCX_EINVAL, "mcx_gen_load_state: the snapshot is of another walker layout");
+    const size_t n = (size_t)b->n, t = (size_t)g->tot;
+    std::vector<int64_t> off(n);
+    r.get(off.data(), sizeof(int64_t) * n);
+    if (off != g->off)
+        return fail(MCX_EINVAL, "mcx_gen_load_state: the snapshot's bin layout differs");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    auto h2d = [&](void* dst, size_t len) -> hipError_t {
+        return hipMemcpyAsync(dst, r.take(len), len, hipMemcpyHostToDevice, st);
+    };
+    HIPCHK(h2d(g->d_lw, sizeof(double) * t));
+    HIPCHK(h2d(g->d_hist, sizeof(unsigned long long) * t));
+    const uint8_t* lf = r.take(sizeof(double) * n);
+    std::memcpy(g->logf.data(), lf, sizeof(double) * n);
+    HIPCHK(hipMemcpyAsync(g->d_logf, g->logf.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+    HIPCHK(h2d(g->d_steps, sizeof(unsigned long long) * n));
+    HIPCHK(h2d(g->d_acc, sizeof(unsigned long long) * n));
+    return sync_check(b, "mcx_gen_load_state");
 }

@@ -286,3 +286,88 @@ extern "C" mcx_status mcx_pt_get_rounds(mcx_pt* p, uint64_t* rounds)
                           b->ctx->stream));
     return sync_check(b, "mcx_pt_get_rounds");
 }
+
+// ---------------------------------------------------------------------------
+// device-state snapshot (checkpoint / resume): ladder positions, counters,
+// stages and round counters; restores into an object of the same shape.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+size_t pt_state_bytes(const mcx_pt* p)
+{
+    const size_t n = (size_t)p->n_sets * p->R, e = (size_t)p->n_sets * (p->R - 1);
+    return sizeof(StateHeader) + sizeof(double) * n + sizeof(int32_t) * 2 * n +
+           sizeof(int32_t) * 2 * e + sizeof(int32_t) * p->n_sets + sizeof(uint64_t) * p->n_sets;
+}
+
+} // namespace
+
+extern "C" mcx_status mcx_pt_state_size(mcx_pt* p, size_t* bytes)
+{
+    if (!p || !bytes)
+        return fail(MCX_EINVAL, "mcx_pt_state_size: NULL argument");
+    *bytes = pt_state_bytes(p);
+    return MCX_OK;
+}
+
+extern "C" mcx_status mcx_pt_save_state(mcx_pt* p, void* buf, size_t bytes)
+{
+    if (!p || !buf)
+        return fail(MCX_EINVAL, "mcx_pt_save_state: NULL argument");
+    if (bytes < pt_state_bytes(p))
+        return fail(MCX_EINVAL, "mcx_pt_save_state: buffer of %zu bytes, %zu needed", bytes,
+                    pt_state_bytes(p));
+    mcx_batch* b = p->b;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    const size_t n = (size_t)p->n_sets * p->R, e = (size_t)p->n_sets * (p->R - 1);
+    StateHeader h = make_state_header(2, 0);
+    h.shape[0] = p->n_sets;
+    h.shape[1] = p->R;
+    BlobWriter w(buf, bytes);
+    w.put(&h, sizeof h);
+    w.put(p->ladder.data(), sizeof(double) * n);
+    hipStream_t st = b->ctx->stream;
+    auto d2h = [&](const void* src, size_t len) -> hipError_t {
+        return hipMemcpyAsync(w.reserve(len), src, len, hipMemcpyDeviceToHost, st);
+    };
+    HIPCHK(d2h(p->d_idx, sizeof(int32_t) * n));
+    HIPCHK(d2h(p->d_pos, sizeof(int32_t) * n));
+    HIPCHK(d2h(p->d_steps, sizeof(int32_t) * e));
+    HIPCHK(d2h(p->d_acc, sizeof(int32_t) * e));
+    HIPCHK(d2h(p->d_stage, sizeof(int32_t) * p->n_sets));
+    HIPCHK(d2h(p->d_round, sizeof(uint64_t) * p->n_sets));
+    return sync_check(b, "mcx_pt_save_state");
+}
+
+extern "C" mcx_status mcx_pt_load_state(mcx_pt* p, const void* buf, size_t bytes)
+{
+    if (!p || !buf)
+        return fail(MCX_EINVAL, "mcx_pt_load_state: NULL argument");
+    BlobReader r(buf, bytes);
+    StateHeader h;
+    if (!r.get(&h, sizeof h) || std::memcmp(h.magic, "MCXS", 4) != 0 || h.version != 1 || h.kind != 2)
+        return fail(MCX_EINVAL, "mcx_pt_load_state: not a replica-exchange snapshot of this library");
+    if (h.shape[0] != p->n_sets || h.shape[1] != p->R || bytes < pt_state_bytes(p))
+        return fail(MCX_EINVAL, "mcx_pt_load_state: the snapshot is of %lld sets x %lld replicas",
+                    (long long)h.shape[0], (long long)h.shape[1]);
+    const size_t n = (size_t)p->n_sets * p->R, e = (size_t)p->n_sets * (p->R - 1);
+    std::vector<double> ladder(n);
+    r.get(ladder.data(), sizeof(double) * n);
+    for (size_t i = 0; i < n; ++i)
+        if (ladder[i] != p->ladder[i])
+            return fail(MCX_EINVAL, "mcx_pt_load_state: the snapshot's ladder differs at %zu", i);
+    mcx_batch* b = p->b;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    auto h2d = [&](void* dst, size_t len) -> hipError_t {
+        return hipMemcpyAsync(dst, r.take(len), len, hipMemcpyHostToDevice, st);
+    };
+    HIPCHK(h2d(p->d_idx, sizeof(int32_t) * n));
+    HIPCHK(h2d(p->d_pos, sizeof(int32_t) * n));
+    HIPCHK(h2d(p->d_steps, sizeof(int32_t) * e));
+    HIPCHK(h2d(p->d_acc, sizeof(int32_t) * e));
+    HIPCHK(h2d(p->d_stage, sizeof(int32_t) * p->n_sets));
+    HIPCHK(h2d(p->d_round, sizeof(uint64_t) * p->n_sets));
+    return sync_check(b, "mcx_pt_load_state");
+}

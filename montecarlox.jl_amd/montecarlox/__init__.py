@@ -13,6 +13,8 @@ from .algorithms import (AbstractHeatBath, AbstractImportanceSampling, AbstractM
                          Glauber, HeatBath, ImportanceSampling, Metropolis, Multicanonical, Philox,
                          WangLandau, acceptance_rate, ensemble, reset_)
 from .batch import LatticeBatch
+from .checkpointing import (CheckpointSession, checkpoint_, finalize_, init_checkpoint,
+                            restore_checkpoint)
 from .ensembles import (BinnedObject, BoltzmannEnsemble, MulticanonicalEnsemble,
                         WangLandauEnsemble, get_centers, get_edges, get_values, logweight, set_,
                         update_)

@@ -18,6 +18,7 @@ ISING, BLUME_CAPEL = 0, 1
 METROPOLIS, HEATBATH, MULTICANONICAL, WANGLANDAU = 0, 1, 2, 3
 INIT_UP, INIT_DOWN, INIT_ZERO, INIT_RANDOM = 0, 1, 2, 3
 RUN_RECORD, RUN_PLAIN = 1, 2
+STATE_SERIES = 1  # MCX_STATE_SERIES: snapshots carry the recorded series
 N_MOMENTS = 6
 
 
@@ -118,6 +119,9 @@ SIGNATURES = {
     "mcx_batch_get_sweeps": [_P, _pu64],
     "mcx_batch_set_sweeps": [_P, _pu64],
     "mcx_batch_set_streams": [_P, _u64, _pu32],
+    "mcx_batch_state_size": [_P, _u32, C.POINTER(C.c_size_t)],
+    "mcx_batch_save_state": [_P, _u32, _P, C.c_size_t],
+    "mcx_batch_load_state": [_P, _P, C.c_size_t],
 }
 OPTIONAL = {}  # filled by modules that extend the ABI (pt, gen, server)
 

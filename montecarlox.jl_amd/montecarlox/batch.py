@@ -201,3 +201,19 @@ class LatticeBatch:
         call("mcx_batch_set_streams", self._h, int(seed), _ptr(k, C.c_uint32) if k is not None else None)
         self.seed = int(seed)
         self.keys = k
+
+    # -- snapshots (checkpoint / resume) -------------------------------------------
+    def save_state(self, series: bool = True) -> np.ndarray:
+        """The batch's device state (chain records, counters, moments, tables,
+        spins and, with series=True, the recorded series) as a uint8 array."""
+        flags = _lib.STATE_SERIES if series else 0
+        n = C.c_size_t()
+        call("mcx_batch_state_size", self._h, flags, C.byref(n))
+        buf = np.empty(n.value, np.uint8)
+        call("mcx_batch_save_state", self._h, flags, buf.ctypes.data_as(C.c_void_p), buf.size)
+        return buf
+
+    def load_state(self, blob):
+        """Restore a snapshot of a batch of the same model, dim and chain sizes."""
+        b = np.ascontiguousarray(blob, np.uint8)
+        call("mcx_batch_load_state", self._h, b.ctypes.data_as(C.c_void_p), b.size)

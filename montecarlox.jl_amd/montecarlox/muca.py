@@ -52,6 +52,9 @@ for _n, _a in {
     "mcx_gen_get_counters": [_P, _pi64, _pi64],
     "mcx_gen_group_info": [_P, C.c_int32, _pi32, _pi32, _pi64],
     "mcx_muca_update_values": [_P, _pd, _pd, C.c_int64],
+    "mcx_gen_state_size": [_P, C.POINTER(C.c_size_t)],
+    "mcx_gen_save_state": [_P, _P, C.c_size_t],
+    "mcx_gen_load_state": [_P, _P, C.c_size_t],
 }.items():
     _lib.declare(_n, _a)
 
@@ -199,6 +202,23 @@ class GeneralizedWalkers:
         out = np.empty(3, np.float64)
         call("mcx_gen_flatness", self._h, int(group), _ptr(out, C.c_double))
         return float(out[0]), float(out[1]), int(out[2])
+
+    def save_state(self) -> np.ndarray:
+        """Log-weights, histograms, ln f and counters of every walker (uint8);
+        host-side changes to the ensembles are uploaded first."""
+        self.sync_in()
+        n = C.c_size_t()
+        call("mcx_gen_state_size", self._h, C.byref(n))
+        buf = np.empty(n.value, np.uint8)
+        call("mcx_gen_save_state", self._h, buf.ctypes.data_as(C.c_void_p), buf.size)
+        return buf
+
+    def load_state(self, blob):
+        """Restore a snapshot of walkers of the same kind and bin layout; the
+        host ensembles and counters are refreshed from it."""
+        b = np.ascontiguousarray(blob, np.uint8)
+        call("mcx_gen_load_state", self._h, b.ctypes.data_as(C.c_void_p), b.size)
+        self.sync_out()
 
     def close(self):
         if getattr(self, "_h", None):

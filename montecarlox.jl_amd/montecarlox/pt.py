@@ -39,6 +39,9 @@ for _n, _a in {
     "mcx_pt_get_ladder": [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
     "mcx_pt_get_counts": [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
     "mcx_pt_get_rounds": [_P, C.POINTER(C.c_uint64)],
+    "mcx_pt_state_size": [_P, C.POINTER(C.c_size_t)],
+    "mcx_pt_save_state": [_P, _P, C.c_size_t],
+    "mcx_pt_load_state": [_P, _P, C.c_size_t],
 }.items():
     _lib.declare(_n, _a)
 
@@ -160,6 +163,20 @@ class ParallelTempering:
                 a.accepted += int(acc1[k] - acc0[k])
         self._sync()
         return self.systems.batch.last_run_ms()
+
+    def save_state(self) -> np.ndarray:
+        """Ladder positions, exchange counters, stages and round counters (uint8)."""
+        n = C.c_size_t()
+        call("mcx_pt_state_size", self._h, C.byref(n))
+        buf = np.empty(n.value, np.uint8)
+        call("mcx_pt_save_state", self._h, buf.ctypes.data_as(C.c_void_p), buf.size)
+        return buf
+
+    def load_state(self, blob):
+        """Restore a snapshot of a ladder of the same shape and betas."""
+        b = np.ascontiguousarray(blob, np.uint8)
+        call("mcx_pt_load_state", self._h, b.ctypes.data_as(C.c_void_p), b.size)
+        self._sync()
 
     def reset_(self):
         call("mcx_pt_reset", self._h)
