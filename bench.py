@@ -153,20 +153,33 @@ def kernel_lines(mcx, np, oracle, pt_rounds, threads, cpu):
     R, S = 64, 256
     set_L = np.random.Generator(np.random.PCG64(11)).choice(np.array([16, 24, 32, 48]), size=S)
     Ls = np.repeat(set_L, R)
-    sys_ = SpinSystemBatch("ising", [16] * 3, n=len(Ls), L=Ls)
+    sys_ = SpinSystemBatch("ising", [16] * 3, n=len(Ls), L=Ls, series_capacity=pt_rounds * 10)
     pt = mcx.ParallelTempering(sys_, mcx.set_betas(R, 0.20, 0.24, "geometric"), n_sets=S, seed=11)
     sys_.init_("random", [a.rng for a in pt.algs])
     pt.run_(2, 10)
-    ms = pt.run_(pt_rounds, 10)
+    # per-temperature E/M series on the device, the input of the adaptive
+    # exchange interval (tau_int per ladder index)
+    pt.set_series_order("temperature")
+    ms = pt.run_(pt_rounds, 10, record=True)
     att = float((Ls.astype(np.int64) ** 3).sum()) * pt_rounds * 10
     acc = float(np.mean(mcx.acceptance_rates(pt)))
+    t0 = time.perf_counter()
+    taus = pt.tau_int(min_points=400, max_lag=200)
+    tau_ms = (time.perf_counter() - t0) * 1e3
+    sw = np.full((S, R), 10, np.int64)
+    mcx.optimize_exchange_interval_(pt, None, sw, base_sweeps=10, min_sweeps=1, max_sweeps=400,
+                                    min_points=400, max_lag=200)
     line = {"workload": f"{S} replica-exchange sets x {R} replicas, 3D Ising, L per set from "
                         "{16,24,32,48} (PCG64 seed 11: " +
                         ",".join(f"{L}:{int((set_L == L).sum())}" for L in (16, 24, 32, 48)) +
                         "), set_betas(64, 0.20, 0.24, geometric), checkerboard Metropolis, "
                         f"{pt_rounds} rounds of 10 sweeps + one exchange",
             "value": round(att / (ms * 1e6), 3), "unit": "attempts/ns", "ms": round(ms, 2),
-            "exchange_acceptance": round(acc, 4), "roofline": _roof(att, ms)}
+            "exchange_acceptance": round(acc, 4), "roofline": _roof(att, ms),
+            "tau_int": {"series": int(taus.size), "samples": pt_rounds * 10, "max_lag": 200,
+                        "ms": round(tau_ms, 2), "median": round(float(np.nanmedian(taus)), 3),
+                        "max": round(float(np.nanmax(taus)), 3),
+                        "retuned_sweeps_min_max": [int(sw.min()), int(sw.max())]}}
     pt.close()
     sys_.batch.close()
     if cpu:

@@ -70,6 +70,16 @@ typedef int32_t mcx_status;
 #define MCX_RUN_GRAPH 4u  /* mcx_pt_run: capture every round in one hipGraph and launch it once */
 
 /* per-chain FP64 moments, in this order */
+/* observables of the recorded series (mcx_batch_tau_int) */
+#define MCX_OBS_ENERGY 0   /* E_pair + D*Q (Ising: E_pair) */
+#define MCX_OBS_M 1        /* magnetization */
+#define MCX_OBS_ABS_M 2    /* |magnetization| */
+#define MCX_OBS_Q 3        /* sum s_i^2 (Blume-Capel) */
+
+/* order of the recorded series under replica exchange (mcx_pt_set_series_order) */
+#define MCX_ORDER_REPLICA 0     /* slot k = chain k */
+#define MCX_ORDER_TEMPERATURE 1 /* slot set*R + i - 1 = whichever replica holds ladder index i */
+
 #define MCX_MOM_COUNT 0
 #define MCX_MOM_E 1
 #define MCX_MOM_E2 2
@@ -231,6 +241,13 @@ mcx_status mcx_pt_get_ladder(mcx_pt* pt, int32_t* indices, int32_t* stages);
 mcx_status mcx_pt_get_counts(mcx_pt* pt, int32_t* steps, int32_t* accepted);
 /* Exchange rounds done per set (position of the PT random stream). */
 mcx_status mcx_pt_get_rounds(mcx_pt* pt, uint64_t* rounds);
+/* Record series and moments per temperature instead of per replica: the
+ * (index(pt), E) samples of pt_Ising2D_fullpower_mpi.jl:88-98 grouped by ladder
+ * index (parallel_tempering.jl:53-70, _group_samples), kept on the device.
+ * With MCX_ORDER_TEMPERATURE, series / moment slot set*R + i - 1 holds, in
+ * sweep order, the observables of whichever replica of the set held ladder
+ * index i at that sweep.  Changing the order clears the series and moments. */
+mcx_status mcx_pt_set_series_order(mcx_pt* pt, int32_t order);
 
 /* ------------------------------------------------------------------------ */
 /* Multicanonical / Wang-Landau walkers                                      */
@@ -294,6 +311,21 @@ mcx_status mcx_gen_set_histogram(mcx_gen* g, int32_t chain, const int64_t* hist,
  * Float64 histogram): lw[i] -= log(h[i]) where h[i] > 0, evaluated on the
  * device (src/ensembles/multicanonical.jl:32-44). */
 mcx_status mcx_muca_update_values(mcx_ctx* ctx, double* lw, const double* h, int64_t n);
+
+/* ------------------------------------------------------------------------ */
+/* Integrated autocorrelation times of the recorded series                   */
+/* ------------------------------------------------------------------------ */
+/* Replaces integrated_autocorrelation_times(traces; min_points, max_lag, c)
+ * (src/measurements/autocorrelations.jl:84-110, per trace :28-73) for the
+ * series of every slot over sweeps [first, first + count) of the recording,
+ * without copying them off the device.  taus[n_chains]: NaN for every slot if
+ * count < min_points or the lag window is empty; max_lag < 0 means no cap
+ * (count / 2).  The lag sums are exact (integer series, 128-bit sums), so the
+ * result is the reference's up to the rounding of its FP64 sums.  In
+ * temperature order (mcx_pt_set_series_order) slot set*R + i - 1 is ladder
+ * index i: the taus of optimize_exchange_interval! (parallel_tempering.jl:83-121). */
+mcx_status mcx_batch_tau_int(mcx_batch* b, int32_t observable, int64_t first, int64_t count,
+                             int32_t min_points, int32_t max_lag, double c, double* taus);
 
 /* ------------------------------------------------------------------------ */
 /* Device-state snapshots (checkpoint / resume)                             */

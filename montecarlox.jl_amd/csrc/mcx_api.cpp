@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 
 using namespace mcx;
 
@@ -627,6 +628,8 @@ mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t
         a.accepted = b->d_acc;
         a.nsweeps = nsweeps;
         a.rec_pos = b->rec_len;
+        a.by_table = b->by_table ? 1 : 0;
+        a.cap = b->cap;
         if (want_last && nsweeps > 0) {
             a.last_E = b->d_last;
             a.last_M = b->d_last + b->n;
@@ -665,6 +668,9 @@ mcx_status check_run(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t flags,
     if ((flags & MCX_RUN_RECORD) && b->rec_len + nsweeps > b->cap)
         return fail(MCX_EINVAL, "%s: recording %lld sweeps exceeds the series capacity (%lld of %lld used)",
                     who, (long long)nsweeps, (long long)b->rec_len, (long long)b->cap);
+    if ((flags & MCX_RUN_RECORD) && (flags & MCX_RUN_PLAIN) && b->by_table)
+        return fail(MCX_ESTATE, "%s: the plain path records in chain order; this batch records in "
+                                "temperature order (mcx_pt_set_series_order)", who);
     return MCX_OK;
 }
 
@@ -850,4 +856,70 @@ extern "C" mcx_status mcx_batch_set_streams(mcx_batch* b, uint64_t seed, const u
     if (s)
         return s;
     return sync_check(b, "mcx_batch_set_streams");
+}
+
+// ---------------------------------------------------------------------------
+// integrated autocorrelation times of the recorded series
+// (src/measurements/autocorrelations.jl:28-110)
+// ---------------------------------------------------------------------------
+
+extern "C" mcx_status mcx_batch_tau_int(mcx_batch* b, int32_t observable, int64_t first,
+                                        int64_t count, int32_t min_points, int32_t max_lag,
+                                        double c, double* taus)
+{
+    if (!b || !taus)
+        return fail(MCX_EINVAL, "mcx_batch_tau_int: NULL argument");
+    if (observable < MCX_OBS_ENERGY || observable > MCX_OBS_Q)
+        return fail(MCX_EINVAL, "mcx_batch_tau_int: unknown observable %d", observable);
+    if (observable == MCX_OBS_Q && b->model != MCX_BLUME_CAPEL)
+        return fail(MCX_EINVAL, "mcx_batch_tau_int: MCX_OBS_Q needs a Blume-Capel batch");
+    if (min_points < 2)
+        return fail(MCX_EINVAL, "min_points must be >= 2");
+    if (!(c > 0))
+        return fail(MCX_EINVAL, "c must be positive");
+    if (first < 0 || count < 0 || first + count > b->rec_len)
+        return fail(MCX_EBOUNDS, "mcx_batch_tau_int: sweeps [%lld, %lld) outside the %lld recorded",
+                    (long long)first, (long long)(first + count), (long long)b->rec_len);
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    const int64_t cap = count / 2;
+    const int64_t lag_cap = max_lag < 0 ? cap : std::min<int64_t>(max_lag, cap);
+    if (count < min_points || lag_cap < 1) {
+        for (int i = 0; i < b->n; ++i)
+            taus[i] = nan;
+        return MCX_OK;
+    }
+    // the 128-bit lag sums hold n^3 * max|x|^2 (|E_pair| <= dim N, |M|, Q <= N)
+    int64_t max_sites = 0;
+    for (int i = 0; i < b->n; ++i)
+        max_sites = std::max(max_sites, b->nsites[i]);
+    double vmax = (double)max_sites * (observable == MCX_OBS_ENERGY ? (double)b->dim : 1.0);
+    if (observable == MCX_OBS_ENERGY && b->model == MCX_BLUME_CAPEL)
+        vmax = std::max(vmax, (double)max_sites);
+    if (3.0 * std::log2((double)count) + 2.0 * std::log2(vmax + 1.0) > 124.0)
+        return fail(MCX_EINVAL, "mcx_batch_tau_int: %lld samples of lattices of %lld sites exceed "
+                                "the exact 128-bit lag sums; use a shorter window",
+                    (long long)count, (long long)max_sites);
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    double* d_tau = nullptr;
+    HIPCHK(hipMalloc((void**)&d_tau, sizeof(double) * b->n));
+    TauArgs a{};
+    a.X = observable == MCX_OBS_ENERGY ? b->d_E : observable == MCX_OBS_Q ? b->d_Q : b->d_M;
+    a.Y = (observable == MCX_OBS_ENERGY && b->model == MCX_BLUME_CAPEL) ? b->d_Q : nullptr;
+    a.chains = b->d_chains;
+    a.cap = b->cap;
+    a.first = first;
+    a.n = count;
+    a.absx = observable == MCX_OBS_ABS_M ? 1 : 0;
+    a.lag_cap = (int32_t)lag_cap;
+    a.c = c;
+    a.n_traces = b->n;
+    a.taus = d_tau;
+    hipError_t e = launch_tau_int(a, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(taus, d_tau, sizeof(double) * b->n, hipMemcpyDeviceToHost, st);
+    mcx_status s = e == hipSuccess ? sync_check(b, "mcx_batch_tau_int")
+                                   : fail(MCX_EHIP, "mcx_batch_tau_int: %s", hipGetErrorString(e));
+    hipFree(d_tau);
+    return s;
 }

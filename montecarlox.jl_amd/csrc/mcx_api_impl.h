@@ -44,6 +44,7 @@ struct mcx_batch {
     std::vector<Launch> launches;
     int64_t cap = 0, rec_len = 0, pool_bytes = 0, max_sites = 0;
     bool tables_ready = false;
+    bool by_table = false; // series / moments recorded per table slot (temperature order)
     double last_ms = 0;
     int8_t* d_spins = nullptr;
     mcx::ChainRec* d_chains = nullptr;

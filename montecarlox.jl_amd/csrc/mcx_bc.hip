@@ -337,6 +337,9 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
     for (int item = blockIdx.x; item < a.n_items; item += gridDim.x) {
         const int ch = a.items[item];
         const ChainRec cr = a.chains[ch];
+        // output slot: the chain, or its ladder slot when recording in temperature order
+        const int64_t oslot = a.by_table ? (int64_t)cr.table : (int64_t)ch;
+        const int64_t sbase = a.by_table ? oslot * a.cap : cr.series_off;
         const long long t_item = a.item_ticks && tid == 0 ? wall_clock64() : 0;
         const uint32_t k0 = (uint32_t)cr.seed;
         const uint32_t c3 = (uint32_t)(cr.seed >> 32);
@@ -361,7 +364,7 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
         // FP64 moments do not depend on how a run is split into launches
         double mE = 0, mE2 = 0, mAM = 0, mM2 = 0, mM4 = 0;
         if (a.record && tid == 0) {
-            const double* mo = a.moments + (int64_t)ch * 6;
+            const double* mo = a.moments + oslot * 6;
             mE = mo[1];
             mE2 = mo[2];
             mAM = mo[3];
@@ -451,7 +454,7 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
             if (series && ((s & 63) == 63 || s == nsw - 1)) {
                 __syncthreads();
                 const int64_t s0 = s & ~(int64_t)63;
-                const int64_t at0 = cr.series_off + a.rec_pos + s0;
+                const int64_t at0 = sbase + a.rec_pos + s0;
                 for (int t = tid; t <= (int)(s - s0); t += B) {
                     a.ser_E[at0 + t] = ring[t];
                     a.ser_M[at0 + t] = ring[64 + t];
@@ -472,7 +475,7 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
             atomicAdd(&a.accepted[ch], acc);
         if (tid == 0) {
             if (a.record) {
-                double* mo = a.moments + (int64_t)ch * 6;
+                double* mo = a.moments + oslot * 6;
                 mo[0] += (double)nsw;
                 mo[1] = mE;
                 mo[2] = mE2;

@@ -76,6 +76,8 @@ struct SweepArgs {
     const int64_t* item_nsweeps; // if set: sweeps of item k (overrides nsweeps)
     int64_t* item_ticks;         // if set: wall-clock ticks item k took
     int32_t lanes_per_word;      // Blume-Capel: lanes sharing one plane word (1, 2, 4, 8; 0 = 1)
+    int32_t by_table;            // series / moments of chain k go to slot chains[k].table
+    int64_t cap;                 // series capacity per slot (slot j starts at j * cap)
 };
 
 struct PlainArgs {
@@ -138,6 +140,20 @@ hipError_t launch_bc_pack(int dim, int unpack, const TileArgs& a, int max_words,
 // clears the sums.
 hipError_t launch_tiles_fold(int model, const TileArgs& a, double* moments, int32_t* last_E,
                              int32_t* last_M, int32_t* last_Q, hipStream_t st);
+// Integrated autocorrelation times of recorded series, one workgroup per slot.
+struct TauArgs {
+    const int32_t* X;        // first series (E_pair, M or Q); slot j at j * cap + first
+    const int32_t* Y;        // Q for the Blume-Capel energy E_pair + D Q (nullptr: X alone)
+    const ChainRec* chains;  // D of slot j
+    int64_t cap, first, n;   // capacity per slot, window start, samples per series
+    int32_t absx;            // use |x| (|M|)
+    int32_t lag_cap;         // largest lag, 1 <= lag_cap <= n / 2
+    double c;                // window constant
+    int32_t n_traces;
+    double* taus;            // [n_traces]
+};
+hipError_t launch_tau_int(const TauArgs& a, hipStream_t st);
+
 hipError_t launch_init_spins(int model, int mode, ChainRec* chains, int32_t n_chains,
                              int64_t max_sites, int8_t* spins, hipStream_t st);
 hipError_t launch_observables(ChainRec* chains, int32_t n_chains, int64_t max_sites,

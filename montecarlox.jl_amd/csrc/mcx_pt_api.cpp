@@ -149,6 +149,7 @@ extern "C" mcx_status mcx_pt_destroy(mcx_pt* p)
     hipSetDevice(p->b->ctx->device);
     hipStreamSynchronize(p->b->ctx->stream);
     p->b->pt = nullptr;
+    p->b->by_table = false;
     pt_free(p);
     delete p;
     return MCX_OK;
@@ -285,6 +286,23 @@ extern "C" mcx_status mcx_pt_get_rounds(mcx_pt* p, uint64_t* rounds)
     HIPCHK(hipMemcpyAsync(rounds, p->d_round, sizeof(uint64_t) * p->n_sets, hipMemcpyDeviceToHost,
                           b->ctx->stream));
     return sync_check(b, "mcx_pt_get_rounds");
+}
+
+extern "C" mcx_status mcx_pt_set_series_order(mcx_pt* p, int32_t order)
+{
+    if (!p)
+        return fail(MCX_EINVAL, "mcx_pt_set_series_order: NULL handle");
+    if (order != MCX_ORDER_REPLICA && order != MCX_ORDER_TEMPERATURE)
+        return fail(MCX_EINVAL, "mcx_pt_set_series_order: order must be MCX_ORDER_REPLICA or "
+                                "MCX_ORDER_TEMPERATURE (got %d)", order);
+    mcx_batch* b = p->b;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    // a series (and its moments) is in one order throughout
+    mcx_status s = mcx_batch_reset_series(b);
+    if (s)
+        return s;
+    b->by_table = order == MCX_ORDER_TEMPERATURE;
+    return MCX_OK;
 }
 
 // ---------------------------------------------------------------------------

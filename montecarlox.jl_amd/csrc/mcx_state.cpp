@@ -59,6 +59,7 @@ extern "C" mcx_status mcx_batch_save_state(mcx_batch* b, uint32_t flags, void* b
     h.shape[4] = b->cap;
     h.shape[5] = b->rec_len;
     h.shape[6] = b->tables_ready ? 1 : 0;
+    h.shape[7] = b->by_table ? 1 : 0;
     BlobWriter w(buf, bytes);
     std::vector<double> betas(n, 0.0);
     for (size_t i = 0; i < n && i < b->betas.size(); ++i)
@@ -157,5 +158,6 @@ extern "C" mcx_status mcx_batch_load_state(mcx_batch* b, const void* buf, size_t
     b->betas = betas;
     b->rec_len = rec_len;
     b->tables_ready = h.shape[6] != 0;
+    b->by_table = h.shape[7] != 0;
     return MCX_OK;
 }

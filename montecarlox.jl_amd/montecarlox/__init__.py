@@ -20,7 +20,8 @@ from .ensembles import (BinnedObject, BoltzmannEnsemble, MulticanonicalEnsemble,
                         update_)
 from .muca import (GeneralizedWalkers, ParallelChains, ParallelMulticanonical, ThreadsBackend,
                    distribute_logweight_, merge_histograms_)
-from .pt import ParallelTempering, ReplicaExchange, acceptance_rates, index, set_betas
+from .pt import (ParallelTempering, ReplicaExchange, acceptance_rates, index,
+                 optimize_exchange_interval_, set_betas)
 from .serving import Job, Server, poisson_workload
 from .spinsystems import (BlumeCapel, Ising, IsingLatticeOptim, SpinSystemBatch, energy, init_,
                           magnetization, spin_flip_, sweep_)

@@ -16,6 +16,8 @@ from ._lib import call
 
 _MODELS = {"ising": _lib.ISING, "blume_capel": _lib.BLUME_CAPEL, "blumecapel": _lib.BLUME_CAPEL}
 _ALGS = {"metropolis": _lib.METROPOLIS, "heatbath": _lib.HEATBATH, "heat_bath": _lib.HEATBATH}
+_OBS = {"energy": 0, "e": 0, "magnetization": 1, "m": 1, "abs_magnetization": 2, "|m|": 2,
+        "q": 3, "sum_sq": 3}
 _INITS = {"up": _lib.INIT_UP, "down": _lib.INIT_DOWN, "zero": _lib.INIT_ZERO,
           "random": _lib.INIT_RANDOM}
 
@@ -201,6 +203,23 @@ class LatticeBatch:
         call("mcx_batch_set_streams", self._h, int(seed), _ptr(k, C.c_uint32) if k is not None else None)
         self.seed = int(seed)
         self.keys = k
+
+    def tau_int(self, observable: str = "energy", first: int = 0, count: Optional[int] = None,
+                min_points: int = 2, max_lag: Optional[int] = None, c: float = 5.0) -> np.ndarray:
+        """integrated_autocorrelation_times(traces; min_points, max_lag, c)
+        (autocorrelations.jl:84-110) of every slot's recorded series over sweeps
+        [first, first + count), computed on the device.  observable: "energy"
+        (E_pair + D*Q), "magnetization", "abs_magnetization" or "q"."""
+        try:
+            obs = _OBS[observable.lower()]
+        except KeyError:
+            raise _lib.MCXArgumentError(f"unknown observable {observable!r}") from None
+        if count is None:
+            count = self.series_length() - first
+        out = np.empty(self.n, np.float64)
+        call("mcx_batch_tau_int", self._h, obs, int(first), int(count), int(min_points),
+             -1 if max_lag is None else int(max_lag), float(c), _ptr(out, C.c_double))
+        return out
 
     # -- snapshots (checkpoint / resume) -------------------------------------------
     def save_state(self, series: bool = True) -> np.ndarray:

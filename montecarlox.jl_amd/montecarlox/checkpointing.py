@@ -149,7 +149,7 @@ class _Encoder:
             p = s._pt
             rec["pt"] = {"ladder": self.array(p.ladder), "n_sets": p.n_sets,
                          "algs": self.value(p.algs), "state": self.array(p.save_state()),
-                         "pt_id": self._id(p)}
+                         "order": p.series_order, "pt_id": self._id(p)}
         if s._walkers is not None:
             w = s._walkers
             rec["walkers"] = {"algs": self.value(w.algs), "state": self.array(w.save_state()),
@@ -248,6 +248,7 @@ class _Decoder:
             p = r["pt"]
             algs = self.value(p["algs"])
             pt = ParallelTempering(s, self.array(p["ladder"]), n_sets=int(p["n_sets"]), algs=algs)
+            pt.series_order = p["order"]  # the batch snapshot carries the device side
             self.memo[p["pt_id"]] = pt
             self.pt_loads.append((pt, self.array(p["state"])))
         if r["walkers"] is not None:

@@ -5,6 +5,8 @@
   ParallelTempering(betas; seed, rng)                  src/algorithms/parallel_tempering.jl:269-298
   set_betas(n, bmin, bmax, mode)                       parallel_tempering.jl:391-407
   acceptance_rates / acceptance_rate / index / reset!  replica_exchange.jl:48-101
+  optimize_exchange_interval!, _retune_exchange_sweeps!
+                                                       parallel_tempering.jl:53-135
 
 The exchange itself runs in libmcx_hip.so (one thread per ladder); this class
 mirrors the device state after every call.
@@ -39,6 +41,7 @@ for _n, _a in {
     "mcx_pt_get_ladder": [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
     "mcx_pt_get_counts": [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
     "mcx_pt_get_rounds": [_P, C.POINTER(C.c_uint64)],
+    "mcx_pt_set_series_order": [_P, C.c_int32],
     "mcx_pt_state_size": [_P, C.POINTER(C.c_size_t)],
     "mcx_pt_save_state": [_P, _P, C.c_size_t],
     "mcx_pt_load_state": [_P, _P, C.c_size_t],
@@ -46,6 +49,7 @@ for _n, _a in {
     _lib.declare(_n, _a)
 
 RUN_GRAPH = 4
+_ORDERS = {"replica": 0, "temperature": 1}
 
 
 def _ptr(a, t):
@@ -107,6 +111,7 @@ class ParallelTempering:
         self._h = C.c_void_p()
         call("mcx_pt_create", systems.batch.handle, C.byref(d), C.byref(self._h))
         self._alg = algorithm_code(self.algs[0])
+        self.series_order = "replica"
         systems._pt = self
         self._sync()
 
@@ -164,6 +169,28 @@ class ParallelTempering:
         self._sync()
         return self.systems.batch.last_run_ms()
 
+    def set_series_order(self, order: str):
+        """"temperature": record series and moments per ladder slot (slot
+        s*R + i - 1 holds whichever replica of set s is at ladder index i);
+        "replica": per chain.  Clears the recorded series and moments."""
+        o = order.lstrip(":").lower()
+        if o not in _ORDERS:
+            raise MCXArgumentError(f"unknown series order {order!r}; use :replica or :temperature")
+        call("mcx_pt_set_series_order", self._h, _ORDERS[o])
+        self.series_order = o
+        return self
+
+    def tau_int(self, observable: str = "energy", first: int = 0, count: Optional[int] = None,
+                min_points: int = 2, max_lag: Optional[int] = None, c: float = 5.0) -> np.ndarray:
+        """tau_int per ladder index from the temperature-ordered device series:
+        shape (n_sets, R), or (R,) for one set."""
+        if self.series_order != "temperature":
+            raise _lib.MCXStateError("tau_int per ladder index needs set_series_order(:temperature) "
+                                     "before the recording")
+        t = self.systems.batch.tau_int(observable, first, count, min_points, max_lag, c)
+        t = t.reshape(self.n_sets, self.R)
+        return t if self.n_sets > 1 else t[0]
+
     def save_state(self) -> np.ndarray:
         """Ladder positions, exchange counters, stages and round counters (uint8)."""
         n = C.c_size_t()
@@ -219,3 +246,73 @@ def acceptance_rates(rx) -> np.ndarray:
 def acceptance_rate_pt(rx) -> float:
     tot = rx.steps.sum()
     return float(rx.accepted.sum() / tot) if tot > 0 else 0.0
+
+
+def _group_samples(samples, n: int):
+    """Per-ladder-index traces from pre-grouped traces or flat (index, value)
+    tuples (parallel_tempering.jl:53-70)."""
+    samples = list(samples)
+    if samples and all(isinstance(s, tuple) and len(s) == 2 for s in samples):
+        grouped = [[] for _ in range(n)]
+        for idx, e in samples:
+            if not 1 <= int(idx) <= n:
+                raise MCXArgumentError(f"sample index {idx} out of bounds for {n} ladders")
+            grouped[int(idx) - 1].append(float(e))
+        return grouped
+    return samples
+
+
+def _retune_exchange_sweeps_(sweeps, taus, base_sweeps: int, min_sweeps: int, max_sweeps: int):
+    """sweeps[i] = clamp(round(base * tau_i / median(finite taus)), min, max);
+    a non-finite tau keeps scale 1 (parallel_tempering.jl:123-135).  Rounding
+    is half to even, as Julia's round(Int, x)."""
+    taus = np.asarray(taus, np.float64)
+    finite = taus[np.isfinite(taus)]
+    tau_ref = float(np.median(finite)) if finite.size else 1.0
+    for i in range(len(sweeps)):
+        scale = taus[i] / tau_ref if np.isfinite(taus[i]) else 1.0
+        target = int(np.rint(base_sweeps * scale))
+        sweeps[i] = min(max(target, min_sweeps), max_sweeps)
+    return sweeps
+
+
+def optimize_exchange_interval_(pt, local_samples, sweeps_after_exchange, base_sweeps: int,
+                                min_sweeps: int = 1, max_sweeps: int = 2 ** 63 - 1,
+                                min_points: int = 400, max_lag: Optional[int] = 200,
+                                observable: str = "energy", first: int = 0,
+                                count: Optional[int] = None):
+    """optimize_exchange_interval!(pt, local_samples, sweeps_after_exchange;
+    base_sweeps, min_sweeps, max_sweeps, min_points, max_lag)
+    (parallel_tempering.jl:83-121): retune the post-exchange sweeps per ladder
+    index from tau_int of the energy at each index; returns
+    sweeps_after_exchange[index(pt)] (one value per replica).
+
+    local_samples: per-index traces or flat (index, value) tuples as in the
+    reference, or None for the temperature-ordered series recorded on the
+    device (set_series_order(:temperature); sweeps [first, first + count)),
+    whose tau_int is computed there.  sweeps_after_exchange (numpy int array,
+    updated in place) has length R, or shape (n_sets, R) with device samples:
+    every set is retuned from its own ladder."""
+    R, S = pt.R, pt.n_sets
+    sw = sweeps_after_exchange
+    if local_samples is None:
+        taus = np.asarray(pt.tau_int(observable, first, count, min_points, max_lag)).reshape(S, R)
+        rows = np.asarray(sw).reshape(S, R) if np.ndim(sw) == 2 else None
+        if rows is None and (S != 1 or len(sw) != R):
+            raise MCXArgumentError("sweeps_after_exchange must have length size(pt) (n_sets x R)")
+        for s in range(S):
+            row = sw[s] if rows is not None else sw
+            _retune_exchange_sweeps_(row, taus[s], base_sweeps, min_sweeps, max_sweeps)
+    else:
+        if S != 1:
+            raise MCXArgumentError("host samples describe one ladder; use a one-set object or the "
+                                   "device series")
+        if len(sw) != R:
+            raise MCXArgumentError("sweeps_after_exchange must have length size(pt)")
+        from .measurements import integrated_autocorrelation_times
+        grouped = _group_samples(local_samples, R)
+        taus = integrated_autocorrelation_times(grouped, min_points=min_points, max_lag=max_lag)
+        _retune_exchange_sweeps_(sw, taus, base_sweeps, min_sweeps, max_sweeps)
+    out = np.asarray(sw).reshape(S, R)
+    sel = np.take_along_axis(out, pt.indices.reshape(S, R) - 1, axis=1)
+    return sel if S > 1 else sel[0]

@@ -108,6 +108,10 @@ def _declare(L):
     L.orc_refpath_run.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _i32p, _f64p, nf64, nf64,
                                   C.c_int64, C.c_double, C.c_double, C.c_uint64, C.c_int64, C.c_int]
     L.orc_refpath_run.restype = C.c_int64
+    L.orc_tau_int.argtypes = [_i32p, n32, C.c_int64, C.c_double, C.c_int, C.c_int32, C.c_double]
+    L.orc_tau_int.restype = C.c_double
+    L.orc_tau_int_centred.argtypes = [_f64p, C.c_int64, C.c_int32, C.c_double]
+    L.orc_tau_int_centred.restype = C.c_double
 
 
 # --------------------------------------------------------------------------
@@ -247,3 +251,37 @@ def refpath_run(model, dim, Ls, betas, seed, nsweeps, nthreads, D=None, lw=None,
     return lib().orc_refpath_run(model, dim, 0 if lwv is None else 1, len(Ls), Ls, betas, Dv, lwv,
                                  0 if lwv is None else len(lwv), float(bin_start), float(bin_step),
                                  seed, nsweeps, nthreads)
+
+
+def _lag_cap(n, max_lag):
+    return n // 2 if max_lag is None else min(int(max_lag), n // 2)
+
+
+def tau_int(x, y=None, D=0.0, absx=False, max_lag=None, c=5.0):
+    """integrated_autocorrelation_time of an integer series (exact lag sums);
+    with y: of the energy x + D*y."""
+    x = np.ascontiguousarray(x, np.int32)
+    yy = None if y is None else np.ascontiguousarray(y, np.int32)
+    return lib().orc_tau_int(x, yy, x.size, float(D), int(absx), _lag_cap(x.size, max_lag), c)
+
+
+def tau_int_centred(x, max_lag=None, c=5.0):
+    """integrated_autocorrelation_time as the reference evaluates it (centred FP64)."""
+    x = np.ascontiguousarray(x, np.float64)
+    return lib().orc_tau_int_centred(x, x.size, _lag_cap(x.size, max_lag), c)
+
+
+def tau_int_times(traces, ys=None, D=None, absx=False, min_points=2, max_lag=None, c=5.0):
+    """integrated_autocorrelation_times (autocorrelations.jl:84-110) over integer
+    traces: NaN below min_points or for an empty lag window."""
+    out = np.full(len(traces), np.nan)
+    for k, x in enumerate(traces):
+        n = len(x)
+        if n < min_points:
+            continue
+        cap = _lag_cap(n, max_lag)
+        if cap < 1:
+            continue
+        out[k] = tau_int(x, None if ys is None else ys[k], 0.0 if D is None else D[k], absx,
+                         cap, c)
+    return out

@@ -834,3 +834,100 @@ int64_t orc_refpath_run(int model, int dim, int muca, int nchains, const int32_t
     free(th);
     return total;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Integrated autocorrelation time (src/measurements/autocorrelations.jl)    */
+/* ------------------------------------------------------------------------ */
+
+typedef __int128 orc_i128;
+
+static int64_t orc_tau_at(const int32_t* x, int64_t i, int absx)
+{
+    int64_t v = x[i];
+    return absx && v < 0 ? -v : v;
+}
+
+/* n^2 sum_{i<n-t} (x_i - Sx/n)(y_{i+t} - Sy/n), exactly */
+static orc_i128 orc_tau_cross(const int32_t* x, int ax, const int32_t* y, int ay, int64_t n,
+                              int64_t t)
+{
+    orc_i128 P = 0;
+    int64_t Sx = 0, Sy = 0, Hx = 0, Ty = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        Sx += orc_tau_at(x, i, ax);
+        Sy += orc_tau_at(y, i, ay);
+    }
+    for (int64_t i = 0; i + t < n; ++i) {
+        P += (orc_i128)(orc_tau_at(x, i, ax) * orc_tau_at(y, i + t, ay));
+        Hx += orc_tau_at(x, i, ax);
+        Ty += orc_tau_at(y, i + t, ay);
+    }
+    orc_i128 nn = n;
+    return nn * nn * P - nn * (orc_i128)Sy * Hx - nn * (orc_i128)Sx * Ty +
+           (orc_i128)(n - t) * (orc_i128)Sx * (orc_i128)Sy;
+}
+
+static double orc_tau_N(const int32_t* x, const int32_t* y, int64_t n, double D, int absx,
+                        int64_t t)
+{
+    if (!y)
+        return (double)orc_tau_cross(x, absx, x, absx, n, t);
+    orc_i128 xx = orc_tau_cross(x, 0, x, 0, n, t);
+    orc_i128 xy = orc_tau_cross(x, 0, y, 0, n, t);
+    orc_i128 yx = orc_tau_cross(y, 0, x, 0, n, t);
+    orc_i128 yy = orc_tau_cross(y, 0, y, 0, n, t);
+    return ((double)xx + D * (double)(xy + yx)) + (D * D) * (double)yy;
+}
+
+double orc_tau_int(const int32_t* x, const int32_t* y, int64_t n, double D, int absx,
+                   int32_t lag_cap, double c)
+{
+    /* integrated_autocorrelation_time, autocorrelations.jl:28-73: tau = 1/2 +
+     * sum C(t), stopped at C(t) <= 0 or t > c * tau; 1/2 for zero variance */
+    if (y && D == 0.0)
+        y = NULL;
+    double C0 = orc_tau_N(x, y, n, D, absx, 0) / (double)n;
+    if (!(C0 > 0))
+        return 0.5;
+    double tau = 0.5;
+    for (int64_t t = 1; t <= lag_cap; ++t) {
+        double C = (orc_tau_N(x, y, n, D, absx, t) / (double)(n - t)) / C0;
+        if (C <= 0)
+            break;
+        double tau_next = tau + C;
+        if ((double)t > c * tau_next)
+            break;
+        tau = tau_next;
+    }
+    return tau > 0.5 ? tau : 0.5;
+}
+
+double orc_tau_int_centred(const double* x, int64_t n, int32_t lag_cap, double c)
+{
+    /* autocorrelations.jl:47-73 as written: mu = sum(x) / n, centred samples,
+     * C0 = dot(c, c) / n, cov(t) = dot(c[1:n-t], c[1+t:n]) / (n - t) */
+    double mu = 0;
+    for (int64_t i = 0; i < n; ++i)
+        mu += x[i];
+    mu /= (double)n;
+    double C0 = 0;
+    for (int64_t i = 0; i < n; ++i)
+        C0 += (x[i] - mu) * (x[i] - mu);
+    C0 /= (double)n;
+    if (!(C0 > 0))
+        return 0.5;
+    double tau = 0.5;
+    for (int64_t t = 1; t <= lag_cap; ++t) {
+        double cov = 0;
+        for (int64_t i = 0; i + t < n; ++i)
+            cov += (x[i] - mu) * (x[i + t] - mu);
+        double C = (cov / (double)(n - t)) / C0;
+        if (C <= 0)
+            break;
+        double tau_next = tau + C;
+        if ((double)t > c * tau_next)
+            break;
+        tau = tau_next;
+    }
+    return tau > 0.5 ? tau : 0.5;
+}

@@ -121,6 +121,18 @@ int64_t orc_refpath_run(int model, int dim, int muca, int nchains, const int32_t
                         double bin_start, double bin_step, uint64_t seed, int64_t nsweeps,
                         int nthreads);
 
+/* Integrated autocorrelation time (src/measurements/autocorrelations.jl:28-73)
+ * of one integer series: x (|x| if absx), or the energy x + D*y when y is not
+ * NULL; lag window 1..lag_cap (1 <= lag_cap <= n/2), window constant c.  The
+ * centred lag sums are evaluated exactly in 128-bit integers,
+ *   N(t) = n^2 sum_{i<n-t} (x_i - mean)(x_{i+t} - mean),
+ * and C(t) = (N(t) / (n - t)) / (N(0) / n) in FP64. */
+double orc_tau_int(const int32_t* x, const int32_t* y, int64_t n, double D, int absx,
+                   int32_t lag_cap, double c);
+/* The reference's own evaluation, for an FP64 series: centred samples,
+ * C0 = dot(c, c) / n, cov(t) = dot(c[0:n-t], c[t:n]) / (n - t), sequential sums. */
+double orc_tau_int_centred(const double* x, int64_t n, int32_t lag_cap, double c);
+
 #ifdef __cplusplus
 }
 #endif
