@@ -57,6 +57,10 @@ typedef int32_t mcx_status;
 #define MCX_HEATBATH 1
 #define MCX_MULTICANONICAL 2
 #define MCX_WANGLANDAU 3
+#define MCX_GLAUBER 4     /* Glauber(rng; beta): logistic acceptance, metropolis.jl:108-127 */
+/* walker kind: Metropolis(rng, CustomEnsemble(BoltzmannEnsemble(beta), MulticanonicalEnsemble(bins
+ * over sum s_i^2))) on Blume-Capel, docs/src/examples/spin_systems/muca_BlumeCapel.jl:43-89 */
+#define MCX_MUCA_PAIR_SPIN2 5
 
 /* initial states, init!(sys, type) */
 #define MCX_INIT_UP 0
@@ -254,13 +258,15 @@ mcx_status mcx_pt_set_series_order(mcx_pt* pt, int32_t order);
 /* ------------------------------------------------------------------------ */
 
 typedef struct {
-    int32_t kind;                  /* MCX_MULTICANONICAL or MCX_WANGLANDAU                     */
+    int32_t kind;                  /* MCX_MULTICANONICAL, MCX_WANGLANDAU or MCX_MUCA_PAIR_SPIN2 */
     int32_t n_groups;              /* walkers of one group share a binning (and lattice size) */
     const int32_t* group_of_chain; /* n_chains entries, or NULL = all in group 0; the first  */
                                    /* chain of a group is its root                            */
     const double* bin_start;       /* per group: DiscreteBinning(start, step, num)            */
     const double* bin_step;
     const int64_t* n_bins;
+    const double* pair_beta;       /* MCX_MUCA_PAIR_SPIN2: n_chains betas of the Boltzmann    */
+                                   /* pair term; NULL for the other kinds                     */
 } mcx_gen_desc;
 
 /* Replaces Multicanonical(rng, bins) / WangLandau(rng, bins; logf) per chain
@@ -286,8 +292,15 @@ mcx_status mcx_gen_get_logf(mcx_gen* g, double* logf);
  * and the call returns MCX_EBOUNDS.  flags: MCX_RUN_RECORD. */
 mcx_status mcx_gen_run(mcx_gen* g, int64_t nsweeps, uint32_t flags);
 /* reset!(alg): histogram and steps/accepted to zero (algorithms/multicanonical.jl:27-33);
- * chain < 0: every walker. */
+ * for MCX_MUCA_PAIR_SPIN2 walkers (Metropolis algorithms) steps/accepted only, as
+ * reset!(::AbstractImportanceSampling) (importance_sampling.jl:104-110); chain < 0:
+ * every walker. */
 mcx_status mcx_gen_reset(mcx_gen* g, int32_t chain);
+/* The histogram alone to zero: reset_histogram!(::CustomEnsemble) of
+ * muca_BlumeCapel.jl:63-66, or of any walker kind; chain < 0: every walker. */
+mcx_status mcx_gen_reset_histogram(mcx_gen* g, int32_t chain);
+/* MCX_MUCA_PAIR_SPIN2: the beta of a walker's Boltzmann pair term (chain < 0: every walker). */
+mcx_status mcx_gen_set_pair_beta(mcx_gen* g, int32_t chain, double beta);
 /* update!(::MulticanonicalEnsemble): lw -= log(h) where h > 0
  * (src/ensembles/multicanonical.jl:32-44); chain < 0: every walker. */
 mcx_status mcx_gen_update(mcx_gen* g, int32_t chain);

@@ -81,12 +81,30 @@ void fill_ising_hb(IsingTable& t, int dim, const double* p)
     }
 }
 
+void fill_ising_gl(IsingTable& t, int dim, const double* p)
+{
+    t.gl_always = t.gl_never = 0;
+    for (int k = 0; k < 8; ++k)
+        t.gl_thr[k] = 0;
+    for (int k = 0; k <= 2 * dim; ++k) {
+        const uint64_t T = threshold32(p[k]);
+        if (T >> 32)
+            t.gl_always |= 1u << k;
+        else if (T == 0)
+            t.gl_never |= 1u << k;
+        t.gl_thr[k] = (uint32_t)T;
+    }
+}
+
 // Metropolis: class k (anti-aligned neighbours) has dE = 2(2 dim - 2k) and
 // p = exp(logweight(BoltzmannEnsemble(beta), dE)) = exp(-beta * dE)
 // (src/ensembles/boltzmann.jl:28).  Heat bath: class k (up neighbours) has
 // local field h = 2k - 2 dim, dE = 2 s h and p_plus = logistic((beta * s) * dE)
 // (SpinSystems/src/ising.jl:49), which equals logistic(beta * 2h) for s = +-1.
-void ising_probabilities(double beta, int dim, double* metro, double* hb)
+// Glauber: class k (anti-aligned neighbours, 0..2 dim) flips with
+// p = logistic(logweight(BoltzmannEnsemble(beta), dE)) = logistic(-beta * dE)
+// (src/algorithms/metropolis.jl:121-127).
+void ising_probabilities(double beta, int dim, double* metro, double* hb, double* gl)
 {
     for (int k = 0; k < dim; ++k) {
         const int dE = 2 * (2 * dim - 2 * k);
@@ -95,7 +113,19 @@ void ising_probabilities(double beta, int dim, double* metro, double* hb)
     for (int k = 0; k <= 2 * dim; ++k) {
         const int h = 2 * k - 2 * dim;
         hb[k] = logistic(beta * (double)(2 * h));
+        const int dE = 2 * (2 * dim - 2 * k);
+        gl[k] = logistic(-beta * (double)dE);
     }
+}
+
+void fill_ising_table(IsingTable& t, int dim, double beta)
+{
+    std::memset(&t, 0, sizeof t);
+    double pm[4], ph[8], pg[8];
+    ising_probabilities(beta, dim, pm, ph, pg);
+    fill_ising_metro(t, dim, pm);
+    fill_ising_hb(t, dim, ph);
+    fill_ising_gl(t, dim, pg);
 }
 
 } // namespace mcx
@@ -227,13 +257,8 @@ mcx_status set_chain_tables(mcx_batch* b, const double* beta)
 {
     if (b->model == MCX_ISING) {
         std::vector<IsingTable> t(b->n);
-        for (int i = 0; i < b->n; ++i) {
-            std::memset(&t[i], 0, sizeof t[i]);
-            double pm[4], ph[8];
-            ising_probabilities(beta[i], b->dim, pm, ph);
-            fill_ising_metro(t[i], b->dim, pm);
-            fill_ising_hb(t[i], b->dim, ph);
-        }
+        for (int i = 0; i < b->n; ++i)
+            fill_ising_table(t[i], b->dim, beta[i]);
         HIPCHK(hipMemcpyAsync(b->d_itab, t.data(), sizeof(IsingTable) * b->n,
                               hipMemcpyHostToDevice, b->ctx->stream));
     } else {
@@ -437,12 +462,12 @@ extern "C" mcx_status mcx_batch_table_layout(mcx_batch* b, int32_t alg, int32_t*
     if (b->model == MCX_ISING) {
         if (alg == MCX_METROPOLIS)
             *n = b->dim;
-        else if (alg == MCX_HEATBATH)
+        else if (alg == MCX_HEATBATH || alg == MCX_GLAUBER)
             *n = 2 * b->dim + 1;
         else
             return fail(MCX_EINVAL, "mcx_batch_table_layout: alg %d has no table", alg);
     } else {
-        if (alg == MCX_METROPOLIS)
+        if (alg == MCX_METROPOLIS || alg == MCX_GLAUBER)
             *n = 6 * (4 * b->dim + 1);
         else if (alg == MCX_HEATBATH)
             *n = 3 * (4 * b->dim + 1);
@@ -500,6 +525,8 @@ extern "C" mcx_status mcx_batch_set_tables(mcx_batch* b, int32_t alg, const doub
         for (int i = 0; i < b->n; ++i) {
             if (alg == MCX_METROPOLIS)
                 fill_ising_metro(t[i], b->dim, p + (int64_t)i * n);
+            else if (alg == MCX_GLAUBER)
+                fill_ising_gl(t[i], b->dim, p + (int64_t)i * n);
             else
                 fill_ising_hb(t[i], b->dim, p + (int64_t)i * n);
         }
@@ -511,6 +538,8 @@ extern "C" mcx_status mcx_batch_set_tables(mcx_batch* b, int32_t alg, const doub
         for (int i = 0; i < b->n; ++i) {
             if (alg == MCX_METROPOLIS)
                 fill_bc_metro_from_p(t[i], b->dim, p + (int64_t)i * n);
+            else if (alg == MCX_GLAUBER)
+                fill_bc_gl_from_p(t[i], b->dim, p + (int64_t)i * n);
             else
                 fill_bc_hb_from_p(t[i], b->dim, p + (int64_t)i * n);
         }
@@ -657,8 +686,9 @@ mcx_status check_run(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t flags,
 {
     if (!b)
         return fail(MCX_EINVAL, "%s: NULL batch", who);
-    if (alg != MCX_METROPOLIS && alg != MCX_HEATBATH)
-        return fail(MCX_EINVAL, "%s: alg must be MCX_METROPOLIS or MCX_HEATBATH (got %d)", who, alg);
+    if (alg != MCX_METROPOLIS && alg != MCX_HEATBATH && alg != MCX_GLAUBER)
+        return fail(MCX_EINVAL, "%s: alg must be MCX_METROPOLIS, MCX_HEATBATH or MCX_GLAUBER (got %d)",
+                    who, alg);
     if (nsweeps < 0)
         return fail(MCX_EINVAL, "%s: nsweeps must be >= 0", who);
     if (flags & ~(MCX_RUN_RECORD | MCX_RUN_PLAIN))

@@ -143,13 +143,16 @@ inline StateHeader make_state_header(uint32_t kind, uint32_t flags)
 mcx_status fail(mcx_status code, const char* fmt, ...);
 uint64_t threshold32(double p);
 double logistic(double x);
-void ising_probabilities(double beta, int dim, double* metro, double* hb);
+void ising_probabilities(double beta, int dim, double* metro, double* hb, double* gl);
 void fill_ising_metro(IsingTable& t, int dim, const double* p);
 void fill_ising_hb(IsingTable& t, int dim, const double* p);
+void fill_ising_gl(IsingTable& t, int dim, const double* p);
+void fill_ising_table(IsingTable& t, int dim, double beta);
 void fill_bc_table(BCTable& t, int dim, double beta, double D);
-void bc_probabilities(int dim, double beta, double D, double* metro, double* hb);
+void bc_probabilities(int dim, double beta, double D, double* metro, double* hb, double* gl);
 void fill_bc_metro_from_p(BCTable& t, int dim, const double* p);
 void fill_bc_hb_from_p(BCTable& t, int dim, const double* p);
+void fill_bc_gl_from_p(BCTable& t, int dim, const double* p);
 size_t lattice_lds_bytes(int model, int dim, int L);
 int lattice_block(int dim, int L);
 mcx_status upload_chains(mcx_batch* b);

@@ -41,7 +41,8 @@ static int8_t propose_host(int s_old, int alt)
 // Probabilities in the layout of mcx_batch_table_layout:
 //   metro[((s_old+1)*2 + alt)*NS + (sum + 2*dim)] = exp(-beta * dE)
 //   hb[(sum + 2*dim)*3 + {0,1,2}] = (w1, w1 + w2, z)
-void bc_probabilities(int dim, double beta, double D, double* metro, double* hb)
+//   gl[same index as metro] = logistic(-beta * dE)  (Glauber, metropolis.jl:121-127)
+void bc_probabilities(int dim, double beta, double D, double* metro, double* hb, double* gl)
 {
     const int NS = 4 * dim + 1;
     for (int so = -1; so <= 1; ++so)
@@ -54,6 +55,7 @@ void bc_probabilities(int dim, double beta, double D, double* metro, double* hb)
                 const double dpair = (double)ds * coupling;
                 const double dE = (-dpair - 0.0) + D * (double)ds2;
                 metro[((so + 1) * 2 + alt) * NS + (sum + 2 * dim)] = std::exp(-beta * dE);
+                gl[((so + 1) * 2 + alt) * NS + (sum + 2 * dim)] = logistic(-beta * dE);
             }
     for (int sum = -2 * dim; sum <= 2 * dim; ++sum) {
         const double coupling = 1.0 * (double)sum;
@@ -72,26 +74,38 @@ void bc_probabilities(int dim, double beta, double D, double* metro, double* hb)
     }
 }
 
-void fill_bc_metro_from_p(BCTable& t, int dim, const double* p)
+// 31-bit thresholds ceil(p * 2^31) on the uniform w >> 1; bit k of `always`
+// marks p >= 1 (accept without comparing).
+static void fill_bc_thr31(uint32_t* thr, uint32_t* always, int dim, const double* p)
 {
     const int n = 6 * (4 * dim + 1);
-    t.metro_always[0] = t.metro_always[1] = t.metro_always[2] = 0;
+    always[0] = always[1] = always[2] = 0;
     for (int k = 0; k < 80; ++k)
-        t.metro_thr[k] = 0;
+        thr[k] = 0;
     for (int k = 0; k < n; ++k) {
         if (p[k] >= 1) {
-            t.metro_always[k >> 5] |= 1u << (k & 31);
-            t.metro_thr[k] = 0x80000000u;
+            always[k >> 5] |= 1u << (k & 31);
+            thr[k] = 0x80000000u;
         } else {
             const double v = std::ceil(std::ldexp(p[k] > 0 ? p[k] : 0.0, 31));
             if (v >= 2147483648.0) {
-                t.metro_always[k >> 5] |= 1u << (k & 31);
-                t.metro_thr[k] = 0x80000000u;
+                always[k >> 5] |= 1u << (k & 31);
+                thr[k] = 0x80000000u;
             } else {
-                t.metro_thr[k] = (uint32_t)v;
+                thr[k] = (uint32_t)v;
             }
         }
     }
+}
+
+void fill_bc_metro_from_p(BCTable& t, int dim, const double* p)
+{
+    fill_bc_thr31(t.metro_thr, t.metro_always, dim, p);
+}
+
+void fill_bc_gl_from_p(BCTable& t, int dim, const double* p)
+{
+    fill_bc_thr31(t.gl_thr, t.gl_always, dim, p);
 }
 
 void fill_bc_hb_from_p(BCTable& t, int dim, const double* p)
@@ -109,10 +123,11 @@ void fill_bc_hb_from_p(BCTable& t, int dim, const double* p)
 void fill_bc_table(BCTable& t, int dim, double beta, double D)
 {
     std::memset(&t, 0, sizeof t);
-    double metro[78], hb[39];
-    bc_probabilities(dim, beta, D, metro, hb);
+    double metro[78], hb[39], gl[78];
+    bc_probabilities(dim, beta, D, metro, hb, gl);
     fill_bc_metro_from_p(t, dim, metro);
     fill_bc_hb_from_p(t, dim, hb);
+    fill_bc_gl_from_p(t, dim, gl);
 }
 
 // ---------------------------------------------------------------------------
@@ -351,7 +366,8 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
         int8_t* __restrict__ sp = a.spins + cr.spin_off;
         const BCTable* tb = a.btab + cr.table;
         for (int k = tid; k < 83; k += B)
-            tabw[k] = k < 80 ? tb->metro_thr[k] : tb->metro_always[k - 80];
+            tabw[k] = k < 80 ? (a.glauber ? tb->gl_thr[k] : tb->metro_thr[k])
+                             : (a.glauber ? tb->gl_always[k - 80] : tb->metro_always[k - 80]);
         for (int k = tid; k < 39; k += B)
             tabd[k] = tb->hb[k / 3][k % 3];
 
@@ -536,7 +552,9 @@ __global__ void __launch_bounds__(256) k_bc_plain(PlainArgs a, int n_col)
         if (ALG == 0) {
             const int alt = (int)(rw & 1u);
             const int k = ((s_old + 1) * 2 + alt) * NS + sidx;
-            const bool ok = ((tb.metro_always[k >> 5] >> (k & 31)) & 1u) || (rw >> 1) < tb.metro_thr[k];
+            const uint32_t* thr = a.glauber ? tb.gl_thr : tb.metro_thr;
+            const uint32_t* alw = a.glauber ? tb.gl_always : tb.metro_always;
+            const bool ok = ((alw[k >> 5] >> (k & 31)) & 1u) || (rw >> 1) < thr[k];
             s_new = ok ? bc_propose(s_old, alt) : s_old;
         } else {
             const double rr = ((double)rw * 0x1.0p-32) * tb.hb[sidx][2];
@@ -588,7 +606,8 @@ __global__ void __launch_bounds__(256) k_bc_tiles(TileArgs a)
     const ChainRec cr = a.chains[ch];
     const BCTable* tb = a.btab + cr.table;
     for (int k = tid; k < 83; k += blockDim.x)
-        tabw[k] = k < 80 ? tb->metro_thr[k] : tb->metro_always[k - 80];
+        tabw[k] = k < 80 ? (a.glauber ? tb->gl_thr[k] : tb->metro_thr[k])
+                         : (a.glauber ? tb->gl_always[k - 80] : tb->metro_always[k - 80]);
     for (int k = tid; k < 39; k += blockDim.x)
         tabd[k] = tb->hb[k / 3][k % 3];
     __syncthreads();
@@ -667,15 +686,20 @@ __global__ void __launch_bounds__(256) k_bc_pack(TileArgs a, int unpack)
         bc_pack_word<DIM>(sp, P, w, L, nw, half, words);
 }
 
-hipError_t launch_bc_tiles(int dim, int alg, const TileArgs& a, int max_words, hipStream_t st)
+// Metropolis and Glauber share the kernels (ALG 0) and differ in the threshold
+// table, chosen by the glauber flag; heat bath is ALG 1.
+hipError_t launch_bc_tiles(int dim, int alg, const TileArgs& a0, int max_words, hipStream_t st)
 {
+    TileArgs a = a0;
+    a.glauber = alg == MCX_GLAUBER ? 1 : 0;
+    const bool hb = alg == MCX_HEATBATH;
     const int lpw = a.lpw > 1 ? a.lpw : 1;
     const dim3 grid(((int64_t)max_words * lpw + 255) / 256, a.n_items);
-    if (dim == 2 && alg == 0)
+    if (dim == 2 && !hb)
         hipLaunchKernelGGL((k_bc_tiles<2, 0>), grid, dim3(256), 0, st, a);
     else if (dim == 2)
         hipLaunchKernelGGL((k_bc_tiles<2, 1>), grid, dim3(256), 0, st, a);
-    else if (alg == 0)
+    else if (!hb)
         hipLaunchKernelGGL((k_bc_tiles<3, 0>), grid, dim3(256), 0, st, a);
     else
         hipLaunchKernelGGL((k_bc_tiles<3, 1>), grid, dim3(256), 0, st, a);
@@ -692,29 +716,35 @@ hipError_t launch_bc_pack(int dim, int unpack, const TileArgs& a, int max_words,
     return hipGetLastError();
 }
 
-hipError_t launch_bc_sweeps(int dim, int alg, const SweepArgs& a, int block, int grid,
+hipError_t launch_bc_sweeps(int dim, int alg, const SweepArgs& a0, int block, int grid,
                             size_t lds_bytes, hipStream_t st)
 {
-    if (dim == 2 && alg == 0)
+    SweepArgs a = a0;
+    a.glauber = alg == MCX_GLAUBER ? 1 : 0;
+    const bool hb = alg == MCX_HEATBATH;
+    if (dim == 2 && !hb)
         hipLaunchKernelGGL((k_bc_sweeps<2, 0>), dim3(grid), dim3(block), lds_bytes, st, a);
     else if (dim == 2)
         hipLaunchKernelGGL((k_bc_sweeps<2, 1>), dim3(grid), dim3(block), lds_bytes, st, a);
-    else if (alg == 0)
+    else if (!hb)
         hipLaunchKernelGGL((k_bc_sweeps<3, 0>), dim3(grid), dim3(block), lds_bytes, st, a);
     else
         hipLaunchKernelGGL((k_bc_sweeps<3, 1>), dim3(grid), dim3(block), lds_bytes, st, a);
     return hipGetLastError();
 }
 
-hipError_t launch_bc_plain(int dim, int alg, const PlainArgs& a, int n_col, hipStream_t st)
+hipError_t launch_bc_plain(int dim, int alg, const PlainArgs& a0, int n_col, hipStream_t st)
 {
+    PlainArgs a = a0;
+    a.glauber = alg == MCX_GLAUBER ? 1 : 0;
+    const bool hb = alg == MCX_HEATBATH;
     const int block = 256;
     const int grid = (n_col + block - 1) / block;
-    if (dim == 2 && alg == 0)
+    if (dim == 2 && !hb)
         hipLaunchKernelGGL((k_bc_plain<2, 0>), dim3(grid), dim3(block), 0, st, a, n_col);
     else if (dim == 2)
         hipLaunchKernelGGL((k_bc_plain<2, 1>), dim3(grid), dim3(block), 0, st, a, n_col);
-    else if (alg == 0)
+    else if (!hb)
         hipLaunchKernelGGL((k_bc_plain<3, 0>), dim3(grid), dim3(block), 0, st, a, n_col);
     else
         hipLaunchKernelGGL((k_bc_plain<3, 1>), dim3(grid), dim3(block), 0, st, a, n_col);

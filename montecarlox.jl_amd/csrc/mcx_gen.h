@@ -8,7 +8,7 @@ namespace mcx {
 struct GenArgs {
     const int32_t* items;   // walkers of this launch (all from one binning group)
     int32_t n_items;
-    int32_t kind;           // 2 = multicanonical, 3 = Wang-Landau
+    int32_t kind;           // 2 = multicanonical, 3 = Wang-Landau, 5 = pair Boltzmann + muca on sum s^2
     int32_t model;          // 0 Ising (integer energy, div binning), 1 Blume-Capel (FP64, round)
     int32_t dim;
     int32_t record;
@@ -33,6 +33,7 @@ struct GenArgs {
     int32_t* ser_Q;
     int64_t nsweeps;
     int64_t rec_pos;
+    const double* pair_beta; // per walker (kind 5): beta of the Boltzmann pair term
 };
 
 hipError_t launch_gen_sweeps(const GenArgs& a, size_t lds_bytes, hipStream_t st);

@@ -19,6 +19,14 @@
 // unless lr falls within 1e-9 of log(u).  Spin writes and visit counts (order
 // does not matter for either) are applied by the lanes after the walk.
 // Histograms are uint32 in LDS when they fit, otherwise int64 atomics in HBM.
+//
+// KIND 2 (Blume-Capel): the CustomEnsemble of docs/src/examples/spin_systems/
+// muca_BlumeCapel.jl:43-89 -- a Boltzmann weight on the pair term and a
+// multicanonical weight on Q = sum s_i^2:
+//   logweight(H) = logweight(pair, H[1]) + logweight(spin2, H[2])
+//               = (-beta) * P + lw(Q),   H = (J * sum_pair_interactions, sum_spins2),
+// P = sum_<ij> s_i s_j (J = 1), evaluated for the old and the new state as
+// accept!(alg, H_new, H_old) does; the visit goes to Q's bin.
 #include "mcx_device.h"
 #include "mcx_gen.h"
 
@@ -42,9 +50,11 @@ __device__ __forceinline__ int gen_propose(int s_old, int alt)
     return s_old == -1 ? (alt ? 0 : 1) : (s_old == 0 ? (alt ? -1 : 1) : (alt ? -1 : 0));
 }
 
-template <int MODEL, int DIM, bool WL>
+template <int MODEL, int DIM, int KIND>
 __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
 {
+    constexpr bool WL = KIND == 1; // Wang-Landau
+    constexpr bool PS = KIND == 2; // Boltzmann pair term + multicanonical sum s^2
     extern __shared__ __align__(16) unsigned char lds_raw[];
     double* lw = reinterpret_cast<double*>(lds_raw);
     // one chunk's staged attempts and the lane-0 walk's results
@@ -161,7 +171,20 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
         double Eo = -(double)pair + D * (double)qq;
         long long Ei = -(long long)pair + (MODEL == 0 ? 0 : Di * (long long)qq);
         int b_old = 0;
-        bool old_ok = int_e ? ibin(Ei, b_old) : bin_fp(Eo, b_old);
+        bool old_ok = PS ? bin_int((long long)qq, b_old) : (int_e ? ibin(Ei, b_old) : bin_fp(Eo, b_old));
+        const double nbeta = PS ? -a.pair_beta[wk] : 0.0;
+        // bin of Q (integer binning, truncating div), lane-0 state
+        auto qbin0 = [&](int Qv, int& b) -> bool {
+            const int d = Qv - start32;
+            int bb;
+            if (step_pow2)
+                bb = d >= 0 ? (d >> a.step_shift) : -((-d) >> a.step_shift);
+            else
+                bb = d / (int)a.bin_step;
+            const bool in = bb >= 0 && bb < nb;
+            b = in ? bb : 0;
+            return in;
+        };
         int64_t s = 0;
         for (; s < a.nsweeps && err < 0; ++s) {
             const uint64_t sweep = cr.sweep + (uint64_t)s;
@@ -221,7 +244,67 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                     const int cnt = n_col - base < 64 ? n_col - base : 64;
                     // ---- walk the attempts in site order (wave-uniform state) ----
                     int my_vis = -1, my_ok = 0, n_done = cnt;
-                    if (int_e) {
+                    if (PS) {
+                        // lane 0 walks the state (P, Q); every attempt evaluates the
+                        // old and new weights as the reference does (the old one
+                        // is the last accepted new one: lw does not change during
+                        // a multicanonical walk)
+                        st_dE[lane] = t_dq;
+                        st_dp[lane] = t_dpair;
+                        st_mq[lane] = t_mq;
+                        st_luh[lane] = t_luh;
+                        st_lul[lane] = t_lul;
+                        st_u[lane] = t_u;
+                        st_site[lane] = t_site;
+                        __syncthreads();
+                        int err_v = -1;
+                        if (lane == 0) {
+                            int kk = 0;
+                            if (!old_ok) {
+                                n_done = 0;
+                                err_v = st_site[0];
+                                kk = cnt;
+                            }
+                            double w_cur = kk < cnt ? nbeta * (double)pair + lw[b_old] : 0.0;
+                            for (int k = kk; k < cnt; ++k) {
+                                const int Pn = pair + st_dp[k];
+                                const int Qn = qq + st_dE[k];
+                                int bn;
+                                if (!qbin0(Qn, bn)) {
+                                    n_done = k;
+                                    err_v = st_site[k];
+                                    break;
+                                }
+                                const double w_new = nbeta * (double)Pn + lw[bn];
+                                const double lr = w_new - w_cur;
+                                // lr > 0 || u < exp(lr), exp only inside the 1e-9 band
+                                bool ok = lr > 0 || lr > st_luh[k];
+                                if (!ok && !(lr < st_lul[k]))
+                                    ok = st_u[k] < exp(lr);
+                                ++steps;
+                                nacc += ok ? 1u : 0u;
+                                if (ok) {
+                                    const int mq = st_mq[k];
+                                    pair = Pn;
+                                    qq = Qn;
+                                    mm += (mq & 0xffff) - 8;
+                                    b_old = bn;
+                                    w_cur = w_new;
+                                }
+                                st_res[k] = b_old | (ok ? (int)0x80000000 : 0);
+                            }
+                        }
+                        n_done = __builtin_amdgcn_readfirstlane(n_done);
+                        err_v = __builtin_amdgcn_readfirstlane(err_v);
+                        if (err_v >= 0)
+                            err = err_v;
+                        __syncthreads();
+                        if (lane < n_done) {
+                            const int r = st_res[lane];
+                            my_vis = r & 0x7fffffff;
+                            my_ok = (r >> 31) & 1;
+                        }
+                    } else if (int_e) {
                         // Integer walk on lane 0 alone: its state stays in vector
                         // registers and every step is a VALU or LDS operation (no
                         // scalar round trips, no per-attempt branches).  The
@@ -458,14 +541,20 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
     }
 }
 
+// kind -> KIND: 2 multicanonical -> 0, 3 Wang-Landau -> 1, 5 pair + sum s^2 -> 2
 template <int M, int D>
 static void gen_launch(const GenArgs& a, size_t lds_bytes, hipStream_t st)
 {
     const dim3 grid(a.n_items), block(64);
     if (a.kind == 3)
-        hipLaunchKernelGGL((k_gen_sweeps<M, D, true>), grid, block, lds_bytes, st, a);
-    else
-        hipLaunchKernelGGL((k_gen_sweeps<M, D, false>), grid, block, lds_bytes, st, a);
+        hipLaunchKernelGGL((k_gen_sweeps<M, D, 1>), grid, block, lds_bytes, st, a);
+    else if constexpr (M == 1) {
+        if (a.kind == 5)
+            hipLaunchKernelGGL((k_gen_sweeps<1, D, 2>), grid, block, lds_bytes, st, a);
+        else
+            hipLaunchKernelGGL((k_gen_sweeps<M, D, 0>), grid, block, lds_bytes, st, a);
+    } else
+        hipLaunchKernelGGL((k_gen_sweeps<M, D, 0>), grid, block, lds_bytes, st, a);
 }
 
 hipError_t launch_gen_sweeps(const GenArgs& a, size_t lds_bytes, hipStream_t st)

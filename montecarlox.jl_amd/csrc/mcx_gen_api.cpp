@@ -26,7 +26,9 @@ struct mcx_gen {
     int64_t* d_err = nullptr;
     int32_t* d_members = nullptr;
     double* d_flat = nullptr;
+    double* d_pbeta = nullptr; // kind MCX_MUCA_PAIR_SPIN2: beta of each walker's pair term
     std::vector<double> logf;
+    std::vector<double> pbeta;
     // groups run side by side on their own streams (forked from / joined to
     // the context stream)
     static constexpr int kStreams = 4;
@@ -55,6 +57,7 @@ void gen_free(mcx_gen* g)
     hipFree(g->d_err);
     hipFree(g->d_members);
     hipFree(g->d_flat);
+    hipFree(g->d_pbeta);
 }
 
 mcx_status check_chain(mcx_gen* g, int32_t chain, const char* who)
@@ -92,10 +95,26 @@ extern "C" mcx_status mcx_gen_create(mcx_batch* b, const mcx_gen_desc* d, mcx_ge
     if (!b || !d || !out)
         return fail(MCX_EINVAL, "mcx_gen_create: NULL argument");
     *out = nullptr;
-    if (d->kind != MCX_MULTICANONICAL && d->kind != MCX_WANGLANDAU)
-        return fail(MCX_EINVAL, "mcx_gen_create: kind must be MCX_MULTICANONICAL or MCX_WANGLANDAU");
+    if (d->kind != MCX_MULTICANONICAL && d->kind != MCX_WANGLANDAU && d->kind != MCX_MUCA_PAIR_SPIN2)
+        return fail(MCX_EINVAL, "mcx_gen_create: kind must be MCX_MULTICANONICAL, MCX_WANGLANDAU or "
+                                "MCX_MUCA_PAIR_SPIN2");
     if (d->n_groups < 1 || !d->bin_start || !d->bin_step || !d->n_bins)
         return fail(MCX_EINVAL, "mcx_gen_create: need n_groups >= 1 and per-group binnings");
+    if (d->kind == MCX_MUCA_PAIR_SPIN2) {
+        if (b->model != MCX_BLUME_CAPEL)
+            return fail(MCX_EINVAL, "mcx_gen_create: MCX_MUCA_PAIR_SPIN2 walkers need a Blume-Capel "
+                                    "batch (sum s^2 is constant for Ising)");
+        if (!d->pair_beta)
+            return fail(MCX_EINVAL, "mcx_gen_create: MCX_MUCA_PAIR_SPIN2 needs pair_beta");
+        for (int i = 0; i < b->n; ++i)
+            if (!std::isfinite(d->pair_beta[i]))
+                return fail(MCX_EINVAL, "mcx_gen_create: pair_beta[%d] is not finite", i);
+        for (int k = 0; k < d->n_groups; ++k)
+            if (d->bin_start[k] != std::floor(d->bin_start[k]) || !(d->bin_step[k] >= 1) ||
+                d->bin_step[k] != std::floor(d->bin_step[k]) || std::fabs(d->bin_start[k]) > 1e9)
+                return fail(MCX_EINVAL, "mcx_gen_create: group %d: sum s^2 is an integer; start and "
+                                        "step must be integral (step >= 1)", k);
+    }
     auto* g = new mcx_gen();
     g->b = b;
     g->kind = d->kind;
@@ -155,6 +174,9 @@ extern "C" mcx_status mcx_gen_create(mcx_batch* b, const mcx_gen_desc* d, mcx_ge
         flat.insert(flat.end(), g->members[k].begin(), g->members[k].end());
     }
     g->logf.assign(b->n, 1.0);
+    g->pbeta.assign(b->n, 0.0);
+    if (d->kind == MCX_MUCA_PAIR_SPIN2)
+        g->pbeta.assign(d->pair_beta, d->pair_beta + b->n);
     HIPCHK(hipSetDevice(b->ctx->device));
     hipError_t e = hipSuccess;
     auto alloc = [&](void** p, size_t bytes) {
@@ -170,6 +192,7 @@ extern "C" mcx_status mcx_gen_create(mcx_batch* b, const mcx_gen_desc* d, mcx_ge
     alloc((void**)&g->d_err, sizeof(int64_t) * b->n);
     alloc((void**)&g->d_members, sizeof(int32_t) * b->n);
     alloc((void**)&g->d_flat, sizeof(double) * 3);
+    alloc((void**)&g->d_pbeta, sizeof(double) * b->n);
     for (int k = 0; k < mcx_gen::kStreams && e == hipSuccess; ++k) {
         e = hipStreamCreateWithFlags(&g->gs[k], hipStreamNonBlocking);
         if (e == hipSuccess)
@@ -190,6 +213,8 @@ extern "C" mcx_status mcx_gen_create(mcx_batch* b, const mcx_gen_desc* d, mcx_ge
         e = hipMemcpyAsync(g->d_off, g->off.data(), sizeof(int64_t) * b->n, hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(g->d_logf, g->logf.data(), sizeof(double) * b->n, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(g->d_pbeta, g->pbeta.data(), sizeof(double) * b->n, hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(g->d_members, flat.data(), sizeof(int32_t) * b->n, hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
@@ -353,6 +378,7 @@ extern "C" mcx_status mcx_gen_run(mcx_gen* g, int64_t nsweeps, uint32_t flags)
         a.ser_Q = b->d_Q;
         a.nsweeps = nsweeps;
         a.rec_pos = b->rec_len;
+        a.pair_beta = g->d_pbeta;
         HIPCHK(launch_gen_sweeps(a, gen_lds(g->nbins[k], ns, a.hist_in_lds), g->gs[oi % nstr]));
     }
     for (int q = 0; q < nstr; ++q) {
@@ -380,10 +406,17 @@ extern "C" mcx_status mcx_gen_run(mcx_gen* g, int64_t nsweeps, uint32_t flags)
     return MCX_OK;
 }
 
-extern "C" mcx_status mcx_gen_reset(mcx_gen* g, int32_t chain)
+namespace {
+
+mcx_status gen_reset(mcx_gen* g, int32_t chain, bool hist, bool counters, const char* who)
 {
     if (!g)
-        return fail(MCX_EINVAL, "mcx_gen_reset: NULL handle");
+        return fail(MCX_EINVAL, "%s: NULL handle", who);
+    if (chain >= 0) {
+        mcx_status s = check_chain(g, chain, who);
+        if (s)
+            return s;
+    }
     mcx_batch* b = g->b;
     HIPCHK(hipSetDevice(b->ctx->device));
     hipStream_t st = b->ctx->stream;
@@ -391,11 +424,51 @@ extern "C" mcx_status mcx_gen_reset(mcx_gen* g, int32_t chain)
         if (chain >= 0 && chain != i)
             continue;
         const int64_t nb = g->nbins[g->group_of[i]];
-        HIPCHK(hipMemsetAsync(g->d_hist + g->off[i], 0, sizeof(unsigned long long) * nb, st));
-        HIPCHK(hipMemsetAsync(g->d_steps + i, 0, sizeof(unsigned long long), st));
-        HIPCHK(hipMemsetAsync(g->d_acc + i, 0, sizeof(unsigned long long), st));
+        if (hist)
+            HIPCHK(hipMemsetAsync(g->d_hist + g->off[i], 0, sizeof(unsigned long long) * nb, st));
+        if (counters) {
+            HIPCHK(hipMemsetAsync(g->d_steps + i, 0, sizeof(unsigned long long), st));
+            HIPCHK(hipMemsetAsync(g->d_acc + i, 0, sizeof(unsigned long long), st));
+        }
     }
-    return sync_check(b, "mcx_gen_reset");
+    return sync_check(b, who);
+}
+
+} // namespace
+
+extern "C" mcx_status mcx_gen_reset(mcx_gen* g, int32_t chain)
+{
+    // reset!(alg): the multicanonical algorithm clears its histogram too; a
+    // Metropolis algorithm over a CustomEnsemble only its counters
+    return gen_reset(g, chain, !g || g->kind != MCX_MUCA_PAIR_SPIN2, true, "mcx_gen_reset");
+}
+
+extern "C" mcx_status mcx_gen_reset_histogram(mcx_gen* g, int32_t chain)
+{
+    return gen_reset(g, chain, true, false, "mcx_gen_reset_histogram");
+}
+
+extern "C" mcx_status mcx_gen_set_pair_beta(mcx_gen* g, int32_t chain, double beta)
+{
+    if (!g)
+        return fail(MCX_EINVAL, "mcx_gen_set_pair_beta: NULL handle");
+    if (g->kind != MCX_MUCA_PAIR_SPIN2)
+        return fail(MCX_ESTATE, "mcx_gen_set_pair_beta: only MCX_MUCA_PAIR_SPIN2 walkers have a "
+                                "pair term");
+    if (chain >= 0) {
+        mcx_status s = check_chain(g, chain, "mcx_gen_set_pair_beta");
+        if (s)
+            return s;
+    }
+    if (!std::isfinite(beta))
+        return fail(MCX_EINVAL, "mcx_gen_set_pair_beta: beta must be finite");
+    for (int i = 0; i < g->b->n; ++i)
+        if (chain < 0 || chain == i)
+            g->pbeta[i] = beta;
+    HIPCHK(hipSetDevice(g->b->ctx->device));
+    HIPCHK(hipMemcpyAsync(g->d_pbeta, g->pbeta.data(), sizeof(double) * g->b->n,
+                          hipMemcpyHostToDevice, g->b->ctx->stream));
+    return sync_check(g->b, "mcx_gen_set_pair_beta");
 }
 
 extern "C" mcx_status mcx_gen_update(mcx_gen* g, int32_t chain)

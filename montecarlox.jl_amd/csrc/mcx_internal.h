@@ -40,7 +40,9 @@ struct IsingTable {
     uint32_t metro_always, metro_never;
     uint32_t hb_thr[8];    // class = number of up neighbours (0..2*dim)
     uint32_t hb_always, hb_never;
-    uint32_t pad[2];
+    uint32_t gl_thr[8];    // Glauber: class = number of anti-aligned neighbours (0..2*dim)
+    uint32_t gl_always, gl_never;
+    uint32_t pad[4];
 };
 
 // Blume-Capel acceptance for one (beta, D).  Metropolis: per (s_old, alt,
@@ -53,6 +55,9 @@ struct BCTable {
     uint32_t metro_always[3];
     uint32_t pad;
     double hb[13][3];
+    uint32_t gl_thr[80];    // Glauber: as metro_thr with p = logistic(-beta * dE)
+    uint32_t gl_always[3];
+    uint32_t gl_pad;
 };
 
 struct SweepArgs {
@@ -78,6 +83,7 @@ struct SweepArgs {
     int32_t lanes_per_word;      // Blume-Capel: lanes sharing one plane word (1, 2, 4, 8; 0 = 1)
     int32_t by_table;            // series / moments of chain k go to slot chains[k].table
     int64_t cap;                 // series capacity per slot (slot j starts at j * cap)
+    int32_t glauber;             // Blume-Capel: Glauber thresholds instead of Metropolis ones
 };
 
 struct PlainArgs {
@@ -93,6 +99,7 @@ struct PlainArgs {
     uint64_t sweep;     // sweep counter value for this sweep
     int32_t chain;
     int32_t colour;
+    int32_t glauber;    // Blume-Capel: Glauber thresholds instead of Metropolis ones
 };
 
 constexpr int kTileObsReplicas = 16;
@@ -119,6 +126,7 @@ struct TileArgs {
     int32_t max_s;
     int32_t lpw;               // Blume-Capel: lanes sharing one plane word (1, 2, 4, 8; 0 = 1)
     unsigned long long* accepted;
+    int32_t glauber;           // Blume-Capel: Glauber thresholds instead of Metropolis ones
 };
 
 // Launchers (defined in the .hip translation units).

@@ -5,6 +5,9 @@
 // src/algorithms/importance_sampling.jl:80-85):
 //   Metropolis: dE = 2 s_i sum_j s_j; accept iff -beta*dE > 0 or u < exp(-beta*dE)
 //   heat bath:  s_new = +1 iff u < logistic(beta * s_i * dE); flip iff s_new != s_i
+//   Glauber:    flip iff u < logistic(-beta * dE) (src/algorithms/metropolis.jl:121-127),
+//               u from the Metropolis stream
+// (ALG template argument: 0 Metropolis, 1 heat bath, 2 Glauber)
 // visited in checkerboard order (colour 0 then colour 1; one colour's sites
 // have disjoint neighbourhoods, so the parallel update of a colour equals the
 // sequential one).
@@ -23,6 +26,7 @@
 // colour per chain per sweep, one thread per site, int8 spins in HBM.  It
 // evaluates the same bit-sliced uniform and therefore gives identical results.
 #include "mcx_device.h"
+#include "../../include/mcx_hip.h"
 
 namespace mcx {
 
@@ -194,6 +198,35 @@ __device__ __forceinline__ uint32_t ising_word(uint32_t s0, const NbrWords& n, u
         }
         const uint32_t lt = bitserial_less<NMC>(need, m, thr, group, sw_lo, w2, c3, k0, k1);
         return s0 ^ ((freem | forced | lt) & valid);
+    } else if (ALG == 2) {
+        // Glauber: every class k (anti-aligned neighbours, 0..2 dim) has a threshold
+        uint32_t b0, b1, b2;
+        if (DIM == 2)
+            count4(s0 ^ n.U, s0 ^ n.Dn, s0 ^ n.A, s0 ^ n.Bw, b0, b1, b2);
+        else
+            count6(s0 ^ n.U, s0 ^ n.Dn, s0 ^ n.A, s0 ^ n.Bw, s0 ^ n.Zu, s0 ^ n.Zd, b0, b1, b2);
+        uint32_t m[NHC];
+        m[0] = count_eq<0>(b0, b1, b2) & valid;
+        m[1] = count_eq<1>(b0, b1, b2) & valid;
+        m[2] = count_eq<2>(b0, b1, b2) & valid;
+        m[3] = count_eq<3>(b0, b1, b2) & valid;
+        m[4] = count_eq<4>(b0, b1, b2) & valid;
+        if (DIM == 3) {
+            m[NHC - 2] = count_eq<5>(b0, b1, b2) & valid;
+            m[NHC - 1] = count_eq<6>(b0, b1, b2) & valid;
+        }
+        uint32_t forced = 0, need = 0;
+        uint32_t thr[NHC];
+#pragma unroll
+        for (int k = 0; k < NHC; ++k) {
+            thr[k] = tb.gl_thr[k];
+            if ((tb.gl_always >> k) & 1u)
+                forced |= m[k];
+            else if (!((tb.gl_never >> k) & 1u))
+                need |= m[k];
+        }
+        const uint32_t lt = bitserial_less<NHC>(need, m, thr, group, sw_lo, w2, c3, k0, k1);
+        return s0 ^ ((forced | lt) & valid);
     } else {
         uint32_t b0, b1, b2;
         if (DIM == 2)
@@ -237,7 +270,7 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int nwaves = (B + 63) >> 6;
-    constexpr uint32_t purpose = ALG == 0 ? P_METRO : P_HB;
+    constexpr uint32_t purpose = ALG == 1 ? P_HB : P_METRO;
 
     for (int item = blockIdx.x; item < a.n_items; item += gridDim.x) {
         const int ch = a.items[item];
@@ -458,6 +491,18 @@ __global__ void __launch_bounds__(256) k_ising_plain(PlainArgs a, int n_col)
                                          ctr_word2(a.sweep, 0, P_METRO), c3, k0, cr.key) != 0u;
             if (ok)
                 snew = (int8_t)-si;
+        } else if (ALG == 2) {
+            const int n_anti = (2 * DIM - si * nb) >> 1;
+            bool ok;
+            if ((tb.gl_always >> n_anti) & 1u)
+                ok = true;
+            else if ((tb.gl_never >> n_anti) & 1u)
+                ok = false;
+            else
+                ok = bitserial_less_site(tb.gl_thr[n_anti], group, t, sw_lo,
+                                         ctr_word2(a.sweep, 0, P_METRO), c3, k0, cr.key) != 0u;
+            if (ok)
+                snew = (int8_t)-si;
         } else {
             const int n_up = (nb + 2 * DIM) >> 1;
             bool up;
@@ -505,7 +550,7 @@ __global__ void __launch_bounds__(256) k_ising_tiles(TileArgs a)
     const int item = blockIdx.y;
     if (a.s >= a.item_nsw[item])
         return;
-    constexpr uint32_t purpose = ALG == 0 ? P_METRO : P_HB;
+    constexpr uint32_t purpose = ALG == 1 ? P_HB : P_METRO;
     const int ch = a.items[item];
     const ChainRec cr = a.chains[ch];
     const int L = cr.L;
@@ -568,17 +613,24 @@ __global__ void __launch_bounds__(256) k_ising_pack(TileArgs a, int unpack)
         ising_pack_word<DIM>(sp, P, w, L, nw, half, words);
 }
 
+// algorithm code -> kernel template: 0 Metropolis, 1 heat bath, 2 Glauber
+static int ising_alg(int alg)
+{
+    return alg == MCX_HEATBATH ? 1 : alg == MCX_GLAUBER ? 2 : 0;
+}
+
 hipError_t launch_ising_tiles(int dim, int alg, const TileArgs& a, int max_words, hipStream_t st)
 {
     const dim3 grid((max_words + 255) / 256, a.n_items);
-    if (dim == 2 && alg == 0)
-        hipLaunchKernelGGL((k_ising_tiles<2, 0>), grid, dim3(256), 0, st, a);
-    else if (dim == 2)
-        hipLaunchKernelGGL((k_ising_tiles<2, 1>), grid, dim3(256), 0, st, a);
-    else if (alg == 0)
-        hipLaunchKernelGGL((k_ising_tiles<3, 0>), grid, dim3(256), 0, st, a);
-    else
-        hipLaunchKernelGGL((k_ising_tiles<3, 1>), grid, dim3(256), 0, st, a);
+    const int k = ising_alg(alg) + (dim == 3 ? 3 : 0);
+    switch (k) {
+    case 0: hipLaunchKernelGGL((k_ising_tiles<2, 0>), grid, dim3(256), 0, st, a); break;
+    case 1: hipLaunchKernelGGL((k_ising_tiles<2, 1>), grid, dim3(256), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_ising_tiles<2, 2>), grid, dim3(256), 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_ising_tiles<3, 0>), grid, dim3(256), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_ising_tiles<3, 1>), grid, dim3(256), 0, st, a); break;
+    default: hipLaunchKernelGGL((k_ising_tiles<3, 2>), grid, dim3(256), 0, st, a); break;
+    }
     return hipGetLastError();
 }
 
@@ -595,29 +647,31 @@ hipError_t launch_ising_pack(int dim, int unpack, const TileArgs& a, int max_wor
 hipError_t launch_ising_sweeps(int dim, int alg, const SweepArgs& a, int block, int grid,
                                size_t lds_bytes, hipStream_t st)
 {
-    if (dim == 2 && alg == 0)
-        hipLaunchKernelGGL((k_ising_sweeps<2, 0>), dim3(grid), dim3(block), lds_bytes, st, a);
-    else if (dim == 2)
-        hipLaunchKernelGGL((k_ising_sweeps<2, 1>), dim3(grid), dim3(block), lds_bytes, st, a);
-    else if (alg == 0)
-        hipLaunchKernelGGL((k_ising_sweeps<3, 0>), dim3(grid), dim3(block), lds_bytes, st, a);
-    else
-        hipLaunchKernelGGL((k_ising_sweeps<3, 1>), dim3(grid), dim3(block), lds_bytes, st, a);
+    const dim3 g(grid), bl(block);
+    const int k = ising_alg(alg) + (dim == 3 ? 3 : 0);
+    switch (k) {
+    case 0: hipLaunchKernelGGL((k_ising_sweeps<2, 0>), g, bl, lds_bytes, st, a); break;
+    case 1: hipLaunchKernelGGL((k_ising_sweeps<2, 1>), g, bl, lds_bytes, st, a); break;
+    case 2: hipLaunchKernelGGL((k_ising_sweeps<2, 2>), g, bl, lds_bytes, st, a); break;
+    case 3: hipLaunchKernelGGL((k_ising_sweeps<3, 0>), g, bl, lds_bytes, st, a); break;
+    case 4: hipLaunchKernelGGL((k_ising_sweeps<3, 1>), g, bl, lds_bytes, st, a); break;
+    default: hipLaunchKernelGGL((k_ising_sweeps<3, 2>), g, bl, lds_bytes, st, a); break;
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_ising_plain(int dim, int alg, const PlainArgs& a, int n_col, hipStream_t st)
 {
-    const int block = 256;
-    const int grid = (n_col + block - 1) / block;
-    if (dim == 2 && alg == 0)
-        hipLaunchKernelGGL((k_ising_plain<2, 0>), dim3(grid), dim3(block), 0, st, a, n_col);
-    else if (dim == 2)
-        hipLaunchKernelGGL((k_ising_plain<2, 1>), dim3(grid), dim3(block), 0, st, a, n_col);
-    else if (alg == 0)
-        hipLaunchKernelGGL((k_ising_plain<3, 0>), dim3(grid), dim3(block), 0, st, a, n_col);
-    else
-        hipLaunchKernelGGL((k_ising_plain<3, 1>), dim3(grid), dim3(block), 0, st, a, n_col);
+    const dim3 g((n_col + 255) / 256), bl(256);
+    const int k = ising_alg(alg) + (dim == 3 ? 3 : 0);
+    switch (k) {
+    case 0: hipLaunchKernelGGL((k_ising_plain<2, 0>), g, bl, 0, st, a, n_col); break;
+    case 1: hipLaunchKernelGGL((k_ising_plain<2, 1>), g, bl, 0, st, a, n_col); break;
+    case 2: hipLaunchKernelGGL((k_ising_plain<2, 2>), g, bl, 0, st, a, n_col); break;
+    case 3: hipLaunchKernelGGL((k_ising_plain<3, 0>), g, bl, 0, st, a, n_col); break;
+    case 4: hipLaunchKernelGGL((k_ising_plain<3, 1>), g, bl, 0, st, a, n_col); break;
+    default: hipLaunchKernelGGL((k_ising_plain<3, 2>), g, bl, 0, st, a, n_col); break;
+    }
     return hipGetLastError();
 }
 

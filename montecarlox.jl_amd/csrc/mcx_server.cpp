@@ -235,8 +235,9 @@ mcx_status validate(const mcx_server* s, const mcx_job_desc* d)
         return fail(MCX_EINVAL,
                     "mcx_server_submit: L = %d; the two-colour checkerboard needs an even L >= 4",
                     d->L);
-    if (d->alg != MCX_METROPOLIS && d->alg != MCX_HEATBATH)
-        return fail(MCX_EINVAL, "mcx_server_submit: alg must be MCX_METROPOLIS or MCX_HEATBATH");
+    if (d->alg != MCX_METROPOLIS && d->alg != MCX_HEATBATH && d->alg != MCX_GLAUBER)
+        return fail(MCX_EINVAL, "mcx_server_submit: alg must be MCX_METROPOLIS, MCX_HEATBATH or "
+                                "MCX_GLAUBER");
     if (!std::isfinite(d->beta) || d->beta < 0 || !std::isfinite(d->D))
         return fail(MCX_EINVAL, "mcx_server_submit: beta must be finite and >= 0, D finite");
     if (d->nsweeps < 1)
@@ -301,11 +302,7 @@ hipError_t setup_jobs(mcx_server* s, std::vector<Job>& fresh)
         r.D = j.d.model == MCX_ISING ? 0.0 : j.d.D;
         e = hipMemcpyAsync(s->d_chains + j.slot, &r, sizeof r, hipMemcpyHostToDevice, st);
         if (e == hipSuccess && j.d.model == MCX_ISING) {
-            std::memset(&it[i], 0, sizeof it[i]);
-            double pm[4], ph[8];
-            ising_probabilities(j.d.beta, j.d.dim, pm, ph);
-            fill_ising_metro(it[i], j.d.dim, pm);
-            fill_ising_hb(it[i], j.d.dim, ph);
+            fill_ising_table(it[i], j.d.dim, j.d.beta);
             e = hipMemcpyAsync(s->d_itab + j.slot, &it[i], sizeof it[i], hipMemcpyHostToDevice,
                                st);
         } else if (e == hipSuccess) {

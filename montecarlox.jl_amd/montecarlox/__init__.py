@@ -15,9 +15,9 @@ from .algorithms import (AbstractHeatBath, AbstractImportanceSampling, AbstractM
 from .batch import LatticeBatch
 from .checkpointing import (CheckpointSession, checkpoint_, finalize_, init_checkpoint,
                             restore_checkpoint)
-from .ensembles import (BinnedObject, BoltzmannEnsemble, MulticanonicalEnsemble,
-                        WangLandauEnsemble, get_centers, get_edges, get_values, logweight, set_,
-                        update_)
+from .ensembles import (BinnedObject, BoltzmannEnsemble, CustomEnsemble, MulticanonicalEnsemble,
+                        WangLandauEnsemble, get_centers, get_edges, get_values, logweight,
+                        reset_histogram_, set_, update_)
 from .muca import (GeneralizedWalkers, ParallelChains, ParallelMulticanonical, ThreadsBackend,
                    distribute_logweight_, merge_histograms_)
 from .pt import (ParallelTempering, ReplicaExchange, acceptance_rates, index,

@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(os.path.dirname(_PKG), "libmcx_hip.so")
 # status codes
 MCX_OK, MCX_EINVAL, MCX_EBOUNDS, MCX_EDOMAIN, MCX_ENOMEM, MCX_EHIP, MCX_ESTATE = range(7)
 ISING, BLUME_CAPEL = 0, 1
-METROPOLIS, HEATBATH, MULTICANONICAL, WANGLANDAU = 0, 1, 2, 3
+METROPOLIS, HEATBATH, MULTICANONICAL, WANGLANDAU, GLAUBER = 0, 1, 2, 3, 4
+MUCA_PAIR_SPIN2 = 5  # Boltzmann pair term + multicanonical sum s^2 (Blume-Capel walkers)
 INIT_UP, INIT_DOWN, INIT_ZERO, INIT_RANDOM = 0, 1, 2, 3
 RUN_RECORD, RUN_PLAIN = 1, 2
 STATE_SERIES = 1  # MCX_STATE_SERIES: snapshots carry the recorded series

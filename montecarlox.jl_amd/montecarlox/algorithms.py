@@ -79,7 +79,8 @@ class Metropolis(AbstractMetropolis):
 
 
 class Glauber(Metropolis):
-    """Glauber(rng; beta): logistic acceptance (metropolis.jl:218-237)."""
+    """Glauber(rng; beta) / Glauber(rng, ensemble): logistic acceptance
+    p = logistic(logweight(ensemble, dE)) (metropolis.jl:108-127)."""
 
 
 class AbstractHeatBath(AbstractAlgorithm):
@@ -152,7 +153,9 @@ def algorithm_code(alg) -> int:
     if isinstance(alg, HeatBath):
         return _lib.HEATBATH
     if isinstance(alg, Glauber):
-        raise MCXArgumentError("Glauber sweeps are not on the GPU path")
+        if not isinstance(alg.ensemble, BoltzmannEnsemble):
+            raise MCXArgumentError("GPU Glauber sweeps need a BoltzmannEnsemble")
+        return _lib.GLAUBER
     if isinstance(alg, Metropolis):
         if not isinstance(alg.ensemble, BoltzmannEnsemble):
             raise MCXArgumentError("GPU Metropolis sweeps need a BoltzmannEnsemble")

@@ -15,7 +15,8 @@ from . import _lib
 from ._lib import call
 
 _MODELS = {"ising": _lib.ISING, "blume_capel": _lib.BLUME_CAPEL, "blumecapel": _lib.BLUME_CAPEL}
-_ALGS = {"metropolis": _lib.METROPOLIS, "heatbath": _lib.HEATBATH, "heat_bath": _lib.HEATBATH}
+_ALGS = {"metropolis": _lib.METROPOLIS, "heatbath": _lib.HEATBATH, "heat_bath": _lib.HEATBATH,
+         "glauber": _lib.GLAUBER}
 _OBS = {"energy": 0, "e": 0, "magnetization": 1, "m": 1, "abs_magnetization": 2, "|m|": 2,
         "q": 3, "sum_sq": 3}
 _INITS = {"up": _lib.INIT_UP, "down": _lib.INIT_DOWN, "zero": _lib.INIT_ZERO,

@@ -34,8 +34,8 @@ import numpy as np
 from ._lib import MCXArgumentError
 from .algorithms import Glauber, HeatBath, ImportanceSampling, Metropolis, Philox
 from .batch import LatticeBatch
-from .ensembles import (BinnedObject, BoltzmannEnsemble, ContinuousBinning, DiscreteBinning,
-                        MulticanonicalEnsemble, WangLandauEnsemble)
+from .ensembles import (BinnedObject, BoltzmannEnsemble, ContinuousBinning, CustomEnsemble,
+                        DiscreteBinning, MulticanonicalEnsemble, WangLandauEnsemble)
 
 FORMAT = "montecarlox-checkpoint"
 FORMAT_VERSION = 1
@@ -45,7 +45,7 @@ def _host_classes():
     from .muca import ParallelChains, ThreadsBackend
     return {c.__name__: c for c in (
         Philox, Metropolis, Glauber, HeatBath, ImportanceSampling, BoltzmannEnsemble,
-        MulticanonicalEnsemble, WangLandauEnsemble, BinnedObject, DiscreteBinning,
+        MulticanonicalEnsemble, WangLandauEnsemble, CustomEnsemble, BinnedObject, DiscreteBinning,
         ContinuousBinning, ParallelChains, ThreadsBackend)}
 
 

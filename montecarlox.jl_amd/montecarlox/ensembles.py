@@ -263,6 +263,42 @@ class WangLandauEnsemble(AbstractEnsemble):
         return None
 
 
+class CustomEnsemble(AbstractEnsemble):
+    """The two-part ensemble of docs/src/examples/spin_systems/muca_BlumeCapel.jl:43-68:
+    a Boltzmann weight on the pair term and a multicanonical weight on sum s^2,
+    logweight(H) = logweight(pair, H[0]) + logweight(spin2, H[1]) for
+    H = (J * sum_pair_interactions, sum_spins2); visits go to spin2's histogram.
+    On the GPU, Metropolis(rng, CustomEnsemble(...)) drives Blume-Capel walkers."""
+
+    def __init__(self, pair: BoltzmannEnsemble, spin2: MulticanonicalEnsemble,
+                 record_visits: bool = True):
+        if not isinstance(pair, BoltzmannEnsemble) or not isinstance(spin2, MulticanonicalEnsemble):
+            raise MCXArgumentError("CustomEnsemble(pair::BoltzmannEnsemble, spin2::MulticanonicalEnsemble)")
+        self.pair = pair
+        self.spin2 = spin2
+        self.record_visits = bool(record_visits)
+
+    def logweight(self, H):
+        return self.pair.logweight(H[0]) + self.spin2.logweight(H[1])
+
+    def __call__(self, H):
+        return self.logweight(H)
+
+
+def reset_histogram_(obj):
+    """reset_histogram!(ens or alg) of muca_BlumeCapel.jl:63-69: zero spin2's
+    histogram (on the device when the ensemble drives device walkers)."""
+    ens = getattr(obj, "ensemble", obj)
+    if isinstance(ens, CustomEnsemble):
+        dev = getattr(ens, "_dev", None)
+        if dev is not None:
+            dev[0].reset_histogram_(dev[1])
+        else:
+            ens.spin2.histogram.values[...] = 0
+        return None
+    raise MCXArgumentError("reset_histogram! takes a CustomEnsemble or an algorithm using one")
+
+
 def logweight(ens, x=None):
     if x is None:
         return ens.logweight if not isinstance(ens, BoltzmannEnsemble) else ens

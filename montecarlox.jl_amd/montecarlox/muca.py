@@ -6,6 +6,8 @@
                                                            src/ensembles/{multicanonical,wang_landau}.jl
   ParallelMulticanonical, merge_histograms!, distribute_logweight!
                                                            src/algorithms/parallel_multicanonical.jl
+  Metropolis(rng, CustomEnsemble(pair, spin2)) on Blume-Capel, reset_histogram!
+                                                           docs/src/examples/spin_systems/muca_BlumeCapel.jl:43-89
 
 GeneralizedWalkers binds the ensembles of a list of ImportanceSampling
 algorithms to device walkers (mcx_gen).  While bound, the device holds the
@@ -21,14 +23,16 @@ import numpy as np
 
 from . import _lib
 from ._lib import MCXArgumentError, call
-from .algorithms import ImportanceSampling
-from .ensembles import BinnedObject, DiscreteBinning, MulticanonicalEnsemble, WangLandauEnsemble
+from .algorithms import ImportanceSampling, Metropolis
+from .ensembles import (BinnedObject, CustomEnsemble, DiscreteBinning, MulticanonicalEnsemble,
+                        WangLandauEnsemble)
 
 
 class GenDesc(C.Structure):
     _fields_ = [("kind", C.c_int32), ("n_groups", C.c_int32),
                 ("group_of_chain", C.POINTER(C.c_int32)), ("bin_start", C.POINTER(C.c_double)),
-                ("bin_step", C.POINTER(C.c_double)), ("n_bins", C.POINTER(C.c_int64))]
+                ("bin_step", C.POINTER(C.c_double)), ("n_bins", C.POINTER(C.c_int64)),
+                ("pair_beta", C.POINTER(C.c_double))]
 
 
 _P = C.c_void_p
@@ -52,6 +56,8 @@ for _n, _a in {
     "mcx_gen_get_counters": [_P, _pi64, _pi64],
     "mcx_gen_group_info": [_P, C.c_int32, _pi32, _pi32, _pi64],
     "mcx_muca_update_values": [_P, _pd, _pd, C.c_int64],
+    "mcx_gen_reset_histogram": [_P, C.c_int32],
+    "mcx_gen_set_pair_beta": [_P, C.c_int32, C.c_double],
     "mcx_gen_state_size": [_P, C.POINTER(C.c_size_t)],
     "mcx_gen_save_state": [_P, _P, C.c_size_t],
     "mcx_gen_load_state": [_P, _P, C.c_size_t],
@@ -72,6 +78,19 @@ def muca_update_values(lw: np.ndarray, h: np.ndarray, ctx=None):
     call("mcx_muca_update_values", ctx.handle, _ptr(lw, C.c_double), _ptr(hh, C.c_double), lw.size)
 
 
+def is_walker_alg(a) -> bool:
+    """Algorithms that run as device walkers: multicanonical / Wang-Landau
+    ImportanceSampling, or Metropolis over a CustomEnsemble."""
+    return isinstance(a, ImportanceSampling) or (type(a) is Metropolis and
+                                                 isinstance(a.ensemble, CustomEnsemble))
+
+
+def _binned(a):
+    """The ensemble holding the walker's log-weights and histogram."""
+    e = a.ensemble
+    return e.spin2 if isinstance(e, CustomEnsemble) else e
+
+
 def _binning(ens):
     b = ens.logweight.bins
     if not isinstance(b, DiscreteBinning):
@@ -89,16 +108,28 @@ class GeneralizedWalkers:
         if len(algs) != len(systems):
             raise MCXArgumentError("one algorithm per system")
         kinds = {type(a.ensemble) for a in algs}
-        if kinds == {MulticanonicalEnsemble}:
+        if kinds == {MulticanonicalEnsemble} and all(isinstance(a, ImportanceSampling) for a in algs):
             self.kind = _lib.MULTICANONICAL
-        elif kinds == {WangLandauEnsemble}:
+        elif kinds == {WangLandauEnsemble} and all(isinstance(a, ImportanceSampling) for a in algs):
             self.kind = _lib.WANGLANDAU
+        elif kinds == {CustomEnsemble} and all(type(a) is Metropolis for a in algs):
+            self.kind = _lib.MUCA_PAIR_SPIN2
+            for a in algs:
+                b = a.ensemble.spin2.logweight.bins
+                if not isinstance(b, DiscreteBinning) or b.step < 1 or \
+                        float(b.start) != int(b.start) or float(b.step) != int(b.step) or \
+                        (b.step != 1 and not isinstance(b.step, (int, np.integer))):
+                    raise MCXArgumentError("sum s^2 is an integer: bin it with an integer range "
+                                           "start:step:stop")
+                if not a.ensemble.record_visits:
+                    raise MCXArgumentError("device walkers record every visit (record_visits=true)")
         else:
-            raise MCXArgumentError("all walkers use MulticanonicalEnsemble or all WangLandauEnsemble")
+            raise MCXArgumentError("all walkers use MulticanonicalEnsemble, all WangLandauEnsemble, "
+                                   "or all Metropolis over a CustomEnsemble")
         self.systems, self.algs = systems, algs
         keys, groups, self.group_of = [], [], []
         for a in algs:
-            bk = _binning(a.ensemble)
+            bk = _binning(_binned(a))
             if bk not in keys:
                 keys.append(bk)
                 groups.append([])
@@ -111,21 +142,34 @@ class GeneralizedWalkers:
         nb = np.array([k[2] for k in keys], np.int64)
         gof = np.array(self.group_of, np.int32)
         systems.bind_streams([a.rng for a in algs])
+        pb = None
+        if self.kind == _lib.MUCA_PAIR_SPIN2:
+            pb = np.array([float(a.ensemble.pair.beta) for a in algs], np.float64)
         d = GenDesc(self.kind, len(keys), _ptr(gof, C.c_int32), _ptr(start, C.c_double),
-                    _ptr(step, C.c_double), _ptr(nb, C.c_int64))
+                    _ptr(step, C.c_double), _ptr(nb, C.c_int64),
+                    _ptr(pb, C.c_double) if pb is not None else None)
         self._h = C.c_void_p()
         call("mcx_gen_create", systems.batch.handle, C.byref(d), C.byref(self._h))
         self._last_lw = [None] * len(algs)
         self._last_h = [None] * len(algs)
         self._last_logf = [None] * len(algs)
+        self._last_pb = None if pb is None else pb.copy()
         for k, a in enumerate(algs):
             a.ensemble._dev = (self, k)
+            if self.kind == _lib.MUCA_PAIR_SPIN2:
+                a.ensemble.spin2._dev = (self, k)
         self.sync_in()
 
     # ---- host <-> device ---------------------------------------------------------
     def sync_in(self):
+        if self._last_pb is not None:
+            for k, a in enumerate(self.algs):
+                b = float(a.ensemble.pair.beta)
+                if b != self._last_pb[k]:
+                    call("mcx_gen_set_pair_beta", self._h, k, b)
+                    self._last_pb[k] = b
         for k, a in enumerate(self.algs):
-            e = a.ensemble
+            e = _binned(a)
             lw = np.ascontiguousarray(e.logweight.values, np.float64)
             if self._last_lw[k] is None or not np.array_equal(lw, self._last_lw[k]):
                 call("mcx_gen_set_logweight", self._h, k, _ptr(lw, C.c_double), lw.size)
@@ -149,7 +193,7 @@ class GeneralizedWalkers:
         ac = np.empty(n, np.int64)
         call("mcx_gen_get_counters", self._h, _ptr(st, C.c_int64), _ptr(ac, C.c_int64))
         for k, a in enumerate(self.algs):
-            e = a.ensemble
+            e = _binned(a)
             lw = np.empty(len(e.logweight.values), np.float64)
             call("mcx_gen_get_logweight", self._h, k, _ptr(lw, C.c_double), lw.size)
             e.logweight.values[:] = lw
@@ -174,6 +218,11 @@ class GeneralizedWalkers:
     def reset_(self, k: int = -1):
         self.sync_in()
         call("mcx_gen_reset", self._h, int(k))
+        self.sync_out()
+
+    def reset_histogram_(self, k: int = -1):
+        self.sync_in()
+        call("mcx_gen_reset_histogram", self._h, int(k))
         self.sync_out()
 
     def update_(self, k: int = -1):
@@ -224,6 +273,8 @@ class GeneralizedWalkers:
         if getattr(self, "_h", None):
             for a in self.algs:
                 a.ensemble._dev = None
+                if isinstance(a.ensemble, CustomEnsemble):
+                    a.ensemble.spin2._dev = None
             call("mcx_gen_destroy", self._h)
             self._h = C.c_void_p()
 

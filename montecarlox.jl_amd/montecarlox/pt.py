@@ -22,7 +22,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import MCXArgumentError, call
-from .algorithms import HeatBath, Metropolis, Philox, algorithm_code
+from .algorithms import Glauber, HeatBath, Metropolis, Philox, algorithm_code
 from .ensembles import BoltzmannEnsemble
 
 
@@ -100,7 +100,7 @@ class ParallelTempering:
         if b.size != n:
             raise MCXArgumentError(f"size(betas) must be R={R} or n_sets*R={n}")
         self.systems, self.n_sets, self.R, self.ladder = systems, n_sets, R, b
-        cls = HeatBath if alg == "heatbath" else Metropolis
+        cls = {"heatbath": HeatBath, "glauber": Glauber}.get(alg, Metropolis)
         if algs is None:
             algs = [cls(Philox(seed, s * R + r + 1), beta=float(b[s * R + r]))
                     for s in range(n_sets) for r in range(R)]

@@ -17,7 +17,7 @@ from typing import List, Optional
 import numpy as np
 
 from . import _lib
-from ._lib import (BLUME_CAPEL, HEATBATH, ISING, METROPOLIS, JobDesc, JobResult,
+from ._lib import (BLUME_CAPEL, GLAUBER, HEATBATH, ISING, METROPOLIS, JobDesc, JobResult,
                    MCXArgumentError, ServerDesc, ServerStats, call)
 
 _P = C.c_void_p
@@ -32,7 +32,7 @@ for _n, _a in {
     _lib.declare(_n, _a)
 
 _MODELS = {"ising": ISING, "blume_capel": BLUME_CAPEL, "blumecapel": BLUME_CAPEL}
-_ALGS = {"metropolis": METROPOLIS, "heatbath": HEATBATH}
+_ALGS = {"metropolis": METROPOLIS, "heatbath": HEATBATH, "glauber": GLAUBER}
 
 
 @dataclass
@@ -56,7 +56,7 @@ class Job:
             raise MCXArgumentError(f"unknown model {self.model!r}")
         a = _ALGS.get(self.alg.lower())
         if a is None:
-            raise MCXArgumentError(f"unknown algorithm {self.alg!r} (metropolis or heatbath)")
+            raise MCXArgumentError(f"unknown algorithm {self.alg!r} (metropolis, heatbath or glauber)")
         return JobDesc(model=m, dim=self.dim, L=self.L, alg=a,
                        beta=float(self.beta), D=float(self.D), seed=self.seed & (2**64 - 1),
                        key=self.key & 0xFFFFFFFF, nsweeps=self.nsweeps, tag=self.tag)

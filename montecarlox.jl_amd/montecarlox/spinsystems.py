@@ -99,7 +99,8 @@ class SpinSystemBatch:
         algs = list(algs) if isinstance(algs, (list, tuple)) else [algs] * self.batch.n
         if len(algs) != self.batch.n:
             raise MCXArgumentError("one algorithm per chain")
-        if all(isinstance(a, ImportanceSampling) for a in algs):
+        from .muca import is_walker_alg
+        if all(is_walker_alg(a) for a in algs):
             if plain:
                 raise MCXArgumentError("multicanonical / Wang-Landau walkers have no plain path")
             return self.walkers(algs).run_(nsweeps, record=record)

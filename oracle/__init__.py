@@ -108,6 +108,12 @@ def _declare(L):
     L.orc_refpath_run.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _i32p, _f64p, nf64, nf64,
                                   C.c_int64, C.c_double, C.c_double, C.c_uint64, C.c_int64, C.c_int]
     L.orc_refpath_run.restype = C.c_int64
+    L.orc_pair_spin2_sweeps.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_int64, _f64p, _i64p, C.c_uint64, C.c_uint32,
+                                        C.c_uint64, C.c_int64, _i8p, C.POINTER(C.c_int64),
+                                        C.POINTER(C.c_int64), n32, n32, n32,
+                                        C.POINTER(C.c_int64)]
+    L.orc_pair_spin2_sweeps.restype = C.c_int
     L.orc_tau_int.argtypes = [_i32p, n32, C.c_int64, C.c_double, C.c_int, C.c_int32, C.c_double]
     L.orc_tau_int.restype = C.c_double
     L.orc_tau_int_centred.argtypes = [_f64p, C.c_int64, C.c_int32, C.c_double]
@@ -197,6 +203,24 @@ def generalized_sweeps(model, wl, dim, L, lw, hist, spins, seed, chain_key, swee
                                       spins, C.byref(steps), C.byref(acc), E, M, Q, C.byref(err))
     if rc == 2:
         raise BoundsError(f"energy bin out of range at site {err.value}")
+    return E, M, Q, steps.value, acc.value
+
+
+def pair_spin2_sweeps(dim, L, beta, lw, hist, spins, seed, chain_key, sweep0, nsweeps,
+                      bin_start, bin_step):
+    """Blume-Capel under the CustomEnsemble (Boltzmann pair term, multicanonical
+    sum s^2) of muca_BlumeCapel.jl, in place on spins and hist (int64); lw is
+    read only.  Returns (E_pair, M, Q, steps, accepted)."""
+    E = np.zeros(nsweeps, np.int32)
+    M = np.zeros(nsweeps, np.int32)
+    Q = np.zeros(nsweeps, np.int32)
+    steps, acc, err = C.c_int64(0), C.c_int64(0), C.c_int64(-1)
+    rc = lib().orc_pair_spin2_sweeps(dim, L, float(beta), float(bin_start), float(bin_step),
+                                     len(lw), np.ascontiguousarray(lw, np.float64), hist, seed,
+                                     chain_key, sweep0, nsweeps, spins, C.byref(steps),
+                                     C.byref(acc), E, M, Q, C.byref(err))
+    if rc == 2:
+        raise BoundsError(f"sum s^2 bin out of range at site {err.value}")
     return E, M, Q, steps.value, acc.value
 
 

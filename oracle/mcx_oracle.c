@@ -320,6 +320,16 @@ void orc_canonical_sweeps(int model, int alg, int dim, int L, double beta, doubl
                         spins[i] = (int8_t)-si;
                         ++acc;
                     }
+                } else if (model == 0 && alg == 2) {
+                    /* accept!(::Glauber, dE), metropolis.jl:121-127: rand < logistic(lr),
+                     * the uniform from the Metropolis stream */
+                    int dE = 2 * si * nb;
+                    double lr = -beta * (double)dE;
+                    uint32_t w = cached_bitsliced(&pc, seed, chain_key, g, t, sweep, ORC_P_METRO);
+                    if (ldexp((double)w, -32) < orc_logistic(lr)) {
+                        spins[i] = (int8_t)-si;
+                        ++acc;
+                    }
                 } else if (model == 0) {
                     int dE = 2 * si * nb;
                     double p_plus = orc_logistic(beta * (double)si * (double)dE);
@@ -329,7 +339,9 @@ void orc_canonical_sweeps(int model, int alg, int dim, int L, double beta, doubl
                         spins[i] = s_new;
                         ++acc;
                     }
-                } else if (alg == 0) {
+                } else if (alg == 0 || alg == 2) {
+                    /* Metropolis (alg 0) or Glauber (alg 2): same proposal, acceptance
+                     * exp(lr) or logistic(lr) */
                     uint32_t w = orc_site_word(seed, chain_key, g, t, sweep, ORC_P_BC_METRO);
                     int8_t s_new = bc_propose(si, (int)(w & 1u));
                     int ds = s_new - si;
@@ -338,7 +350,8 @@ void orc_canonical_sweeps(int model, int alg, int dim, int L, double beta, doubl
                     double dpair = (double)ds * coupling;
                     double dE = (-dpair - 0.0) + D * (double)ds2; /* delta_energy, h = 0 */
                     double lr = -beta * dE;
-                    int ok = lr > 0 || ldexp((double)(w >> 1), -31) < exp(lr);
+                    double u = ldexp((double)(w >> 1), -31);
+                    int ok = alg == 2 ? u < orc_logistic(lr) : (lr > 0 || u < exp(lr));
                     if (ok) {
                         spins[i] = s_new;
                         ++acc;
@@ -460,6 +473,69 @@ int orc_generalized_sweeps(int model, int wl, int dim, int L, double D,
                     spins[i] = s_new;
                     pair += dpair_i;
                     q += dq;
+                }
+            }
+        }
+        record(&G, spins, k, E_pair, M, Q);
+    }
+    return 0;
+}
+
+int orc_pair_spin2_sweeps(int dim, int L, double beta, double bin_start, double bin_step,
+                          int64_t nbins, const double* lw, int64_t* hist, uint64_t seed,
+                          uint32_t chain_key, uint64_t sweep0, int64_t nsweeps, int8_t* spins,
+                          int64_t* steps, int64_t* accepted, int32_t* E_pair, int32_t* M,
+                          int32_t* Q, int64_t* err_site)
+{
+    /* docs/src/examples/spin_systems/muca_BlumeCapel.jl:43-89: spin_flip! builds
+     * H = (J * sum_pair_interactions, sum_spins2) for the old and the proposed
+     * state (J = 1) and calls accept!(alg, H_new, H_old) of a Metropolis
+     * algorithm whose CustomEnsemble has logweight(H) = logweight(pair, H[1]) +
+     * logweight(spin2, H[2]) = -beta * H[1] + lw(H[2]); the visit goes to the
+     * spin2 histogram (record_visit!, :59-62; importance_sampling.jl:69-78). */
+    geo_t G = make_geo(dim, L);
+    double sum_pair = (double)orc_pair_sum(dim, L, spins); /* sum_pair_interactions */
+    int64_t sum_sq = orc_sum_sq(dim, L, spins);            /* sum_spins2 */
+    for (int64_t k = 0; k < nsweeps; ++k) {
+        uint64_t sweep = sweep0 + (uint64_t)k;
+        for (int colour = 0; colour < 2; ++colour) {
+            for (int64_t i = 0; i < G.N; ++i) {
+                if (colour_of(&G, i) != colour)
+                    continue;
+                uint32_t g;
+                int t, c;
+                orc_site_group(dim, L, i, &g, &t, &c);
+                uint32_t w = orc_site_word(seed, chain_key, g, t, sweep, ORC_P_GEN);
+                int8_t si = spins[i];
+                int nb = nbr_sum(&G, spins, i);
+                int8_t s_new = bc_propose(si, (int)(w & 1u));
+                int ds = s_new - si;
+                int ds2 = s_new * s_new - si * si;
+                double dpair = (double)ds * (1.0 * (double)nb); /* propose_changes */
+                double H1_old = 1.0 * sum_pair;
+                int64_t H2_old = sum_sq;
+                double H1_new = H1_old + 1.0 * dpair;
+                int64_t H2_new = H2_old + ds2;
+                int64_t b_new, b_old;
+                int ok_new = bin_int(H2_new, (int64_t)bin_start, (int64_t)bin_step, nbins, &b_new);
+                int ok_old = bin_int(H2_old, (int64_t)bin_start, (int64_t)bin_step, nbins, &b_old);
+                if (!ok_new || !ok_old) {
+                    if (err_site)
+                        *err_site = i;
+                    return 2;
+                }
+                double lw_new = (-beta * H1_new) + lw[b_new];
+                double lw_old = (-beta * H1_old) + lw[b_old];
+                double lr = lw_new - lw_old;
+                double u = ldexp((double)(w >> 1), -31);
+                ++*steps;
+                int acc = lr > 0 || u < exp(lr);
+                *accepted += acc;
+                hist[acc ? b_new : b_old] += 1;
+                if (acc) {
+                    spins[i] = s_new;
+                    sum_pair += dpair;
+                    sum_sq = H2_new;
                 }
             }
         }

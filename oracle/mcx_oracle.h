@@ -59,8 +59,10 @@ void orc_init_random(int model, int dim, int L, uint64_t seed, uint32_t chain_ke
                      uint64_t sweep, int8_t* spins);
 
 /* Canonical sweeps on one chain.
- *   model 0 (Ising): alg 0 Metropolis, 1 heat bath.
- *   model 1 (Blume-Capel): alg 0 Metropolis, 1 heat bath; D = crystal field.
+ *   model 0 (Ising): alg 0 Metropolis, 1 heat bath, 2 Glauber.
+ *   model 1 (Blume-Capel): alg 0 Metropolis, 1 heat bath, 2 Glauber; D = crystal field.
+ * Glauber (src/algorithms/metropolis.jl:121-127) accepts iff u < logistic(-beta dE),
+ * u from the Metropolis stream of the model.
  * Per sweep, after both colours: E_pair[s] = -sum s_i s_j, M[s] = sum s_i,
  * Q[s] = sum s_i^2 (any of the three may be NULL).
  * *accepted accumulates accepted moves (Metropolis) / changed sites (heat bath). */
@@ -101,6 +103,17 @@ void orc_refpath_series_ising2d(int L, double beta, uint64_t seed, int64_t nswee
 
 /* Multicanonical weight recursion: lw[i] -= log(h[i]) where h[i] > 0. */
 void orc_muca_update(int64_t nbins, double* lw, const double* h);
+
+/* Blume-Capel (J = 1, h = 0) under Metropolis(rng, CustomEnsemble(BoltzmannEnsemble(beta),
+ * MulticanonicalEnsemble(bins over sum s_i^2))): the spin_flip! / accept! of
+ * docs/src/examples/spin_systems/muca_BlumeCapel.jl:43-89, in checkerboard order
+ * with the site words of ORC_P_GEN.  Integer binning of sum s^2 (truncating div).
+ * Returns 0, or 2 with *err_site when sum s^2 leaves the binned domain. */
+int orc_pair_spin2_sweeps(int dim, int L, double beta, double bin_start, double bin_step,
+                          int64_t nbins, const double* lw, int64_t* hist, uint64_t seed,
+                          uint32_t chain_key, uint64_t sweep0, int64_t nsweeps, int8_t* spins,
+                          int64_t* steps, int64_t* accepted, int32_t* E_pair, int32_t* M,
+                          int32_t* Q, int64_t* err_site);
 
 /* ---- reference CPU path (timed as the CPU baseline) ----------------------
  * Random-site single-spin-flip Metropolis exactly as spin_flip!(IsingLatticeOptim,

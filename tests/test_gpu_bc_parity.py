@@ -15,7 +15,7 @@ def _run(mcx, orc, dim, alg, Ls, betas, Ds, nsweeps, seed=3, plain=False, runs=N
     b.init("random")
     start = [b.get_spins(k) for k in range(n)]
     for p in (runs or [nsweeps]):
-        b.run("heatbath" if alg else "metropolis", p, record=True, plain=plain)
+        b.run({0: "metropolis", 1: "heatbath", 2: "glauber"}[alg], p, record=True, plain=plain)
     _, acc = b.counters()
     mom = b.moments()
     for k in range(n):
@@ -46,6 +46,18 @@ def test_bc_2d_heatbath(mcx, orc, L):
 @pytest.mark.parametrize("alg", [0, 1])
 def test_bc_3d(mcx, orc, alg):
     _run(mcx, orc, 3, alg, [4, 6], [0.4, 0.9], [0.2, 1.1], 5)
+
+
+@pytest.mark.parametrize("L", [4, 16, 34])
+def test_bc_2d_glauber(mcx, orc, L):
+    # Glauber on Blume-Capel: the Metropolis proposal with logistic acceptance
+    _run(mcx, orc, 2, 2, [L] * 3, [0.3, 0.7, 1.4], [0.0, 1.0, 1.9], 8)
+
+
+def test_bc_glauber_3d_plain_extreme(mcx, orc):
+    _run(mcx, orc, 3, 2, [4, 6], [0.4, 0.9], [0.2, 1.1], 5)
+    _run(mcx, orc, 2, 2, [16, 6], [0.6, 1.0], [1.0, 0.3], 6, plain=True)
+    _run(mcx, orc, 2, 2, [8, 8, 8], [0.0, 5.0, 60.0], [1.0, 1.0, 1.9], 6)
 
 
 def test_bc_plain_and_split_runs(mcx, orc):

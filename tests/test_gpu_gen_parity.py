@@ -164,3 +164,104 @@ def test_wang_landau_converges_on_gpu(mcx):
     rmse = math.sqrt(np.mean((est[ok] - ref[ok]) ** 2))
     print("rmse", rmse)
     assert rmse < 0.1
+
+
+def _pair_spin2_walkers(mcx, dim, L, n, T, seed, lw0, start=0, cap=0):
+    """Metropolis(rng, CustomEnsemble(BoltzmannEnsemble(T=T),
+    MulticanonicalEnsemble(start:1:N))) per system, muca_BlumeCapel.jl:43-76."""
+    from montecarlox.spinsystems import SpinSystemBatch
+    N = L ** dim
+    sys = SpinSystemBatch("blume_capel", [L] * dim, n=n, series_capacity=cap)
+    algs = []
+    for k in range(n):
+        spin2 = mcx.MulticanonicalEnsemble(range(start, N + 1))
+        spin2.logweight.values[:] = lw0[k]
+        ens = mcx.CustomEnsemble(mcx.BoltzmannEnsemble(T=T), spin2, True)
+        algs.append(mcx.Metropolis(mcx.Philox(seed, k), ens))
+    sys.init_("random", [a.rng for a in algs])
+    return sys, algs
+
+
+@pytest.mark.parametrize("dim,L,n,T", [(2, 8, 3, 0.9), (2, 16, 2, 0.6), (3, 4, 2, 1.5),
+                                       (2, 6, 2, 0.9)])
+def test_pair_spin2_walkers_match_oracle(mcx, orc, dim, L, n, T):
+    """The muca_BlumeCapel.jl iteration on device walkers -- sweeps, update! of
+    the spin2 weights, reset!(alg) (counters only), reset_histogram!, more
+    sweeps -- against the oracle's sequential restatement, bit for bit (the
+    update's device log within 4 ulp, then continued from the device weights)."""
+    N, seed, n1, n2 = L ** dim, 31, 25, 15
+    rng = np.random.default_rng(L * 10 + dim)
+    lw0 = rng.normal(0, 0.4, size=(n, N + 1))
+    print(f"pair+spin2 walkers dim={dim} L={L} n={n} T={T}", flush=True)
+    sys, algs = _pair_spin2_walkers(mcx, dim, L, n, T, seed, lw0, cap=n1 + n2)
+    init = [sys.batch.get_spins(k) for k in range(n)]
+    mcx.sweep_(sys, algs, n1, record=True)
+    beta = 1.0 / T
+    spins = [s.copy() for s in init]
+    hists = [np.zeros(N + 1, np.int64) for _ in range(n)]
+    ser = []
+    for k in range(n):
+        E, M, Q, st, ac = orc.pair_spin2_sweeps(dim, L, beta, lw0[k], hists[k], spins[k], seed, k,
+                                                0, n1, 0, 1)
+        ser.append((E, M, Q))
+        assert np.array_equal(sys.batch.get_spins(k), spins[k]), f"walker {k} spins"
+        assert np.array_equal(algs[k].ensemble.spin2.histogram.values, hists[k])
+        assert (algs[k].steps, algs[k].accepted) == (st, ac)
+    # update!(ensemble(alg).spin2), reset!(alg), reset_histogram!(alg)
+    lw1 = []
+    for k, a in enumerate(algs):
+        mcx.update_(a.ensemble.spin2)
+        ref = lw0[k].copy()
+        orc.muca_update(ref, hists[k].astype(np.float64))
+        got = a.ensemble.spin2.logweight.values.copy()
+        assert np.all(np.abs(got - ref) <= 4 * np.spacing(np.abs(ref)) + 1e-300)
+        lw1.append(got)
+        mcx.reset_(a)
+        assert a.steps == 0 and a.accepted == 0
+        assert np.array_equal(a.ensemble.spin2.histogram.values, hists[k])  # kept by reset!
+        mcx.reset_histogram_(a)
+        assert not a.ensemble.spin2.histogram.values.any()
+    mcx.sweep_(sys, algs, n2, record=True)
+    for k in range(n):
+        h = np.zeros(N + 1, np.int64)
+        E, M, Q, st, ac = orc.pair_spin2_sweeps(dim, L, beta, lw1[k], h, spins[k], seed, k, n1, n2,
+                                                0, 1)
+        Eg, Mg, Qg = sys.batch.series(k)
+        assert np.array_equal(Eg, np.concatenate([ser[k][0], E]))
+        assert np.array_equal(Mg, np.concatenate([ser[k][1], M]))
+        assert np.array_equal(Qg, np.concatenate([ser[k][2], Q]))
+        assert np.array_equal(sys.batch.get_spins(k), spins[k])
+        assert np.array_equal(algs[k].ensemble.spin2.histogram.values, h)
+        assert (algs[k].steps, algs[k].accepted) == (st, ac)
+
+
+def test_pair_spin2_bounds_and_rules(mcx, orc):
+    """sum s^2 leaving a restricted domain stops the walker as the reference's
+    BoundsError would; invalid setups are refused."""
+    print("pair+spin2 bounds", flush=True)
+    L, N = 4, 16
+    lw0 = np.zeros((1, N + 1 - 12))
+    sys, algs = _pair_spin2_walkers(mcx, 2, L, 1, 0.9, 5, lw0, start=12)
+    s = sys.batch.get_spins(0)
+    with pytest.raises(mcx.MCXBoundsError):
+        mcx.sweep_(sys, algs, 50)
+    ref = s.copy()
+    h = np.zeros(N + 1 - 12, np.int64)
+    with pytest.raises(orc.BoundsError):
+        orc.pair_spin2_sweeps(2, L, 1 / 0.9, lw0[0], h, ref, 5, 0, 0, 50, 12, 1)
+    # float bins with step 2 would round where the integer range divides
+    from montecarlox.spinsystems import SpinSystemBatch
+    sys2 = SpinSystemBatch("blume_capel", [4, 4], n=1)
+    e = mcx.MulticanonicalEnsemble(mcx.BinnedObject(np.arange(0.0, 17.0, 2.0).tolist(), 0.0))
+    a = mcx.Metropolis(mcx.Philox(1), mcx.CustomEnsemble(mcx.BoltzmannEnsemble(beta=1.0), e))
+    sys2.init_("random", [a.rng])
+    with pytest.raises(mcx.MCXArgumentError):
+        mcx.sweep_(sys2, [a], 1)
+    # Ising: sum s^2 = N is constant
+    from montecarlox.spinsystems import SpinSystemBatch as SSB
+    sys3 = SSB("ising", [4, 4], n=1)
+    a3 = mcx.Metropolis(mcx.Philox(1), mcx.CustomEnsemble(
+        mcx.BoltzmannEnsemble(beta=1.0), mcx.MulticanonicalEnsemble(range(0, 17))))
+    sys3.init_("random", [a3.rng])
+    with pytest.raises(mcx.MCXArgumentError):
+        mcx.sweep_(sys3, [a3], 1)

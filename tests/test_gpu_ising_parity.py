@@ -10,7 +10,8 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-ISING, METRO, HB = 0, 0, 1
+ISING, METRO, HB, GLAUBER = 0, 0, 1, 2
+NAMES = {METRO: "metropolis", HB: "heatbath", GLAUBER: "glauber"}
 
 
 def _run_case(mcx, orc, dim, alg, Ls, betas, nsweeps, seed=7, plain=False, keys=None,
@@ -24,7 +25,7 @@ def _run_case(mcx, orc, dim, alg, Ls, betas, nsweeps, seed=7, plain=False, keys=
     parts = [nsweeps] if runs == (None,) else list(runs)
     assert sum(parts) == nsweeps
     for p in parts:
-        b.run("heatbath" if alg == HB else "metropolis", p, record=True, plain=plain)
+        b.run(NAMES[alg], p, record=True, plain=plain)
     steps, acc = b.counters()
     for k in range(n):
         key = k if keys is None else keys[k]
@@ -61,6 +62,25 @@ def test_3d_metropolis(mcx, orc, L):
 @pytest.mark.parametrize("L", [4, 8, 14])
 def test_3d_heatbath(mcx, orc, L):
     _run_case(mcx, orc, 3, HB, [L] * 2, [0.1, 0.2216544], 6)
+
+
+@pytest.mark.parametrize("L", [4, 16, 66, 130])
+def test_2d_glauber_sizes(mcx, orc, L):
+    # Glauber(rng; beta): logistic acceptance (metropolis.jl:108-127)
+    _run_case(mcx, orc, 2, GLAUBER, [L] * 4, [0.2, 0.4406868, 0.7, 1.0], 10)
+
+
+@pytest.mark.parametrize("L", [4, 8, 14])
+def test_3d_glauber(mcx, orc, L):
+    _run_case(mcx, orc, 3, GLAUBER, [L] * 3, [0.15, 0.2216544, 0.35], 6)
+
+
+def test_glauber_extremes_plain_and_split(mcx, orc):
+    # beta = 0: p = 1/2 everywhere; beta = 100: logistic rounds to 0 and 1
+    _run_case(mcx, orc, 2, GLAUBER, [16] * 4, [0.0, 1e-12, 5.0, 100.0], 8)
+    _run_case(mcx, orc, 2, GLAUBER, [32, 18], [0.44, 0.3], 5, plain=True)
+    _run_case(mcx, orc, 3, GLAUBER, [8], [0.22], 3, plain=True)
+    _run_case(mcx, orc, 2, GLAUBER, [32, 32], [0.44, 0.5], 13, runs=(5, 8))
 
 
 def test_extreme_betas(mcx, orc):

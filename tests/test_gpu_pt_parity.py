@@ -91,6 +91,20 @@ def test_pt_heatbath(mcx, orc):
         assert np.array_equal(out[k], spins[k])
 
 
+def test_pt_glauber(mcx, orc):
+    n_sets, R, L, seed, rounds, sb = 2, 4, 8, 15, 6, 3
+    ladder = mcx.set_betas(R, 0.3, 0.6, "uniform")
+    Ls = [L] * (n_sets * R)
+    pt, init, out = _gpu_pt(mcx, 2, Ls, n_sets, R, np.tile(ladder, n_sets), seed, rounds, sb,
+                            alg="glauber")
+    spins, idx, steps, acc, stage = _oracle_pt(orc, 2, Ls, n_sets, R, np.tile(ladder, n_sets),
+                                               seed, np.arange(1, n_sets * R + 1), rounds, sb,
+                                               alg=2, init_spins=init)
+    assert np.array_equal(pt.indices, idx) and np.array_equal(pt.accepted, acc)
+    for k in range(n_sets * R):
+        assert np.array_equal(out[k], spins[k])
+
+
 def test_update_with_given_energies(mcx):
     # equal ladder betas: log ratio 0, every attempted pair swaps (replica_exchange.jl:158-178)
     from montecarlox.spinsystems import SpinSystemBatch

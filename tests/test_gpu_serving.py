@@ -8,7 +8,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 MODEL = {"ising": 0, "blume_capel": 1}
-ALG = {"metropolis": 0, "heatbath": 1}
+ALG = {"metropolis": 0, "heatbath": 1, "glauber": 2}
 
 
 def _jobs(mcx):
@@ -26,6 +26,9 @@ def _jobs(mcx):
         J("blume_capel", 4, 2.0, 500, "metropolis", D=1.9, seed=5, key=9),
         J("ising", 256, 0.45, 6, "metropolis", seed=5, key=10),        # 1024-thread class
         J("blume_capel", 130, 0.6, 8, "heatbath", D=0.7, seed=5, key=11),
+        J("ising", 24, 0.4406868, 90, "glauber", seed=5, key=12),
+        J("blume_capel", 18, 0.8, 70, "glauber", D=0.8, seed=5, key=13),
+        J("ising", 6, 0.25, 120, "glauber", dim=3, seed=9, key=14),
     ]
 
 

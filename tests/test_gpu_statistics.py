@@ -3,14 +3,20 @@ reference CPU path.
 
 * 2D Ising L=16: <E> and C = beta^2 var(E) from 64 independent GPU chains
   against Kaufman's exact finite-lattice partition function (FP64 from a
-  40-digit evaluation), within 2 sigma, Metropolis and heat bath, three betas.
+  40-digit evaluation), Metropolis, heat bath and Glauber, three betas.
 * 2D Ising L=16 at beta_c: <E>, <|m|>, C and the Binder cumulant U4 of the GPU
   chains against the reference CPU path (random-site Metropolis with
-  Xoshiro256++, restated in C), within 2 sigma of the combined error.
+  Xoshiro256++, restated in C), in units of the combined error.
 
 Errors come from the scatter of independent chains (each chain's mean is one
-sample), so the correlation time does not enter them.  Seeds are fixed.
+sample), so the correlation time does not enter them.  Seeds are fixed.  The
+file makes 22 z-tests; a false-alarm rate of 1 % for the whole family gives
+each one the two-sided level 0.01 / 22, i.e. |z| < 3.5 (Bonferroni).  A
+per-test 2-sigma cut would fail somewhere in the family about two times in
+three with correct sampling.
 """
+
+Z_MAX = 3.5
 import math
 
 import numpy as np
@@ -61,12 +67,12 @@ def _observables(m, beta):
     }
 
 
-@pytest.mark.parametrize("alg", ["metropolis", "heatbath"])
+@pytest.mark.parametrize("alg", ["metropolis", "heatbath", "glauber"])
 @pytest.mark.parametrize("beta", [0.3, BETA_C, 0.55])
 def test_l16_matches_kaufman(mcx, alg, beta):
     from oracle import exact
     Ek, _, Ck = exact.kaufman_observables(L, beta)
-    seed = 1000 + int(beta * 1e4) + (7 if alg == "heatbath" else 0)
+    seed = 1000 + int(beta * 1e4) + {"metropolis": 0, "heatbath": 7, "glauber": 13}[alg]
     print(f"statistics: L=16 {alg} beta={beta} vs Kaufman", flush=True)
     m = _gpu_chain_moments(mcx, alg, beta, seed)
     obs = _observables(m, beta)
@@ -74,7 +80,7 @@ def test_l16_matches_kaufman(mcx, alg, beta):
     ze, zc = (e - Ek) / de, (c - Ck) / dc
     print(f"  <E> {e:.4f} +- {de:.4f} (exact {Ek:.4f}, z={ze:+.2f});"
           f" C {c:.3f} +- {dc:.3f} (exact {Ck:.3f}, z={zc:+.2f})", flush=True)
-    assert abs(ze) < 2 and abs(zc) < 2
+    assert abs(ze) < Z_MAX and abs(zc) < Z_MAX
 
 
 def test_l16_matches_reference_cpu_path(mcx, orc):
@@ -95,4 +101,4 @@ def test_l16_matches_reference_cpu_path(mcx, orc):
         (a, da), (b, db) = gpu[k], cpu[k]
         z = (a - b) / math.hypot(da, db)
         print(f"  {k}: gpu {a:.5f} +- {da:.5f}  cpu {b:.5f} +- {db:.5f}  z={z:+.2f}", flush=True)
-        assert abs(z) < 2, k
+        assert abs(z) < Z_MAX, k

@@ -148,6 +148,7 @@ def test_job_descriptor_validation_on_host(mcx):
     with pytest.raises(mcx.MCXArgumentError):
         mcx.Job("potts", 16, 0.4, 10).desc()
     with pytest.raises(mcx.MCXArgumentError):
-        mcx.Job("ising", 16, 0.4, 10, alg="glauber").desc()
+        mcx.Job("ising", 16, 0.4, 10, alg="kawasaki").desc()
+    assert mcx.Job("ising", 16, 0.4, 10, alg="glauber").desc().alg == 4  # MCX_GLAUBER
     d = mcx.Job("blume_capel", 16, 0.4, 10, alg="heatbath", D=1.5, seed=2**40 + 3, key=7).desc()
     assert (d.model, d.alg, d.seed, d.key, d.D) == (1, 1, 2**40 + 3, 7, 1.5)

@@ -13,7 +13,7 @@ import pytest
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 ISING, BC = 0, 1
-METRO, HB = 0, 1
+METRO, HB, GLAUBER = 0, 1, 2
 
 
 def _load(name):
@@ -116,7 +116,8 @@ def test_blume_capel_bookkeeping_known_answers(orc):
 def test_energy_series_equal_full_recompute(orc):
     # cached == full after random updates (test_ising.jl:149-172, test_blume_capel.jl:87-107)
     for model, alg, dim, L, D in ((ISING, METRO, 2, 6, 0), (ISING, HB, 3, 4, 0),
-                                  (BC, METRO, 2, 6, 0.7), (BC, HB, 2, 4, 1.3)):
+                                  (BC, METRO, 2, 6, 0.7), (BC, HB, 2, 4, 1.3),
+                                  (ISING, GLAUBER, 3, 4, 0), (BC, GLAUBER, 2, 6, 0.9)):
         s = orc.init_random(model, dim, L, 11, 2)
         for k in range(4):
             E, M, Q, _ = orc.canonical_sweeps(model, alg, dim, L, 0.45, s, 11, 2, 10 * k, 1, D=D)
@@ -175,7 +176,7 @@ def _chain_stats(E, M, N, beta, nb=40):
     return out
 
 
-@pytest.mark.parametrize("alg", [METRO, HB])
+@pytest.mark.parametrize("alg", [METRO, HB, GLAUBER])
 @pytest.mark.parametrize("beta", [0.3, 0.4406868, 0.6])
 def test_ising_4x4_sampling_matches_exact(orc, alg, beta):
     from oracle import exact
@@ -189,7 +190,7 @@ def test_ising_4x4_sampling_matches_exact(orc, alg, beta):
         assert abs(v - ex[k]) < 4 * err + 1e-12, (k, v, ex[k], err)
 
 
-@pytest.mark.parametrize("alg", [METRO, HB])
+@pytest.mark.parametrize("alg", [METRO, HB, GLAUBER])
 @pytest.mark.parametrize("beta,D", [(0.5, 0.5), (1.1, 1.6), (0.8, 0.0)])
 def test_blume_capel_4x4_sampling_matches_exact(orc, alg, beta, D):
     from montecarlox.measurements import binning_error
@@ -364,3 +365,71 @@ def test_pt_round_schedule(orc):
     for r in range(6):
         st = orc.pt_round(betas, x, idx, steps, acc, r % 2, 1, keys, r)
     assert list(idx) == [1, 2, 3, 4] and acc.sum() == 0 and steps.sum() == 9
+
+
+def test_glauber_acceptance_is_the_logistic_rule(orc):
+    """accept!(::Glauber) (metropolis.jl:121-127): a flip is taken iff
+    u < logistic(-beta dE).  On one sweep of a 2D lattice, replay every site
+    of colour 0 with the site's own uniform and compare; logistic satisfies
+    detailed balance p(dE) / p(-dE) = exp(-beta dE)."""
+    for beta in (0.2, 0.7, 3.0):
+        for dE in (-8, -4, 0, 4, 8):
+            pa, pb = orc.logistic(-beta * dE), orc.logistic(beta * dE)
+            assert pa / pb == pytest.approx(math.exp(-beta * dE), rel=1e-13)
+    L, beta, seed, key = 8, 0.45, 77, 3
+    s = orc.init_random(ISING, 2, L, seed, key)
+    before = s.copy()
+    orc.canonical_sweeps(ISING, GLAUBER, 2, L, beta, s, seed, key, 0, 1)
+    # colour 0 sites see only colour-1 neighbours, which are unchanged when colour 0 runs
+    for i in range(L * L):
+        x, y = i % L, i // L
+        g, t, colour = orc.site_group(2, L, i)
+        if colour != 0:
+            continue
+        nb = (int(before[y * L + (x + 1) % L]) + int(before[y * L + (x - 1) % L])
+              + int(before[((y + 1) % L) * L + x]) + int(before[((y - 1) % L) * L + x]))
+        dE = 2 * int(before[i]) * nb
+        u = orc.bitsliced_word(seed, key, g, t, 0, 1) / 2.0 ** 32  # Metropolis stream
+        flipped = s[i] != before[i]
+        assert flipped == (u < orc.logistic(-beta * dE)), i
+
+
+@pytest.mark.parametrize("beta,lw_kind", [(1 / 0.9, "flat"), (0.5, "tilted"), (1 / 0.9, "tilted")])
+def test_pair_spin2_sampling_matches_exact(orc, beta, lw_kind):
+    """The CustomEnsemble walk of muca_BlumeCapel.jl (weight exp(-beta * H[1] +
+    lw(H[2])) with H = (sum_pair_interactions, sum_spins2)) samples, on the
+    4x4 lattice, the distribution fixed by the exact joint density of states:
+    <P>, <Q>, <|m|> and P(Q) within 4 sigma."""
+    from montecarlox.measurements import binning_error
+    N, L = 16, 4
+    dos = _bc_dos()
+    P = np.array([k[0] for k in dos], np.float64)
+    Mv = np.array([k[1] for k in dos], np.float64)
+    Qv = np.array([k[2] for k in dos], np.int64)
+    g = np.array(list(dos.values()), np.float64)
+    lw = np.zeros(N + 1)
+    if lw_kind == "tilted":
+        q = np.arange(N + 1, dtype=np.float64)
+        lw = 0.3 * q - 0.02 * q * q
+    x = np.log(g) - beta * P + lw[Qv]
+    w = np.exp(x - x.max())
+    w /= w.sum()
+    ex = {"P": (w * P).sum(), "Q": (w * Qv).sum(), "absm": (w * np.abs(Mv) / N).sum()}
+    pq = np.bincount(Qv, weights=w, minlength=N + 1)
+    s = orc.init_random(BC, 2, L, 21, 4)
+    hist = np.zeros(N + 1, np.int64)
+    orc.pair_spin2_sweeps(2, L, beta, lw, hist, s, 21, 4, 0, 500, 0, 1)
+    hist[:] = 0
+    E, M, Q, steps, acc = orc.pair_spin2_sweeps(2, L, beta, lw, hist, s, 21, 4, 500, 60000, 0, 1)
+    assert steps == 60000 * N and hist.sum() == steps
+    for name, v in (("P", -E.astype(np.float64)), ("Q", Q.astype(np.float64)),
+                    ("absm", np.abs(M) / float(N))):
+        err = binning_error(v, 40)
+        assert abs(v.mean() - ex[name]) < 4 * err, (name, v.mean(), ex[name], err)
+    # per-attempt visits: P(Q) of the visited states
+    freq = hist / hist.sum()
+    big = pq > 0.02
+    se = np.sqrt(pq * (1 - pq) / (60000 / 20.0))  # ~20 sweeps of correlation, generous
+    assert np.all(np.abs(freq[big] - pq[big]) < 4 * se[big]), (freq[big], pq[big])
+    if beta > 1:
+        assert ex["P"] < 0  # the pair weight -beta * H[1] favours anti-aligned pairs
