@@ -34,6 +34,8 @@ struct GenArgs {
     int64_t nsweeps;
     int64_t rec_pos;
     const double* pair_beta; // per walker (kind 5): beta of the Boltzmann pair term
+    int32_t win;            // weights held in LDS: a window of win bins (win = nbins: all)
+    int32_t max_dbin;       // largest bin change of one attempt (window reach per chunk)
 };
 
 hipError_t launch_gen_sweeps(const GenArgs& a, size_t lds_bytes, hipStream_t st);

@@ -60,7 +60,10 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
     // one chunk's staged attempts and the lane-0 walk's results
     __shared__ int st_dE[64], st_dp[64], st_mq[64], st_site[64], st_res[64];
     __shared__ double st_luh[64], st_lul[64], st_u[64];
-    uint32_t* hl = reinterpret_cast<uint32_t*>(lw + a.nbins);
+    // the weights in LDS are the window [wbase, wbase + wsz) of the walker's
+    // table (the whole table when it fits: wsz = nbins, wbase = 0)
+    const int wsz = a.win;
+    uint32_t* hl = reinterpret_cast<uint32_t*>(lw + wsz);
     int8_t* sp = reinterpret_cast<int8_t*>(hl + (a.hist_in_lds ? a.nbins : 0));
     const int lane = threadIdx.x;
     const int64_t nb = a.nbins;
@@ -79,11 +82,9 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
         int8_t* sp_g = a.spins + cr.spin_off;
         const double logf = a.logf[wk];
 
-        for (int64_t b = lane; b < nb; b += 64) {
-            lw[b] = lw_g[b];
-            if (a.hist_in_lds)
+        if (a.hist_in_lds)
+            for (int64_t b = lane; b < nb; b += 64)
                 hl[b] = 0;
-        }
         for (int64_t i = lane; i < N; i += 64)
             sp[i] = sp_g[i];
         __syncthreads();
@@ -173,6 +174,22 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
         int b_old = 0;
         bool old_ok = PS ? bin_int((long long)qq, b_old) : (int_e ? ibin(Ei, b_old) : bin_fp(Eo, b_old));
         const double nbeta = PS ? -a.pair_beta[wk] : 0.0;
+        // ---- weight window: a chunk of 64 attempts moves the walker by at most
+        // 64 * max_dbin bins, so a window that holds `reach` bins on both sides of
+        // the chunk's starting bin serves every weight the chunk reads ----
+        const int reach = 66 * a.max_dbin;
+        int wbase = 0;
+        auto place_window = [&](int bcur) {
+            int nbase = bcur - wsz / 2;
+            nbase = nbase < 0 ? 0 : nbase;
+            nbase = nbase > (int)nb - wsz ? (int)nb - wsz : nbase;
+            return nbase;
+        };
+        if (wsz < nb && old_ok)
+            wbase = place_window(b_old);
+        for (int b = lane; b < wsz; b += 64)
+            lw[b] = lw_g[wbase + b];
+        __syncthreads();
         // bin of Q (integer binning, truncating div), lane-0 state
         auto qbin0 = [&](int Qv, int& b) -> bool {
             const int d = Qv - start32;
@@ -191,6 +208,27 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
             for (int c = 0; c < 2 && err < 0; ++c) {
                 const int n_col = rows * half;
                 for (int base = 0; base < n_col && err < 0; base += 64) {
+                    if (wsz < nb) {
+                        const int bcur = __builtin_amdgcn_readfirstlane(b_old);
+                        const bool okc = __builtin_amdgcn_readfirstlane(old_ok ? 1 : 0) != 0;
+                        if (okc && (bcur - wbase < reach || wbase + wsz - 1 - bcur < reach)) {
+                            const int nbase = place_window(bcur);
+                            if (nbase != wbase) {
+                                __syncthreads();
+                                // the old window goes back whole before the new one
+                                // is read: the two overlap
+                                if (WL) {
+                                    for (int b = lane; b < wsz; b += 64)
+                                        lw_g[wbase + b] = lw[b];
+                                    __syncthreads();
+                                }
+                                for (int b = lane; b < wsz; b += 64)
+                                    lw[b] = lw_g[nbase + b];
+                                wbase = nbase;
+                                __syncthreads();
+                            }
+                        }
+                    }
                     // ---- stage: lane k prepares attempt base + k (the sites of one
                     // colour are independent given the other colour) ----
                     const int o = base + lane;
@@ -265,7 +303,7 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                                 err_v = st_site[0];
                                 kk = cnt;
                             }
-                            double w_cur = kk < cnt ? nbeta * (double)pair + lw[b_old] : 0.0;
+                            double w_cur = kk < cnt ? nbeta * (double)pair + lw[b_old - wbase] : 0.0;
                             for (int k = kk; k < cnt; ++k) {
                                 const int Pn = pair + st_dp[k];
                                 const int Qn = qq + st_dE[k];
@@ -275,7 +313,7 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                                     err_v = st_site[k];
                                     break;
                                 }
-                                const double w_new = nbeta * (double)Pn + lw[bn];
+                                const double w_new = nbeta * (double)Pn + lw[bn - wbase];
                                 const double lr = w_new - w_cur;
                                 // lr > 0 || u < exp(lr), exp only inside the 1e-9 band
                                 bool ok = lr > 0 || lr > st_luh[k];
@@ -331,13 +369,13 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                                 const int d = E - start32;
                                 return MODEL == 0 ? (d + ((d >> 31) & smask)) >> a.step_shift : d;
                             };
-                            auto binc = [&](int E) -> int { // clamped into the domain
-                                const int b = bin32(E);
-                                return (unsigned)b < nbu ? b : 0;
+                            auto binc = [&](int E) -> int { // window index, clamped into it
+                                const int b = bin32(E) - wbase;
+                                return (unsigned)b < (unsigned)wsz ? b : 0;
                             };
                             int E32 = (int)Ei;
                             if (b_cur != b_old && old_ok) {
-                                lw_cur = lw[b_old];
+                                lw_cur = lw[b_old - wbase];
                                 b_cur = b_old;
                             }
                             int kk = 0;
@@ -383,7 +421,7 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                                     if (WL) {
                                         // unchanged value when the attempt did not run
                                         lw_cur = go ? lw_cur - logf : lw_cur;
-                                        lw[b_old] = lw_cur;
+                                        lw[b_old - wbase] = lw_cur;
                                     }
                                     lw_pre = ok ? lw_a : lw_r;
                                     dE_c = dE_n;
@@ -409,7 +447,7 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                                     err_v = st_site[stop];
                                     break;
                                 }
-                                const double lw_new = bb == b_old ? lw_cur : lw[bb];
+                                const double lw_new = bb == b_old ? lw_cur : lw[bb - wbase];
                                 const double lr = lw_new - lw_cur;
                                 const bool ok = lr > 0 || st_u[stop] < exp(lr);
                                 ++steps;
@@ -426,7 +464,7 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                                 st_res[stop] = b_old | (ok ? (int)0x80000000 : 0);
                                 if (WL) {
                                     lw_cur = lw_cur - logf;
-                                    lw[b_old] = lw_cur;
+                                    lw[b_old - wbase] = lw_cur;
                                 }
                                 b_cur = b_old;
                                 kk = stop + 1;
@@ -456,8 +494,8 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                             n_done = k;
                             break;
                         }
-                        const double lw_old = b_old == b_cur ? lw_cur : lw[b_old];
-                        const double lw_new = b_new == b_old ? lw_old : lw[b_new];
+                        const double lw_old = b_old == b_cur ? lw_cur : lw[b_old - wbase];
+                        const double lw_new = b_new == b_old ? lw_old : lw[b_new - wbase];
                         const double lr = lw_new - lw_old;
                         // lr > 0 || u < exp(lr); log(u) from the staging settles
                         // all but a 1e-9-wide band, where exp decides exactly
@@ -485,7 +523,7 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                         if (wl) {
                             lw_vis = lw_vis - logf;
                             if (lane == 0)
-                                lw[vis] = lw_vis;
+                                lw[vis - wbase] = lw_vis;
                         }
                         if (ok) {
                             pair += readlane_i32(t_dpair, k);
@@ -524,12 +562,13 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
         __syncthreads();
         for (int64_t i = lane; i < N; i += 64)
             sp_g[i] = sp[i];
-        for (int64_t b = lane; b < nb; b += 64) {
-            if (wl)
-                lw_g[b] = lw[b];
-            if (a.hist_in_lds && hl[b])
-                hist_g[b] += hl[b];
-        }
+        if (wl)
+            for (int b = lane; b < wsz; b += 64)
+                lw_g[wbase + b] = lw[b];
+        if (a.hist_in_lds)
+            for (int64_t b = lane; b < nb; b += 64)
+                if (hl[b])
+                    hist_g[b] += hl[b];
         if (lane == 0) {
             a.steps[wk] += steps;
             a.accepted[wk] += nacc;

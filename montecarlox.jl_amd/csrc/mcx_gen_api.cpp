@@ -81,11 +81,49 @@ mcx_status check_group(mcx_gen* g, int32_t group, const char* who)
 // static LDS of the walker kernel: one chunk's staged attempts (64 x 5 ints + 3 doubles)
 constexpr size_t kGenStaticLds = 64 * (5 * 4 + 3 * 8);
 
-// dynamic LDS of one walker: weights, optional histogram, spins
-size_t gen_lds(int64_t nbins, int64_t nsites, bool hist_in_lds)
+// dynamic LDS of one walker: weights (a window of win bins), optional
+// histogram (whole table), spins
+size_t gen_lds(int64_t win, int64_t nbins, int64_t nsites, bool hist_in_lds)
 {
-    return 8 * (size_t)nbins + (hist_in_lds ? 4 * (size_t)nbins : 0)
+    return 8 * (size_t)win + (hist_in_lds ? 4 * (size_t)nbins : 0)
          + (size_t)((nsites + 15) / 16 * 16);
+}
+
+// Weight tables above this size live in HBM and the walker keeps a window of
+// them in LDS, so that walkers of every lattice size fit side by side on a CU.
+constexpr size_t kWindowAbove = 24 * 1024;
+constexpr int64_t kMinWindow = 2048;
+
+struct GenWindow {
+    int64_t win;   // bins held in LDS (= nbins: the whole table)
+    int max_dbin;  // largest bin change of one attempt
+};
+
+// The window must hold every bin 66 attempts can reach on either side of the
+// current one (a chunk of 64 plus the walk's lookahead).
+GenWindow gen_window(const mcx_gen* g, int k)
+{
+    const mcx_batch* b = g->b;
+    double max_de;
+    if (g->kind == MCX_MUCA_PAIR_SPIN2) {
+        max_de = 1.0; // Q = sum s^2 changes by at most 1
+    } else if (b->model == MCX_ISING) {
+        max_de = 4.0 * b->dim; // |2 s sum_j s_j|
+    } else {
+        double dmax = 0;
+        for (int c : g->members[k])
+            dmax = std::max(dmax, std::fabs(b->rec[c].D));
+        max_de = 4.0 * b->dim + dmax; // |ds| * |sum| + |D| |ds2|
+    }
+    GenWindow w;
+    w.max_dbin = (int)std::ceil(max_de / std::fabs(g->step[k])) + 2;
+    const int64_t nb = g->nbins[k];
+    const int64_t need = 2 * 66 * (int64_t)w.max_dbin + 64;
+    int64_t win = std::max(kMinWindow, (need + 63) / 64 * 64);
+    if (8 * (size_t)nb <= kWindowAbove || win >= nb)
+        win = nb;
+    w.win = win;
+    return w;
 }
 
 } // namespace
@@ -155,10 +193,19 @@ extern "C" mcx_status mcx_gen_create(mcx_batch* b, const mcx_gen_desc* d, mcx_ge
                 delete g;
                 return fail(MCX_EINVAL, "mcx_gen_create: group %d mixes lattice sizes", k);
             }
-        if (gen_lds(g->nbins[k], b->nsites[g->members[k][0]], false) + kGenStaticLds > b->ctx->lds_max) {
+        if (std::fabs(g->start[k]) > 1e15 || (double)g->nbins[k] > 2e9) {
             delete g;
-            return fail(MCX_EINVAL, "mcx_gen_create: group %d: %lld bins do not fit the device LDS",
-                        k, (long long)g->nbins[k]);
+            return fail(MCX_EINVAL, "mcx_gen_create: group %d: binning beyond 32-bit bin indices", k);
+        }
+    }
+    for (int k = 0; k < d->n_groups; ++k) {
+        const GenWindow w = gen_window(g, k);
+        if (gen_lds(w.win, g->nbins[k], b->nsites[g->members[k][0]], false) + kGenStaticLds >
+            b->ctx->lds_max) {
+            delete g;
+            return fail(MCX_EINVAL, "mcx_gen_create: group %d: the lattice (%lld sites) and a window "
+                                    "of %lld bins do not fit the device LDS",
+                        k, (long long)b->nsites[g->members[k][0]], (long long)w.win);
         }
     }
     g->off.resize(b->n);
@@ -350,7 +397,12 @@ extern "C" mcx_status mcx_gen_run(mcx_gen* g, int64_t nsweeps, uint32_t flags)
         a.dim = b->dim;
         a.record = record ? 1 : 0;
         const int64_t ns = b->nsites[g->members[k][0]];
-        a.hist_in_lds = gen_lds(g->nbins[k], ns, true) + kGenStaticLds <= b->ctx->lds_max ? 1 : 0;
+        const GenWindow gw = gen_window(g, k);
+        a.win = (int32_t)gw.win;
+        a.max_dbin = gw.max_dbin;
+        // a windowed walker counts visits in HBM; a whole table may count in LDS
+        a.hist_in_lds = gw.win == g->nbins[k] &&
+                        gen_lds(gw.win, g->nbins[k], ns, true) + kGenStaticLds <= b->ctx->lds_max ? 1 : 0;
         a.bin_start = g->start[k];
         a.bin_step = g->step[k];
         {
@@ -379,7 +431,7 @@ extern "C" mcx_status mcx_gen_run(mcx_gen* g, int64_t nsweeps, uint32_t flags)
         a.nsweeps = nsweeps;
         a.rec_pos = b->rec_len;
         a.pair_beta = g->d_pbeta;
-        HIPCHK(launch_gen_sweeps(a, gen_lds(g->nbins[k], ns, a.hist_in_lds), g->gs[oi % nstr]));
+        HIPCHK(launch_gen_sweeps(a, gen_lds(gw.win, g->nbins[k], ns, a.hist_in_lds), g->gs[oi % nstr]));
     }
     for (int q = 0; q < nstr; ++q) {
         HIPCHK(hipEventRecord(g->join[q], g->gs[q]));

@@ -92,6 +92,25 @@ def test_histogram_in_hbm_path(mcx, orc):
     _check(mcx, orc, "blume_capel", 2, L, 1, "muca", -N * 1.0, 1.0, 4 * N + 1, 3, 2, D=1.0)
 
 
+@pytest.mark.parametrize("kind,lw_scale", [("wl", 0.0), ("muca", 0.0), ("muca", 1.5)])
+def test_windowed_weights_blume_capel(mcx, orc, kind, lw_scale):
+    """4097 bins (33 KB of weights): the walker keeps a window of 2048 bins in
+    LDS and slides it as its energy moves (Wang-Landau with ln f = 1 wanders
+    across the whole range); results equal the oracle's."""
+    L, N = 32, 1024
+    _check(mcx, orc, "blume_capel", 2, L, 2, kind, -1.0 * N, 1.0, 4 * N + 1, 13, 60, D=1.0,
+           logf=1.0, lw_scale=lw_scale)
+
+
+def test_windowed_weights_ising_and_fp64_bins(mcx, orc):
+    # Ising L=64, step 4: 4097 bins (integer walk); Blume-Capel 3D with step 0.5:
+    # 13825 bins, FP64 energies (rounded bins, the generic walk)
+    _check(mcx, orc, "ising", 2, 64, 2, "wl", -2 * 4096, 4, 4097, 17, 30, logf=1.0, lw_scale=0.0)
+    L, N = 12, 1728
+    _check(mcx, orc, "blume_capel", 3, L, 1, "wl", -1.0 * N, 0.5, 8 * N + 1, 19, 25, D=0.5,
+           logf=1.0, lw_scale=0.0)
+
+
 def test_split_runs(mcx, orc):
     _check(mcx, orc, "ising", 2, 8, 2, "wl", -128, 4, 65, 4, 9, runs=[3, 6], logf=0.5)
 
@@ -183,7 +202,7 @@ def _pair_spin2_walkers(mcx, dim, L, n, T, seed, lw0, start=0, cap=0):
 
 
 @pytest.mark.parametrize("dim,L,n,T", [(2, 8, 3, 0.9), (2, 16, 2, 0.6), (3, 4, 2, 1.5),
-                                       (2, 6, 2, 0.9)])
+                                       (2, 6, 2, 0.9), (2, 64, 1, 0.9)])
 def test_pair_spin2_walkers_match_oracle(mcx, orc, dim, L, n, T):
     """The muca_BlumeCapel.jl iteration on device walkers -- sweeps, update! of
     the spin2 weights, reset!(alg) (counters only), reset_histogram!, more
