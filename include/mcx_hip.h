@@ -388,8 +388,9 @@ typedef struct {
     int32_t n_streams;  /* streams the kernel classes are spread over (0 = 4, <= 8)  */
     double quantum_ms;  /* target device time of one quantum (0 = 2 ms)             */
     int64_t tile_sites; /* lattices with at least this many sites run spread over    */
-                        /* many workgroups (bit planes in the pool, one launch per    */
-                        /* colour and sweep); 0 = 2^17 Ising / 2^14 Blume-Capel,      */
+                        /* many workgroups (bit planes in the pool, bands of rows or  */
+                        /* planes with halos, several sweeps per launch);             */
+                        /* 0 = 2^15 Ising / 2^14 Blume-Capel,                         */
                         /* < 0 = never.  Other lattices run one workgroup each, from  */
                         /* LDS.  A job takes N bytes, plus N/4 (Ising) or N/2         */
                         /* (Blume-Capel) bytes of planes on the tiled path.           */

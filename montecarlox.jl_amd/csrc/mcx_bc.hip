@@ -143,16 +143,16 @@ struct BCNbr {
     uint32_t occ[6], up[6]; // U, Dn, A, Bw, Zu, Zd
 };
 
+// Neighbour occupancy / up words of word q of plane row iA, given the plane
+// rows of its y-1, y+1 (and z-1, z+1) neighbours in the other colour's planes.
 template <int DIM>
-__device__ __forceinline__ void bc_load_nbrs(const uint32_t* __restrict__ Oo,
-                                             const uint32_t* __restrict__ So, int L, int nw,
-                                             int half, int r, int q, int y, int z, int p, int W,
-                                             uint32_t valid, BCNbr& n)
+__device__ __forceinline__ void bc_nbrs_rows(const uint32_t* __restrict__ Oo,
+                                             const uint32_t* __restrict__ So, int nw, int half,
+                                             int iA0, int iU0, int iD0, int iZu0, int iZd0, int q,
+                                             int p, int W, uint32_t valid, BCNbr& n)
 {
-    const int base = DIM == 3 ? z * L : 0;
-    const int yu = y == 0 ? L - 1 : y - 1;
-    const int yd = y == L - 1 ? 0 : y + 1;
-    const int iU = (base + yu) * nw + q, iD = (base + yd) * nw + q, iA = r * nw + q;
+    const int r = iA0;
+    const int iU = iU0 * nw + q, iD = iD0 * nw + q, iA = r * nw + q;
     n.occ[0] = Oo[iU];
     n.up[0] = So[iU];
     n.occ[1] = Oo[iD];
@@ -160,9 +160,7 @@ __device__ __forceinline__ void bc_load_nbrs(const uint32_t* __restrict__ Oo,
     n.occ[2] = Oo[iA];
     n.up[2] = So[iA];
     if (DIM == 3) {
-        const int zu = z == 0 ? L - 1 : z - 1;
-        const int zd = z == L - 1 ? 0 : z + 1;
-        const int iZu = (zu * L + y) * nw + q, iZd = (zd * L + y) * nw + q;
+        const int iZu = iZu0 * nw + q, iZd = iZd0 * nw + q;
         n.occ[4] = Oo[iZu];
         n.up[4] = So[iZu];
         n.occ[5] = Oo[iZd];
@@ -189,6 +187,21 @@ __device__ __forceinline__ void bc_load_nbrs(const uint32_t* __restrict__ Oo,
         n.occ[3] = ((A0 << 1) | ((Oo[r * nw + qp] >> (Wp - 1)) & 1u)) & valid;
         n.up[3] = ((A1 << 1) | ((So[r * nw + qp] >> (Wp - 1)) & 1u)) & valid;
     }
+}
+
+template <int DIM>
+__device__ __forceinline__ void bc_load_nbrs(const uint32_t* __restrict__ Oo,
+                                             const uint32_t* __restrict__ So, int L, int nw,
+                                             int half, int r, int q, int y, int z, int p, int W,
+                                             uint32_t valid, BCNbr& n)
+{
+    const int base = DIM == 3 ? z * L : 0;
+    const int yu = y == 0 ? L - 1 : y - 1;
+    const int yd = y == L - 1 ? 0 : y + 1;
+    const int zu = DIM == 3 ? (z == 0 ? L - 1 : z - 1) : 0;
+    const int zd = DIM == 3 ? (z == L - 1 ? 0 : z + 1) : 0;
+    bc_nbrs_rows<DIM>(Oo, So, nw, half, r, base + yu, base + yd, zu * L + y, zd * L + y, q, p, W,
+                      valid, n);
 }
 
 // Bit-sliced (sum_j s_j + 2*dim) for 32 sites: each neighbour adds 2 if up,
@@ -346,8 +359,6 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
     const int wave = tid >> 6;
     const int nwaves = (B + 63) >> 6;
     constexpr uint32_t purpose = ALG == 0 ? P_BC_METRO : P_BC_HB;
-    const int lpw = a.lanes_per_word > 1 ? a.lanes_per_word : 1;
-    const int lpart = tid & (lpw - 1);
 
     for (int item = blockIdx.x; item < a.n_items; item += gridDim.x) {
         const int ch = a.items[item];
@@ -363,6 +374,15 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
         const int nw = (half + 31) >> 5;
         const int rows = DIM == 3 ? L * L : L;
         const int words = rows * nw;
+        // lanes per word: as asked, or (negative) the most, up to 8, that keep
+        // words x lanes within the workgroup; uniform over the workgroup
+        int lpw = a.lanes_per_word > 1 ? a.lanes_per_word : 1;
+        if (a.lanes_per_word < 0) {
+            lpw = 8;
+            while (lpw > 1 && words * lpw > B)
+                lpw >>= 1;
+        }
+        const int lpart = tid & (lpw - 1);
         int8_t* __restrict__ sp = a.spins + cr.spin_off;
         const BCTable* tb = a.btab + cr.table;
         for (int k = tid; k < 83; k += B)
@@ -686,6 +706,139 @@ __global__ void __launch_bounds__(256) k_bc_pack(TileArgs a, int unpack)
         bc_pack_word<DIM>(sp, P, w, L, nw, half, words);
 }
 
+// Multi-workgroup path with temporal blocking (the scheme of k_ising_band):
+// one band of a.band slabs plus 2T halo slabs on either side in LDS, T sweeps
+// per launch, half-sweep k updating the local slabs [k + 1, R - 2 - k], the
+// band written back, per-sweep observables over the band.  a.lpw lanes per
+// plane word, as in k_bc_tiles.  LDS: [occ c0, occ c1, up c0, up c1][R slabs][rows per slab][nw].
+template <int DIM, int ALG>
+__global__ void __launch_bounds__(kBandThreads) k_bc_band(TileArgs a)
+{
+    extern __shared__ uint32_t bq[];
+    __shared__ uint32_t tabw[84];
+    __shared__ double tabd[39];
+    const int item = blockIdx.y;
+    const int64_t nsw = a.item_nsw[item];
+    if (a.s >= nsw)
+        return;
+    constexpr uint32_t purpose = ALG == 0 ? P_BC_METRO : P_BC_HB;
+    const int tid = threadIdx.x;
+    const int nthr = (int)blockDim.x;
+    const int ch = a.items[item];
+    const ChainRec cr = a.chains[ch];
+    const int L = cr.L;
+    const int B = a.band;
+    const int s0 = blockIdx.x * B;
+    if (s0 >= L)
+        return;
+    const BCTable* tb = a.btab + cr.table;
+    for (int k = tid; k < 83; k += nthr)
+        tabw[k] = k < 80 ? (a.glauber ? tb->gl_thr[k] : tb->metro_thr[k])
+                         : (a.glauber ? tb->gl_always[k - 80] : tb->metro_always[k - 80]);
+    for (int k = tid; k < 39; k += nthr)
+        tabd[k] = tb->hb[k / 3][k % 3];
+    const int half = L >> 1;
+    const int nw = (half + 31) >> 5;
+    const int spr = DIM == 3 ? L : 1;
+    const int H = 2 * a.T;
+    const int Bb = B < L - s0 ? B : L - s0;
+    const int R = Bb + 2 * H;
+    const int rw = spr * nw;
+    const int nloc = R * rw;
+    const int words = (DIM == 3 ? L * L : L) * nw;
+    uint32_t* P = reinterpret_cast<uint32_t*>(a.base + a.item_plane[item]);
+    // global slab of local slab 0, in [0, L): the halo may be longer than the lattice
+    const int g0 = ((s0 - H) % L + L) % L;
+    for (int i = tid; i < 4 * nloc; i += nthr) {
+        const int pl = i / nloc;
+        const int li = i - pl * nloc;
+        const int ls = li / rw;
+        bq[i] = P[pl * words + ((g0 + ls) % L) * rw + (li - ls * rw)];
+    }
+    __syncthreads();
+    const uint32_t k0 = (uint32_t)cr.seed, c3 = (uint32_t)(cr.seed >> 32);
+    const int lpw = a.lpw > 1 ? a.lpw : 1;
+    const int part = tid & (lpw - 1);
+    const int64_t left = nsw - a.s;
+    const int Tn = left < (int64_t)a.T ? (int)left : a.T;
+    for (int t = 0; t < Tn; ++t) {
+        const uint64_t sweep = cr.sweep + (uint64_t)(a.s + t);
+        const uint32_t w2 = ctr_word2(sweep, 0, purpose);
+        uint32_t flips = 0;
+        int32_t pr = 0, mm = 0, qq = 0;
+        for (int c = 0; c < 2; ++c) {
+            const int k = 2 * t + c;
+            const int lo = k + 1, n_upd = (R - 2 - k - lo + 1) * rw;
+            uint32_t* Oc = bq + c * nloc;
+            uint32_t* Sc = bq + (2 + c) * nloc;
+            const uint32_t* Oo = bq + (c ^ 1) * nloc;
+            const uint32_t* So = bq + (2 + (c ^ 1)) * nloc;
+            // the lanes of one word share i, so the group stays converged for its shuffles
+            for (int i = tid / lpw; i < n_upd; i += nthr / lpw) {
+                const int ls = lo + i / rw;
+                const int rem = i - (ls - lo) * rw;
+                const int yl = DIM == 3 ? rem / nw : 0;
+                const int q = rem - yl * nw;
+                const int gs = (g0 + ls) % L;
+                const int y = DIM == 3 ? yl : gs;
+                const int z = DIM == 3 ? gs : 0;
+                const int r = DIM == 3 ? z * L + y : y;
+                const int lr = ls * spr + yl;
+                int iU, iD, iZu = 0, iZd = 0;
+                if (DIM == 2) {
+                    iU = lr - 1;
+                    iD = lr + 1;
+                } else {
+                    iU = ls * L + (yl == 0 ? L - 1 : yl - 1);
+                    iD = ls * L + (yl == L - 1 ? 0 : yl + 1);
+                    iZu = lr - L;
+                    iZd = lr + L;
+                }
+                const int p = (c + y + z) & 1;
+                const int W = word_width(q, half);
+                const uint32_t valid = width_mask(W);
+                uint32_t occ = Oc[lr * nw + q], up = Sc[lr * nw + q];
+                BCNbr n;
+                bc_nbrs_rows<DIM>(Oo, So, nw, half, lr, iU, iD, iZu, iZd, q, p, W, valid, n);
+                uint32_t v0, v1, v2, v3;
+                bc_shifted_sum<DIM>(n, v0, v1, v2, v3);
+                const uint32_t group = (uint32_t)((r * 2 + c) * nw + q);
+                const uint32_t nchanged =
+                    bc_word<DIM, ALG>(occ, up, v0, v1, v2, v3, W, tabw, tabd, group, (uint32_t)sweep,
+                                      w2, c3, k0, cr.key, part, lpw);
+                if (part == 0) {
+                    Oc[lr * nw + q] = occ;
+                    Sc[lr * nw + q] = up;
+                }
+                if (ls >= H && ls < H + Bb) {
+                    flips += nchanged;
+                    if (c == 1 && part == 0) {
+                        const uint32_t dn = occ & ~up;
+                        const uint32_t upo = occ & up;
+#pragma unroll
+                        for (int kk = 0; kk < 2 * DIM; ++kk) {
+                            const uint32_t both = occ & n.occ[kk];
+                            const uint32_t anti = both & (up ^ n.up[kk]);
+                            pr += __popc(both) - 2 * __popc(anti);
+                        }
+                        const uint32_t Ao = n.occ[2], Au = n.up[2];
+                        mm += __popc(upo) - __popc(dn) + __popc(Ao & Au) - __popc(Ao & ~Au);
+                        qq += __popc(occ) + __popc(Ao);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        tile_block_add_s(a, item, a.s + t, pr, mm, qq, (int32_t)flips);
+    }
+    for (int i = tid; i < 4 * Bb * rw; i += nthr) {
+        const int pl = i / (Bb * rw);
+        const int li = i - pl * Bb * rw;
+        const int b = li / rw;
+        P[pl * words + ((s0 + b) % L) * rw + (li - b * rw)] = bq[pl * nloc + (H + b) * rw + (li - b * rw)];
+    }
+}
+
 // Metropolis and Glauber share the kernels (ALG 0) and differ in the threshold
 // table, chosen by the glauber flag; heat bath is ALG 1.
 hipError_t launch_bc_tiles(int dim, int alg, const TileArgs& a0, int max_words, hipStream_t st)
@@ -703,6 +856,24 @@ hipError_t launch_bc_tiles(int dim, int alg, const TileArgs& a0, int max_words, 
         hipLaunchKernelGGL((k_bc_tiles<3, 0>), grid, dim3(256), 0, st, a);
     else
         hipLaunchKernelGGL((k_bc_tiles<3, 1>), grid, dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bc_band(int dim, int alg, const TileArgs& a0, int max_bands, int threads,
+                          size_t lds, hipStream_t st)
+{
+    TileArgs a = a0;
+    a.glauber = alg == MCX_GLAUBER ? 1 : 0;
+    const bool hb = alg == MCX_HEATBATH;
+    const dim3 grid(max_bands, a.n_items), bl(threads);
+    if (dim == 2 && !hb)
+        hipLaunchKernelGGL((k_bc_band<2, 0>), grid, bl, lds, st, a);
+    else if (dim == 2)
+        hipLaunchKernelGGL((k_bc_band<2, 1>), grid, bl, lds, st, a);
+    else if (!hb)
+        hipLaunchKernelGGL((k_bc_band<3, 0>), grid, bl, lds, st, a);
+    else
+        hipLaunchKernelGGL((k_bc_band<3, 1>), grid, bl, lds, st, a);
     return hipGetLastError();
 }
 

@@ -107,6 +107,38 @@ __device__ __forceinline__ uint32_t count_eq(uint32_t b0, uint32_t b1, uint32_t 
 // Block-level sum of four per-lane values, then one atomic per field into this
 // block's replica of the item's per-sweep sums (same-address atomics from every
 // wave of a big lattice would serialise).  blockDim.x = 256.
+// tile_block_add for sweep s of the quantum, any block of up to 16 waves
+// (the band kernels run several sweeps per launch)
+__device__ __forceinline__ void tile_block_add_s(const TileArgs& a, int item, int64_t s, int32_t f0,
+                                                 int32_t f1, int32_t f2, int32_t f3)
+{
+    __shared__ int32_t red_s[16][4];
+    const int wave = threadIdx.x >> 6;
+    const int nwaves = (int)(blockDim.x >> 6);
+    f0 = wave_sum(f0);
+    f1 = wave_sum(f1);
+    f2 = wave_sum(f2);
+    f3 = wave_sum(f3);
+    if ((threadIdx.x & 63) == 0) {
+        red_s[wave][0] = f0;
+        red_s[wave][1] = f1;
+        red_s[wave][2] = f2;
+        red_s[wave][3] = f3;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int k = threadIdx.x;
+        int32_t v = 0;
+        for (int w = 0; w < nwaves; ++w)
+            v += red_s[w][k];
+        if (v != 0) {
+            int32_t* o = reinterpret_cast<int32_t*>(a.base + a.item_obs[item]);
+            atomicAdd(&o[(s * 4 + k) * kTileObsReplicas + (int)(blockIdx.x % kTileObsReplicas)], v);
+        }
+    }
+    __syncthreads();
+}
+
 __device__ __forceinline__ void tile_block_add(const TileArgs& a, int item, int32_t f0, int32_t f1,
                                                int32_t f2, int32_t f3)
 {

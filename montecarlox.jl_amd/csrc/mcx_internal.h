@@ -80,7 +80,8 @@ struct SweepArgs {
     int32_t* last_Q;
     const int64_t* item_nsweeps; // if set: sweeps of item k (overrides nsweeps)
     int64_t* item_ticks;         // if set: wall-clock ticks item k took
-    int32_t lanes_per_word;      // Blume-Capel: lanes sharing one plane word (1, 2, 4, 8; 0 = 1)
+    int32_t lanes_per_word;      // Blume-Capel: lanes sharing one plane word (1, 2, 4, 8; 0 = 1;
+                                 // negative: per lattice, the most up to 8 within the workgroup)
     int32_t by_table;            // series / moments of chain k go to slot chains[k].table
     int64_t cap;                 // series capacity per slot (slot j starts at j * cap)
     int32_t glauber;             // Blume-Capel: Glauber thresholds instead of Metropolis ones
@@ -127,6 +128,8 @@ struct TileArgs {
     int32_t lpw;               // Blume-Capel: lanes sharing one plane word (1, 2, 4, 8; 0 = 1)
     unsigned long long* accepted;
     int32_t glauber;           // Blume-Capel: Glauber thresholds instead of Metropolis ones
+    int32_t band;              // band kernels: slabs (rows in 2D, z-planes in 3D) per band
+    int32_t T;                 // band kernels: sweeps per launch (halo 2T slabs each side)
 };
 
 // Launchers (defined in the .hip translation units).
@@ -146,6 +149,11 @@ hipError_t launch_bc_pack(int dim, int unpack, const TileArgs& a, int max_words,
 // Folds the per-sweep sums of a series into each item's FP64 moments (sweep by
 // sweep, as the batched kernels do), last-sweep E/M/Q and sweep counter, and
 // clears the sums.
+constexpr int kBandThreads = 1024; // largest band workgroup (a multiple of 64 is launched)
+hipError_t launch_ising_band(int dim, int alg, const TileArgs& a, int max_bands, int threads,
+                             size_t lds, hipStream_t st);
+hipError_t launch_bc_band(int dim, int alg, const TileArgs& a, int max_bands, int threads,
+                          size_t lds, hipStream_t st);
 hipError_t launch_tiles_fold(int model, const TileArgs& a, double* moments, int32_t* last_E,
                              int32_t* last_M, int32_t* last_Q, hipStream_t st);
 // Integrated autocorrelation times of recorded series, one workgroup per slot.

@@ -36,22 +36,20 @@ struct NbrWords {
     uint32_t U, Dn, Zu, Zd;
 };
 
+// Neighbour words of word q of plane row iA, given the plane rows of its
+// y-1, y+1 (and z-1, z+1) neighbours in the other colour's plane Po.
 template <int DIM>
-__device__ __forceinline__ void load_nbr_words(const uint32_t* __restrict__ Po, int L, int nw,
-                                               int half, int r, int q, int y, int z, int p,
-                                               int W, uint32_t valid, NbrWords& n)
+__device__ __forceinline__ void nbr_words_rows(const uint32_t* __restrict__ Po, int nw, int half,
+                                               int iA, int iU, int iD, int iZu, int iZd, int q,
+                                               int p, int W, uint32_t valid, NbrWords& n)
 {
-    const int base = DIM == 3 ? z * L : 0;
-    const int yu = y == 0 ? L - 1 : y - 1;
-    const int yd = y == L - 1 ? 0 : y + 1;
+    const int r = iA;
     n.A = Po[r * nw + q];
-    n.U = Po[(base + yu) * nw + q];
-    n.Dn = Po[(base + yd) * nw + q];
+    n.U = Po[iU * nw + q];
+    n.Dn = Po[iD * nw + q];
     if (DIM == 3) {
-        const int zu = z == 0 ? L - 1 : z - 1;
-        const int zd = z == L - 1 ? 0 : z + 1;
-        n.Zu = Po[(zu * L + y) * nw + q];
-        n.Zd = Po[(zd * L + y) * nw + q];
+        n.Zu = Po[iZu * nw + q];
+        n.Zd = Po[iZd * nw + q];
     } else {
         n.Zu = 0;
         n.Zd = 0;
@@ -70,6 +68,20 @@ __device__ __forceinline__ void load_nbr_words(const uint32_t* __restrict__ Po, 
         const int Wp = word_width(qp, half);
         n.Bw = ((n.A << 1) | ((Po[r * nw + qp] >> (Wp - 1)) & 1u)) & valid;
     }
+}
+
+template <int DIM>
+__device__ __forceinline__ void load_nbr_words(const uint32_t* __restrict__ Po, int L, int nw,
+                                               int half, int r, int q, int y, int z, int p,
+                                               int W, uint32_t valid, NbrWords& n)
+{
+    const int base = DIM == 3 ? z * L : 0;
+    const int yu = y == 0 ? L - 1 : y - 1;
+    const int yd = y == L - 1 ? 0 : y + 1;
+    const int zu = DIM == 3 ? (z == 0 ? L - 1 : z - 1) : 0;
+    const int zd = DIM == 3 ? (z == L - 1 ? 0 : z + 1) : 0;
+    nbr_words_rows<DIM>(Po, nw, half, r, base + yu, base + yd, zu * L + y, zd * L + y, q, p, W,
+                        valid, n);
 }
 
 // Bit-serial "w < thr[class]" for the 32 sites of one word.  eq: sites still
@@ -613,6 +625,120 @@ __global__ void __launch_bounds__(256) k_ising_pack(TileArgs a, int unpack)
         ising_pack_word<DIM>(sp, P, w, L, nw, half, words);
 }
 
+// Multi-workgroup path with temporal blocking.  A work item is one band of
+// a.band slabs (rows in 2D, z-planes in 3D) of one lattice; the workgroup
+// loads the band and H = 2T slabs on either side (periodic) into LDS and runs
+// T sweeps there.  Half-sweep k = 2t + colour updates the local slabs
+// [k + 1, R - 2 - k]: a slab outside that range has a neighbour slab that is
+// stale or was never loaded, so after the T sweeps exactly the band
+// [H, H + B) is right for both colours.  Halo sites are updated by several
+// workgroups with the same counter-based random words, so they agree with
+// their owners; only the band is written back.  Per-sweep observables and
+// flips are summed over the band into the item's per-sweep replicas, as the
+// per-colour kernel does.  LDS: [2 colours][R slabs][rows per slab][nw].
+template <int DIM, int ALG>
+__global__ void __launch_bounds__(kBandThreads) k_ising_band(TileArgs a)
+{
+    extern __shared__ uint32_t bq[];
+    const int item = blockIdx.y;
+    const int64_t nsw = a.item_nsw[item];
+    if (a.s >= nsw)
+        return;
+    constexpr uint32_t purpose = ALG == 1 ? P_HB : P_METRO;
+    const int tid = threadIdx.x;
+    const int nthr = (int)blockDim.x;
+    const int ch = a.items[item];
+    const ChainRec cr = a.chains[ch];
+    const int L = cr.L;
+    const int B = a.band;
+    const int s0 = blockIdx.x * B;
+    if (s0 >= L)
+        return;
+    const int half = L >> 1;
+    const int nw = (half + 31) >> 5;
+    const int spr = DIM == 3 ? L : 1; // plane rows per slab
+    const int H = 2 * a.T;
+    const int Bb = B < L - s0 ? B : L - s0;
+    const int R = Bb + 2 * H;
+    const int rw = spr * nw; // words per slab and colour
+    const int nloc = R * rw;
+    const int words = (DIM == 3 ? L * L : L) * nw;
+    uint32_t* P = reinterpret_cast<uint32_t*>(a.base + a.item_plane[item]);
+    // global slab of local slab 0, in [0, L): the halo may be longer than the lattice
+    const int g0 = ((s0 - H) % L + L) % L;
+    for (int i = tid; i < 2 * nloc; i += nthr) {
+        const int c = i >= nloc ? 1 : 0;
+        const int li = i - c * nloc;
+        const int ls = li / rw;
+        bq[i] = P[c * words + ((g0 + ls) % L) * rw + (li - ls * rw)];
+    }
+    __syncthreads();
+    const IsingTable tb = a.itab[cr.table];
+    const uint32_t k0 = (uint32_t)cr.seed, c3 = (uint32_t)(cr.seed >> 32);
+    const int64_t left = nsw - a.s;
+    const int Tn = left < (int64_t)a.T ? (int)left : a.T;
+    for (int t = 0; t < Tn; ++t) {
+        const uint64_t sweep = cr.sweep + (uint64_t)(a.s + t);
+        const uint32_t w2 = ctr_word2(sweep, 0, purpose);
+        uint32_t flips = 0, nu = 0, na = 0;
+        for (int c = 0; c < 2; ++c) {
+            const int k = 2 * t + c;
+            const int lo = k + 1, n_upd = (R - 2 - k - lo + 1) * rw;
+            uint32_t* Qc = bq + c * nloc;
+            const uint32_t* Qo = bq + (c ^ 1) * nloc;
+            for (int i = tid; i < n_upd; i += nthr) {
+                const int ls = lo + i / rw;
+                const int rem = i - (ls - lo) * rw;
+                const int yl = DIM == 3 ? rem / nw : 0;
+                const int q = rem - yl * nw;
+                const int gs = (g0 + ls) % L;
+                const int y = DIM == 3 ? yl : gs;
+                const int z = DIM == 3 ? gs : 0;
+                const int r = DIM == 3 ? z * L + y : y; // global plane row
+                const int lr = ls * spr + yl;           // local plane row
+                int iU, iD, iZu = 0, iZd = 0;
+                if (DIM == 2) {
+                    iU = lr - 1;
+                    iD = lr + 1;
+                } else {
+                    iU = ls * L + (yl == 0 ? L - 1 : yl - 1);
+                    iD = ls * L + (yl == L - 1 ? 0 : yl + 1);
+                    iZu = lr - L;
+                    iZd = lr + L;
+                }
+                const int p = (c + y + z) & 1;
+                const int W = word_width(q, half);
+                const uint32_t valid = width_mask(W);
+                const uint32_t sw0 = Qc[lr * nw + q];
+                NbrWords n;
+                nbr_words_rows<DIM>(Qo, nw, half, lr, iU, iD, iZu, iZd, q, p, W, valid, n);
+                const uint32_t group = (uint32_t)((r * 2 + c) * nw + q);
+                const uint32_t snew = ising_word<DIM, ALG>(sw0, n, valid, tb, group, (uint32_t)sweep,
+                                                           w2, c3, k0, cr.key);
+                Qc[lr * nw + q] = snew;
+                if (ls >= H && ls < H + Bb) {
+                    flips += (uint32_t)__popc(snew ^ sw0);
+                    if (c == 1) {
+                        na += __popc(snew ^ n.U) + __popc(snew ^ n.Dn) + __popc(snew ^ n.A) +
+                              __popc(snew ^ n.Bw);
+                        if (DIM == 3)
+                            na += __popc(snew ^ n.Zu) + __popc(snew ^ n.Zd);
+                        nu += __popc(snew) + __popc(n.A);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        tile_block_add_s(a, item, a.s + t, (int32_t)nu, (int32_t)na, 0, (int32_t)flips);
+    }
+    for (int i = tid; i < 2 * Bb * rw; i += nthr) {
+        const int c = i >= Bb * rw ? 1 : 0;
+        const int li = i - c * Bb * rw;
+        const int b = li / rw;
+        P[c * words + ((s0 + b) % L) * rw + (li - b * rw)] = bq[c * nloc + (H + b) * rw + (li - b * rw)];
+    }
+}
+
 // algorithm code -> kernel template: 0 Metropolis, 1 heat bath, 2 Glauber
 static int ising_alg(int alg)
 {
@@ -630,6 +756,22 @@ hipError_t launch_ising_tiles(int dim, int alg, const TileArgs& a, int max_words
     case 3: hipLaunchKernelGGL((k_ising_tiles<3, 0>), grid, dim3(256), 0, st, a); break;
     case 4: hipLaunchKernelGGL((k_ising_tiles<3, 1>), grid, dim3(256), 0, st, a); break;
     default: hipLaunchKernelGGL((k_ising_tiles<3, 2>), grid, dim3(256), 0, st, a); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ising_band(int dim, int alg, const TileArgs& a, int max_bands, int threads,
+                             size_t lds, hipStream_t st)
+{
+    const dim3 grid(max_bands, a.n_items), bl(threads);
+    const int k = ising_alg(alg) + (dim == 3 ? 3 : 0);
+    switch (k) {
+    case 0: hipLaunchKernelGGL((k_ising_band<2, 0>), grid, bl, lds, st, a); break;
+    case 1: hipLaunchKernelGGL((k_ising_band<2, 1>), grid, bl, lds, st, a); break;
+    case 2: hipLaunchKernelGGL((k_ising_band<2, 2>), grid, bl, lds, st, a); break;
+    case 3: hipLaunchKernelGGL((k_ising_band<3, 0>), grid, bl, lds, st, a); break;
+    case 4: hipLaunchKernelGGL((k_ising_band<3, 1>), grid, bl, lds, st, a); break;
+    default: hipLaunchKernelGGL((k_ising_band<3, 2>), grid, bl, lds, st, a); break;
     }
     return hipGetLastError();
 }

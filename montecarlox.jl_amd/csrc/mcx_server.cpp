@@ -5,7 +5,8 @@
 // arrive at any time.  A scheduler thread keeps their lattices resident in a
 // device pool bounded by an HBM budget and advances every resident job in
 // quanta.  A quantum is one launch per kernel class (model, dim, algorithm,
-// block size), the classes spread over a few streams so that different kernels
+// workgroup size; the multi-workgroup lattices one class per model, dim and
+// algorithm), the classes spread over a few streams so that different kernels
 // run side by side; within a launch every job runs the number of sweeps that
 // fills its share of the quantum's target time, from the per-sweep time the
 // kernel measured for it (device wall clock) in its previous quanta.  Finished
@@ -43,7 +44,8 @@ double now_ms()
 
 constexpr int64_t kAlign = 256;
 constexpr int kMaxStreams = 8;
-constexpr int kTileMaxSweeps = 256; // sweeps of a tiled job per quantum (per-sweep sum buffer)
+constexpr int kTileMaxSweeps = 512; // sweeps of a tiled job per quantum (per-sweep sum buffer)
+constexpr int kMaxTileClasses = 16; // tiled classes (model, dim, algorithm) of one quantum
 // Lanes per plane word of a tiled Blume-Capel launch: up to 8 (one Philox
 // block each) while the launch stays near 16k lanes; more lanes only repeat
 // the neighbour gathers once the chip is covered.
@@ -53,6 +55,55 @@ int tile_lanes(int max_words)
     while (l > 1 && (int64_t)max_words * l > 16384)
         l /= 2;
     return l;
+}
+
+// Band shape of a tiled class: B slabs per band, T sweeps per launch (a halo
+// of 2T slabs on either side), lpw lanes per plane word and the workgroup
+// size; T = 0: the band and its halo do not fit the LDS, and the class runs
+// the per-colour kernels.
+struct Band {
+    int B, T, lpw, threads;
+};
+
+size_t band_lds(int model, int dim, int L, int B, int T)
+{
+    const int nw = (L / 2 + 31) / 32;
+    const size_t per_slab = (size_t)(dim == 3 ? L : 1) * nw * 4 * (model == MCX_ISING ? 2 : 4);
+    return per_slab * (size_t)(B + 4 * T);
+}
+
+Band band_shape(int model, int dim, int L, size_t lds_max)
+{
+    // static LDS of the band kernels (tables, reductions) and some slack
+    const size_t budget = lds_max > 4096 ? lds_max - 4096 : 0;
+    const int rw = (dim == 3 ? L : 1) * ((L / 2 + 31) / 32); // plane words per slab and colour
+    // Ising: 32 rows (8 planes in 3D) and 8 (4) sweeps per launch, one lane per
+    // word; Blume-Capel, whose word update costs many times more: 2 sweeps
+    // per launch, 8 lanes per word, and a band sized so that the largest
+    // workgroup covers the band and its halo in about one pass.  Measured on
+    // the Poisson serving stream against other shapes and the per-colour path.
+    Band b{};
+    if (model == MCX_ISING) {
+        b = Band{dim == 3 ? 8 : 32, dim == 3 ? 4 : 8, 1, 0};
+    } else {
+        b.T = 2;
+        b.lpw = 8;
+        b.B = std::max(dim == 3 ? 8 : 16, kBandThreads / b.lpw / rw - 4 * b.T);
+    }
+    b.B = std::min(b.B, L);
+    b.T = std::max(1, std::min(b.T, L / 4)); // a halo of 2T slabs: keep it within the lattice
+    while (band_lds(model, dim, L, b.B, b.T) > budget && b.T > 1)
+        --b.T;
+    while (band_lds(model, dim, L, b.B, b.T) > budget && b.B > 1)
+        b.B = std::max(1, b.B / 2);
+    if (band_lds(model, dim, L, b.B, b.T) > budget) {
+        b.T = 0;
+        return b;
+    }
+    // one lane (group) per word of the first half-sweep, in whole waves
+    const int64_t lanes = (int64_t)(b.B + 4 * b.T - 2) * rw * b.lpw;
+    b.threads = (int)std::min<int64_t>(kBandThreads, (lanes + 63) / 64 * 64);
+    return b;
 }
 
 int64_t align_up(int64_t n)
@@ -127,6 +178,17 @@ struct ClassKey {
 
 using ShapeKey = std::tuple<int, int, int, int>; // model, dim, alg, L
 
+// Launch class of an LDS-resident job: its kernel (model, dim, algorithm) and
+// one of two workgroup sizes, 256 for lattices whose own shape needs at most
+// 256 threads and 1024 above; Blume-Capel lanes per word are chosen per
+// lattice in the kernel (lpw -1).  Few classes keep the quantum's launches on
+// as many streams as there are classes, each with the whole quantum.
+ClassKey class_of(const Job& j)
+{
+    return ClassKey{j.d.model, j.d.dim, j.d.alg, j.block <= 256 ? 256 : 1024,
+                    j.d.model == MCX_ISING ? 1 : -1};
+}
+
 } // namespace
 
 struct mcx_server {
@@ -139,7 +201,8 @@ struct mcx_server {
     double quantum_ms = 0;
     double ticks_per_ms = 0; // device wall clock
     int64_t tile_sites[2] = {0, 0}; // per model: lattices this big take the tiled path
-    double tile_ms_per_sweep = 0;   // latest measurement of one tiled sweep (0 = none yet)
+    // latest measurement of one tiled sweep per tiled class (model, dim, alg)
+    std::map<std::tuple<int, int, int>, double> tile_ms_per_sweep;
     // device arrays, one entry per slot
     int8_t* d_pool = nullptr;
     ChainRec* d_chains = nullptr;
@@ -155,7 +218,7 @@ struct mcx_server {
     int64_t* d_tnsw = nullptr;           // [slots]
     int64_t* d_tplane = nullptr;         // [slots]
     int64_t* d_tobs = nullptr;           // [slots]
-    hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
+    hipEvent_t ev_t[kMaxTileClasses][2] = {}; // start / end of each tiled class's launches
     // worker-only state
     std::vector<Job> active;
     PoolAlloc* pool = nullptr;
@@ -371,15 +434,23 @@ int64_t quantum_sweeps(const mcx_server* s, const Job& j, double target)
     return n >= (double)j.remaining ? j.remaining : std::max<int64_t>(1, (int64_t)n);
 }
 
+using TileKey = std::tuple<int, int, int>; // model, dim, alg
+
+// Device time of one tiled class in one quantum: ms for sweeps sweeps.
+struct TileTime {
+    TileKey key;
+    float ms;
+    int64_t sweeps;
+};
+
 // One quantum over every active job: ran[i] = sweeps run by active[i],
 // ticks[i] = the wall-clock ticks its LDS-resident work item took (0 on the
-// tiled path), ms = device time of the quantum; tile_ms / tile_sweeps = time
-// and sweep count of the tiled launches (0 when there were none).
+// tiled path), ms = device time of the quantum, ttime = device time of each
+// tiled class.
 hipError_t run_quantum(mcx_server* s, std::vector<int64_t>& ran, std::vector<int64_t>& ticks,
-                       float& ms, float& tile_ms, int64_t& tile_sweeps)
+                       float& ms, std::vector<TileTime>& ttime)
 {
     const int n = (int)s->active.size();
-    using TileKey = std::tuple<int, int, int>; // model, dim, alg
     std::map<ClassKey, std::vector<int>> classes;
     std::map<TileKey, std::vector<int>> tclasses;
     for (int i = 0; i < n; ++i) {
@@ -387,21 +458,29 @@ hipError_t run_quantum(mcx_server* s, std::vector<int64_t>& ran, std::vector<int
         if (j.tiled)
             tclasses[TileKey{j.d.model, j.d.dim, j.d.alg}].push_back(i);
         else
-            classes[{j.d.model, j.d.dim, j.d.alg, j.block, j.lpw}].push_back(i);
+            classes[class_of(j)].push_back(i);
     }
-    // units of work dealt round-robin over the streams: the tiled launches as
-    // one unit first, then the LDS-resident classes in decreasing size
+    // units of work: each tiled class, then the LDS-resident classes in
+    // decreasing size.  The tiled classes hold the largest lattices, whose
+    // latency is the tail of the job stream: with streams to spare each gets a
+    // stream of its own (and so the whole quantum) and the LDS-resident classes
+    // are dealt round-robin over the others; else every unit is dealt round-robin.
     std::vector<std::pair<ClassKey, std::vector<int>>> cl(classes.begin(), classes.end());
     std::stable_sort(cl.begin(), cl.end(),
                      [](const auto& a, const auto& b) { return a.second.size() > b.second.size(); });
-    const bool has_tiles = !tclasses.empty();
-    const int n_units = (int)cl.size() + (has_tiles ? 1 : 0);
+    const int ntc = (int)tclasses.size();
+    if (ntc > kMaxTileClasses)
+        return hipErrorInvalidValue; // more (model, dim, algorithm) classes than exist
+    const int n_units = (int)cl.size() + ntc;
     const int ns = std::max(1, std::min(s->n_streams, n_units));
-    std::vector<int> per_stream(ns, 0);
+    std::vector<int> ustream(n_units), per_stream(ns, 0);
+    const bool own = ntc > 0 && ns > ntc && n_units > ntc;
     for (int u = 0; u < n_units; ++u)
-        ++per_stream[u % ns];
-    auto unit_stream = [&](int u) { return u % ns; };
-    const int unit0 = has_tiles ? 1 : 0; // unit of LDS class c is unit0 + c
+        ustream[u] = !own ? u % ns : u < ntc ? u : ntc + (u - ntc) % (ns - ntc);
+    for (int u = 0; u < n_units; ++u)
+        ++per_stream[ustream[u]];
+    auto unit_stream = [&](int u) { return ustream[u]; };
+    const int unit0 = ntc; // unit of LDS class c is unit0 + c; of tiled class t, t
 
     ran.assign(n, 0);
     // LDS-resident work list: class by class, the longest items first
@@ -426,32 +505,37 @@ hipError_t run_quantum(mcx_server* s, std::vector<int64_t>& ran, std::vector<int
         }
         span[c].second = (int)v.size();
     }
-    // tiled work list: every tiled job runs the same number of sweeps (or what
-    // it has left), sized from the last measured tiled sweep time
+    // tiled work list: every job of a tiled class runs the same number of
+    // sweeps (or what it has left), sized from the class's last measured sweep
     std::vector<int32_t> titems;
     std::vector<int64_t> tnsw, tplane, tobs;
     std::vector<std::tuple<TileKey, int, int, int>> tspan; // key, first, count, max words
-    tile_sweeps = 0;
-    if (has_tiles) {
-        const double target_ms = s->quantum_ms / per_stream[unit_stream(0)];
-        const double per = s->tile_ms_per_sweep > 0 ? s->tile_ms_per_sweep : 0.02;
-        const int64_t nt =
-            std::max<int64_t>(1, std::min<int64_t>(kTileMaxSweeps, (int64_t)std::floor(target_ms / per)));
-        for (auto& kv : tclasses) {
-            const int first = (int)titems.size();
-            int max_words = 0;
-            for (int i : kv.second) {
-                const Job& j = s->active[i];
-                ran[i] = std::min(nt, j.remaining);
-                tile_sweeps = std::max(tile_sweeps, ran[i]);
-                titems.push_back(j.slot);
-                tnsw.push_back(ran[i]);
-                tplane.push_back(j.plane_off);
-                tobs.push_back(j.obs_off);
-                max_words = std::max(max_words, j.words);
-            }
-            tspan.emplace_back(kv.first, first, (int)kv.second.size(), max_words);
+    std::vector<int> tspan_L;                              // largest L of each class
+    std::vector<int64_t> tspan_sw;                         // sweeps of each class
+    for (auto& kv : tclasses) {
+        const int t = (int)tspan.size();
+        const double target_ms = s->quantum_ms / per_stream[unit_stream(t)];
+        const auto it = s->tile_ms_per_sweep.find(kv.first);
+        const double per = it != s->tile_ms_per_sweep.end() ? it->second : 0.02;
+        const int64_t nt = std::max<int64_t>(
+            1, std::min<int64_t>(kTileMaxSweeps, (int64_t)std::floor(target_ms / per)));
+        const int first = (int)titems.size();
+        int max_words = 0, max_L = 0;
+        int64_t sw = 0;
+        for (int i : kv.second) {
+            const Job& j = s->active[i];
+            max_L = std::max(max_L, j.d.L);
+            ran[i] = std::min(nt, j.remaining);
+            sw = std::max(sw, ran[i]);
+            titems.push_back(j.slot);
+            tnsw.push_back(ran[i]);
+            tplane.push_back(j.plane_off);
+            tobs.push_back(j.obs_off);
+            max_words = std::max(max_words, j.words);
         }
+        tspan.emplace_back(kv.first, first, (int)kv.second.size(), max_words);
+        tspan_L.push_back(max_L);
+        tspan_sw.push_back(sw);
     }
 
     hipStream_t s0 = s->streams[0];
@@ -477,52 +561,58 @@ hipError_t run_quantum(mcx_server* s, std::vector<int64_t>& ran, std::vector<int
         e = hipEventRecord(s->ev_in, s0);
     for (int k = 1; k < ns && e == hipSuccess; ++k)
         e = hipStreamWaitEvent(s->streams[k], s->ev_in, 0);
-    // tiled unit: per sweep, colour 0 then colour 1 of every tiled class, then the fold
-    if (has_tiles && e == hipSuccess) {
-        hipStream_t st = s->streams[unit_stream(0)];
-        e = hipEventRecord(s->ev_t0, st);
-        for (int64_t sw = 0; sw < tile_sweeps && e == hipSuccess; ++sw)
-            for (const auto& ts : tspan) {
-                const TileKey& k = std::get<0>(ts);
-                TileArgs a{};
-                a.items = s->d_titems + std::get<1>(ts);
-                a.item_nsw = s->d_tnsw + std::get<1>(ts);
-                a.item_plane = s->d_tplane + std::get<1>(ts);
-                a.n_items = std::get<2>(ts);
+    // tiled classes, each on the stream of its unit: band kernels (T sweeps per
+    // launch) where the band and its halo fit the LDS, else the per-colour
+    // kernels (one launch per colour and sweep); then the fold
+    for (size_t t = 0; t < tspan.size() && e == hipSuccess; ++t) {
+        const auto& ts = tspan[t];
+        const TileKey& k = std::get<0>(ts);
+        const int model = std::get<0>(k), dim = std::get<1>(k), alg = std::get<2>(k);
+        const int64_t nsw_t = tspan_sw[t];
+        hipStream_t st = s->streams[unit_stream((int)t)];
+        e = hipEventRecord(s->ev_t[t][0], st);
+        TileArgs a{};
+        a.items = s->d_titems + std::get<1>(ts);
+        a.item_nsw = s->d_tnsw + std::get<1>(ts);
+        a.item_plane = s->d_tplane + std::get<1>(ts);
+        a.n_items = std::get<2>(ts);
+        a.chains = s->d_chains;
+        a.itab = s->d_itab;
+        a.btab = s->d_btab;
+        a.base = reinterpret_cast<uint8_t*>(s->d_pool);
+        a.item_obs = s->d_tobs + std::get<1>(ts);
+        a.max_s = kTileMaxSweeps;
+        a.accepted = s->d_acc;
+        const Band b = band_shape(model, dim, tspan_L[t], s->ctx->lds_max);
+        if (b.T != 0) {
+            a.band = b.B;
+            a.lpw = b.lpw;
+            const int max_bands = (tspan_L[t] + b.B - 1) / b.B;
+            for (int64_t sw = 0; sw < nsw_t && e == hipSuccess; sw += b.T) {
                 a.s = sw;
-                a.chains = s->d_chains;
-                a.itab = s->d_itab;
-                a.btab = s->d_btab;
-                a.base = reinterpret_cast<uint8_t*>(s->d_pool);
-                a.item_obs = s->d_tobs + std::get<1>(ts);
-                a.max_s = kTileMaxSweeps;
-                a.lpw = tile_lanes(std::get<3>(ts));
-                a.accepted = s->d_acc;
+                a.T = (int32_t)std::min<int64_t>(b.T, nsw_t - sw);
+                const size_t lds = band_lds(model, dim, tspan_L[t], b.B, a.T);
+                e = model == MCX_ISING
+                        ? launch_ising_band(dim, alg, a, max_bands, b.threads, lds, st)
+                        : launch_bc_band(dim, alg, a, max_bands, b.threads, lds, st);
+            }
+        } else {
+            a.lpw = tile_lanes(std::get<3>(ts));
+            for (int64_t sw = 0; sw < nsw_t && e == hipSuccess; ++sw) {
+                a.s = sw;
                 for (int c = 0; c < 2 && e == hipSuccess; ++c) {
                     a.colour = c;
-                    e = std::get<0>(k) == MCX_ISING
-                            ? launch_ising_tiles(std::get<1>(k), std::get<2>(k), a, std::get<3>(ts), st)
-                            : launch_bc_tiles(std::get<1>(k), std::get<2>(k), a, std::get<3>(ts), st);
+                    e = model == MCX_ISING
+                            ? launch_ising_tiles(dim, alg, a, std::get<3>(ts), st)
+                            : launch_bc_tiles(dim, alg, a, std::get<3>(ts), st);
                 }
-                if (e != hipSuccess)
-                    break;
             }
-        for (size_t t = 0; t < tspan.size() && e == hipSuccess; ++t) {
-            const auto& ts = tspan[t];
-            TileArgs a{};
-            a.items = s->d_titems + std::get<1>(ts);
-            a.item_nsw = s->d_tnsw + std::get<1>(ts);
-            a.n_items = std::get<2>(ts);
-            a.chains = s->d_chains;
-            a.base = reinterpret_cast<uint8_t*>(s->d_pool);
-            a.item_obs = s->d_tobs + std::get<1>(ts);
-            a.max_s = kTileMaxSweeps;
-            a.accepted = s->d_acc;
-            e = launch_tiles_fold(std::get<0>(std::get<0>(ts)), a, s->d_mom, s->d_last,
-                                  s->d_last + s->max_slots, s->d_last + 2 * s->max_slots, st);
         }
         if (e == hipSuccess)
-            e = hipEventRecord(s->ev_t1, st);
+            e = launch_tiles_fold(model, a, s->d_mom, s->d_last, s->d_last + s->max_slots,
+                                  s->d_last + 2 * s->max_slots, st);
+        if (e == hipSuccess)
+            e = hipEventRecord(s->ev_t[t][1], st);
     }
     for (size_t c = 0; c < cl.size() && e == hipSuccess; ++c) {
         const ClassKey& k = cl[c].first;
@@ -564,11 +654,14 @@ hipError_t run_quantum(mcx_server* s, std::vector<int64_t>& ran, std::vector<int
     if (e == hipSuccess)
         e = e2;
     ms = 0;
-    tile_ms = 0;
+    ttime.clear();
     if (e == hipSuccess)
         e = hipEventElapsedTime(&ms, s->ev0, s->ev1);
-    if (e == hipSuccess && has_tiles)
-        e = hipEventElapsedTime(&tile_ms, s->ev_t0, s->ev_t1);
+    for (size_t t = 0; t < tspan.size() && e == hipSuccess; ++t) {
+        float tms = 0;
+        e = hipEventElapsedTime(&tms, s->ev_t[t][0], s->ev_t[t][1]);
+        ttime.push_back({std::get<0>(tspan[t]), tms, tspan_sw[t]});
+    }
     ticks.assign(n, 0);
     for (int q = 0; q < nl; ++q)
         ticks[item_job[q]] = tk[q];
@@ -582,6 +675,7 @@ void worker_main(mcx_server* s)
     std::vector<unsigned long long> acc(n);
     std::vector<int32_t> last(3 * n);
     std::vector<int64_t> ran, ticks;
+    std::vector<TileTime> ttime;
     auto die = [&](const char* what, hipError_t e) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->error = std::string(what) + ": " + hipGetErrorString(e);
@@ -609,12 +703,12 @@ void worker_main(mcx_server* s)
                 return die("job set-up", e);
             s->active.insert(s->active.end(), fresh.begin(), fresh.end());
         }
-        float ms = 0, tile_ms = 0;
-        int64_t tile_sweeps = 0;
-        if ((e = run_quantum(s, ran, ticks, ms, tile_ms, tile_sweeps)) != hipSuccess)
+        float ms = 0;
+        if ((e = run_quantum(s, ran, ticks, ms, ttime)) != hipSuccess)
             return die("quantum", e);
-        if (tile_sweeps > 0 && tile_ms > 0)
-            s->tile_ms_per_sweep = (double)tile_ms / (double)tile_sweeps;
+        for (const TileTime& tt : ttime)
+            if (tt.sweeps > 0 && tt.ms > 0)
+                s->tile_ms_per_sweep[tt.key] = (double)tt.ms / (double)tt.sweeps;
         bool any_done = false;
         for (size_t i = 0; i < s->active.size(); ++i) {
             Job& j = s->active[i];
@@ -694,9 +788,13 @@ void free_device(mcx_server* s)
     hipFree(s->d_tnsw);
     hipFree(s->d_tplane);
     hipFree(s->d_tobs);
-    for (hipEvent_t ev : {s->ev0, s->ev1, s->ev_in, s->ev_t0, s->ev_t1})
+    for (hipEvent_t ev : {s->ev0, s->ev1, s->ev_in})
         if (ev)
             hipEventDestroy(ev);
+    for (auto& pr : s->ev_t)
+        for (hipEvent_t ev : pr)
+            if (ev)
+                hipEventDestroy(ev);
     for (int k = 0; k < kMaxStreams; ++k) {
         if (s->ev_done[k])
             hipEventDestroy(s->ev_done[k]);
@@ -730,14 +828,33 @@ extern "C" mcx_status mcx_server_create(mcx_ctx* ctx, const mcx_server_desc* d, 
     s->n_streams = d->n_streams > 0 ? d->n_streams : 4;
     s->quantum_ms = d->quantum_ms > 0 ? d->quantum_ms : 2.0;
     s->ticks_per_ms = (double)khz;
-    // tiled path from 2^17 Ising / 2^14 Blume-Capel sites on, where one
+    // tiled path from 2^15 Ising / 2^14 Blume-Capel sites on, where one
     // workgroup per lattice gets slower per sweep than the tiled launches
-    s->tile_sites[0] = d->tile_sites > 0 ? d->tile_sites : (d->tile_sites < 0 ? 0 : 1 << 17);
+    s->tile_sites[0] = d->tile_sites > 0 ? d->tile_sites : (d->tile_sites < 0 ? 0 : 1 << 15);
     s->tile_sites[1] = d->tile_sites > 0 ? d->tile_sites : (d->tile_sites < 0 ? 0 : 1 << 14);
     const size_t n = (size_t)s->max_slots;
     hipError_t e = hipSuccess;
+    // One hardware queue per stream.  HIP spreads ordinary streams over a few
+    // shared queues (GPU_MAX_HW_QUEUES), and kernels of two streams on one
+    // queue run one after the other, so two classes of a quantum would not
+    // overlap; a stream created with a CU mask always gets a queue of its own.
+    // The mask enables every CU; where the runtime refuses one, an ordinary
+    // non-blocking stream is used.
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess)
+        ncu = 0;
+    std::vector<uint32_t> cu_mask((size_t)(ncu + 31) / 32, 0u);
+    for (int i = 0; i < ncu; ++i)
+        cu_mask[(size_t)i / 32] |= 1u << (i % 32);
     for (int k = 0; k < s->n_streams && e == hipSuccess; ++k) {
-        e = hipStreamCreateWithFlags(&s->streams[k], hipStreamNonBlocking);
+        e = ncu > 0 ? hipExtStreamCreateWithCUMask(&s->streams[k], (uint32_t)cu_mask.size(),
+                                                   cu_mask.data())
+                    : hipErrorInvalidValue;
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            s->streams[k] = nullptr;
+            e = hipStreamCreateWithFlags(&s->streams[k], hipStreamNonBlocking);
+        }
         if (e == hipSuccess)
             e = hipEventCreateWithFlags(&s->ev_done[k], hipEventDisableTiming);
     }
@@ -765,10 +882,10 @@ extern "C" mcx_status mcx_server_create(mcx_ctx* ctx, const mcx_server_desc* d, 
         e = hipEventCreate(&s->ev1);
     if (e == hipSuccess)
         e = hipEventCreateWithFlags(&s->ev_in, hipEventDisableTiming);
-    if (e == hipSuccess)
-        e = hipEventCreate(&s->ev_t0);
-    if (e == hipSuccess)
-        e = hipEventCreate(&s->ev_t1);
+    for (auto& pr : s->ev_t)
+        for (hipEvent_t& ev : pr)
+            if (e == hipSuccess)
+                e = hipEventCreate(&ev);
 
     if (e == hipSuccess)
         e = hipMemsetAsync(s->d_last, 0, sizeof(int32_t) * 3 * n, s->streams[0]);

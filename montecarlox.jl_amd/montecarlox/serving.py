@@ -80,7 +80,7 @@ class Server:
     max_jobs: resident slots; quantum_ms: target device time of one quantum
     (0 = 2 ms); n_streams: streams the kernel classes run on (0 = 4);
     tile_sites: lattices at least this big run spread over many workgroups
-    (0 = 2^17 Ising / 2^14 Blume-Capel sites, negative = never)."""
+    (0 = 2^15 Ising / 2^14 Blume-Capel sites, negative = never)."""
 
     def __init__(self, hbm_budget: int, max_jobs: int = 1024, quantum_ms: float = 0.0,
                  n_streams: int = 0, tile_sites: int = 0, ctx: Optional[_lib.Context] = None):

@@ -90,6 +90,37 @@ def test_tiled_path_equals_lds_path_and_oracle(mcx, orc):
         assert np.allclose(a["moments"], mom, rtol=1e-12, atol=0), job
 
 
+def test_band_kernels_small_and_odd_lattices(mcx, orc):
+    """The multi-workgroup band kernels (T sweeps per launch, halo of 2T slabs)
+    on lattices smaller than their halo, bands that do not divide L, and 3D
+    z-plane bands, at temperatures where most sites keep flipping: equal to
+    the oracle."""
+    J = mcx.Job
+    jobs = [J("ising", 4, 0.3, 37, "metropolis", seed=3, key=0),
+            J("ising", 6, 0.25, 41, "heatbath", seed=3, key=1),
+            J("ising", 10, 0.35, 29, "glauber", seed=3, key=2),
+            J("ising", 34, 0.3, 23, "metropolis", seed=3, key=3),
+            J("ising", 70, 0.4, 19, "metropolis", seed=3, key=4),
+            J("blume_capel", 4, 0.3, 33, "metropolis", D=0.2, seed=3, key=5),
+            J("blume_capel", 6, 0.4, 27, "heatbath", D=0.5, seed=3, key=6),
+            J("blume_capel", 38, 0.35, 17, "glauber", D=1.0, seed=3, key=7),
+            J("ising", 4, 0.15, 21, "metropolis", dim=3, seed=3, key=8),
+            J("ising", 10, 0.2, 13, "heatbath", dim=3, seed=3, key=9),
+            J("blume_capel", 6, 0.3, 15, "metropolis", dim=3, D=0.4, seed=3, key=10),
+            J("blume_capel", 12, 0.25, 9, "heatbath", dim=3, D=0.8, seed=3, key=11),
+            # one class, its band shape set by L=200 (several 8-lane bands),
+            # applied to a lattice smaller than one band
+            J("blume_capel", 200, 0.35, 7, "metropolis", D=0.7, seed=3, key=12),
+            J("blume_capel", 18, 0.3, 11, "metropolis", D=0.3, seed=3, key=13)]
+    print("serving: band kernels on small / odd lattices", flush=True)
+    with mcx.Server(hbm_budget=1 << 24, max_jobs=14, quantum_ms=0.3, tile_sites=16) as srv:
+        out = srv.run(jobs, timeout_ms=120000)
+    for job, r in zip(jobs, out):
+        E, M, Q, acc, mom = _oracle(orc, job)
+        assert (r["E_pair"], r["M"], r["Q"], r["accepted"]) == (E, M, Q, acc), job
+        assert np.allclose(r["moments"], mom, rtol=1e-12, atol=0), job
+
+
 def test_tiled_path_beyond_lds(mcx, orc):
     """A 3D lattice too big for one CU's LDS runs on the tiled path."""
     job = mcx.Job("blume_capel", 96, 0.5, 3, "heatbath", dim=3, D=0.4, seed=8, key=1)
