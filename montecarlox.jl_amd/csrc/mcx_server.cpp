@@ -78,18 +78,12 @@ Band band_shape(int model, int dim, int L, size_t lds_max)
     const size_t budget = lds_max > 4096 ? lds_max - 4096 : 0;
     const int rw = (dim == 3 ? L : 1) * ((L / 2 + 31) / 32); // plane words per slab and colour
     // Ising: 32 rows (8 planes in 3D) and 8 (4) sweeps per launch, one lane per
-    // word; Blume-Capel, whose word update costs many times more: 2 sweeps
-    // per launch, 8 lanes per word, and a band sized so that the largest
-    // workgroup covers the band and its halo in about one pass.  Measured on
-    // the Poisson serving stream against other shapes and the per-colour path.
-    Band b{};
-    if (model == MCX_ISING) {
-        b = Band{dim == 3 ? 8 : 32, dim == 3 ? 4 : 8, 1, 0};
-    } else {
-        b.T = 2;
-        b.lpw = 8;
-        b.B = std::max(dim == 3 ? 8 : 16, kBandThreads / b.lpw / rw - 4 * b.T);
-    }
+    // word; Blume-Capel, whose word update costs many times more: 16 rows
+    // (8 planes) and 4 (2) sweeps per launch, 8 lanes per word.  Chosen on the
+    // Poisson serving stream against other band heights, depths and lane
+    // counts and against the per-colour path.
+    Band b = model == MCX_ISING ? Band{dim == 3 ? 8 : 32, dim == 3 ? 4 : 8, 1, 0}
+                                : Band{dim == 3 ? 8 : 16, dim == 3 ? 2 : 4, 8, 0};
     b.B = std::min(b.B, L);
     b.T = std::max(1, std::min(b.T, L / 4)); // a halo of 2T slabs: keep it within the lattice
     while (band_lds(model, dim, L, b.B, b.T) > budget && b.T > 1)
