@@ -394,10 +394,15 @@ typedef struct {
                         /* < 0 = never.  Other lattices run one workgroup each, from  */
                         /* LDS.  A job takes N bytes, plus N/4 (Ising) or N/2         */
                         /* (Blume-Capel) bytes of planes on the tiled path.           */
+    int32_t band_sweeps; /* sweeps per band launch on the tiled path: 0 = the built-in */
+                         /* shapes (Ising 8 in 2D / 4 in 3D, Blume-Capel 4 / 2), k > 0 */
+                         /* = k (fewer where L or the LDS is too small), < 0 = no bands: */
+                         /* one launch per colour and sweep, planes in HBM             */
+    int32_t pad_;
 } mcx_server_desc;
 
 typedef struct {
-    int32_t model, dim, L, alg; /* L even >= 4; alg MCX_METROPOLIS or MCX_HEATBATH      */
+    int32_t model, dim, L, alg; /* L even >= 4; alg MCX_METROPOLIS, _HEATBATH, _GLAUBER */
     double beta, D;             /* D: Blume-Capel crystal field (ignored for Ising)     */
     uint64_t seed;              /* Philox key word 0 = low 32 bits, counter word 3 = high */
     uint32_t key;               /* Philox key word 1 (the chain key)                    */

@@ -72,7 +72,7 @@ size_t band_lds(int model, int dim, int L, int B, int T)
     return per_slab * (size_t)(B + 4 * T);
 }
 
-Band band_shape(int model, int dim, int L, size_t lds_max)
+Band band_shape(int model, int dim, int L, size_t lds_max, int band_sweeps)
 {
     // static LDS of the band kernels (tables, reductions) and some slack
     const size_t budget = lds_max > 4096 ? lds_max - 4096 : 0;
@@ -84,6 +84,12 @@ Band band_shape(int model, int dim, int L, size_t lds_max)
     // counts and against the per-colour path.
     Band b = model == MCX_ISING ? Band{dim == 3 ? 8 : 32, dim == 3 ? 4 : 8, 1, 0}
                                 : Band{dim == 3 ? 8 : 16, dim == 3 ? 2 : 4, 8, 0};
+    if (band_sweeps < 0) {
+        b.T = 0; // asked for the per-colour kernels
+        return b;
+    }
+    if (band_sweeps > 0)
+        b.T = std::min(band_sweeps, kTileMaxSweeps);
     b.B = std::min(b.B, L);
     b.T = std::max(1, std::min(b.T, L / 4)); // a halo of 2T slabs: keep it within the lattice
     while (band_lds(model, dim, L, b.B, b.T) > budget && b.T > 1)
@@ -195,6 +201,7 @@ struct mcx_server {
     double quantum_ms = 0;
     double ticks_per_ms = 0; // device wall clock
     int64_t tile_sites[2] = {0, 0}; // per model: lattices this big take the tiled path
+    int band_sweeps = 0;            // mcx_server_desc::band_sweeps
     // latest measurement of one tiled sweep per tiled class (model, dim, alg)
     std::map<std::tuple<int, int, int>, double> tile_ms_per_sweep;
     // device arrays, one entry per slot
@@ -577,7 +584,7 @@ hipError_t run_quantum(mcx_server* s, std::vector<int64_t>& ran, std::vector<int
         a.item_obs = s->d_tobs + std::get<1>(ts);
         a.max_s = kTileMaxSweeps;
         a.accepted = s->d_acc;
-        const Band b = band_shape(model, dim, tspan_L[t], s->ctx->lds_max);
+        const Band b = band_shape(model, dim, tspan_L[t], s->ctx->lds_max, s->band_sweeps);
         if (b.T != 0) {
             a.band = b.B;
             a.lpw = b.lpw;
@@ -826,6 +833,7 @@ extern "C" mcx_status mcx_server_create(mcx_ctx* ctx, const mcx_server_desc* d, 
     // workgroup per lattice gets slower per sweep than the tiled launches
     s->tile_sites[0] = d->tile_sites > 0 ? d->tile_sites : (d->tile_sites < 0 ? 0 : 1 << 15);
     s->tile_sites[1] = d->tile_sites > 0 ? d->tile_sites : (d->tile_sites < 0 ? 0 : 1 << 14);
+    s->band_sweeps = d->band_sweeps;
     const size_t n = (size_t)s->max_slots;
     hipError_t e = hipSuccess;
     // One hardware queue per stream.  HIP spreads ordinary streams over a few

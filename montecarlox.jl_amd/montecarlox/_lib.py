@@ -63,7 +63,8 @@ class GenDesc(C.Structure):
 
 class ServerDesc(C.Structure):
     _fields_ = [("hbm_budget", C.c_int64), ("max_jobs", C.c_int32), ("n_streams", C.c_int32),
-                ("quantum_ms", C.c_double), ("tile_sites", C.c_int64)]
+                ("quantum_ms", C.c_double), ("tile_sites", C.c_int64),
+                ("band_sweeps", C.c_int32), ("pad_", C.c_int32)]
 
 
 class JobDesc(C.Structure):

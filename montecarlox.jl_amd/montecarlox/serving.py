@@ -80,14 +80,17 @@ class Server:
     max_jobs: resident slots; quantum_ms: target device time of one quantum
     (0 = 2 ms); n_streams: streams the kernel classes run on (0 = 4);
     tile_sites: lattices at least this big run spread over many workgroups
-    (0 = 2^15 Ising / 2^14 Blume-Capel sites, negative = never)."""
+    (0 = 2^15 Ising / 2^14 Blume-Capel sites, negative = never);
+    band_sweeps: sweeps per band launch of those lattices (0 = built-in shapes,
+    k > 0 = k, negative = per-colour launches without bands)."""
 
     def __init__(self, hbm_budget: int, max_jobs: int = 1024, quantum_ms: float = 0.0,
-                 n_streams: int = 0, tile_sites: int = 0, ctx: Optional[_lib.Context] = None):
+                 n_streams: int = 0, tile_sites: int = 0, band_sweeps: int = 0,
+                 ctx: Optional[_lib.Context] = None):
         self.ctx = ctx or _lib.default_context()
         d = ServerDesc(hbm_budget=int(hbm_budget), max_jobs=int(max_jobs),
                        n_streams=int(n_streams), quantum_ms=float(quantum_ms),
-                       tile_sites=int(tile_sites))
+                       tile_sites=int(tile_sites), band_sweeps=int(band_sweeps))
         self._h = _P()
         call("mcx_server_create", self.ctx.handle, C.byref(d), C.byref(self._h))
 

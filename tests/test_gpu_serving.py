@@ -73,13 +73,16 @@ def test_server_matches_oracle_alone_and_in_mix(mcx, orc):
             _same(alone, r)
 
 
-def test_tiled_path_equals_lds_path_and_oracle(mcx, orc):
-    """Every lattice on the multi-workgroup path (tile_sites=16) against every
-    lattice on the LDS-resident path (tile_sites=-1): bit-identical results,
-    and both equal to the oracle."""
+@pytest.mark.parametrize("band_sweeps", [0, 3, -1])
+def test_tiled_path_equals_lds_path_and_oracle(mcx, orc, band_sweeps):
+    """Every lattice on the multi-workgroup path (tile_sites=16: band kernels
+    with the built-in depths, bands of 3 sweeps, and the per-colour kernels
+    without bands) against every lattice on the LDS-resident path
+    (tile_sites=-1): bit-identical results, and both equal to the oracle."""
     jobs = _jobs(mcx)
-    print("serving: tiled vs LDS path", flush=True)
-    with mcx.Server(hbm_budget=1 << 22, max_jobs=6, quantum_ms=0.2, tile_sites=16) as srv:
+    print(f"serving: tiled (band_sweeps={band_sweeps}) vs LDS path", flush=True)
+    with mcx.Server(hbm_budget=1 << 22, max_jobs=6, quantum_ms=0.2, tile_sites=16,
+                    band_sweeps=band_sweeps) as srv:
         tiled = srv.run(jobs, timeout_ms=120000)
     with mcx.Server(hbm_budget=1 << 22, max_jobs=6, quantum_ms=0.2, tile_sites=-1) as srv:
         lds = srv.run(jobs, timeout_ms=120000)
