@@ -104,6 +104,8 @@ struct PlainArgs {
 };
 
 constexpr int kTileObsReplicas = 16;
+constexpr int kFoldThreads = 256;   // threads of the fold of one tiled item
+constexpr int kFoldMaxSweeps = 512; // sweeps one fold stages (TileArgs::max_s at most)
 
 // Multi-workgroup path for big lattices: the lattice lives in bit planes in
 // device memory (layout of the LDS planes) and one launch updates one colour

@@ -174,36 +174,43 @@ hipError_t launch_observables(ChainRec* chains, int32_t n_chains, int64_t, const
     return hipGetLastError();
 }
 
-// One thread per work item of a tile series: per-sweep sums -> E/M/Q, FP64
+// One workgroup per work item of a tile series: per-sweep sums -> E/M/Q, FP64
 // moments continued sweep by sweep (the same operations, in the same order, as
 // the batched sweep kernels), last-sweep observables, sweep counter.
-__global__ void k_tiles_fold(TileArgs a, int model, double* moments, int32_t* last_E,
-                             int32_t* last_M, int32_t* last_Q)
+__global__ void __launch_bounds__(kFoldThreads) k_tiles_fold(TileArgs a, int model,
+                                                               double* moments, int32_t* last_E,
+                                                               int32_t* last_M, int32_t* last_Q)
 {
-    const int item = blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= a.n_items)
-        return;
+    // one workgroup per item: the threads sum (and clear) the replicas of
+    // their sweeps, then one thread continues the FP64 moments in sweep order,
+    // as the sweep kernels do, so that they do not depend on the path
+    __shared__ double e_s[kFoldMaxSweeps];
+    __shared__ int32_t m_s[kFoldMaxSweeps];
+    __shared__ int32_t last[3];
+    __shared__ unsigned long long fl[kFoldThreads / 64];
+    const int item = blockIdx.x;
+    const int tid = threadIdx.x;
     const int ch = a.items[item];
-    const int64_t nsw = a.item_nsw[item];
+    const int64_t nsw = a.item_nsw[item] < a.max_s ? a.item_nsw[item] : a.max_s;
     const ChainRec cr = a.chains[ch];
     const int64_t N = (int64_t)cr.L * cr.L * (cr.dim == 3 ? cr.L : 1);
-    double* mo = moments + (int64_t)ch * 6;
-    double mE = mo[1], mE2 = mo[2], mAM = mo[3], mM2 = mo[4], mM4 = mo[5];
-    int32_t E = 0, M = 0, Q = 0;
     int32_t* obs = reinterpret_cast<int32_t*>(a.base + a.item_obs[item]);
     unsigned long long flips = 0;
-    for (int64_t s = 0; s < nsw; ++s) {
+    for (int64_t s = tid; s < nsw; s += kFoldThreads) {
         int32_t f[4];
         for (int k = 0; k < 4; ++k) {
-            int32_t* o = obs + (s * 4 + k) * kTileObsReplicas;
+            int4* o = reinterpret_cast<int4*>(obs + (s * 4 + k) * kTileObsReplicas);
             int32_t v = 0;
-            for (int r = 0; r < kTileObsReplicas; ++r) {
-                v += o[r];
-                o[r] = 0;
+#pragma unroll
+            for (int r = 0; r < kTileObsReplicas / 4; ++r) {
+                const int4 q = o[r];
+                v += q.x + q.y + q.z + q.w;
+                o[r] = make_int4(0, 0, 0, 0);
             }
             f[k] = v;
         }
         flips += (unsigned long long)(uint32_t)f[3];
+        int32_t E, M, Q;
         double e;
         if (model == MCX_ISING) {
             E = (int32_t)(2 * (int64_t)(uint32_t)f[1] - (int64_t)cr.dim * N);
@@ -216,33 +223,58 @@ __global__ void k_tiles_fold(TileArgs a, int model, double* moments, int32_t* la
             Q = f[2];
             e = (double)E + cr.D * (double)Q;
         }
-        const double m = (double)M;
+        e_s[s] = e;
+        m_s[s] = M;
+        if (s == nsw - 1) {
+            last[0] = E;
+            last[1] = M;
+            last[2] = Q;
+        }
+    }
+    // flips: wave sums, then the waves' partials
+    unsigned long long w = flips;
+    for (int off = 32; off > 0; off >>= 1)
+        w += __shfl_xor(w, off, 64);
+    if ((tid & 63) == 0)
+        fl[tid >> 6] = w;
+    __syncthreads();
+    if (tid != 0 || nsw <= 0)
+        return;
+    double* mo = moments + (int64_t)ch * 6;
+    double mE = mo[1], mE2 = mo[2], mAM = mo[3], mM2 = mo[4], mM4 = mo[5];
+    for (int64_t s = 0; s < nsw; ++s) {
+        const double e = e_s[s];
+        const double m = (double)m_s[s];
         mE += e;
         mE2 += e * e;
         mAM += fabs(m);
         mM2 += m * m;
         mM4 += (m * m) * (m * m);
     }
-    if (nsw > 0) {
-        a.accepted[ch] += flips;
-        last_E[ch] = E;
-        last_M[ch] = M;
-        last_Q[ch] = Q;
-        mo[0] += (double)nsw;
-        mo[1] = mE;
-        mo[2] = mE2;
-        mo[3] = mAM;
-        mo[4] = mM2;
-        mo[5] = mM4;
-        a.chains[ch].sweep = cr.sweep + (uint64_t)nsw;
-    }
+    unsigned long long tot = 0;
+    for (int k = 0; k < kFoldThreads / 64; ++k)
+        tot += fl[k];
+    a.accepted[ch] += tot;
+    last_E[ch] = last[0];
+    last_M[ch] = last[1];
+    last_Q[ch] = last[2];
+    mo[0] += (double)nsw;
+    mo[1] = mE;
+    mo[2] = mE2;
+    mo[3] = mAM;
+    mo[4] = mM2;
+    mo[5] = mM4;
+    a.chains[ch].sweep = cr.sweep + (uint64_t)nsw;
 }
 
 hipError_t launch_tiles_fold(int model, const TileArgs& a, double* moments, int32_t* last_E,
                              int32_t* last_M, int32_t* last_Q, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_tiles_fold, dim3((a.n_items + 63) / 64), dim3(64), 0, st, a, model,
-                       moments, last_E, last_M, last_Q);
+    if (a.max_s > kFoldMaxSweeps)
+        return hipErrorInvalidValue; // the per-sweep staging holds kFoldMaxSweeps sweeps
+    if (a.n_items > 0)
+        hipLaunchKernelGGL(k_tiles_fold, dim3(a.n_items), dim3(kFoldThreads), 0, st, a, model,
+                           moments, last_E, last_M, last_Q);
     return hipGetLastError();
 }
 
