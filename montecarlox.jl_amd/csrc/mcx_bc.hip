@@ -281,6 +281,24 @@ __device__ __forceinline__ void bc_unpack_word(int8_t* __restrict__ sp,
     }
 }
 
+// Threshold table of the device kernels, indexed by a site's class
+// i = sidx | o << 4 | u << 5 | alt << 6 (sidx = sum of the neighbour spins +
+// 2*DIM, o / u the site's occupancy and up bits, alt the proposal bit), from
+// the host layout k = ((s_old + 1) * 2 + alt) * NS + sidx.  Classes that no
+// site can be in (sidx >= NS, or up without occupancy) get 0.
+constexpr int kBCClasses = 128; // class index: 4 neighbour-sum bits, o, u, alt
+
+template <int DIM>
+__device__ __forceinline__ uint32_t bc_class_thr(const uint32_t* thr, int i)
+{
+    constexpr int NS = 4 * DIM + 1;
+    const int sidx = i & 15, o = (i >> 4) & 1, u = (i >> 5) & 1, alt = (i >> 6) & 1;
+    if (sidx >= NS || (!o && u))
+        return 0u;
+    const int s_old = o ? (u ? 1 : -1) : 0;
+    return thr[((s_old + 1) * 2 + alt) * NS + sidx];
+}
+
 // One 32-site word of colour c: updates its occupancy / up bits in place from
 // the bit-sliced shifted neighbour sums v0..v3 (the per-site rule above, one
 // Philox block per 4 sites) and returns the number of changed sites this lane
@@ -288,9 +306,13 @@ __device__ __forceinline__ void bc_unpack_word(int8_t* __restrict__ sp,
 // be shared by lpw adjacent lanes (lpw = 1, 2, 4 or 8, lane part = lane % lpw):
 // each takes the Philox blocks part, part + lpw, ... and the lanes then OR
 // their bit changes together, so every lane of the group ends with the whole
-// word.  tabw: 80 Metropolis thresholds + 3 always-masks, tabd: 13 x (w1,
-// w1 + w2, z).  Shared by the LDS-resident batched kernel and the
-// multi-workgroup tile kernel.
+// word.  tabw: the kBCClasses Metropolis (or Glauber) thresholds by class
+// (bc_class_thr; 2^31 for an always-accepted class, above every w >> 1),
+// tabd: 13 x (w1, w1 + w2, z).  Metropolis collects the accept and proposal
+// bits of the sites and moves all accepted sites of the word at once: the
+// proposal (the lower other state if alt, else the upper) changes the
+// occupancy where ~o | (u ^ alt) and the up bit where ~alt | u.  Shared by the
+// LDS-resident batched kernel and the multi-workgroup tile and band kernels.
 template <int DIM, int ALG>
 __device__ __forceinline__ uint32_t bc_word(uint32_t& occ, uint32_t& up, uint32_t v0, uint32_t v1,
                                             uint32_t v2, uint32_t v3, int W,
@@ -299,49 +321,63 @@ __device__ __forceinline__ uint32_t bc_word(uint32_t& occ, uint32_t& up, uint32_
                                             uint32_t sw_lo, uint32_t w2, uint32_t c3, uint32_t k0,
                                             uint32_t k1, int part, int lpw)
 {
-    constexpr int NS = 4 * DIM + 1;
-    const uint32_t occ0 = occ, up0 = up;
-    uint32_t changed = 0;
-    for (int j0 = 4 * part; j0 < W; j0 += 4 * lpw) {
-        const uint4 R = philox4x32_10(8u * group + (uint32_t)(j0 >> 2), sw_lo, w2, c3, k0, k1);
-        const uint32_t Rb[4] = {R.x, R.y, R.z, R.w};
+    uint32_t xo = 0, xu = 0; // bits of the sites this lane changes
+    uint32_t n_changed;
+    if (ALG == 0) {
+        uint32_t okm = 0, altm = 0;
+        for (int j0 = 4 * part; j0 < W; j0 += 4 * lpw) {
+            const uint4 R = philox4x32_10(8u * group + (uint32_t)(j0 >> 2), sw_lo, w2, c3, k0, k1);
+            const uint32_t Rb[4] = {R.x, R.y, R.z, R.w};
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int j = j0 + b;
-            if (j >= W)
-                break;
-            const uint32_t rw = Rb[b];
-            const int o = (occ >> j) & 1, u = (up >> j) & 1;
-            const int s_old = o ? (u ? 1 : -1) : 0;
-            const int sidx = (int)(((v0 >> j) & 1u) | (((v1 >> j) & 1u) << 1)
-                                   | (((v2 >> j) & 1u) << 2) | (((v3 >> j) & 1u) << 3));
-            int s_new;
-            if (ALG == 0) {
-                const int alt = (int)(rw & 1u);
-                const int k = ((s_old + 1) * 2 + alt) * NS + sidx;
-                const bool ok = ((tabw[80 + (k >> 5)] >> (k & 31)) & 1u) || (rw >> 1) < tabw[k];
-                s_new = ok ? bc_propose(s_old, alt) : s_old;
-            } else {
-                const double rr = ((double)rw * 0x1.0p-32) * tabd[sidx * 3 + 2];
-                s_new = rr < tabd[sidx * 3] ? -1 : (rr < tabd[sidx * 3 + 1] ? 0 : 1);
-            }
-            if (s_new != s_old) {
-                changed |= 1u << j;
-                occ = (occ & ~(1u << j)) | ((uint32_t)(s_new != 0) << j);
-                up = (up & ~(1u << j)) | ((uint32_t)(s_new > 0) << j);
+            for (int b = 0; b < 4; ++b) {
+                const int j = j0 + b;
+                if (j >= W)
+                    break;
+                const uint32_t rw = Rb[b];
+                const uint32_t cls = ((v0 >> j) & 1u) | (((v1 >> j) & 1u) << 1) |
+                                     (((v2 >> j) & 1u) << 2) | (((v3 >> j) & 1u) << 3) |
+                                     (((occ >> j) & 1u) << 4) | (((up >> j) & 1u) << 5) |
+                                     ((rw & 1u) << 6);
+                okm |= (uint32_t)((rw >> 1) < tabw[cls]) << j;
+                altm |= (rw & 1u) << j;
             }
         }
+        xo = okm & (~occ | (up ^ altm));
+        xu = okm & (~altm | up);
+        n_changed = (uint32_t)__popc(okm);
+    } else {
+        for (int j0 = 4 * part; j0 < W; j0 += 4 * lpw) {
+            const uint4 R = philox4x32_10(8u * group + (uint32_t)(j0 >> 2), sw_lo, w2, c3, k0, k1);
+            const uint32_t Rb[4] = {R.x, R.y, R.z, R.w};
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int j = j0 + b;
+                if (j >= W)
+                    break;
+                const int sidx = (int)(((v0 >> j) & 1u) | (((v1 >> j) & 1u) << 1) |
+                                       (((v2 >> j) & 1u) << 2) | (((v3 >> j) & 1u) << 3));
+                // s_new = rr < w1 ? -1 : (rr < w1 + w2 ? 0 : +1), and rr < w1
+                // implies rr < w1 + w2
+                const double rr = ((double)Rb[b] * 0x1.0p-32) * tabd[sidx * 3 + 2];
+                const bool lo = rr < tabd[sidx * 3];
+                const bool mid = rr < tabd[sidx * 3 + 1];
+                const uint32_t nz = (lo || !mid) ? 1u : 0u; // s_new != 0
+                const uint32_t nu = mid ? 0u : 1u;          // s_new = +1
+                xo |= (nz ^ ((occ >> j) & 1u)) << j;
+                xu |= (nu ^ ((up >> j) & 1u)) << j;
+            }
+        }
+        n_changed = (uint32_t)__popc(xo | xu);
     }
     if (lpw > 1) {
-        uint32_t d_occ = occ ^ occ0, d_up = up ^ up0;
         for (int off = 1; off < lpw; off <<= 1) {
-            d_occ |= __shfl_xor(d_occ, off, 64);
-            d_up |= __shfl_xor(d_up, off, 64);
+            xo |= __shfl_xor(xo, off, 64);
+            xu |= __shfl_xor(xu, off, 64);
         }
-        occ = occ0 ^ d_occ;
-        up = up0 ^ d_up;
     }
-    return (uint32_t)__popc(changed);
+    occ ^= xo;
+    up ^= xu;
+    return n_changed;
 }
 
 template <int DIM, int ALG>
@@ -349,10 +385,10 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
 {
     extern __shared__ uint32_t lds[];
     int32_t* part = reinterpret_cast<int32_t*>(lds); // [2][16][3]
-    uint32_t* tabw = lds + 96;                       // 80 thresholds + 3 masks
-    double* tabd = reinterpret_cast<double*>(lds + 184); // 39 doubles (8-byte aligned)
-    int32_t* ring = reinterpret_cast<int32_t*>(lds + 264); // E[64], M[64], Q[64] of the last sweeps
-    uint32_t* P = lds + 456;                          // [occ c0, occ c1, up c0, up c1]
+    uint32_t* tabw = lds + 96;                       // kBCClasses thresholds by class
+    double* tabd = reinterpret_cast<double*>(lds + 224); // 39 doubles (8-byte aligned)
+    int32_t* ring = reinterpret_cast<int32_t*>(lds + 304); // E[64], M[64], Q[64] of the last sweeps
+    uint32_t* P = lds + 496;                          // [occ c0, occ c1, up c0, up c1]
     const int tid = threadIdx.x;
     const int B = blockDim.x;
     const int lane = tid & 63;
@@ -385,9 +421,8 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
         const int lpart = tid & (lpw - 1);
         int8_t* __restrict__ sp = a.spins + cr.spin_off;
         const BCTable* tb = a.btab + cr.table;
-        for (int k = tid; k < 83; k += B)
-            tabw[k] = k < 80 ? (a.glauber ? tb->gl_thr[k] : tb->metro_thr[k])
-                             : (a.glauber ? tb->gl_always[k - 80] : tb->metro_always[k - 80]);
+        for (int k = tid; k < kBCClasses; k += B)
+            tabw[k] = bc_class_thr<DIM>(a.glauber ? tb->gl_thr : tb->metro_thr, k);
         for (int k = tid; k < 39; k += B)
             tabd[k] = tb->hb[k / 3][k % 3];
 
@@ -615,7 +650,7 @@ __global__ void __launch_bounds__(256) k_bc_plain(PlainArgs a, int n_col)
 template <int DIM, int ALG>
 __global__ void __launch_bounds__(256) k_bc_tiles(TileArgs a)
 {
-    __shared__ uint32_t tabw[84];
+    __shared__ uint32_t tabw[kBCClasses];
     __shared__ double tabd[39];
     const int item = blockIdx.y;
     if (a.s >= a.item_nsw[item])
@@ -625,9 +660,8 @@ __global__ void __launch_bounds__(256) k_bc_tiles(TileArgs a)
     const int ch = a.items[item];
     const ChainRec cr = a.chains[ch];
     const BCTable* tb = a.btab + cr.table;
-    for (int k = tid; k < 83; k += blockDim.x)
-        tabw[k] = k < 80 ? (a.glauber ? tb->gl_thr[k] : tb->metro_thr[k])
-                         : (a.glauber ? tb->gl_always[k - 80] : tb->metro_always[k - 80]);
+    for (int k = tid; k < kBCClasses; k += blockDim.x)
+        tabw[k] = bc_class_thr<DIM>(a.glauber ? tb->gl_thr : tb->metro_thr, k);
     for (int k = tid; k < 39; k += blockDim.x)
         tabd[k] = tb->hb[k / 3][k % 3];
     __syncthreads();
@@ -715,7 +749,7 @@ template <int DIM, int ALG>
 __global__ void __launch_bounds__(kBandThreads) k_bc_band(TileArgs a)
 {
     extern __shared__ uint32_t bq[];
-    __shared__ uint32_t tabw[84];
+    __shared__ uint32_t tabw[kBCClasses];
     __shared__ double tabd[39];
     const int item = blockIdx.y;
     const int64_t nsw = a.item_nsw[item];
@@ -732,9 +766,8 @@ __global__ void __launch_bounds__(kBandThreads) k_bc_band(TileArgs a)
     if (s0 >= L)
         return;
     const BCTable* tb = a.btab + cr.table;
-    for (int k = tid; k < 83; k += nthr)
-        tabw[k] = k < 80 ? (a.glauber ? tb->gl_thr[k] : tb->metro_thr[k])
-                         : (a.glauber ? tb->gl_always[k - 80] : tb->metro_always[k - 80]);
+    for (int k = tid; k < kBCClasses; k += nthr)
+        tabw[k] = bc_class_thr<DIM>(a.glauber ? tb->gl_thr : tb->metro_thr, k);
     for (int k = tid; k < 39; k += nthr)
         tabd[k] = tb->hb[k / 3][k % 3];
     const int half = L >> 1;
