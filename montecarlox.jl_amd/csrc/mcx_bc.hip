@@ -334,11 +334,16 @@ __device__ __forceinline__ uint32_t bc_word(uint32_t& occ, uint32_t& up, uint32_
                 if (j >= W)
                     break;
                 const uint32_t rw = Rb[b];
-                const uint32_t cls = ((v0 >> j) & 1u) | (((v1 >> j) & 1u) << 1) |
-                                     (((v2 >> j) & 1u) << 2) | (((v3 >> j) & 1u) << 3) |
-                                     (((occ >> j) & 1u) << 4) | (((up >> j) & 1u) << 5) |
-                                     ((rw & 1u) << 6);
-                okm |= (uint32_t)((rw >> 1) < tabw[cls]) << j;
+                const uint32_t uj = (uint32_t)j;
+                uint32_t cls = (rw & 1u) << 6;
+                cls |= __builtin_amdgcn_ubfe(up, uj, 1u) << 5;
+                cls |= __builtin_amdgcn_ubfe(occ, uj, 1u) << 4;
+                cls |= __builtin_amdgcn_ubfe(v3, uj, 1u) << 3;
+                cls |= __builtin_amdgcn_ubfe(v2, uj, 1u) << 2;
+                cls |= __builtin_amdgcn_ubfe(v1, uj, 1u) << 1;
+                cls |= __builtin_amdgcn_ubfe(v0, uj, 1u);
+                // (w >> 1) - thr < 0 exactly when w >> 1 < thr (both below 2^31 + 1)
+                okm |= (((rw >> 1) - tabw[cls]) >> 31) << j;
                 altm |= (rw & 1u) << j;
             }
         }
