@@ -352,11 +352,13 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                         // outcomes of attempt k are loaded while k decides.  An
                         // attempt out of the binned domain, or one whose lr lies
                         // in the 1e-9 band around log(u), stops the fast loop; it
-                        // is settled exactly (with exp) and the loop resumes.
+                        // is settled exactly (with exp) and the loop resumes.  The
+                        // loop records each attempt's bin and outcome; the counters
+                        // and the running sums are added up from those records by
+                        // all lanes after the chunk.
                         st_dE[lane] = t_dE;
-                        st_dp[lane] = t_dpair;
-                        st_mq[lane] = t_mq;
-                        st_luh[lane] = t_luh;
+                        // lr > 0 || lr > log(u) + 1e-9  <=>  lr > min(0, log(u) + 1e-9)
+                        st_luh[lane] = t_luh < 0.0 ? t_luh : 0.0;
                         st_lul[lane] = t_lul;
                         st_u[lane] = t_u;
                         st_site[lane] = t_site;
@@ -389,13 +391,12 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                             while (kk < cnt) {
                                 int stop = cnt;
                                 bool alive = true;
-                                int dE_c = st_dE[kk], dp_c = st_dp[kk], mq_c = st_mq[kk];
+                                int dE_c = st_dE[kk];
                                 double luh_c = st_luh[kk], lul_c = st_lul[kk];
                                 double lw_pre = lw[binc(E32 + dE_c)];
-                                int pair_d = 0, mq_d = 0, n_go = 0, n_ok = 0;
                                 for (int k = kk; k < cnt; ++k) {
                                     const int kn = k + 1 < cnt ? k + 1 : k;
-                                    const int dE_n = st_dE[kn], dp_n = st_dp[kn], mq_n = st_mq[kn];
+                                    const int dE_n = st_dE[kn];
                                     const double luh_n = st_luh[kn], lul_n = st_lul[kn];
                                     const int En = E32 + dE_c;
                                     const int bb = bin32(En);
@@ -405,18 +406,14 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                                     const double lw_r = lw[binc(E32 + dE_n)];
                                     const double lw_new = b_new == b_old ? lw_cur : lw_pre;
                                     const double lr = lw_new - lw_cur;
-                                    const bool acc_c = lr > 0 || lr > luh_c;
+                                    const bool acc_c = lr > luh_c;
                                     const bool go = alive && okn && (acc_c || lr < lul_c);
                                     const bool ok = go && acc_c;
                                     stop = alive && !go ? k : stop;
                                     alive = go;
-                                    n_go += go ? 1 : 0;
-                                    n_ok += ok ? 1 : 0;
                                     E32 = ok ? En : E32;
                                     b_old = ok ? b_new : b_old;
                                     lw_cur = ok ? lw_new : lw_cur;
-                                    pair_d += ok ? dp_c : 0;
-                                    mq_d += ok ? mq_c : (8 | (8 << 16));
                                     st_res[k] = b_old | (ok ? (int)0x80000000 : 0);
                                     if (WL) {
                                         // unchanged value when the attempt did not run
@@ -425,17 +422,9 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                                     }
                                     lw_pre = ok ? lw_a : lw_r;
                                     dE_c = dE_n;
-                                    dp_c = dp_n;
-                                    mq_c = mq_n;
                                     luh_c = luh_n;
                                     lul_c = lul_n;
                                 }
-                                const int iters = cnt - kk;
-                                steps += (unsigned)n_go;
-                                nacc += (unsigned)n_ok;
-                                pair += pair_d;
-                                mm += (mq_d & 0xffff) - 8 * iters;
-                                qq += (mq_d >> 16) - 8 * iters;
                                 b_cur = b_old;
                                 if (stop == cnt)
                                     break;
@@ -450,16 +439,10 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                                 const double lw_new = bb == b_old ? lw_cur : lw[bb - wbase];
                                 const double lr = lw_new - lw_cur;
                                 const bool ok = lr > 0 || st_u[stop] < exp(lr);
-                                ++steps;
-                                nacc += ok ? 1u : 0u;
                                 if (ok) {
-                                    const int mq = st_mq[stop];
                                     E32 = En;
                                     b_old = bb;
                                     lw_cur = lw_new;
-                                    pair += st_dp[stop];
-                                    mm += (mq & 0xffff) - 8;
-                                    qq += (mq >> 16) - 8;
                                 }
                                 st_res[stop] = b_old | (ok ? (int)0x80000000 : 0);
                                 if (WL) {
@@ -481,6 +464,15 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                             my_vis = r & 0x7fffffff;
                             my_ok = (r >> 31) & 1;
                         }
+                        // counters and running sums of the attempts the walk ran
+                        // (every attempt before n_done ran; accepted ones move)
+                        const int a_ok = my_ok;
+                        const int s_ok = wave_sum(a_ok);
+                        pair += wave_sum(a_ok ? t_dpair : 0);
+                        mm += wave_sum(a_ok ? t_dm : 0);
+                        qq += wave_sum(a_ok ? t_dq : 0);
+                        steps += (unsigned)n_done;
+                        nacc += (unsigned)s_ok;
                     } else
                     for (int k = 0; k < cnt; ++k) {
                         int b_new;
