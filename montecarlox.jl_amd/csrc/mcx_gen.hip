@@ -58,8 +58,10 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
     extern __shared__ __align__(16) unsigned char lds_raw[];
     double* lw = reinterpret_cast<double*>(lds_raw);
     // one chunk's staged attempts and the lane-0 walk's results
-    __shared__ int st_dE[64], st_dp[64], st_mq[64], st_site[64], st_res[64];
-    __shared__ double st_luh[64], st_lul[64], st_u[64];
+    // (entry 64 of st_dE / st_luh / st_lul: the integer walk's read-ahead past
+    // the last attempt; its value is never used)
+    __shared__ int st_dE[65], st_dp[64], st_mq[64], st_site[64], st_res[64];
+    __shared__ double st_luh[65], st_lul[65], st_u[64];
     // the weights in LDS are the window [wbase, wbase + wsz) of the walker's
     // table (the whole table when it fits: wsz = nbins, wbase = 0)
     const int wsz = a.win;
@@ -68,6 +70,11 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
     const int lane = threadIdx.x;
     const int64_t nb = a.nbins;
     constexpr bool wl = WL;
+    if (lane == 0) {
+        st_dE[64] = 0;
+        st_luh[64] = 0.0;
+        st_lul[64] = 0.0;
+    }
 
     for (int item = blockIdx.x; item < a.n_items; item += gridDim.x) {
         const int wk = a.items[item];
@@ -395,7 +402,7 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                                 double luh_c = st_luh[kk], lul_c = st_lul[kk];
                                 double lw_pre = lw[binc(E32 + dE_c)];
                                 for (int k = kk; k < cnt; ++k) {
-                                    const int kn = k + 1 < cnt ? k + 1 : k;
+                                    const int kn = k + 1;
                                     const int dE_n = st_dE[kn];
                                     const double luh_n = st_luh[kn], lul_n = st_lul[kn];
                                     const int En = E32 + dE_c;
@@ -404,7 +411,10 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                                     const int b_new = okn ? bb : 0;
                                     const double lw_a = lw[binc(En + dE_n)];
                                     const double lw_r = lw[binc(E32 + dE_n)];
-                                    const double lw_new = b_new == b_old ? lw_cur : lw_pre;
+                                    // a multicanonical walk never writes the weights, so
+                                    // the prefetched weight is exact even for the current
+                                    // bin; a Wang-Landau walk has just lowered that one
+                                    const double lw_new = !WL ? lw_pre : (b_new == b_old ? lw_cur : lw_pre);
                                     const double lr = lw_new - lw_cur;
                                     const bool acc_c = lr > luh_c;
                                     const bool go = alive && okn && (acc_c || lr < lul_c);
