@@ -380,7 +380,7 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                             };
                             auto binc = [&](int E) -> int { // window index, clamped into it
                                 const int b = bin32(E) - wbase;
-                                return (unsigned)b < (unsigned)wsz ? b : 0;
+                                return max(0, min(b, wsz - 1));
                             };
                             int E32 = (int)Ei;
                             if (b_cur != b_old && old_ok) {
@@ -401,6 +401,7 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
                                 int dE_c = st_dE[kk];
                                 double luh_c = st_luh[kk], lul_c = st_lul[kk];
                                 double lw_pre = lw[binc(E32 + dE_c)];
+#pragma unroll 2
                                 for (int k = kk; k < cnt; ++k) {
                                     const int kn = k + 1;
                                     const int dE_n = st_dE[kn];
