@@ -136,7 +136,9 @@ __device__ __forceinline__ void tile_block_add_s(const TileArgs& a, int item, in
             atomicAdd(&o[(s * 4 + k) * kTileObsReplicas + (int)(blockIdx.x % kTileObsReplicas)], v);
         }
     }
-    __syncthreads();
+    // no barrier here: the callers' next writes of red_s come after the two
+    // half-sweep barriers of the next sweep (or never), so the atomics' latency
+    // overlaps the next half-sweep instead of stalling the workgroup
 }
 
 __device__ __forceinline__ void tile_block_add(const TileArgs& a, int item, int32_t f0, int32_t f1,
