@@ -29,6 +29,7 @@ def _jobs(mcx):
         J("ising", 24, 0.4406868, 90, "glauber", seed=5, key=12),
         J("blume_capel", 18, 0.8, 70, "glauber", D=0.8, seed=5, key=13),
         J("ising", 6, 0.25, 120, "glauber", dim=3, seed=9, key=14),
+        J("blume_capel", 8, 0.6, 40, "glauber", dim=3, D=0.6, seed=9, key=15),
     ]
 
 
