@@ -288,6 +288,23 @@ __device__ __forceinline__ void bc_unpack_word(int8_t* __restrict__ sp,
 // site can be in (sidx >= NS, or up without occupancy) get 0.
 constexpr int kBCClasses = 128; // class index: 4 neighbour-sum bits, o, u, alt
 
+// Bit 4p + b of x moves to bit 8b + p (p < 8, b < 4): a 8 x 4 bit-matrix
+// transpose, the bit index rotated right by 2, as four swaps of index bits
+// (0 1), (0 2), (1 3), (2 4).
+__device__ __forceinline__ uint32_t bits_8x4_transpose(uint32_t x)
+{
+    uint32_t t;
+    t = ((x >> 1) ^ x) & 0x22222222u;
+    x ^= t ^ (t << 1);
+    t = ((x >> 3) ^ x) & 0x0a0a0a0au;
+    x ^= t ^ (t << 3);
+    t = ((x >> 6) ^ x) & 0x00cc00ccu;
+    x ^= t ^ (t << 6);
+    t = ((x >> 12) ^ x) & 0x0000f0f0u;
+    x ^= t ^ (t << 12);
+    return x;
+}
+
 template <int DIM>
 __device__ __forceinline__ uint32_t bc_class_thr(const uint32_t* thr, int i)
 {
@@ -328,6 +345,31 @@ __device__ __forceinline__ uint32_t bc_word(uint32_t& occ, uint32_t& up, uint32_
         for (int j0 = 4 * part; j0 < W; j0 += 4 * lpw) {
             const uint4 R = philox4x32_10(8u * group + (uint32_t)(j0 >> 2), sw_lo, w2, c3, k0, k1);
             const uint32_t Rb[4] = {R.x, R.y, R.z, R.w};
+            if (j0 + 4 <= W) {
+                // the block's 4 sites at once: their bits of the 6 planes, plane p
+                // in bits 4p..4p+3, transposed so that site b's class bits are
+                // bits 8b..8b+5
+                const uint32_t u0 = (uint32_t)j0;
+                uint32_t x = __builtin_amdgcn_ubfe(v0, u0, 4u);
+                x |= __builtin_amdgcn_ubfe(v1, u0, 4u) << 4;
+                x |= __builtin_amdgcn_ubfe(v2, u0, 4u) << 8;
+                x |= __builtin_amdgcn_ubfe(v3, u0, 4u) << 12;
+                x |= __builtin_amdgcn_ubfe(occ, u0, 4u) << 16;
+                x |= __builtin_amdgcn_ubfe(up, u0, 4u) << 20;
+                x = bits_8x4_transpose(x);
+                uint32_t ok4 = 0, alt4 = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const uint32_t rw = Rb[b];
+                    const uint32_t cls = __builtin_amdgcn_ubfe(x, 8u * b, 6u) | ((rw & 1u) << 6);
+                    // (w >> 1) - thr < 0 exactly when w >> 1 < thr (both at most 2^31)
+                    ok4 |= (((rw >> 1) - tabw[cls]) >> 31) << b;
+                    alt4 |= (rw & 1u) << b;
+                }
+                okm |= ok4 << u0;
+                altm |= alt4 << u0;
+                continue;
+            }
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const int j = j0 + b;
