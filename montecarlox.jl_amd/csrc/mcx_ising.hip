@@ -103,8 +103,14 @@ __device__ __forceinline__ uint32_t bitserial_less(uint32_t eq, const uint32_t (
             const uint32_t sh = 31u - (4u * chunk + (uint32_t)b);
             uint32_t Tb = 0;
 #pragma unroll
-            for (int k = 0; k < NC; ++k)
-                Tb |= m[k] & (0u - ((thr[k] >> sh) & 1u));
+            for (int k = 0; k < NC; ++k) {
+                // all-ones where the class's threshold has bit sh (wave-uniform);
+                // kept opaque so that the planes combine with and / and-or ops
+                // rather than one select per class
+                uint32_t tm = 0u - ((thr[k] >> sh) & 1u);
+                asm volatile("" : "+s"(tm));
+                Tb |= m[k] & tm;
+            }
             const uint32_t x = Rb[b] ^ Tb;
             lt |= eq & Tb & x;
             eq &= ~x;
