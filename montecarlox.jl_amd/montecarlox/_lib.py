@@ -55,12 +55,6 @@ class BatchDesc(C.Structure):
                 ("D", C.POINTER(C.c_double)), ("series_capacity", C.c_int64)]
 
 
-class GenDesc(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("n_groups", C.c_int32),
-                ("group_of_chain", C.POINTER(C.c_int32)), ("bin_start", C.POINTER(C.c_double)),
-                ("bin_step", C.POINTER(C.c_double)), ("n_bins", C.POINTER(C.c_int64))]
-
-
 class ServerDesc(C.Structure):
     _fields_ = [("hbm_budget", C.c_int64), ("max_jobs", C.c_int32), ("n_streams", C.c_int32),
                 ("quantum_ms", C.c_double), ("tile_sites", C.c_int64),

@@ -1,0 +1,52 @@
+"""The ctypes mirrors of the C ABI's descriptor and result structs have the
+layout that include/mcx_hip.h declares: the same size and the same offset for
+every field, read from a C program compiled against the header with gcc.  The
+program is generated from the ctypes field names, so a field that the header
+lacks or names differently fails to compile."""
+import ctypes as C
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _mirrors():
+    from montecarlox import _lib, muca, pt
+    return [
+        ("mcx_batch_desc", _lib.BatchDesc),
+        ("mcx_pt_desc", pt.PTDesc),
+        ("mcx_gen_desc", muca.GenDesc),
+        ("mcx_server_desc", _lib.ServerDesc),
+        ("mcx_job_desc", _lib.JobDesc),
+        ("mcx_job_result", _lib.JobResult),
+        ("mcx_server_stats_t", _lib.ServerStats),
+    ]
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+def test_ctypes_structs_match_the_header(tmp_path):
+    pairs = _mirrors()
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "mcx_hip.h"',
+             "int main(void) {"]
+    for cname, cls in pairs:
+        lines.append(f'    printf("%zu\\n", sizeof({cname}));')
+        for field, _ in cls._fields_:
+            lines.append(f'    printf("%zu\\n", offsetof({cname}, {field}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), "-o", str(exe),
+                    str(src)], check=True, capture_output=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    k = 0
+    for cname, cls in pairs:
+        assert got[k] == C.sizeof(cls), (cname, cls.__qualname__, got[k], C.sizeof(cls))
+        k += 1
+        for field, _ in cls._fields_:
+            assert got[k] == getattr(cls, field).offset, (cname, field, got[k])
+            k += 1
