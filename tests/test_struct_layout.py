@@ -50,3 +50,40 @@ def test_ctypes_structs_match_the_header(tmp_path):
         for field, _ in cls._fields_:
             assert got[k] == getattr(cls, field).offset, (cname, field, got[k])
             k += 1
+
+
+# Julia field type -> bytes
+_JL_SIZE = {"Int32": 4, "UInt32": 4, "Int64": 8, "UInt64": 8, "Float64": 8}
+
+
+def _julia_structs(text):
+    """struct Name ... end blocks of INTEGRATION.md: name -> [(field, bytes)]."""
+    import re
+    out = {}
+    for m in re.finditer(r"^struct (\w+)(.*?)\bend\b", text, re.M | re.S):
+        fields = []
+        for f, t in re.findall(r"(\w+)::([\w{},]+)", m.group(2)):
+            if t.startswith("Ptr"):
+                n = 8
+            elif t.startswith("NTuple"):
+                k, e = re.match(r"NTuple\{(\d+),(\w+)\}", t).groups()
+                n = int(k) * _JL_SIZE[e]
+            else:
+                n = _JL_SIZE[t]
+            fields.append((f, n))
+        out[m.group(1)] = fields
+    return out
+
+
+def test_julia_bindings_match_the_ctypes_mirrors():
+    """The Julia structs INTEGRATION.md gives a MonteCarloX.jl user have the
+    fields, in order and of the sizes, of the mirrors checked above."""
+    from montecarlox import _lib, pt
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    jl = _julia_structs(text)
+    for name, cls in [("BatchDesc", _lib.BatchDesc), ("PTDesc", pt.PTDesc),
+                      ("ServerDesc", _lib.ServerDesc), ("JobDesc", _lib.JobDesc),
+                      ("JobResult", _lib.JobResult)]:
+        assert name in jl, name
+        want = [(f, C.sizeof(t)) for f, t in cls._fields_]
+        assert jl[name] == want, (name, jl[name], want)
