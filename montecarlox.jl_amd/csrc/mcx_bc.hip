@@ -346,25 +346,26 @@ __device__ __forceinline__ uint32_t bc_word(uint32_t& occ, uint32_t& up, uint32_
             const uint4 R = philox4x32_10(8u * group + (uint32_t)(j0 >> 2), sw_lo, w2, c3, k0, k1);
             const uint32_t Rb[4] = {R.x, R.y, R.z, R.w};
             if (j0 + 4 <= W) {
-                // the block's 4 sites at once: their bits of the 6 planes, plane p
-                // in bits 4p..4p+3, transposed so that site b's class bits are
-                // bits 8b..8b+5
+                // the block's 4 sites at once: their bits of the 6 planes and the
+                // proposal bits, plane p in bits 4p..4p+3, transposed so that
+                // site b's 7-bit class is bits 8b..8b+6
                 const uint32_t u0 = (uint32_t)j0;
+                const uint32_t alt4 = (Rb[0] & 1u) | ((Rb[1] & 1u) << 1) | ((Rb[2] & 1u) << 2) |
+                                      ((Rb[3] & 1u) << 3);
                 uint32_t x = __builtin_amdgcn_ubfe(v0, u0, 4u);
                 x |= __builtin_amdgcn_ubfe(v1, u0, 4u) << 4;
                 x |= __builtin_amdgcn_ubfe(v2, u0, 4u) << 8;
                 x |= __builtin_amdgcn_ubfe(v3, u0, 4u) << 12;
                 x |= __builtin_amdgcn_ubfe(occ, u0, 4u) << 16;
                 x |= __builtin_amdgcn_ubfe(up, u0, 4u) << 20;
+                x |= alt4 << 24;
                 x = bits_8x4_transpose(x);
-                uint32_t ok4 = 0, alt4 = 0;
+                uint32_t ok4 = 0;
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
-                    const uint32_t rw = Rb[b];
-                    const uint32_t cls = __builtin_amdgcn_ubfe(x, 8u * b, 6u) | ((rw & 1u) << 6);
+                    const uint32_t cls = __builtin_amdgcn_ubfe(x, 8u * b, 7u);
                     // (w >> 1) - thr < 0 exactly when w >> 1 < thr (both at most 2^31)
-                    ok4 |= (((rw >> 1) - tabw[cls]) >> 31) << b;
-                    alt4 |= (rw & 1u) << b;
+                    ok4 |= (((Rb[b] >> 1) - tabw[cls]) >> 31) << b;
                 }
                 okm |= ok4 << u0;
                 altm |= alt4 << u0;
