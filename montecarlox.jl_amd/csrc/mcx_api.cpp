@@ -240,6 +240,37 @@ void batch_free(mcx_batch* b)
     hipFree(b->d_items);
     hipFree(b->d_obs);
     hipFree(b->d_last);
+    hipFree(b->d_planes);
+    hipFree(b->d_plane_off);
+}
+
+// Bit-plane storage of every chain (2 planes per colour word for Ising, 4 for
+// Blume-Capel: the LDS layout of the batched kernels), allocated once.
+mcx_status ensure_planes(mcx_batch* b)
+{
+    if (b->d_planes)
+        return MCX_OK;
+    std::vector<int64_t> off(b->n);
+    int64_t total = 0;
+    for (int k = 0; k < b->n; ++k) {
+        const int64_t L = b->rec[k].L;
+        const int64_t nw = (L / 2 + 31) / 32;
+        const int64_t rows = b->dim == 3 ? L * L : L;
+        off[k] = total;
+        total += (b->model == MCX_ISING ? 2 : 4) * rows * nw;
+    }
+    HIPCHK(hipMalloc((void**)&b->d_plane_off, sizeof(int64_t) * (size_t)b->n));
+    hipError_t e = hipMalloc((void**)&b->d_planes, sizeof(uint32_t) * (size_t)std::max<int64_t>(total, 1));
+    if (e != hipSuccess) {
+        hipFree(b->d_plane_off);
+        b->d_plane_off = nullptr;
+        b->d_planes = nullptr;
+        return fail(MCX_EHIP, "plane storage (%lld words): %s", (long long)total,
+                    hipGetErrorString(e));
+    }
+    HIPCHK(hipMemcpy(b->d_plane_off, off.data(), sizeof(int64_t) * (size_t)b->n,
+                     hipMemcpyHostToDevice));
+    return MCX_OK;
 }
 
 // The device copy of the chain records is authoritative (kernels advance the
@@ -603,7 +634,7 @@ extern "C" mcx_status mcx_batch_get_spins(mcx_batch* b, int32_t chain, int8_t* s
 namespace mcx {
 
 mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t flags,
-                            bool want_last)
+                            bool want_last, int planes)
 {
     const bool record = (flags & MCX_RUN_RECORD) != 0;
     hipStream_t st = b->ctx->stream;
@@ -659,6 +690,12 @@ mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t
         a.rec_pos = b->rec_len;
         a.by_table = b->by_table ? 1 : 0;
         a.cap = b->cap;
+        if (planes && b->d_planes) {
+            a.planes = b->d_planes;
+            a.plane_off = b->d_plane_off;
+            a.planes_in = planes & 1;
+            a.planes_out = (planes >> 1) & 1;
+        }
         if (want_last && nsweeps > 0) {
             a.last_E = b->d_last;
             a.last_M = b->d_last + b->n;

@@ -58,6 +58,10 @@ struct mcx_batch {
     int32_t* d_items = nullptr;
     int32_t* d_obs = nullptr;
     int32_t* d_last = nullptr; // [3 * n] E_pair, M, Q after the last sweep of a call
+    // bit planes of every chain in HBM, held between the rounds of one replica-
+    // exchange run instead of the int8 lattice (allocated on first use)
+    uint32_t* d_planes = nullptr;
+    int64_t* d_plane_off = nullptr; // [n] word offset of chain k's planes
     mcx_pt* pt = nullptr;      // replica-exchange object driving the tables, if any
     int n_gen = 0;             // multicanonical / Wang-Landau objects built on this batch
 };
@@ -159,8 +163,12 @@ mcx_status upload_chains(mcx_batch* b);
 mcx_status pull_chains(mcx_batch* b);
 mcx_status set_chain_tables(mcx_batch* b, const double* beta_per_table);
 mcx_status sync_check(mcx_batch* b, const char* what);
+// planes: bit 0 = read the lattices from b->d_planes instead of the int8
+// spins, bit 1 = leave them in b->d_planes instead of the int8 spins (the
+// batched kernels only; ensure_planes first)
 mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t flags,
-                            bool want_last = false);
+                            bool want_last = false, int planes = 0);
+mcx_status ensure_planes(mcx_batch* b);
 mcx_status check_run(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t flags, const char* who);
 
 } // namespace mcx

@@ -474,8 +474,15 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
         for (int k = tid; k < 39; k += B)
             tabd[k] = tb->hb[k / 3][k % 3];
 
-        for (int w = tid; w < words; w += B)
-            bc_pack_word<DIM>(sp, P, w, L, nw, half, words);
+        // int8 lattice (or the planes a previous launch left) -> planes
+        if (a.planes_in) {
+            const uint32_t* src = a.planes + a.plane_off[ch];
+            for (int w = tid; w < 4 * words; w += B)
+                P[w] = src[w];
+        } else {
+            for (int w = tid; w < words; w += B)
+                bc_pack_word<DIM>(sp, P, w, L, nw, half, words);
+        }
         __syncthreads();
 
         unsigned long long acc = 0;
@@ -587,8 +594,15 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
             a.last_Q[ch] = ring[128 + ((nsw - 1) & 63)];
         }
 
-        for (int w = tid; w < words; w += B)
-            bc_unpack_word<DIM>(sp, P, w, L, nw, half, words);
+        // planes -> int8 lattice (or left as planes for the next launch)
+        if (a.planes_out) {
+            uint32_t* dst = a.planes + a.plane_off[ch];
+            for (int w = tid; w < 4 * words; w += B)
+                dst[w] = P[w];
+        } else {
+            for (int w = tid; w < words; w += B)
+                bc_unpack_word<DIM>(sp, P, w, L, nw, half, words);
+        }
         acc = wave_sum(acc);
         if (lane == 0 && acc)
             atomicAdd(&a.accepted[ch], acc);

@@ -85,6 +85,10 @@ struct SweepArgs {
     int32_t by_table;            // series / moments of chain k go to slot chains[k].table
     int64_t cap;                 // series capacity per slot (slot j starts at j * cap)
     int32_t glauber;             // Blume-Capel: Glauber thresholds instead of Metropolis ones
+    uint32_t* planes;            // if set: bit-plane storage of the chains, at plane_off[chain]
+    const int64_t* plane_off;
+    int32_t planes_in;           // read the lattice from planes instead of packing the int8 spins
+    int32_t planes_out;          // leave it in planes instead of unpacking to the int8 spins
 };
 
 struct PlainArgs {

@@ -308,9 +308,15 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
         int8_t* __restrict__ sp = a.spins + cr.spin_off;
         const IsingTable tb = a.itab[cr.table];
 
-        // ---- int8 lattice -> bit planes (up = 1) ----
-        for (int w = tid; w < words; w += B)
-            ising_pack_word<DIM>(sp, P, w, L, nw, half, words);
+        // ---- int8 lattice (or the planes a previous launch left) -> bit planes (up = 1) ----
+        if (a.planes_in) {
+            const uint32_t* src = a.planes + a.plane_off[ch];
+            for (int w = tid; w < 2 * words; w += B)
+                P[w] = src[w];
+        } else {
+            for (int w = tid; w < words; w += B)
+                ising_pack_word<DIM>(sp, P, w, L, nw, half, words);
+        }
         __syncthreads();
 
         unsigned long long acc = 0;
@@ -410,9 +416,15 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
             a.last_Q[ch] = (int32_t)N;
         }
 
-        // ---- bit planes -> int8 lattice ----
-        for (int w = tid; w < words; w += B)
-            ising_unpack_word<DIM>(sp, P, w, L, nw, half, words);
+        // ---- bit planes -> int8 lattice (or left as planes for the next launch) ----
+        if (a.planes_out) {
+            uint32_t* dst = a.planes + a.plane_off[ch];
+            for (int w = tid; w < 2 * words; w += B)
+                dst[w] = P[w];
+        } else {
+            for (int w = tid; w < words; w += B)
+                ising_unpack_word<DIM>(sp, P, w, L, nw, half, words);
+        }
         acc = wave_sum(acc);
         if (lane == 0 && acc)
             atomicAdd(&a.accepted[ch], acc);

@@ -203,6 +203,18 @@ extern "C" mcx_status mcx_pt_run(mcx_pt* p, int32_t alg, int64_t n_rounds, int64
     const bool graph = (flags & MCX_RUN_GRAPH) != 0;
     const uint32_t rflags = flags & MCX_RUN_RECORD;
     PTArgs a = pt_args(p);
+    // between the rounds of this run the lattices stay bit planes in HBM: the
+    // first round packs the int8 spins, the last one unpacks them (the exchange
+    // reads the energies the sweeps leave, not the spins)
+    const bool keep_planes = n_rounds > 1;
+    if (keep_planes) {
+        s = ensure_planes(b);
+        if (s)
+            return s;
+    }
+    auto planes_of = [&](int64_t k) {
+        return keep_planes ? (k > 0 ? 1 : 0) | (k < n_rounds - 1 ? 2 : 0) : 0;
+    };
     HIPCHK(hipEventRecord(b->ctx->ev0, st));
     if (graph && n_rounds > 0) {
         // one graph holding every round: sweep launch(es) then the exchange
@@ -211,7 +223,7 @@ extern "C" mcx_status mcx_pt_run(mcx_pt* p, int32_t alg, int64_t n_rounds, int64
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         mcx_status rs = MCX_OK;
         for (int64_t k = 0; k < n_rounds && rs == MCX_OK; ++k) {
-            rs = run_sweeps_async(b, alg, sweeps_between, rflags, true);
+            rs = run_sweeps_async(b, alg, sweeps_between, rflags, true, planes_of(k));
             if (rs == MCX_OK && launch_pt_exchange(a, st) != hipSuccess)
                 rs = fail(MCX_EHIP, "mcx_pt_run: exchange launch failed during capture");
         }
@@ -232,7 +244,7 @@ extern "C" mcx_status mcx_pt_run(mcx_pt* p, int32_t alg, int64_t n_rounds, int64
             return fail(MCX_EHIP, "mcx_pt_run: graph execution failed: %s", hipGetErrorString(ie));
     } else {
         for (int64_t k = 0; k < n_rounds; ++k) {
-            s = run_sweeps_async(b, alg, sweeps_between, rflags, true);
+            s = run_sweeps_async(b, alg, sweeps_between, rflags, true, planes_of(k));
             if (s)
                 return s;
             HIPCHK(launch_pt_exchange(a, st));
