@@ -5,6 +5,14 @@
 
 namespace mcx {
 
+// threads of a walker's workgroup: wave 0 walks, wave 1 stages the next chunk
+constexpr int kGenThreads = 128;
+
+// static LDS of the walker kernel: two chunks' stages (ints dE[65], dp, mq,
+// site, snew[64]; doubles luh[65], lul[65], u[64]), the walk's results
+// (64 ints) and two error slots
+constexpr size_t kGenStaticLds = 2 * (4 * (65 + 4 * 64) + 8 * (65 + 65 + 64)) + 4 * 64 + 8;
+
 struct GenArgs {
     const int32_t* items;   // walkers of this launch (all from one binning group)
     int32_t n_items;

@@ -10,15 +10,21 @@
 // sequential chain: sites are attempted in checkerboard order (colour 0 in
 // increasing site index, then colour 1).
 //
-// One wavefront per walker.  Sites of one colour never neighbour each other,
-// so for 64 consecutive sites of the colour the lanes compute dE, the proposal,
-// the uniform and its log in parallel and keep them in registers; the wave then
-// walks the 64 attempts in order, reading each from its lane (v_readlane).  The
-// walker's FP64 log-weights live in LDS; the weight and bin of the current
-// energy are kept in registers, so an attempt costs one LDS load and no exp
-// unless lr falls within 1e-9 of log(u).  Spin writes and visit counts (order
-// does not matter for either) are applied by the lanes after the walk.
-// Histograms are uint32 in LDS when they fit, otherwise int64 atomics in HBM.
+// Two wavefronts per walker.  Sites of one colour never neighbour each other,
+// so for 64 consecutive sites of the colour the lanes of the staging wave
+// (wave 1) compute dE, the proposal, the uniform and its log in parallel and
+// store them in an LDS stage; the walking wave (wave 0) walks the 64 attempts
+// in order from the stage.  The stage is double-buffered: while wave 0 walks
+// chunk i, wave 1 stages chunk i + 1 of the same colour, which reads only
+// spins of the other colour and the chunk's own sites, none of which the walk
+// of chunk i changes.  The first chunk of a colour is staged once the previous
+// colour's spins are written.  One workgroup barrier per chunk hands the
+// buffers over.  The walker's FP64 log-weights live in LDS; the weight and
+// bin of the current energy are kept in registers, so an attempt costs one LDS
+// load and no exp unless lr falls within 1e-9 of log(u).  Spin writes and
+// visit counts (order does not matter for either) are applied by wave 0's
+// lanes after the walk.  Histograms are uint32 in LDS when they fit,
+// otherwise int64 atomics in HBM.
 //
 // KIND 2 (Blume-Capel): the CustomEnsemble of docs/src/examples/spin_systems/
 // muca_BlumeCapel.jl:43-89 -- a Boltzmann weight on the pair term and a
@@ -45,35 +51,57 @@ __device__ __forceinline__ double readlane_f64(double v, int k)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// Orders the LDS and global accesses of the calling wavefront's lanes before
+// it against those after it (what a workgroup barrier does, without waiting
+// for the other wavefront of the workgroup).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ __forceinline__ double uniform_f64(double v)
+{
+    return readlane_f64(v, 0);
+}
+
 __device__ __forceinline__ int gen_propose(int s_old, int alt)
 {
     return s_old == -1 ? (alt ? 0 : 1) : (s_old == 0 ? (alt ? -1 : 1) : (alt ? -1 : 0));
 }
 
 template <int MODEL, int DIM, int KIND>
-__global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
+__global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
 {
     constexpr bool WL = KIND == 1; // Wang-Landau
     constexpr bool PS = KIND == 2; // Boltzmann pair term + multicanonical sum s^2
     extern __shared__ __align__(16) unsigned char lds_raw[];
     double* lw = reinterpret_cast<double*>(lds_raw);
-    // one chunk's staged attempts and the lane-0 walk's results
-    // (entry 64 of st_dE / st_luh / st_lul: the integer walk's read-ahead past
-    // the last attempt; its value is never used)
-    __shared__ int st_dE[65], st_dp[64], st_mq[64], st_site[64], st_res[64];
-    __shared__ double st_luh[65], st_lul[65], st_u[64];
+    // two chunks' staged attempts (stage [i & 1] holds chunk i of the colour)
+    // and the lane-0 walk's results.  Entry 64 of st_dE / st_luh / st_lul is the
+    // integer walk's read-ahead past the last attempt; its value is never used.
+    // The FP64 walk keeps log(u) in st_luh and the FP64 energy change in st_lul.
+    __shared__ int st_dE[2][65], st_dp[2][64], st_mq[2][64], st_site[2][64], st_snew[2][64];
+    __shared__ int st_res[64];
+    __shared__ double st_luh[2][65], st_lul[2][65], st_u[2][64];
+    // the walk's error site, one slot per barrier parity (wave 0 writes the slot
+    // of the barrier it is about to enter; both waves read it after that barrier)
+    __shared__ int sh_err[2];
     // the weights in LDS are the window [wbase, wbase + wsz) of the walker's
     // table (the whole table when it fits: wsz = nbins, wbase = 0)
     const int wsz = a.win;
     uint32_t* hl = reinterpret_cast<uint32_t*>(lw + wsz);
     int8_t* sp = reinterpret_cast<int8_t*>(hl + (a.hist_in_lds ? a.nbins : 0));
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(tid >> 6); // 0 walks, 1 stages
     const int64_t nb = a.nbins;
     constexpr bool wl = WL;
-    if (lane == 0) {
-        st_dE[64] = 0;
-        st_luh[64] = 0.0;
-        st_lul[64] = 0.0;
+    if (tid < 2) {
+        st_dE[tid][64] = 0;
+        st_luh[tid][64] = 0.0;
+        st_lul[tid][64] = 0.0;
     }
 
     for (int item = blockIdx.x; item < a.n_items; item += gridDim.x) {
@@ -90,13 +118,16 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
         const double logf = a.logf[wk];
 
         if (a.hist_in_lds)
-            for (int64_t b = lane; b < nb; b += 64)
+            for (int64_t b = tid; b < nb; b += kGenThreads)
                 hl[b] = 0;
-        for (int64_t i = lane; i < N; i += 64)
+        for (int64_t i = tid; i < N; i += kGenThreads)
             sp[i] = sp_g[i];
+        if (tid < 2)
+            sh_err[tid] = -1;
         __syncthreads();
 
-        // full recompute of the cached sums (_recompute_cached!)
+        // full recompute of the cached sums (_recompute_cached!), by each wave
+        // over the whole lattice
         long long pair0 = 0, mm0 = 0, qq0 = 0;
         for (int64_t i = lane; i < N; i += 64) {
             const int x = (int)(i % L);
@@ -194,7 +225,7 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
         };
         if (wsz < nb && old_ok)
             wbase = place_window(b_old);
-        for (int b = lane; b < wsz; b += 64)
+        for (int b = tid; b < wsz; b += kGenThreads)
             lw[b] = lw_g[wbase + b];
         __syncthreads();
         // bin of Q (integer binning, truncating div), lane-0 state
@@ -209,352 +240,381 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
             b = in ? bb : 0;
             return in;
         };
+        // ---- wave 0: walk chunk [base, base + 64) of the colour from stage `buf`
+        // (the window of weights first re-centred if the walker nears its edge) ----
+        auto walk_chunk = [&](int base, int buf, int n_col) {
+            if (wsz < nb) {
+                const int bcur = __builtin_amdgcn_readfirstlane(b_old);
+                const bool okc = __builtin_amdgcn_readfirstlane(old_ok ? 1 : 0) != 0;
+                if (okc && (bcur - wbase < reach || wbase + wsz - 1 - bcur < reach)) {
+                    const int nbase = place_window(bcur);
+                    if (nbase != wbase) {
+                        wave_sync();
+                        // the old window goes back whole before the new one
+                        // is read: the two overlap
+                        if (WL) {
+                            for (int b = lane; b < wsz; b += 64)
+                                lw_g[wbase + b] = lw[b];
+                            wave_sync();
+                        }
+                        for (int b = lane; b < wsz; b += 64)
+                            lw[b] = lw_g[nbase + b];
+                        wbase = nbase;
+                        wave_sync();
+                    }
+                }
+            }
+            const int cnt = n_col - base < 64 ? n_col - base : 64;
+            // ---- walk the attempts in site order (wave-uniform state) ----
+            int my_vis = -1, my_ok = 0, n_done = cnt;
+            if (PS) {
+                // lane 0 walks the state (P, Q); every attempt evaluates the
+                // old and new weights as the reference does (the old one
+                // is the last accepted new one: lw does not change during
+                // a multicanonical walk)
+                int err_v = -1;
+                if (lane == 0) {
+                    int kk = 0;
+                    if (!old_ok) {
+                        n_done = 0;
+                        err_v = st_site[buf][0];
+                        kk = cnt;
+                    }
+                    double w_cur = kk < cnt ? nbeta * (double)pair + lw[b_old - wbase] : 0.0;
+                    for (int k = kk; k < cnt; ++k) {
+                        const int Pn = pair + st_dp[buf][k];
+                        const int Qn = qq + st_dE[buf][k];
+                        int bn;
+                        if (!qbin0(Qn, bn)) {
+                            n_done = k;
+                            err_v = st_site[buf][k];
+                            break;
+                        }
+                        const double w_new = nbeta * (double)Pn + lw[bn - wbase];
+                        const double lr = w_new - w_cur;
+                        // lr > 0 || u < exp(lr), exp only inside the 1e-9 band
+                        bool ok = lr > 0 || lr > st_luh[buf][k];
+                        if (!ok && !(lr < st_lul[buf][k]))
+                            ok = st_u[buf][k] < exp(lr);
+                        ++steps;
+                        nacc += ok ? 1u : 0u;
+                        if (ok) {
+                            const int mq = st_mq[buf][k];
+                            pair = Pn;
+                            qq = Qn;
+                            mm += (mq & 0xffff) - 8;
+                            b_old = bn;
+                            w_cur = w_new;
+                        }
+                        st_res[k] = b_old | (ok ? (int)0x80000000 : 0);
+                    }
+                }
+                n_done = __builtin_amdgcn_readfirstlane(n_done);
+                err_v = __builtin_amdgcn_readfirstlane(err_v);
+                if (err_v >= 0)
+                    err = err_v;
+                wave_sync();
+                if (lane < n_done) {
+                    const int r = st_res[lane];
+                    my_vis = r & 0x7fffffff;
+                    my_ok = (r >> 31) & 1;
+                }
+            } else if (int_e) {
+                // Integer walk on lane 0 alone: its state stays in vector
+                // registers and every step is a VALU or LDS operation (no
+                // scalar round trips, no per-attempt branches).  The
+                // walker always sits in the bin it last visited, so
+                // lw_cur holds that bin's weight; attempt k+1's staged
+                // values and the weights of its proposed bin under both
+                // outcomes of attempt k are loaded while k decides.  An
+                // attempt out of the binned domain, or one whose lr lies
+                // in the 1e-9 band around log(u), stops the fast loop; it
+                // is settled exactly (with exp) and the loop resumes.  The
+                // loop records each attempt's bin and outcome; the counters
+                // and the running sums are added up from those records by
+                // all lanes after the chunk.
+                int err_v = -1;
+                if (lane == 0) {
+                    // bin of an energy (32-bit): Ising floor-free div by 2^s
+                    // rounding toward zero as div does; Blume-Capel E - start
+                    auto bin32 = [&](int E) -> int {
+                        const int d = E - start32;
+                        return MODEL == 0 ? (d + ((d >> 31) & smask)) >> a.step_shift : d;
+                    };
+                    auto binc = [&](int E) -> int { // window index, clamped into it
+                        const int b = bin32(E) - wbase;
+                        return max(0, min(b, wsz - 1));
+                    };
+                    int E32 = (int)Ei;
+                    if (b_cur != b_old && old_ok) {
+                        lw_cur = lw[b_old - wbase];
+                        b_cur = b_old;
+                    }
+                    int kk = 0;
+                    if (!old_ok) {
+                        // the walker's current energy lies outside the binned
+                        // domain: its first attempt fails
+                        n_done = 0;
+                        err_v = st_site[buf][0];
+                        kk = cnt;
+                    }
+                    while (kk < cnt) {
+                        int stop = cnt;
+                        bool alive = true;
+                        int dE_c = st_dE[buf][kk];
+                        double luh_c = st_luh[buf][kk], lul_c = st_lul[buf][kk];
+                        double lw_pre = lw[binc(E32 + dE_c)];
+#pragma unroll 2
+                        for (int k = kk; k < cnt; ++k) {
+                            const int kn = k + 1;
+                            const int dE_n = st_dE[buf][kn];
+                            const double luh_n = st_luh[buf][kn], lul_n = st_lul[buf][kn];
+                            const int En = E32 + dE_c;
+                            const int bb = bin32(En);
+                            const bool okn = (unsigned)bb < nbu;
+                            const int b_new = okn ? bb : 0;
+                            const double lw_a = lw[binc(En + dE_n)];
+                            const double lw_r = lw[binc(E32 + dE_n)];
+                            // a multicanonical walk never writes the weights, so
+                            // the prefetched weight is exact even for the current
+                            // bin; a Wang-Landau walk has just lowered that one
+                            const double lw_new = !WL ? lw_pre : (b_new == b_old ? lw_cur : lw_pre);
+                            const double lr = lw_new - lw_cur;
+                            const bool acc_c = lr > luh_c;
+                            const bool go = alive && okn && (acc_c || lr < lul_c);
+                            const bool ok = go && acc_c;
+                            stop = alive && !go ? k : stop;
+                            alive = go;
+                            E32 = ok ? En : E32;
+                            b_old = ok ? b_new : b_old;
+                            lw_cur = ok ? lw_new : lw_cur;
+                            st_res[k] = b_old | (ok ? (int)0x80000000 : 0);
+                            if (WL) {
+                                // unchanged value when the attempt did not run
+                                lw_cur = go ? lw_cur - logf : lw_cur;
+                                lw[b_old - wbase] = lw_cur;
+                            }
+                            lw_pre = ok ? lw_a : lw_r;
+                            dE_c = dE_n;
+                            luh_c = luh_n;
+                            lul_c = lul_n;
+                        }
+                        b_cur = b_old;
+                        if (stop == cnt)
+                            break;
+                        // the attempt that stopped the fast loop, exactly
+                        const int En = E32 + st_dE[buf][stop];
+                        const int bb = bin32(En);
+                        if (!((unsigned)bb < nbu)) {
+                            n_done = stop;
+                            err_v = st_site[buf][stop];
+                            break;
+                        }
+                        const double lw_new = bb == b_old ? lw_cur : lw[bb - wbase];
+                        const double lr = lw_new - lw_cur;
+                        const bool ok = lr > 0 || st_u[buf][stop] < exp(lr);
+                        if (ok) {
+                            E32 = En;
+                            b_old = bb;
+                            lw_cur = lw_new;
+                        }
+                        st_res[stop] = b_old | (ok ? (int)0x80000000 : 0);
+                        if (WL) {
+                            lw_cur = lw_cur - logf;
+                            lw[b_old - wbase] = lw_cur;
+                        }
+                        b_cur = b_old;
+                        kk = stop + 1;
+                    }
+                    Ei = E32;
+                }
+                n_done = __builtin_amdgcn_readfirstlane(n_done);
+                err_v = __builtin_amdgcn_readfirstlane(err_v);
+                if (err_v >= 0)
+                    err = err_v;
+                wave_sync();
+                if (lane < n_done) {
+                    const int r = st_res[lane];
+                    my_vis = r & 0x7fffffff;
+                    my_ok = (r >> 31) & 1;
+                }
+                // counters and running sums of the attempts the walk ran
+                // (every attempt before n_done ran; accepted ones move)
+                const int a_ok = my_ok;
+                const int s_ok = wave_sum(a_ok);
+                pair += wave_sum(a_ok ? st_dp[buf][lane] : 0);
+                mm += wave_sum(a_ok ? (st_mq[buf][lane] & 0xffff) - 8 : 0);
+                qq += wave_sum(a_ok ? (st_mq[buf][lane] >> 16) - 8 : 0);
+                steps += (unsigned)n_done;
+                nacc += (unsigned)s_ok;
+            } else
+            for (int k = 0; k < cnt; ++k) {
+                int b_new;
+                bool okn;
+                if (MODEL == 0)
+                    okn = bin_int(-(long long)pair + __builtin_amdgcn_readfirstlane(st_dE[buf][k]), b_new);
+                else
+                    okn = bin_fp(Eo + uniform_f64(st_lul[buf][k]), b_new);
+                if (!okn || !old_ok) {
+                    err = st_site[buf][k];
+                    n_done = k;
+                    break;
+                }
+                const double lw_old = b_old == b_cur ? lw_cur : lw[b_old - wbase];
+                const double lw_new = b_new == b_old ? lw_old : lw[b_new - wbase];
+                const double lr = lw_new - lw_old;
+                // lr > 0 || u < exp(lr); log(u) from the staging settles
+                // all but a 1e-9-wide band, where exp decides exactly
+                bool ok = __builtin_amdgcn_readfirstlane(lr > 0 ? 1 : 0) != 0;
+                if (!ok) {
+                    const double lu = uniform_f64(st_luh[buf][k]);
+                    // 1 = certain accept, 0 = certain reject, 2 = evaluate exp
+                    const int cls = lr > lu + 1e-9 && lu > -700.0 ? 1
+                                  : (lr < lu - 1e-9 ? 0 : 2);
+                    const int clsu = __builtin_amdgcn_readfirstlane(cls);
+                    if (clsu == 2)
+                        ok = __builtin_amdgcn_readfirstlane(
+                                 uniform_f64(st_u[buf][k]) < exp(lr) ? 1 : 0) != 0;
+                    else
+                        ok = clsu == 1;
+                }
+                ++steps;
+                nacc += ok ? 1u : 0u;
+                const int vis = ok ? b_new : b_old;
+                if (lane == k) {
+                    my_vis = vis;
+                    my_ok = ok ? 1 : 0;
+                }
+                double lw_vis = ok ? lw_new : lw_old;
+                if (wl) {
+                    lw_vis = lw_vis - logf;
+                    if (lane == 0)
+                        lw[vis - wbase] = lw_vis;
+                }
+                if (ok) {
+                    pair += __builtin_amdgcn_readfirstlane(st_dp[buf][k]);
+                    mm += __builtin_amdgcn_readfirstlane((st_mq[buf][k] & 0xffff) - 8);
+                    qq += __builtin_amdgcn_readfirstlane((st_mq[buf][k] >> 16) - 8);
+                    if (MODEL == 0) {
+                        b_old = b_new;
+                    } else {
+                        Eo = -(double)pair + D * (double)qq;
+                        old_ok = bin_fp(Eo, b_old);
+                    }
+                }
+                b_cur = vis;
+                lw_cur = lw_vis;
+            }            // ---- deferred, in parallel: spins and visit counts (order-free) ----
+            if (lane < n_done) {
+                if (my_ok)
+                    sp[st_site[buf][lane]] = (int8_t)st_snew[buf][lane];
+                if (a.hist_in_lds)
+                    atomicAdd(&hl[my_vis], 1u);
+                else
+                    atomicAdd(&hist_g[my_vis], 1ull);
+            }
+        };
+        // ---- stage chunk [base, base + 64) of colour c into stage `buf`: lane k
+        // prepares attempt base + k (the sites of one colour are independent
+        // given the other colour) ----
+        auto stage = [&](int c, int base, int buf, uint64_t sweep) {
+            const int n_col = rows * half;
+            const int o = base + lane;
+            int t_site = 0, t_snew = 0, t_dE = 0, t_dpair = 0, t_dm = 0, t_dq = 0;
+            double t_dEf = 0.0, t_u = 0.0, t_lu = 0.0;
+            int t_mq = 8 | (8 << 16); // packed dm+8 | (dq+8) << 16
+            double t_luh = INFINITY, t_lul = -INFINITY;
+            if (o < n_col) {
+                const int r = o / half;
+                const int j = o - r * half;
+                const int y = DIM == 3 ? r % L : r;
+                const int z = DIM == 3 ? r / L : 0;
+                const int x = 2 * j + ((c + y + z) & 1);
+                const int64_t rowb = (int64_t)L * r;
+                const int64_t zb = DIM == 3 ? (int64_t)z * L * L : 0;
+                int nbs = sp[rowb + (x == 0 ? L - 1 : x - 1)] + sp[rowb + (x == L - 1 ? 0 : x + 1)]
+                        + sp[zb + (int64_t)(y == 0 ? L - 1 : y - 1) * L + x]
+                        + sp[zb + (int64_t)(y == L - 1 ? 0 : y + 1) * L + x];
+                if (DIM == 3)
+                    nbs += sp[(int64_t)(z == 0 ? L - 1 : z - 1) * L * L + (int64_t)y * L + x]
+                         + sp[(int64_t)(z == L - 1 ? 0 : z + 1) * L * L + (int64_t)y * L + x];
+                const int si = sp[rowb + x];
+                const uint32_t g = (uint32_t)((r * 2 + c) * nw + (j >> 5));
+                const uint32_t w = site_word(cr.seed, cr.key, g, j & 31, sweep, P_GEN);
+                t_site = (int)(rowb + x);
+                if (MODEL == 0) {
+                    t_dE = 2 * si * nbs;
+                    t_dpair = -2 * si * nbs;
+                    t_dm = -2 * si;
+                    t_snew = -si;
+                    t_u = (double)w * 0x1.0p-32;
+                } else {
+                    const int sn = gen_propose(si, (int)(w & 1u));
+                    const int ds = sn - si;
+                    const int ds2 = sn * sn - si * si;
+                    t_dEf = (-((double)ds * (1.0 * (double)nbs)) - 0.0) + D * (double)ds2;
+                    t_dE = int_e ? -(ds * nbs) + (int)Di * ds2 : 0;
+                    t_dpair = ds * nbs;
+                    t_dm = ds;
+                    t_dq = ds2;
+                    t_snew = sn;
+                    t_u = (double)(w >> 1) * 0x1.0p-31;
+                }
+                t_lu = log(t_u);
+                // u < exp(lr) is certain for lr > log(u) + 1e-9 and false for
+                // lr < log(u) - 1e-9 (log and exp err by ~1e-16 relative)
+                t_luh = t_lu > -700.0 ? t_lu + 1e-9 : INFINITY;
+                t_lul = t_lu - 1e-9;
+                t_mq = (t_dm + 8) | ((t_dq + 8) << 16);
+            }
+            st_site[buf][lane] = t_site;
+            st_snew[buf][lane] = t_snew;
+            st_dp[buf][lane] = t_dpair;
+            st_mq[buf][lane] = t_mq;
+            st_u[buf][lane] = t_u;
+            if (PS) {
+                st_dE[buf][lane] = t_dq;
+                st_luh[buf][lane] = t_luh;
+                st_lul[buf][lane] = t_lul;
+            } else if (int_e) {
+                st_dE[buf][lane] = t_dE;
+                // lr > 0 || lr > log(u) + 1e-9  <=>  lr > min(0, log(u) + 1e-9)
+                st_luh[buf][lane] = t_luh < 0.0 ? t_luh : 0.0;
+                st_lul[buf][lane] = t_lul;
+            } else {
+                st_dE[buf][lane] = t_dE;
+                st_luh[buf][lane] = t_lu;
+                st_lul[buf][lane] = t_dEf;
+            }
+        };
+
+        // barriers entered so far in the walk (parity selects the sh_err slot)
+        int nbar = 0;
         int64_t s = 0;
         for (; s < a.nsweeps && err < 0; ++s) {
             const uint64_t sweep = cr.sweep + (uint64_t)s;
             for (int c = 0; c < 2 && err < 0; ++c) {
                 const int n_col = rows * half;
-                for (int base = 0; base < n_col && err < 0; base += 64) {
-                    if (wsz < nb) {
-                        const int bcur = __builtin_amdgcn_readfirstlane(b_old);
-                        const bool okc = __builtin_amdgcn_readfirstlane(old_ok ? 1 : 0) != 0;
-                        if (okc && (bcur - wbase < reach || wbase + wsz - 1 - bcur < reach)) {
-                            const int nbase = place_window(bcur);
-                            if (nbase != wbase) {
-                                __syncthreads();
-                                // the old window goes back whole before the new one
-                                // is read: the two overlap
-                                if (WL) {
-                                    for (int b = lane; b < wsz; b += 64)
-                                        lw_g[wbase + b] = lw[b];
-                                    __syncthreads();
-                                }
-                                for (int b = lane; b < wsz; b += 64)
-                                    lw[b] = lw_g[nbase + b];
-                                wbase = nbase;
-                                __syncthreads();
-                            }
-                        }
-                    }
-                    // ---- stage: lane k prepares attempt base + k (the sites of one
-                    // colour are independent given the other colour) ----
-                    const int o = base + lane;
-                    int t_site = 0, t_snew = 0, t_dE = 0, t_dpair = 0, t_dm = 0, t_dq = 0;
-                    double t_dEf = 0.0, t_u = 0.0, t_lu = 0.0;
-                    int t_mq = 8 | (8 << 16); // packed dm+8 | (dq+8) << 16
-                    double t_luh = INFINITY, t_lul = -INFINITY;
-                    if (o < n_col) {
-                        const int r = o / half;
-                        const int j = o - r * half;
-                        const int y = DIM == 3 ? r % L : r;
-                        const int z = DIM == 3 ? r / L : 0;
-                        const int x = 2 * j + ((c + y + z) & 1);
-                        const int64_t rowb = (int64_t)L * r;
-                        const int64_t zb = DIM == 3 ? (int64_t)z * L * L : 0;
-                        int nbs = sp[rowb + (x == 0 ? L - 1 : x - 1)] + sp[rowb + (x == L - 1 ? 0 : x + 1)]
-                                + sp[zb + (int64_t)(y == 0 ? L - 1 : y - 1) * L + x]
-                                + sp[zb + (int64_t)(y == L - 1 ? 0 : y + 1) * L + x];
-                        if (DIM == 3)
-                            nbs += sp[(int64_t)(z == 0 ? L - 1 : z - 1) * L * L + (int64_t)y * L + x]
-                                 + sp[(int64_t)(z == L - 1 ? 0 : z + 1) * L * L + (int64_t)y * L + x];
-                        const int si = sp[rowb + x];
-                        const uint32_t g = (uint32_t)((r * 2 + c) * nw + (j >> 5));
-                        const uint32_t w = site_word(cr.seed, cr.key, g, j & 31, sweep, P_GEN);
-                        t_site = (int)(rowb + x);
-                        if (MODEL == 0) {
-                            t_dE = 2 * si * nbs;
-                            t_dpair = -2 * si * nbs;
-                            t_dm = -2 * si;
-                            t_snew = -si;
-                            t_u = (double)w * 0x1.0p-32;
-                        } else {
-                            const int sn = gen_propose(si, (int)(w & 1u));
-                            const int ds = sn - si;
-                            const int ds2 = sn * sn - si * si;
-                            t_dEf = (-((double)ds * (1.0 * (double)nbs)) - 0.0) + D * (double)ds2;
-                            t_dE = int_e ? -(ds * nbs) + (int)Di * ds2 : 0;
-                            t_dpair = ds * nbs;
-                            t_dm = ds;
-                            t_dq = ds2;
-                            t_snew = sn;
-                            t_u = (double)(w >> 1) * 0x1.0p-31;
-                        }
-                        t_lu = log(t_u);
-                        // u < exp(lr) is certain for lr > log(u) + 1e-9 and false for
-                        // lr < log(u) - 1e-9 (log and exp err by ~1e-16 relative)
-                        t_luh = t_lu > -700.0 ? t_lu + 1e-9 : INFINITY;
-                        t_lul = t_lu - 1e-9;
-                        t_mq = (t_dm + 8) | ((t_dq + 8) << 16);
-                    }
-                    const int cnt = n_col - base < 64 ? n_col - base : 64;
-                    // ---- walk the attempts in site order (wave-uniform state) ----
-                    int my_vis = -1, my_ok = 0, n_done = cnt;
-                    if (PS) {
-                        // lane 0 walks the state (P, Q); every attempt evaluates the
-                        // old and new weights as the reference does (the old one
-                        // is the last accepted new one: lw does not change during
-                        // a multicanonical walk)
-                        st_dE[lane] = t_dq;
-                        st_dp[lane] = t_dpair;
-                        st_mq[lane] = t_mq;
-                        st_luh[lane] = t_luh;
-                        st_lul[lane] = t_lul;
-                        st_u[lane] = t_u;
-                        st_site[lane] = t_site;
-                        __syncthreads();
-                        int err_v = -1;
-                        if (lane == 0) {
-                            int kk = 0;
-                            if (!old_ok) {
-                                n_done = 0;
-                                err_v = st_site[0];
-                                kk = cnt;
-                            }
-                            double w_cur = kk < cnt ? nbeta * (double)pair + lw[b_old - wbase] : 0.0;
-                            for (int k = kk; k < cnt; ++k) {
-                                const int Pn = pair + st_dp[k];
-                                const int Qn = qq + st_dE[k];
-                                int bn;
-                                if (!qbin0(Qn, bn)) {
-                                    n_done = k;
-                                    err_v = st_site[k];
-                                    break;
-                                }
-                                const double w_new = nbeta * (double)Pn + lw[bn - wbase];
-                                const double lr = w_new - w_cur;
-                                // lr > 0 || u < exp(lr), exp only inside the 1e-9 band
-                                bool ok = lr > 0 || lr > st_luh[k];
-                                if (!ok && !(lr < st_lul[k]))
-                                    ok = st_u[k] < exp(lr);
-                                ++steps;
-                                nacc += ok ? 1u : 0u;
-                                if (ok) {
-                                    const int mq = st_mq[k];
-                                    pair = Pn;
-                                    qq = Qn;
-                                    mm += (mq & 0xffff) - 8;
-                                    b_old = bn;
-                                    w_cur = w_new;
-                                }
-                                st_res[k] = b_old | (ok ? (int)0x80000000 : 0);
-                            }
-                        }
-                        n_done = __builtin_amdgcn_readfirstlane(n_done);
-                        err_v = __builtin_amdgcn_readfirstlane(err_v);
-                        if (err_v >= 0)
-                            err = err_v;
-                        __syncthreads();
-                        if (lane < n_done) {
-                            const int r = st_res[lane];
-                            my_vis = r & 0x7fffffff;
-                            my_ok = (r >> 31) & 1;
-                        }
-                    } else if (int_e) {
-                        // Integer walk on lane 0 alone: its state stays in vector
-                        // registers and every step is a VALU or LDS operation (no
-                        // scalar round trips, no per-attempt branches).  The
-                        // walker always sits in the bin it last visited, so
-                        // lw_cur holds that bin's weight; attempt k+1's staged
-                        // values and the weights of its proposed bin under both
-                        // outcomes of attempt k are loaded while k decides.  An
-                        // attempt out of the binned domain, or one whose lr lies
-                        // in the 1e-9 band around log(u), stops the fast loop; it
-                        // is settled exactly (with exp) and the loop resumes.  The
-                        // loop records each attempt's bin and outcome; the counters
-                        // and the running sums are added up from those records by
-                        // all lanes after the chunk.
-                        st_dE[lane] = t_dE;
-                        // lr > 0 || lr > log(u) + 1e-9  <=>  lr > min(0, log(u) + 1e-9)
-                        st_luh[lane] = t_luh < 0.0 ? t_luh : 0.0;
-                        st_lul[lane] = t_lul;
-                        st_u[lane] = t_u;
-                        st_site[lane] = t_site;
-                        __syncthreads();
-                        int err_v = -1;
-                        if (lane == 0) {
-                            // bin of an energy (32-bit): Ising floor-free div by 2^s
-                            // rounding toward zero as div does; Blume-Capel E - start
-                            auto bin32 = [&](int E) -> int {
-                                const int d = E - start32;
-                                return MODEL == 0 ? (d + ((d >> 31) & smask)) >> a.step_shift : d;
-                            };
-                            auto binc = [&](int E) -> int { // window index, clamped into it
-                                const int b = bin32(E) - wbase;
-                                return max(0, min(b, wsz - 1));
-                            };
-                            int E32 = (int)Ei;
-                            if (b_cur != b_old && old_ok) {
-                                lw_cur = lw[b_old - wbase];
-                                b_cur = b_old;
-                            }
-                            int kk = 0;
-                            if (!old_ok) {
-                                // the walker's current energy lies outside the binned
-                                // domain: its first attempt fails
-                                n_done = 0;
-                                err_v = st_site[0];
-                                kk = cnt;
-                            }
-                            while (kk < cnt) {
-                                int stop = cnt;
-                                bool alive = true;
-                                int dE_c = st_dE[kk];
-                                double luh_c = st_luh[kk], lul_c = st_lul[kk];
-                                double lw_pre = lw[binc(E32 + dE_c)];
-#pragma unroll 2
-                                for (int k = kk; k < cnt; ++k) {
-                                    const int kn = k + 1;
-                                    const int dE_n = st_dE[kn];
-                                    const double luh_n = st_luh[kn], lul_n = st_lul[kn];
-                                    const int En = E32 + dE_c;
-                                    const int bb = bin32(En);
-                                    const bool okn = (unsigned)bb < nbu;
-                                    const int b_new = okn ? bb : 0;
-                                    const double lw_a = lw[binc(En + dE_n)];
-                                    const double lw_r = lw[binc(E32 + dE_n)];
-                                    // a multicanonical walk never writes the weights, so
-                                    // the prefetched weight is exact even for the current
-                                    // bin; a Wang-Landau walk has just lowered that one
-                                    const double lw_new = !WL ? lw_pre : (b_new == b_old ? lw_cur : lw_pre);
-                                    const double lr = lw_new - lw_cur;
-                                    const bool acc_c = lr > luh_c;
-                                    const bool go = alive && okn && (acc_c || lr < lul_c);
-                                    const bool ok = go && acc_c;
-                                    stop = alive && !go ? k : stop;
-                                    alive = go;
-                                    E32 = ok ? En : E32;
-                                    b_old = ok ? b_new : b_old;
-                                    lw_cur = ok ? lw_new : lw_cur;
-                                    st_res[k] = b_old | (ok ? (int)0x80000000 : 0);
-                                    if (WL) {
-                                        // unchanged value when the attempt did not run
-                                        lw_cur = go ? lw_cur - logf : lw_cur;
-                                        lw[b_old - wbase] = lw_cur;
-                                    }
-                                    lw_pre = ok ? lw_a : lw_r;
-                                    dE_c = dE_n;
-                                    luh_c = luh_n;
-                                    lul_c = lul_n;
-                                }
-                                b_cur = b_old;
-                                if (stop == cnt)
-                                    break;
-                                // the attempt that stopped the fast loop, exactly
-                                const int En = E32 + st_dE[stop];
-                                const int bb = bin32(En);
-                                if (!((unsigned)bb < nbu)) {
-                                    n_done = stop;
-                                    err_v = st_site[stop];
-                                    break;
-                                }
-                                const double lw_new = bb == b_old ? lw_cur : lw[bb - wbase];
-                                const double lr = lw_new - lw_cur;
-                                const bool ok = lr > 0 || st_u[stop] < exp(lr);
-                                if (ok) {
-                                    E32 = En;
-                                    b_old = bb;
-                                    lw_cur = lw_new;
-                                }
-                                st_res[stop] = b_old | (ok ? (int)0x80000000 : 0);
-                                if (WL) {
-                                    lw_cur = lw_cur - logf;
-                                    lw[b_old - wbase] = lw_cur;
-                                }
-                                b_cur = b_old;
-                                kk = stop + 1;
-                            }
-                            Ei = E32;
-                        }
-                        n_done = __builtin_amdgcn_readfirstlane(n_done);
-                        err_v = __builtin_amdgcn_readfirstlane(err_v);
-                        if (err_v >= 0)
-                            err = err_v;
-                        __syncthreads();
-                        if (lane < n_done) {
-                            const int r = st_res[lane];
-                            my_vis = r & 0x7fffffff;
-                            my_ok = (r >> 31) & 1;
-                        }
-                        // counters and running sums of the attempts the walk ran
-                        // (every attempt before n_done ran; accepted ones move)
-                        const int a_ok = my_ok;
-                        const int s_ok = wave_sum(a_ok);
-                        pair += wave_sum(a_ok ? t_dpair : 0);
-                        mm += wave_sum(a_ok ? t_dm : 0);
-                        qq += wave_sum(a_ok ? t_dq : 0);
-                        steps += (unsigned)n_done;
-                        nacc += (unsigned)s_ok;
-                    } else
-                    for (int k = 0; k < cnt; ++k) {
-                        int b_new;
-                        bool okn;
-                        if (MODEL == 0)
-                            okn = bin_int(-(long long)pair + readlane_i32(t_dE, k), b_new);
-                        else
-                            okn = bin_fp(Eo + readlane_f64(t_dEf, k), b_new);
-                        if (!okn || !old_ok) {
-                            err = readlane_i32(t_site, k);
-                            n_done = k;
-                            break;
-                        }
-                        const double lw_old = b_old == b_cur ? lw_cur : lw[b_old - wbase];
-                        const double lw_new = b_new == b_old ? lw_old : lw[b_new - wbase];
-                        const double lr = lw_new - lw_old;
-                        // lr > 0 || u < exp(lr); log(u) from the staging settles
-                        // all but a 1e-9-wide band, where exp decides exactly
-                        bool ok = __builtin_amdgcn_readfirstlane(lr > 0 ? 1 : 0) != 0;
-                        if (!ok) {
-                            const double lu = readlane_f64(t_lu, k);
-                            // 1 = certain accept, 0 = certain reject, 2 = evaluate exp
-                            const int cls = lr > lu + 1e-9 && lu > -700.0 ? 1
-                                          : (lr < lu - 1e-9 ? 0 : 2);
-                            const int clsu = __builtin_amdgcn_readfirstlane(cls);
-                            if (clsu == 2)
-                                ok = __builtin_amdgcn_readfirstlane(
-                                         readlane_f64(t_u, k) < exp(lr) ? 1 : 0) != 0;
-                            else
-                                ok = clsu == 1;
-                        }
-                        ++steps;
-                        nacc += ok ? 1u : 0u;
-                        const int vis = ok ? b_new : b_old;
-                        if (lane == k) {
-                            my_vis = vis;
-                            my_ok = ok ? 1 : 0;
-                        }
-                        double lw_vis = ok ? lw_new : lw_old;
-                        if (wl) {
-                            lw_vis = lw_vis - logf;
-                            if (lane == 0)
-                                lw[vis - wbase] = lw_vis;
-                        }
-                        if (ok) {
-                            pair += readlane_i32(t_dpair, k);
-                            mm += readlane_i32(t_dm, k);
-                            qq += readlane_i32(t_dq, k);
-                            if (MODEL == 0) {
-                                b_old = b_new;
-                            } else {
-                                Eo = -(double)pair + D * (double)qq;
-                                old_ok = bin_fp(Eo, b_old);
-                            }
-                        }
-                        b_cur = vis;
-                        lw_cur = lw_vis;
-                    }
-                    // ---- deferred, in parallel: spins and visit counts (order-free) ----
-                    if (lane < n_done) {
-                        if (my_ok)
-                            sp[t_site] = (int8_t)t_snew;
-                        if (a.hist_in_lds)
-                            atomicAdd(&hl[my_vis], 1u);
-                        else
-                            atomicAdd(&hist_g[my_vis], 1ull);
+                // the colour's first chunk, once the other colour's spins are written
+                if (wid == 1)
+                    stage(c, 0, 0, sweep);
+                __syncthreads();
+                ++nbar;
+                for (int base = 0, buf = 0; base < n_col && err < 0; base += 64, buf ^= 1) {
+                    if (wid == 1) {
+                        if (base + 64 < n_col)
+                            stage(c, base + 64, buf ^ 1, sweep);
+                    } else {
+                        walk_chunk(base, buf, n_col);
+                        if (err >= 0 && lane == 0)
+                            sh_err[nbar & 1] = (int)err;
                     }
                     __syncthreads();
+                    err = __builtin_amdgcn_readfirstlane(sh_err[nbar & 1]);
+                    ++nbar;
                 }
             }
-            if (err < 0 && a.record && lane == 0) {
+            if (err < 0 && a.record && tid == 0) {
                 const int64_t at = cr.series_off + a.rec_pos + s;
                 a.ser_E[at] = (int32_t)(-pair);
                 a.ser_M[at] = (int32_t)mm;
@@ -563,16 +623,17 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
             }
         }
         __syncthreads();
-        for (int64_t i = lane; i < N; i += 64)
+        for (int64_t i = tid; i < N; i += kGenThreads)
             sp_g[i] = sp[i];
-        if (wl)
+        // (the window's position is wave 0's state)
+        if (wl && wid == 0)
             for (int b = lane; b < wsz; b += 64)
                 lw_g[wbase + b] = lw[b];
         if (a.hist_in_lds)
-            for (int64_t b = lane; b < nb; b += 64)
+            for (int64_t b = tid; b < nb; b += kGenThreads)
                 if (hl[b])
                     hist_g[b] += hl[b];
-        if (lane == 0) {
+        if (tid == 0) {
             a.steps[wk] += steps;
             a.accepted[wk] += nacc;
             a.err_site[wk] = err;
@@ -587,7 +648,7 @@ __global__ void __launch_bounds__(64) k_gen_sweeps(GenArgs a)
 template <int M, int D>
 static void gen_launch(const GenArgs& a, size_t lds_bytes, hipStream_t st)
 {
-    const dim3 grid(a.n_items), block(64);
+    const dim3 grid(a.n_items), block(kGenThreads);
     if (a.kind == 3)
         hipLaunchKernelGGL((k_gen_sweeps<M, D, 1>), grid, block, lds_bytes, st, a);
     else if constexpr (M == 1) {

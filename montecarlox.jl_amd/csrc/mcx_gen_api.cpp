@@ -78,9 +78,6 @@ mcx_status check_group(mcx_gen* g, int32_t group, const char* who)
     return MCX_OK;
 }
 
-// static LDS of the walker kernel: one chunk's staged attempts (64 x 5 ints + 3 doubles)
-constexpr size_t kGenStaticLds = 64 * (5 * 4 + 3 * 8);
-
 // dynamic LDS of one walker: weights (a window of win bins), optional
 // histogram (whole table), spins
 size_t gen_lds(int64_t win, int64_t nbins, int64_t nsites, bool hist_in_lds)

@@ -123,6 +123,38 @@ def test_bounds_error(mcx):
     assert algs[0].steps == 0 and algs[0].ensemble.histogram.values.sum() == 0
 
 
+@pytest.mark.parametrize("model,kind,start,step,nbins,D", [
+    # the oracle stops these after 195, 264, 5601 and 1772 attempts (chunks of 64)
+    ("ising", "muca", -40, 4, 21, 0.0),            # integer walk
+    ("ising", "wl", -60, 4, 31, 0.0),
+    ("blume_capel", "muca", 10.0, 0.1, 801, 0.3),  # FP64 energies: the generic walk
+    ("blume_capel", "wl", 10.0, 0.1, 801, 0.3),
+])
+def test_bounds_error_mid_run(mcx, orc, model, kind, start, step, nbins, D):
+    """Flat weights accept every move, so the walker's energy wanders out of a
+    narrow domain somewhere in a later chunk or sweep; the run stops at the
+    same attempt as the oracle's (same site) with the same spins."""
+    import re
+    L, seed = 16, 21
+    sys, algs = _walkers(mcx, model, 2, L, 1, kind, start, step, nbins, seed,
+                         np.zeros((1, nbins)), 0.5, D)
+    s = sys.batch.get_spins(0)
+    print(f"gen bounds mid-run model={model} kind={kind}", flush=True)
+    with pytest.raises(mcx.MCXBoundsError) as ei:
+        mcx.sweep_(sys, algs, 40)
+    h = np.zeros(nbins, np.int64)
+    code = {"ising": 0, "blume_capel": 1}[model]
+    with pytest.raises(orc.BoundsError) as eo:
+        orc.generalized_sweeps(code, 1 if kind == "wl" else 0, 2, L, np.zeros(nbins), h, s, seed,
+                               0, 0, 40, bin_start=start, bin_step=step, logf=0.5, D=D)
+    site_dev = int(re.search(r"site (\d+)", str(ei.value)).group(1))
+    site_orc = int(re.search(r"site (\d+)", str(eo.value)).group(1))
+    assert site_dev == site_orc
+    assert np.array_equal(s, sys.batch.get_spins(0))
+    # the walk got past the first chunk (the error is not the initial state's)
+    assert h.sum() > 64
+
+
 def test_flat_weights_accept_everything(mcx):
     sys, algs = _walkers(mcx, "ising", 2, 6, 2, "muca", -72, 4, 37, 8)
     mcx.sweep_(sys, algs, 10)
