@@ -325,8 +325,12 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                 // scalar round trips, no per-attempt branches).  The
                 // walker always sits in the bin it last visited, so
                 // lw_cur holds that bin's weight; attempt k+1's staged
-                // values and the weights of its proposed bin under both
-                // outcomes of attempt k are loaded while k decides.  An
+                // values are loaded while k decides.  The weight of k+1's
+                // proposed bin is written as a choice between the two
+                // outcomes of k, which the compiler turns into one load from
+                // the chosen address (loading both and choosing measured
+                // slower: more instructions on the chain than LDS latency
+                // saved).  An
                 // attempt out of the binned domain, or one whose lr lies
                 // in the 1e-9 band around log(u), stops the fast loop; it
                 // is settled exactly (with exp) and the loop resumes.  The
