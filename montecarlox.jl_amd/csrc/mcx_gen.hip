@@ -83,7 +83,8 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
     // integer walk's read-ahead past the last attempt; its value is never used.
     // The FP64 walk keeps log(u) in st_luh and the FP64 energy change in st_lul.
     __shared__ int st_dE[2][65], st_dp[2][64], st_mq[2][64], st_site[2][64], st_snew[2][64];
-    __shared__ int st_res[64];
+    // the walk's outcomes per attempt, and how many attempts it ran, per stage
+    __shared__ int st_res[2][64], sh_nd[2];
     __shared__ double st_luh[2][65], st_lul[2][65], st_u[2][64];
     // the walk's error site, one slot per barrier parity (wave 0 writes the slot
     // of the barrier it is about to enter; both waves read it after that barrier)
@@ -188,6 +189,8 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                                        : (Dr == D && fabs(D) * (double)N < lim && a.bin_step == 1.0 &&
                                           rint(a.bin_start) == a.bin_start));
         const long long Di = (long long)Dr;
+        // the integer walk (lane 0 walks E; the staging wave adds up each chunk)
+        const bool int_walk = int_e && !PS;
         const long long start_i = (long long)a.bin_start;
         const int start32 = (int)start_i;
         const int smask = step_pow2 ? (1 << a.step_shift) - 1 : 0;
@@ -306,7 +309,7 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                             b_old = bn;
                             w_cur = w_new;
                         }
-                        st_res[k] = b_old | (ok ? (int)0x80000000 : 0);
+                        st_res[buf][k] = b_old | (ok ? (int)0x80000000 : 0);
                     }
                 }
                 n_done = __builtin_amdgcn_readfirstlane(n_done);
@@ -315,7 +318,7 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                     err = err_v;
                 wave_sync();
                 if (lane < n_done) {
-                    const int r = st_res[lane];
+                    const int r = st_res[buf][lane];
                     my_vis = r & 0x7fffffff;
                     my_ok = (r >> 31) & 1;
                 }
@@ -392,7 +395,7 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                             E32 = ok ? En : E32;
                             b_old = ok ? b_new : b_old;
                             lw_cur = ok ? lw_new : lw_cur;
-                            st_res[k] = b_old | (ok ? (int)0x80000000 : 0);
+                            st_res[buf][k] = b_old | (ok ? (int)0x80000000 : 0);
                             if (WL) {
                                 // unchanged value when the attempt did not run
                                 lw_cur = go ? lw_cur - logf : lw_cur;
@@ -422,7 +425,7 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                             b_old = bb;
                             lw_cur = lw_new;
                         }
-                        st_res[stop] = b_old | (ok ? (int)0x80000000 : 0);
+                        st_res[buf][stop] = b_old | (ok ? (int)0x80000000 : 0);
                         if (WL) {
                             lw_cur = lw_cur - logf;
                             lw[b_old - wbase] = lw_cur;
@@ -438,19 +441,14 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                     err = err_v;
                 wave_sync();
                 if (lane < n_done) {
-                    const int r = st_res[lane];
+                    const int r = st_res[buf][lane];
                     my_vis = r & 0x7fffffff;
                     my_ok = (r >> 31) & 1;
                 }
-                // counters and running sums of the attempts the walk ran
-                // (every attempt before n_done ran; accepted ones move)
-                const int a_ok = my_ok;
-                const int s_ok = wave_sum(a_ok);
-                pair += wave_sum(a_ok ? st_dp[buf][lane] : 0);
-                mm += wave_sum(a_ok ? (st_mq[buf][lane] & 0xffff) - 8 : 0);
-                qq += wave_sum(a_ok ? (st_mq[buf][lane] >> 16) - 8 : 0);
-                steps += (unsigned)n_done;
-                nacc += (unsigned)s_ok;
+                // the counters, running sums and visit counts of the chunk
+                // are added up by the staging wave during the next walk
+                if (lane == 0)
+                    sh_nd[buf] = n_done;
             } else
             for (int k = 0; k < cnt; ++k) {
                 int b_new;
@@ -508,10 +506,37 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                 }
                 b_cur = vis;
                 lw_cur = lw_vis;
-            }            // ---- deferred, in parallel: spins and visit counts (order-free) ----
+            }
+            // ---- deferred, in parallel: spins and visit counts (order-free) ----
             if (lane < n_done) {
                 if (my_ok)
                     sp[st_site[buf][lane]] = (int8_t)st_snew[buf][lane];
+                if (!int_walk) {
+                    if (a.hist_in_lds)
+                        atomicAdd(&hl[my_vis], 1u);
+                    else
+                        atomicAdd(&hist_g[my_vis], 1ull);
+                }
+            }
+        };
+        // ---- staging wave, integer walk: the counters, running sums and visit
+        // counts of a walked chunk (stage `pb`), from its recorded outcomes
+        // (every attempt before n_done ran; accepted ones move) ----
+        auto add_up = [&](int pb) {
+            const int nd = __builtin_amdgcn_readfirstlane(sh_nd[pb]);
+            int my_ok = 0, my_vis = 0;
+            if (lane < nd) {
+                const int r = st_res[pb][lane];
+                my_vis = r & 0x7fffffff;
+                my_ok = (r >> 31) & 1;
+            }
+            const int s_ok = wave_sum(my_ok);
+            pair += wave_sum(my_ok ? st_dp[pb][lane] : 0);
+            mm += wave_sum(my_ok ? (st_mq[pb][lane] & 0xffff) - 8 : 0);
+            qq += wave_sum(my_ok ? (st_mq[pb][lane] >> 16) - 8 : 0);
+            steps += (unsigned)nd;
+            nacc += (unsigned)s_ok;
+            if (lane < nd) {
                 if (a.hist_in_lds)
                     atomicAdd(&hl[my_vis], 1u);
                 else
@@ -594,18 +619,31 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
 
         // barriers entered so far in the walk (parity selects the sh_err slot)
         int nbar = 0;
+        // staging wave, integer walk: the stage whose chunk was walked last and
+        // is not yet added up (-1: none).  It is added up before that stage is
+        // staged again, and at the end of each sweep before the series record.
+        int pend = -1;
         int64_t s = 0;
         for (; s < a.nsweeps && err < 0; ++s) {
             const uint64_t sweep = cr.sweep + (uint64_t)s;
             for (int c = 0; c < 2 && err < 0; ++c) {
                 const int n_col = rows * half;
                 // the colour's first chunk, once the other colour's spins are written
-                if (wid == 1)
+                if (wid == 1) {
+                    if (pend >= 0)
+                        add_up(pend);
+                    pend = -1;
                     stage(c, 0, 0, sweep);
+                }
                 __syncthreads();
                 ++nbar;
                 for (int base = 0, buf = 0; base < n_col && err < 0; base += 64, buf ^= 1) {
                     if (wid == 1) {
+                        // the previous chunk's stage is added up before it is
+                        // staged again
+                        if (pend >= 0)
+                            add_up(pend);
+                        pend = int_walk ? buf : -1;
                         if (base + 64 < n_col)
                             stage(c, base + 64, buf ^ 1, sweep);
                     } else {
@@ -618,7 +656,12 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                     ++nbar;
                 }
             }
-            if (err < 0 && a.record && tid == 0) {
+            if (wid == 1 && pend >= 0) {
+                add_up(pend);
+                pend = -1;
+            }
+            // (the integer walk's running sums are the staging wave's)
+            if (err < 0 && a.record && tid == (int_walk ? 64 : 0)) {
                 const int64_t at = cr.series_off + a.rec_pos + s;
                 a.ser_E[at] = (int32_t)(-pair);
                 a.ser_M[at] = (int32_t)mm;
@@ -637,7 +680,7 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
             for (int64_t b = tid; b < nb; b += kGenThreads)
                 if (hl[b])
                     hist_g[b] += hl[b];
-        if (tid == 0) {
+        if (tid == (int_walk ? 64 : 0)) {
             a.steps[wk] += steps;
             a.accepted[wk] += nacc;
             a.err_site[wk] = err;
