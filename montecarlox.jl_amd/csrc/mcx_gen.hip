@@ -21,10 +21,11 @@
 // colour's spins are written.  One workgroup barrier per chunk hands the
 // buffers over.  The walker's FP64 log-weights live in LDS; the weight and
 // bin of the current energy are kept in registers, so an attempt costs one LDS
-// load and no exp unless lr falls within 1e-9 of log(u).  Spin writes and
-// visit counts (order does not matter for either) are applied by wave 0's
-// lanes after the walk.  Histograms are uint32 in LDS when they fit,
-// otherwise int64 atomics in HBM.
+// load and no exp unless lr falls within 1e-9 of log(u).  Spin writes (order
+// does not matter) are applied by wave 0's lanes after the walk; in the
+// integer walk the chunk's counters, running sums and visit counts are added
+// up by wave 1 from the recorded outcomes while wave 0 walks the next chunk.
+// Histograms are uint32 in LDS when they fit, otherwise int64 atomics in HBM.
 //
 // KIND 2 (Blume-Capel): the CustomEnsemble of docs/src/examples/spin_systems/
 // muca_BlumeCapel.jl:43-89 -- a Boltzmann weight on the pair term and a
