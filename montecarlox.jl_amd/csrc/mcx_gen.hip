@@ -21,10 +21,10 @@
 // colour's spins are written.  One workgroup barrier per chunk hands the
 // buffers over.  The walker's FP64 log-weights live in LDS; the weight and
 // bin of the current energy are kept in registers, so an attempt costs one LDS
-// load and no exp unless lr falls within 1e-9 of log(u).  Spin writes (order
-// does not matter) are applied by wave 0's lanes after the walk; in the
-// integer walk the chunk's counters, running sums and visit counts are added
-// up by wave 1 from the recorded outcomes while wave 0 walks the next chunk.
+// load and no exp unless lr falls within 1e-9 of log(u).  In the integer walk
+// the chunk's spin writes, counters, running sums and visit counts are done
+// by wave 1 from the recorded outcomes while wave 0 walks the next chunk; the
+// other walks apply spins and visits with wave 0's lanes after the walk.
 // Histograms are uint32 in LDS when they fit, otherwise int64 atomics in HBM.
 //
 // KIND 2 (Blume-Capel): the CustomEnsemble of docs/src/examples/spin_systems/
@@ -440,14 +440,9 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                 err_v = __builtin_amdgcn_readfirstlane(err_v);
                 if (err_v >= 0)
                     err = err_v;
-                wave_sync();
-                if (lane < n_done) {
-                    const int r = st_res[buf][lane];
-                    my_vis = r & 0x7fffffff;
-                    my_ok = (r >> 31) & 1;
-                }
-                // the counters, running sums and visit counts of the chunk
-                // are added up by the staging wave during the next walk
+                // the chunk's spin writes, counters, running sums and visit
+                // counts are done by the staging wave during the next walk
+                // (my_ok stays 0 here: nothing is deferred below)
                 if (lane == 0)
                     sh_nd[buf] = n_done;
             } else
@@ -520,9 +515,12 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                 }
             }
         };
-        // ---- staging wave, integer walk: the counters, running sums and visit
-        // counts of a walked chunk (stage `pb`), from its recorded outcomes
-        // (every attempt before n_done ran; accepted ones move) ----
+        // ---- staging wave, integer walk: the spin writes, counters, running
+        // sums and visit counts of a walked chunk (stage `pb`), from its
+        // recorded outcomes (every attempt before n_done ran; accepted ones
+        // move).  The spins are written before this wave stages again, and
+        // staging reads them only from the other colour or from sites of
+        // later chunks. ----
         auto add_up = [&](int pb) {
             const int nd = __builtin_amdgcn_readfirstlane(sh_nd[pb]);
             int my_ok = 0, my_vis = 0;
@@ -538,6 +536,8 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
             steps += (unsigned)nd;
             nacc += (unsigned)s_ok;
             if (lane < nd) {
+                if (my_ok)
+                    sp[st_site[pb][lane]] = (int8_t)st_snew[pb][lane];
                 if (a.hist_in_lds)
                     atomicAdd(&hl[my_vis], 1u);
                 else
