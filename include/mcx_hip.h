@@ -137,7 +137,7 @@ mcx_status mcx_batch_destroy(mcx_batch* b);
 mcx_status mcx_batch_num_sites(mcx_batch* b, int32_t chain, int64_t* n_sites);
 
 /* Replaces: Metropolis(rng; beta) / HeatBath(rng; beta) per chain
- * (src/algorithms/metropolis.jl:205-206, heat_bath.jl:254): one beta >= 0 per
+ * (src/algorithms/metropolis.jl:95-96, heat_bath.jl:17): one beta >= 0 per
  * chain; the library tabulates exp(-beta*dE) and logistic(beta*s*dE) exactly
  * as logweight(BoltzmannEnsemble) (src/ensembles/boltzmann.jl:28) and
  * logistic (src/infrastructure/utils.jl:82-88) evaluate them. */
@@ -198,7 +198,7 @@ mcx_status mcx_batch_get_moments(mcx_batch* b, double* out);
 mcx_status mcx_batch_get_counters(mcx_batch* b, int64_t* steps, int64_t* accepted);
 mcx_status mcx_batch_reset_counters(mcx_batch* b);
 
-/* Replaces the alg.rng field (Metropolis(rng; beta), src/algorithms/metropolis.jl:173-206):
+/* Replaces the alg.rng field (Metropolis(rng; beta), src/algorithms/metropolis.jl:63-96):
  * rebinds the chains' Philox streams to key (seed & 0xffffffff, chain_keys[i])
  * with counter word 3 = seed >> 32 (chain_keys NULL = chain index).  The
  * stream position (sweep counter) is kept. */
@@ -220,7 +220,7 @@ typedef struct {
 } mcx_pt_desc;
 
 /* Replaces ReplicaExchange(ThreadsBackend(R), algs) / ParallelTempering(betas; seed, rng)
- * (src/algorithms/replica_exchange.jl:52-58, parallel_tempering.jl:254-298) for
+ * (src/algorithms/replica_exchange.jl:52-58, parallel_tempering.jl:24-53) for
  * n_sets independent ladders over the chains of a batch (n_sets*R == n_chains).
  * Replica r starts on ladder index r+1 (indices = collect(1:n)), stage 0. */
 mcx_status mcx_pt_create(mcx_batch* b, const mcx_pt_desc* desc, mcx_pt** out);
@@ -270,8 +270,8 @@ typedef struct {
 } mcx_gen_desc;
 
 /* Replaces Multicanonical(rng, bins) / WangLandau(rng, bins; logf) per chain
- * (src/algorithms/multicanonical.jl:9-25, wang_landau.jl:44-51) and
- * ParallelMulticanonical(ThreadsBackend(n), algs) (parallel_multicanonical.jl:91-99)
+ * (src/algorithms/multicanonical.jl:9-25, algorithms/wang_landau.jl:10-17) and
+ * ParallelMulticanonical(ThreadsBackend(n), algs) (parallel_multicanonical.jl:21-28)
  * over the chains of a batch.  Each walker owns FP64 log-weights (initially 0)
  * and an int64 histogram.  Ising energies bin with integer div, Blume-Capel
  * (Float64) energies with round-half-even, as _binindex (binned_object.jl:18-24). */
@@ -339,6 +339,19 @@ mcx_status mcx_muca_update_values(mcx_ctx* ctx, double* lw, const double* h, int
  * index i: the taus of optimize_exchange_interval! (parallel_tempering.jl:83-121). */
 mcx_status mcx_batch_tau_int(mcx_batch* b, int32_t observable, int64_t first, int64_t count,
                              int32_t min_points, int32_t max_lag, double c, double* taus);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving or retuning loops,
+ * harnesses that forbid hipMalloc): mcx_batch_tau_int with the same arguments
+ * and results, staging the taus in a static device array of
+ * MCX_TAU_NOALLOC_MAX_SLOTS doubles (512 KiB per GPU, held until the process
+ * ends) instead of a per-call allocation; it calls no allocating routine.
+ * Batches of more slots get MCX_EINVAL.  Every call on one GPU shares that
+ * GPU's array, so calls that overlap on one GPU (from several host threads or
+ * contexts) race; calls on one context are ordered by its stream. */
+#define MCX_TAU_NOALLOC_MAX_SLOTS 65536
+mcx_status mcx_batch_tau_int_noalloc(mcx_batch* b, int32_t observable, int64_t first,
+                                     int64_t count, int32_t min_points, int32_t max_lag,
+                                     double c, double* taus);
 
 /* ------------------------------------------------------------------------ */
 /* Device-state snapshots (checkpoint / resume)                             */

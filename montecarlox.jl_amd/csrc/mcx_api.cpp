@@ -170,7 +170,16 @@ extern "C" mcx_status mcx_ctx_create(int32_t device, mcx_ctx** out)
         e = hipEventCreate(&c->ev0);
     if (e == hipSuccess)
         e = hipEventCreate(&c->ev1);
+    // the allocation-free entry points' static device array on this device
+    if (e == hipSuccess)
+        e = tau_scratch_address(device, &c->tau_scratch);
     if (e != hipSuccess) {
+        if (c->ev1)
+            hipEventDestroy(c->ev1);
+        if (c->ev0)
+            hipEventDestroy(c->ev0);
+        if (c->stream)
+            hipStreamDestroy(c->stream);
         delete c;
         return fail(MCX_EHIP, "mcx_ctx_create: %s", hipGetErrorString(e));
     }
@@ -930,22 +939,29 @@ extern "C" mcx_status mcx_batch_set_streams(mcx_batch* b, uint64_t seed, const u
 // (src/measurements/autocorrelations.jl:28-110)
 // ---------------------------------------------------------------------------
 
-extern "C" mcx_status mcx_batch_tau_int(mcx_batch* b, int32_t observable, int64_t first,
-                                        int64_t count, int32_t min_points, int32_t max_lag,
-                                        double c, double* taus)
+namespace {
+
+static_assert(mcx::kTauScratchSlots == MCX_TAU_NOALLOC_MAX_SLOTS,
+              "the static tau array and the public limit must agree");
+
+// Shared body of mcx_batch_tau_int / _noalloc: the taus are staged in d_out
+// (n_chains doubles on the batch's device; nullptr = allocate one here).
+mcx_status batch_tau_int(mcx_batch* b, int32_t observable, int64_t first, int64_t count,
+                         int32_t min_points, int32_t max_lag, double c, double* taus,
+                         double* d_out, const char* who)
 {
     if (!b || !taus)
-        return fail(MCX_EINVAL, "mcx_batch_tau_int: NULL argument");
+        return fail(MCX_EINVAL, "%s: NULL argument", who);
     if (observable < MCX_OBS_ENERGY || observable > MCX_OBS_Q)
-        return fail(MCX_EINVAL, "mcx_batch_tau_int: unknown observable %d", observable);
+        return fail(MCX_EINVAL, "%s: unknown observable %d", who, observable);
     if (observable == MCX_OBS_Q && b->model != MCX_BLUME_CAPEL)
-        return fail(MCX_EINVAL, "mcx_batch_tau_int: MCX_OBS_Q needs a Blume-Capel batch");
+        return fail(MCX_EINVAL, "%s: MCX_OBS_Q needs a Blume-Capel batch", who);
     if (min_points < 2)
         return fail(MCX_EINVAL, "min_points must be >= 2");
     if (!(c > 0))
         return fail(MCX_EINVAL, "c must be positive");
     if (first < 0 || count < 0 || first + count > b->rec_len)
-        return fail(MCX_EBOUNDS, "mcx_batch_tau_int: sweeps [%lld, %lld) outside the %lld recorded",
+        return fail(MCX_EBOUNDS, "%s: sweeps [%lld, %lld) outside the %lld recorded", who,
                     (long long)first, (long long)(first + count), (long long)b->rec_len);
     const double nan = std::numeric_limits<double>::quiet_NaN();
     const int64_t cap = count / 2;
@@ -963,13 +979,14 @@ extern "C" mcx_status mcx_batch_tau_int(mcx_batch* b, int32_t observable, int64_
     if (observable == MCX_OBS_ENERGY && b->model == MCX_BLUME_CAPEL)
         vmax = std::max(vmax, (double)max_sites);
     if (3.0 * std::log2((double)count) + 2.0 * std::log2(vmax + 1.0) > 124.0)
-        return fail(MCX_EINVAL, "mcx_batch_tau_int: %lld samples of lattices of %lld sites exceed "
-                                "the exact 128-bit lag sums; use a shorter window",
+        return fail(MCX_EINVAL, "%s: %lld samples of lattices of %lld sites exceed "
+                                "the exact 128-bit lag sums; use a shorter window", who,
                     (long long)count, (long long)max_sites);
     HIPCHK(hipSetDevice(b->ctx->device));
     hipStream_t st = b->ctx->stream;
-    double* d_tau = nullptr;
-    HIPCHK(hipMalloc((void**)&d_tau, sizeof(double) * b->n));
+    double* d_tau = d_out;
+    if (!d_tau)
+        HIPCHK(hipMalloc((void**)&d_tau, sizeof(double) * b->n));
     TauArgs a{};
     a.X = observable == MCX_OBS_ENERGY ? b->d_E : observable == MCX_OBS_Q ? b->d_Q : b->d_M;
     a.Y = (observable == MCX_OBS_ENERGY && b->model == MCX_BLUME_CAPEL) ? b->d_Q : nullptr;
@@ -985,8 +1002,35 @@ extern "C" mcx_status mcx_batch_tau_int(mcx_batch* b, int32_t observable, int64_
     hipError_t e = launch_tau_int(a, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(taus, d_tau, sizeof(double) * b->n, hipMemcpyDeviceToHost, st);
-    mcx_status s = e == hipSuccess ? sync_check(b, "mcx_batch_tau_int")
-                                   : fail(MCX_EHIP, "mcx_batch_tau_int: %s", hipGetErrorString(e));
-    hipFree(d_tau);
+    mcx_status s = e == hipSuccess ? sync_check(b, who)
+                                   : fail(MCX_EHIP, "%s: %s", who, hipGetErrorString(e));
+    if (!d_out)
+        hipFree(d_tau);
     return s;
+}
+
+} // namespace
+
+extern "C" mcx_status mcx_batch_tau_int(mcx_batch* b, int32_t observable, int64_t first,
+                                        int64_t count, int32_t min_points, int32_t max_lag,
+                                        double c, double* taus)
+{
+    return batch_tau_int(b, observable, first, count, min_points, max_lag, c, taus, nullptr,
+                         "mcx_batch_tau_int");
+}
+
+extern "C" mcx_status mcx_batch_tau_int_noalloc(mcx_batch* b, int32_t observable, int64_t first,
+                                                int64_t count, int32_t min_points,
+                                                int32_t max_lag, double c, double* taus)
+{
+    const char* who = "mcx_batch_tau_int_noalloc";
+    if (!b || !taus)
+        return fail(MCX_EINVAL, "%s: NULL argument", who);
+    if (b->n > MCX_TAU_NOALLOC_MAX_SLOTS)
+        return fail(MCX_EINVAL, "%s: %d series exceed the static limit of %d; use "
+                                "mcx_batch_tau_int", who, b->n, MCX_TAU_NOALLOC_MAX_SLOTS);
+    if (!b->ctx->tau_scratch)
+        return fail(MCX_ESTATE, "%s: the context has no static tau array", who);
+    return batch_tau_int(b, observable, first, count, min_points, max_lag, c, taus,
+                         b->ctx->tau_scratch, who);
 }

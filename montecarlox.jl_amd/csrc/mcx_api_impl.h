@@ -27,6 +27,7 @@ struct mcx_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int n_users = 0; // batches and servers built on this context
+    double* tau_scratch = nullptr; // this device's static tau staging array (no allocation)
 };
 
 struct mcx_batch {

@@ -175,6 +175,12 @@ struct TauArgs {
     double* taus;            // [n_traces]
 };
 hipError_t launch_tau_int(const TauArgs& a, hipStream_t st);
+// Allocation-free tau: a static device array of kTauScratchSlots doubles per
+// GPU stages the taus; tau_scratch_address gives its address on `device`
+// (hipGetSymbolAddress, done once per device and cached).
+constexpr int kTauScratchSlots = 65536;
+constexpr int kMaxDevices = 64;
+hipError_t tau_scratch_address(int device, double** out);
 
 hipError_t launch_init_spins(int model, int mode, ChainRec* chains, int32_t n_chains,
                              int64_t max_sites, int8_t* spins, hipStream_t st);

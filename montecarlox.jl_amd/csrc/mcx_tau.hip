@@ -20,6 +20,8 @@
 // window closes; the next block is computed only if it does not.
 #include "mcx_internal.h"
 
+#include <mutex>
+
 namespace mcx {
 
 namespace {
@@ -250,6 +252,49 @@ hipError_t launch_tau_int(const TauArgs& a, hipStream_t st)
 {
     hipLaunchKernelGGL(k_tau_int, dim3(a.n_traces), dim3(kTauThreads), 0, st, a);
     return hipGetLastError();
+}
+
+} // namespace mcx
+
+// Output staging of the allocation-free tau entry point: one FP64 per slot,
+// kTauScratchSlots at most (512 KiB on each GPU the library runs on, held
+// until the process ends).  Every call on a GPU shares that GPU's copy, so
+// calls that overlap on one GPU race.  Host code never passes `scratch`
+// itself to a kernel or a copy: in host code the name is a stand-in, not a
+// device address.  It uses the per-device address that tau_scratch_address()
+// obtains once per device with hipGetSymbolAddress.
+static __device__ __align__(256) double scratch[mcx::kTauScratchSlots];
+
+namespace mcx {
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+} // namespace
+
+hipError_t tau_scratch_address(int device, double** out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess)
+            return e;
+        if (cur != device && (e = hipSetDevice(device)) != hipSuccess)
+            return e;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device)
+            hipSetDevice(cur);
+        if (e != hipSuccess)
+            return e;
+        g_scratch_ptr[device] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
 }
 
 } // namespace mcx

@@ -116,6 +116,7 @@ SIGNATURES = {
     "mcx_batch_set_sweeps": [_P, _pu64],
     "mcx_batch_set_streams": [_P, _u64, _pu32],
     "mcx_batch_tau_int": [_P, _i32, _i64, _i64, _i32, _i32, _f64, _pf64],
+    "mcx_batch_tau_int_noalloc": [_P, _i32, _i64, _i64, _i32, _i32, _f64, _pf64],
     "mcx_batch_state_size": [_P, _u32, C.POINTER(C.c_size_t)],
     "mcx_batch_save_state": [_P, _u32, _P, C.c_size_t],
     "mcx_batch_load_state": [_P, _P, C.c_size_t],
