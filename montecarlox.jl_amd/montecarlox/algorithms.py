@@ -4,7 +4,7 @@ An algorithm carries its random stream, its ensemble and its counters; the
 GPU library applies it.  Counters are mirrored from the device after each
 device call.
   ImportanceSampling / accept rules   src/algorithms/importance_sampling.jl:23-110
-  Metropolis, Glauber                 src/algorithms/metropolis.jl:173-237
+  Metropolis, Glauber                 src/algorithms/metropolis.jl:63-127
   HeatBath                            src/algorithms/heat_bath.jl:11-17
 """
 from __future__ import annotations
@@ -58,7 +58,7 @@ class AbstractMetropolis(AbstractImportanceSampling):
 
 
 class Metropolis(AbstractMetropolis):
-    """Metropolis(rng; beta) / Metropolis(rng, ensemble) (metropolis.jl:189-206)."""
+    """Metropolis(rng; beta) / Metropolis(rng, ensemble) (metropolis.jl:79-96)."""
 
     def __init__(self, rng: Philox, ensemble=None, beta=None, **kw):
         b2 = kw.pop("β", None)

@@ -333,7 +333,7 @@ def _walkers_of(pc: ParallelChains):
 
 
 def merge_histograms_(pc: ParallelChains):
-    """Sum every chain's histogram into the root chain's (parallel_multicanonical.jl:108-117)."""
+    """Sum every chain's histogram into the root chain's (parallel_multicanonical.jl:38-47)."""
     w, g = _walkers_of(pc)
     if w is None:
         raise MCXArgumentError("merge_histograms! runs on device walkers: run the chains first")
@@ -341,7 +341,7 @@ def merge_histograms_(pc: ParallelChains):
 
 
 def distribute_logweight_(pc: ParallelChains):
-    """Copy the root's log-weights to every chain (parallel_multicanonical.jl:129-138)."""
+    """Copy the root's log-weights to every chain (parallel_multicanonical.jl:59-68)."""
     w, g = _walkers_of(pc)
     if w is None:
         raise MCXArgumentError("distribute_logweight! runs on device walkers: run the chains first")

@@ -2,8 +2,8 @@
 
   ReplicaExchange fields indices/steps/accepted/stage  src/algorithms/replica_exchange.jl:13-19
   update!(rx, xs)                                      replica_exchange.jl:158-178
-  ParallelTempering(betas; seed, rng)                  src/algorithms/parallel_tempering.jl:269-298
-  set_betas(n, bmin, bmax, mode)                       parallel_tempering.jl:391-407
+  ParallelTempering(betas; seed, rng)                  src/algorithms/parallel_tempering.jl:24-53
+  set_betas(n, bmin, bmax, mode)                       parallel_tempering.jl:146-162
   acceptance_rates / acceptance_rate / index / reset!  replica_exchange.jl:48-101
   optimize_exchange_interval!, _retune_exchange_sweeps!
                                                        parallel_tempering.jl:53-135

@@ -1,7 +1,7 @@
 """Continuous-batching job server (libmcx_hip.so's mcx_server_*).
 
 No reference counterpart: the reference runs one chain per Julia thread to
-completion (ParallelChains(ThreadsBackend(n), ...), src/parallel/parallel_chains.jl:99-105).
+completion (ParallelChains(ThreadsBackend(n), ...), src/infrastructure/parallel_chains.jl:99-105).
 Here a scheduler thread inside the library keeps the lattices of many jobs
 resident in a device pool bounded by an HBM budget, advances them together in
 quanta and admits new jobs as soon as finished ones free their space.  A job's
