@@ -267,14 +267,19 @@ typedef struct {
     const int64_t* n_bins;
     const double* pair_beta;       /* MCX_MUCA_PAIR_SPIN2: n_chains betas of the Boltzmann    */
                                    /* pair term; NULL for the other kinds                     */
+    const int32_t* bin_integer;    /* per group: 1 = an integer-typed DiscreteBinning, 0 = a  */
+                                   /* Float64 one; NULL = integer for Ising, Float64 for      */
+                                   /* Blume-Capel                                             */
 } mcx_gen_desc;
 
 /* Replaces Multicanonical(rng, bins) / WangLandau(rng, bins; logf) per chain
  * (src/algorithms/multicanonical.jl:9-25, algorithms/wang_landau.jl:10-17) and
  * ParallelMulticanonical(ThreadsBackend(n), algs) (parallel_multicanonical.jl:21-28)
  * over the chains of a batch.  Each walker owns FP64 log-weights (initially 0)
- * and an int64 histogram.  Ising energies bin with integer div, Blume-Capel
- * (Float64) energies with round-half-even, as _binindex (binned_object.jl:18-24). */
+ * and an int64 histogram.  Bins follow _binindex (binned_object.jl:18-24):
+ * Ising (Int) energies on an integer binning with truncating div, every other
+ * case (Float64 binning, Blume-Capel's Float64 energies) with round-half-even
+ * of (E - start) / step. */
 mcx_status mcx_gen_create(mcx_batch* b, const mcx_gen_desc* desc, mcx_gen** out);
 mcx_status mcx_gen_destroy(mcx_gen* g);
 /* ensemble(alg).logweight.values / .histogram.values of one walker. */
@@ -312,8 +317,52 @@ mcx_status mcx_gen_distribute(mcx_gen* g, int32_t group);
  * the group's walkers. */
 mcx_status mcx_gen_average_logweight(mcx_gen* g, int32_t group);
 /* No reference counterpart (Wang-Landau flatness test): {min, mean, count} of
- * the root histogram over bins reached (h > 0 or lw != 0). */
+ * the group root's histogram over the group's reached bins.  A bin is reached
+ * once any flatness evaluation of the group (this call or mcx_gen_iterate)
+ * has seen it non-zero; the mark persists when histograms are reset, so a bin
+ * the walkers stopped visiting counts as 0.  Bins never visited (forbidden
+ * energies, the untouched edges of the domain) do not count. */
 mcx_status mcx_gen_flatness(mcx_gen* g, int32_t group, double* min_mean_count);
+
+/* The weight iteration as one call, per binning group over its walkers:
+ *  - multicanonical walkers (MCX_MULTICANONICAL, MCX_MUCA_PAIR_SPIN2): each
+ *    update is the loop body of muca_Ising2D.jl:80-91 / muca_BlumeCapel.jl:103-112
+ *    across the group as ParallelMulticanonical does it
+ *    (parallel_chains_threads.jl:180-201): therm_sweeps sweeps, reset!(alg)
+ *    (histogram and counters), `sweeps` sweeps, merge_histograms! into the root
+ *    (parallel_multicanonical.jl:38-47), update! of the root's weights
+ *    (ensembles/multicanonical.jl:32-44), distribute_logweight! (:59-68);
+ *  - Wang-Landau walkers (no reference counterpart for the schedule): each
+ *    update runs `sweeps` sweeps of the group's active walkers (lw[x] -= ln f per
+ *    attempt, algorithms/wang_landau.jl:29-37), then the root takes the mean of
+ *    the group's log-weights and distributes it; the group's summed histogram
+ *    is flat when min >= flatness * mean over its reached bins (as
+ *    mcx_gen_flatness), and then ln f <- power * ln f for the group
+ *    (update!(::WangLandauEnsemble; power), ensembles/wang_landau.jl:23) and
+ *    its histograms are zeroed; a group whose ln f falls below logf_final stops.
+ * The call returns after max_updates updates or when every Wang-Landau group
+ * has stopped. */
+typedef struct {
+    int64_t sweeps;       /* sweeps of every walker per update (>= 1), e.g. 10^4         */
+    int64_t therm_sweeps; /* multicanonical: sweeps before reset! in each update (>= 0) */
+    int64_t max_updates;  /* >= 1                                                       */
+    double flatness;      /* Wang-Landau: (0, 1], e.g. 0.8                              */
+    double logf_final;    /* Wang-Landau: > 0, e.g. 1e-6                                */
+    double power;         /* Wang-Landau: (0, 1), e.g. 0.5                              */
+    uint32_t flags;       /* 0                                                          */
+    int32_t pad;
+} mcx_gen_schedule;
+
+typedef struct {
+    int64_t updates;      /* updates done by the call                                  */
+    int64_t stages;       /* Wang-Landau: ln f reductions done by the call, all groups */
+    int64_t sweeps;       /* sweeps run by the longest-running walker                  */
+    int32_t active;       /* Wang-Landau: groups still above logf_final                */
+    int32_t pad;
+    double device_ms;     /* device time of the sweeps                                 */
+} mcx_gen_progress;
+
+mcx_status mcx_gen_iterate(mcx_gen* g, const mcx_gen_schedule* sched, mcx_gen_progress* out);
 /* alg.steps / alg.accepted of every walker. */
 mcx_status mcx_gen_get_counters(mcx_gen* g, int64_t* steps, int64_t* accepted);
 mcx_status mcx_gen_group_info(mcx_gen* g, int32_t group, int32_t* root, int32_t* count,

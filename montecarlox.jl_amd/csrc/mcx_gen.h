@@ -19,11 +19,12 @@ struct GenArgs {
     const int32_t* items;   // walkers of this launch (all from one binning group)
     int32_t n_items;
     int32_t kind;           // 2 = multicanonical, 3 = Wang-Landau, 5 = pair Boltzmann + muca on sum s^2
-    int32_t model;          // 0 Ising (integer energy, div binning), 1 Blume-Capel (FP64, round)
+    int32_t model;          // 0 Ising (integer energy), 1 Blume-Capel (FP64 energy)
     int32_t dim;
     int32_t record;
     int32_t hist_in_lds;    // 1: uint32 histogram staged in LDS, else int64 atomics in HBM
-    int32_t pad;
+    int32_t int_bins;       // Ising energies on an integer binning: truncating div, else the
+                            // FP64 round-half-even of (E - start) / step (binned_object.jl:18-24)
     double bin_start, bin_step;
     double inv_step;        // 1 / bin_step when step_pow2
     int32_t step_pow2;      // bin_step is a power of two (multiply / shift instead of divide)
@@ -63,8 +64,14 @@ hipError_t launch_gen_average(double* lw, const int64_t* lw_off, const int32_t* 
 hipError_t launch_gen_zero_hist(unsigned long long* hist, const int64_t* lw_off,
                                 const int32_t* walkers, int32_t n_walkers, int64_t nbins,
                                 hipStream_t st);
-hipError_t launch_gen_flatness(const unsigned long long* hist, const double* lw,
-                               const int64_t* lw_off, int32_t root, int64_t nbins,
-                               double* out, hipStream_t st);
+// Histogram statistics of a set of walkers of one group: per bin the sum of
+// their histograms; `reached` (one byte per bin) is OR-ed with sum > 0, and
+// over the reached bins out = {min sum, mean sum, bins} (FP64).  `part` holds
+// kGenStatBlocks * 3 uint64 partials.
+constexpr int kGenStatBlocks = 64;
+hipError_t launch_gen_hist_stats(const unsigned long long* hist, const int64_t* lw_off,
+                                 const int32_t* walkers, int32_t n_walkers, int64_t nbins,
+                                 uint8_t* reached, unsigned long long* part, double* out,
+                                 hipStream_t st);
 
 } // namespace mcx

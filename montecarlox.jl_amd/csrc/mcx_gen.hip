@@ -153,8 +153,9 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
         const double D = cr.D;
 
         // bin index as _binindex evaluates it (binned_object.jl:18-24): integer
-        // energies (Ising) truncating div, FP64 energies round-half-even.  A
-        // power-of-two step divides exactly as a multiply / shift.
+        // energies (Ising) on an integer binning truncating div, everything
+        // else round-half-even of (E - start) / step in FP64.  A power-of-two
+        // step divides exactly as a multiply / shift.
         const bool step_pow2 = a.step_pow2 != 0;
         // (the walk's state is wave-uniform: bins and decisions are moved to
         // scalar registers so that its branches are scalar branches)
@@ -185,10 +186,12 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
         const double Dr = rint(D);
         // (32-bit: energies, offsets from the bin start and D * Q stay below 2^29)
         const double lim = 536870912.0;
+        const bool unit_bins = a.bin_step == 1.0 && rint(a.bin_start) == a.bin_start;
         const bool int_e = fabs(a.bin_start) < lim && 3.0 * (double)N < lim &&
-                           (MODEL == 0 ? step_pow2
-                                       : (Dr == D && fabs(D) * (double)N < lim && a.bin_step == 1.0 &&
-                                          rint(a.bin_start) == a.bin_start));
+                           (MODEL == 0 ? step_pow2 && (a.int_bins || unit_bins)
+                                       : (Dr == D && fabs(D) * (double)N < lim && unit_bins));
+        // Ising energies on an integer binning take the truncating div everywhere
+        const bool div_bins = MODEL == 0 && a.int_bins;
         const long long Di = (long long)Dr;
         // the integer walk (lane 0 walks E; the staging wave adds up each chunk)
         const bool int_walk = int_e && !PS;
@@ -211,10 +214,12 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
         unsigned long long steps = 0, nacc = 0;
         // energy of the cached sums (_cached_energy) and its bin; recomputed
         // only when a move is accepted
-        double Eo = -(double)pair + D * (double)qq;
+        double Eo = MODEL == 0 ? -(double)pair : -(double)pair + D * (double)qq;
         long long Ei = -(long long)pair + (MODEL == 0 ? 0 : Di * (long long)qq);
         int b_old = 0;
-        bool old_ok = PS ? bin_int((long long)qq, b_old) : (int_e ? ibin(Ei, b_old) : bin_fp(Eo, b_old));
+        bool old_ok = PS ? bin_int((long long)qq, b_old)
+                         : (int_e ? ibin(Ei, b_old)
+                                  : (div_bins ? bin_int(Ei, b_old) : bin_fp(Eo, b_old)));
         const double nbeta = PS ? -a.pair_beta[wk] : 0.0;
         // ---- weight window: a chunk of 64 attempts moves the walker by at most
         // 64 * max_dbin bins, so a window that holds `reach` bins on both sides of
@@ -449,7 +454,7 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
             for (int k = 0; k < cnt; ++k) {
                 int b_new;
                 bool okn;
-                if (MODEL == 0)
+                if (div_bins)
                     okn = bin_int(-(long long)pair + __builtin_amdgcn_readfirstlane(st_dE[buf][k]), b_new);
                 else
                     okn = bin_fp(Eo + uniform_f64(st_lul[buf][k]), b_new);
@@ -494,7 +499,10 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                     mm += __builtin_amdgcn_readfirstlane((st_mq[buf][k] & 0xffff) - 8);
                     qq += __builtin_amdgcn_readfirstlane((st_mq[buf][k] >> 16) - 8);
                     if (MODEL == 0) {
+                        // (Ising energies are exact in FP64: the new bin is the
+                        // bin of the new cached energy)
                         b_old = b_new;
+                        Eo = -(double)pair;
                     } else {
                         Eo = -(double)pair + D * (double)qq;
                         old_ok = bin_fp(Eo, b_old);
@@ -577,6 +585,7 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                     t_dpair = -2 * si * nbs;
                     t_dm = -2 * si;
                     t_snew = -si;
+                    t_dEf = (double)t_dE;
                     t_u = (double)w * 0x1.0p-32;
                 } else {
                     const int sn = gen_propose(si, (int)(w & 1u));
@@ -813,25 +822,36 @@ __global__ void k_gen_zero_hist(unsigned long long* hist, const int64_t* lw_off,
         h[b] = 0;
 }
 
-// Flatness of the root histogram over the bins the walk has reached (lw
-// differs from its start or h > 0): out = {min h, mean h, bins counted}.
-__global__ void k_gen_flatness(const unsigned long long* hist, const double* lw,
-                               const int64_t* lw_off, int32_t root, int64_t nbins, double* out)
+// Histogram statistics of a walker set (the group's walkers, or its root
+// alone): per bin the sum of their histograms in 64-bit integers; a bin counts
+// once it has been seen non-zero by any evaluation (`reached`, so that a bin
+// the walkers left stays in the flatness test), and over the counted bins each
+// block writes {min, sum, count} partials.  Integer sums: the result does not
+// depend on the order of the additions.
+__global__ void k_gen_hist_stats(const unsigned long long* hist, const int64_t* lw_off,
+                                 const int32_t* walkers, int32_t n_walkers, int64_t nbins,
+                                 uint8_t* reached, unsigned long long* part)
 {
-    __shared__ double smin[4], ssum[4], scnt[4];
-    const unsigned long long* h = hist + lw_off[root];
-    const double* l = lw + lw_off[root];
-    double mn = 1e300, sm = 0, ct = 0;
-    for (int64_t b = threadIdx.x; b < nbins; b += blockDim.x) {
-        if (h[b] > 0 || l[b] != 0.0) {
-            const double v = (double)h[b];
-            mn = v < mn ? v : mn;
-            sm += v;
+    __shared__ unsigned long long smin[4], ssum[4], scnt[4];
+    unsigned long long mn = ~0ull, sm = 0, ct = 0;
+    for (int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < nbins;
+         b += (int64_t)gridDim.x * blockDim.x) {
+        unsigned long long h = 0;
+        for (int k = 0; k < n_walkers; ++k)
+            h += hist[lw_off[walkers[k]] + b];
+        bool r = reached[b] != 0;
+        if (h > 0 && !r) {
+            reached[b] = 1;
+            r = true;
+        }
+        if (r) {
+            mn = h < mn ? h : mn;
+            sm += h;
             ct += 1;
         }
     }
     for (int off = 32; off >= 1; off >>= 1) {
-        const double o = __shfl_xor(mn, off, 64);
+        const unsigned long long o = __shfl_xor(mn, off, 64);
         mn = o < mn ? o : mn;
         sm += __shfl_xor(sm, off, 64);
         ct += __shfl_xor(ct, off, 64);
@@ -849,10 +869,26 @@ __global__ void k_gen_flatness(const unsigned long long* hist, const double* lw,
             sm += ssum[k];
             ct += scnt[k];
         }
-        out[0] = ct > 0 ? mn : 0.0;
-        out[1] = ct > 0 ? sm / ct : 0.0;
-        out[2] = ct;
+        part[3 * blockIdx.x] = mn;
+        part[3 * blockIdx.x + 1] = sm;
+        part[3 * blockIdx.x + 2] = ct;
     }
+}
+
+// out = {min, mean, bins} over the counted bins (0, 0, 0 when none)
+__global__ void k_gen_hist_stats_final(const unsigned long long* part, int n_part, double* out)
+{
+    if (threadIdx.x != 0)
+        return;
+    unsigned long long mn = ~0ull, sm = 0, ct = 0;
+    for (int k = 0; k < n_part; ++k) {
+        mn = part[3 * k] < mn ? part[3 * k] : mn;
+        sm += part[3 * k + 1];
+        ct += part[3 * k + 2];
+    }
+    out[0] = ct > 0 ? (double)mn : 0.0;
+    out[1] = ct > 0 ? (double)sm / (double)ct : 0.0;
+    out[2] = (double)ct;
 }
 
 static dim3 bins_grid(int64_t nbins, int y)
@@ -904,12 +940,16 @@ hipError_t launch_gen_zero_hist(unsigned long long* hist, const int64_t* lw_off,
     return hipGetLastError();
 }
 
-hipError_t launch_gen_flatness(const unsigned long long* hist, const double* lw,
-                               const int64_t* lw_off, int32_t root, int64_t nbins, double* out,
-                               hipStream_t st)
+hipError_t launch_gen_hist_stats(const unsigned long long* hist, const int64_t* lw_off,
+                                 const int32_t* walkers, int32_t n_walkers, int64_t nbins,
+                                 uint8_t* reached, unsigned long long* part, double* out,
+                                 hipStream_t st)
 {
-    hipLaunchKernelGGL(k_gen_flatness, dim3(1), dim3(256), 0, st, hist, lw, lw_off, root, nbins,
-                       out);
+    int64_t blocks = (nbins + 255) / 256;
+    blocks = blocks < 1 ? 1 : (blocks > kGenStatBlocks ? kGenStatBlocks : blocks);
+    hipLaunchKernelGGL(k_gen_hist_stats, dim3((unsigned)blocks), dim3(256), 0, st, hist, lw_off,
+                       walkers, n_walkers, nbins, reached, part);
+    hipLaunchKernelGGL(k_gen_hist_stats_final, dim3(1), dim3(64), 0, st, part, (int)blocks, out);
     return hipGetLastError();
 }
 

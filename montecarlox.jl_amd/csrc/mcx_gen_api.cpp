@@ -16,6 +16,8 @@ struct mcx_gen {
     std::vector<int32_t> group_of;
     std::vector<double> start, step;
     std::vector<int64_t> nbins, off, member_off;
+    std::vector<int32_t> int_bins; // per group: integer-typed binning (Ising: truncating div)
+    std::vector<int64_t> roff;     // per group: offset of its reached-bin mask
     std::vector<std::vector<int32_t>> members; // per group; members[g][0] is the root
     double* d_lw = nullptr;
     unsigned long long* d_hist = nullptr;
@@ -25,7 +27,10 @@ struct mcx_gen {
     unsigned long long* d_acc = nullptr;
     int64_t* d_err = nullptr;
     int32_t* d_members = nullptr;
-    double* d_flat = nullptr;
+    uint8_t* d_reached = nullptr;         // per group: bins seen non-zero by a flatness test
+    unsigned long long* d_part = nullptr; // per group: kGenStatBlocks * 3 partials
+    double* d_stats = nullptr;            // per group: {min, mean, bins}
+    int64_t rtot = 0;
     double* d_pbeta = nullptr; // kind MCX_MUCA_PAIR_SPIN2: beta of each walker's pair term
     std::vector<double> logf;
     std::vector<double> pbeta;
@@ -56,7 +61,9 @@ void gen_free(mcx_gen* g)
     hipFree(g->d_acc);
     hipFree(g->d_err);
     hipFree(g->d_members);
-    hipFree(g->d_flat);
+    hipFree(g->d_reached);
+    hipFree(g->d_part);
+    hipFree(g->d_stats);
     hipFree(g->d_pbeta);
 }
 
@@ -157,6 +164,10 @@ extern "C" mcx_status mcx_gen_create(mcx_batch* b, const mcx_gen_desc* d, mcx_ge
     g->start.assign(d->bin_start, d->bin_start + d->n_groups);
     g->step.assign(d->bin_step, d->bin_step + d->n_groups);
     g->nbins.assign(d->n_bins, d->n_bins + d->n_groups);
+    g->int_bins.assign(d->n_groups, b->model == MCX_ISING ? 1 : 0);
+    if (d->bin_integer)
+        for (int k = 0; k < d->n_groups; ++k)
+            g->int_bins[k] = d->bin_integer[k] ? 1 : 0;
     g->group_of.resize(b->n);
     g->members.resize(d->n_groups);
     for (int i = 0; i < b->n; ++i) {
@@ -178,11 +189,11 @@ extern "C" mcx_status mcx_gen_create(mcx_batch* b, const mcx_gen_desc* d, mcx_ge
             delete g;
             return fail(MCX_EINVAL, "mcx_gen_create: group %d: invalid binning", k);
         }
-        if (b->model == MCX_ISING &&
+        if (b->model == MCX_ISING && g->int_bins[k] &&
             (g->start[k] != std::floor(g->start[k]) || g->step[k] != std::floor(g->step[k]))) {
             delete g;
-            return fail(MCX_EINVAL, "mcx_gen_create: group %d: Ising energies are integers; "
-                                    "start and step must be integral", k);
+            return fail(MCX_EINVAL, "mcx_gen_create: group %d: an integer binning needs an "
+                                    "integral start and step", k);
         }
         const int L0 = b->rec[g->members[k][0]].L;
         for (int c : g->members[k])
@@ -212,6 +223,10 @@ extern "C" mcx_status mcx_gen_create(mcx_batch* b, const mcx_gen_desc* d, mcx_ge
         tot += (g->nbins[g->group_of[i]] + 31) / 32 * 32;
     }
     g->tot = tot;
+    for (int k = 0; k < d->n_groups; ++k) {
+        g->roff.push_back(g->rtot);
+        g->rtot += (g->nbins[k] + 255) / 256 * 256;
+    }
     std::vector<int32_t> flat;
     for (int k = 0; k < g->n_groups; ++k) {
         g->member_off.push_back((int64_t)flat.size());
@@ -235,7 +250,9 @@ extern "C" mcx_status mcx_gen_create(mcx_batch* b, const mcx_gen_desc* d, mcx_ge
     alloc((void**)&g->d_acc, sizeof(unsigned long long) * b->n);
     alloc((void**)&g->d_err, sizeof(int64_t) * b->n);
     alloc((void**)&g->d_members, sizeof(int32_t) * b->n);
-    alloc((void**)&g->d_flat, sizeof(double) * 3);
+    alloc((void**)&g->d_reached, (size_t)g->rtot);
+    alloc((void**)&g->d_part, sizeof(unsigned long long) * 3 * kGenStatBlocks * d->n_groups);
+    alloc((void**)&g->d_stats, sizeof(double) * 3 * d->n_groups);
     alloc((void**)&g->d_pbeta, sizeof(double) * b->n);
     for (int k = 0; k < mcx_gen::kStreams && e == hipSuccess; ++k) {
         e = hipStreamCreateWithFlags(&g->gs[k], hipStreamNonBlocking);
@@ -249,6 +266,8 @@ extern "C" mcx_status mcx_gen_create(mcx_batch* b, const mcx_gen_desc* d, mcx_ge
         e = hipMemsetAsync(g->d_lw, 0, sizeof(double) * tot, st);
     if (e == hipSuccess)
         e = hipMemsetAsync(g->d_hist, 0, sizeof(unsigned long long) * tot, st);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(g->d_reached, 0, (size_t)g->rtot, st);
     if (e == hipSuccess)
         e = hipMemsetAsync(g->d_steps, 0, sizeof(unsigned long long) * b->n, st);
     if (e == hipSuccess)
@@ -356,35 +375,32 @@ extern "C" mcx_status mcx_gen_get_logf(mcx_gen* g, double* out)
     return MCX_OK;
 }
 
-extern "C" mcx_status mcx_gen_run(mcx_gen* g, int64_t nsweeps, uint32_t flags)
+namespace {
+
+// nsweeps sweeps of the walkers of every group with active[k] (nullptr: all);
+// adds the device time of the call to *ms when ms is set
+mcx_status gen_run(mcx_gen* g, int64_t nsweeps, uint32_t flags, const std::vector<char>* active,
+                   const char* who, double* ms_acc = nullptr)
 {
-    if (!g)
-        return fail(MCX_EINVAL, "mcx_gen_run: NULL handle");
     mcx_batch* b = g->b;
-    if (nsweeps < 0)
-        return fail(MCX_EINVAL, "mcx_gen_run: nsweeps must be >= 0");
-    if (flags & ~MCX_RUN_RECORD)
-        return fail(MCX_EINVAL, "mcx_gen_run: only MCX_RUN_RECORD is supported");
     const bool record = (flags & MCX_RUN_RECORD) != 0;
-    if (record && b->rec_len + nsweeps > b->cap)
-        return fail(MCX_EINVAL, "mcx_gen_run: recording %lld sweeps exceeds the series capacity",
-                    (long long)nsweeps);
     HIPCHK(hipSetDevice(b->ctx->device));
     hipStream_t st = b->ctx->stream;
     HIPCHK(hipMemsetAsync(g->d_err, 0xff, sizeof(int64_t) * b->n, st));
     HIPCHK(hipEventRecord(b->ctx->ev0, st));
     // groups by decreasing lattice size: the longest walks start first
-    std::vector<int> order(g->n_groups);
+    std::vector<int> order;
     for (int k = 0; k < g->n_groups; ++k)
-        order[k] = k;
+        if (!active || (*active)[k])
+            order.push_back(k);
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
         return b->nsites[g->members[x][0]] > b->nsites[g->members[y][0]];
     });
-    const int nstr = std::min(g->n_groups, (int)mcx_gen::kStreams);
+    const int nstr = std::min((int)order.size(), (int)mcx_gen::kStreams);
     HIPCHK(hipEventRecord(g->fork, st));
     for (int q = 0; q < nstr; ++q)
         HIPCHK(hipStreamWaitEvent(g->gs[q], g->fork, 0));
-    for (int oi = 0; oi < g->n_groups; ++oi) {
+    for (size_t oi = 0; oi < order.size(); ++oi) {
         const int k = order[oi];
         GenArgs a{};
         a.items = g->d_members + g->member_off[k];
@@ -393,6 +409,7 @@ extern "C" mcx_status mcx_gen_run(mcx_gen* g, int64_t nsweeps, uint32_t flags)
         a.model = b->model;
         a.dim = b->dim;
         a.record = record ? 1 : 0;
+        a.int_bins = g->int_bins[k];
         const int64_t ns = b->nsites[g->members[k][0]];
         const GenWindow gw = gen_window(g, k);
         a.win = (int32_t)gw.win;
@@ -400,6 +417,9 @@ extern "C" mcx_status mcx_gen_run(mcx_gen* g, int64_t nsweeps, uint32_t flags)
         // a windowed walker counts visits in HBM; a whole table may count in LDS
         a.hist_in_lds = gw.win == g->nbins[k] &&
                         gen_lds(gw.win, g->nbins[k], ns, true) + kGenStaticLds <= b->ctx->lds_max ? 1 : 0;
+        // the uint32 LDS counts of one launch hold at most nsweeps * N visits per bin
+        if ((double)nsweeps * (double)ns >= 4294967296.0)
+            a.hist_in_lds = 0;
         a.bin_start = g->start[k];
         a.bin_step = g->step[k];
         {
@@ -435,12 +455,14 @@ extern "C" mcx_status mcx_gen_run(mcx_gen* g, int64_t nsweeps, uint32_t flags)
         HIPCHK(hipStreamWaitEvent(st, g->join[q], 0));
     }
     HIPCHK(hipEventRecord(b->ctx->ev1, st));
-    mcx_status s = sync_check(b, "mcx_gen_run");
+    mcx_status s = sync_check(b, who);
     if (s)
         return s;
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1));
     b->last_ms = ms;
+    if (ms_acc)
+        *ms_acc += ms;
     s = pull_chains(b);
     if (s)
         return s;
@@ -450,9 +472,26 @@ extern "C" mcx_status mcx_gen_run(mcx_gen* g, int64_t nsweeps, uint32_t flags)
     HIPCHK(hipMemcpy(err.data(), g->d_err, sizeof(int64_t) * b->n, hipMemcpyDeviceToHost));
     for (int i = 0; i < b->n; ++i)
         if (err[i] >= 0)
-            return fail(MCX_EBOUNDS, "mcx_gen_run: chain %d: the energy of the attempt at site %lld "
-                                     "lies outside the binned domain", i, (long long)err[i]);
+            return fail(MCX_EBOUNDS, "%s: chain %d: the energy of the attempt at site %lld "
+                                     "lies outside the binned domain", who, i, (long long)err[i]);
     return MCX_OK;
+}
+
+} // namespace
+
+extern "C" mcx_status mcx_gen_run(mcx_gen* g, int64_t nsweeps, uint32_t flags)
+{
+    if (!g)
+        return fail(MCX_EINVAL, "mcx_gen_run: NULL handle");
+    mcx_batch* b = g->b;
+    if (nsweeps < 0)
+        return fail(MCX_EINVAL, "mcx_gen_run: nsweeps must be >= 0");
+    if (flags & ~MCX_RUN_RECORD)
+        return fail(MCX_EINVAL, "mcx_gen_run: only MCX_RUN_RECORD is supported");
+    if ((flags & MCX_RUN_RECORD) && b->rec_len + nsweeps > b->cap)
+        return fail(MCX_EINVAL, "mcx_gen_run: recording %lld sweeps exceeds the series capacity",
+                    (long long)nsweeps);
+    return gen_run(g, nsweeps, flags, nullptr, "mcx_gen_run");
 }
 
 namespace {
@@ -582,6 +621,20 @@ extern "C" mcx_status mcx_gen_average_logweight(mcx_gen* g, int32_t group)
     return sync_check(g->b, "mcx_gen_average_logweight");
 }
 
+namespace {
+
+// {min, mean, bins} of the summed histograms of n walkers of group k (the
+// first n of its member list) over the group's reached bins, into
+// d_stats[3k..3k+2] (device); the reached mask is updated
+hipError_t group_stats_async(mcx_gen* g, int k, int n, hipStream_t st)
+{
+    return launch_gen_hist_stats(g->d_hist, g->d_off, g->d_members + g->member_off[k], n,
+                                 g->nbins[k], g->d_reached + g->roff[k],
+                                 g->d_part + 3 * kGenStatBlocks * k, g->d_stats + 3 * k, st);
+}
+
+} // namespace
+
 extern "C" mcx_status mcx_gen_flatness(mcx_gen* g, int32_t group, double* out3)
 {
     mcx_status s = check_group(g, group, "mcx_gen_flatness");
@@ -591,10 +644,111 @@ extern "C" mcx_status mcx_gen_flatness(mcx_gen* g, int32_t group, double* out3)
         return fail(MCX_EINVAL, "mcx_gen_flatness: NULL output");
     HIPCHK(hipSetDevice(g->b->ctx->device));
     hipStream_t st = g->b->ctx->stream;
-    HIPCHK(launch_gen_flatness(g->d_hist, g->d_lw, g->d_off, g->members[group][0], g->nbins[group],
-                               g->d_flat, st));
-    HIPCHK(hipMemcpyAsync(out3, g->d_flat, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
+    HIPCHK(group_stats_async(g, group, 1, st)); // the root alone
+    HIPCHK(hipMemcpyAsync(out3, g->d_stats + 3 * group, sizeof(double) * 3, hipMemcpyDeviceToHost,
+                          st));
     return sync_check(g->b, "mcx_gen_flatness");
+}
+
+extern "C" mcx_status mcx_gen_iterate(mcx_gen* g, const mcx_gen_schedule* sc, mcx_gen_progress* out)
+{
+    const char* who = "mcx_gen_iterate";
+    if (!g || !sc)
+        return fail(MCX_EINVAL, "%s: NULL argument", who);
+    if (sc->sweeps < 1 || sc->therm_sweeps < 0 || sc->max_updates < 1 || sc->flags != 0)
+        return fail(MCX_EINVAL, "%s: need sweeps >= 1, therm_sweeps >= 0, max_updates >= 1 and "
+                                "flags == 0", who);
+    const bool wl = g->kind == MCX_WANGLANDAU;
+    if (wl && (!(sc->flatness > 0 && sc->flatness <= 1) || !(sc->logf_final > 0) ||
+               !(sc->power > 0 && sc->power < 1)))
+        return fail(MCX_EINVAL, "%s: Wang-Landau needs 0 < flatness <= 1, logf_final > 0 and "
+                                "0 < power < 1", who);
+    mcx_gen_progress pr{};
+    mcx_batch* b = g->b;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    // a Wang-Landau group runs while its root's ln f is not below logf_final
+    std::vector<char> active(g->n_groups, 1);
+    auto count_active = [&]() {
+        int n = 0;
+        for (int k = 0; k < g->n_groups; ++k) {
+            if (wl && !(g->logf[g->members[k][0]] >= sc->logf_final))
+                active[k] = 0;
+            n += active[k] ? 1 : 0;
+        }
+        return n;
+    };
+    mcx_status s = MCX_OK;
+    std::vector<double> stats(3 * (size_t)g->n_groups);
+    while (pr.updates < sc->max_updates && count_active() > 0) {
+        if (!wl) {
+            // ParallelMulticanonical: thermalise, reset!, record, merge, update!, distribute
+            if (sc->therm_sweeps > 0 &&
+                (s = gen_run(g, sc->therm_sweeps, 0, &active, who, &pr.device_ms)) != MCX_OK)
+                return s;
+            for (int k = 0; k < g->n_groups; ++k)
+                HIPCHK(launch_gen_zero_hist(g->d_hist, g->d_off, g->d_members + g->member_off[k],
+                                            (int32_t)g->members[k].size(), g->nbins[k], st));
+            HIPCHK(hipMemsetAsync(g->d_steps, 0, sizeof(unsigned long long) * b->n, st));
+            HIPCHK(hipMemsetAsync(g->d_acc, 0, sizeof(unsigned long long) * b->n, st));
+            if ((s = gen_run(g, sc->sweeps, 0, &active, who, &pr.device_ms)) != MCX_OK)
+                return s;
+            for (int k = 0; k < g->n_groups; ++k) {
+                const int32_t* m = g->d_members + g->member_off[k];
+                const int32_t n = (int32_t)g->members[k].size();
+                HIPCHK(launch_gen_merge(g->d_hist, g->d_off, m, n, g->nbins[k], st));
+                HIPCHK(launch_gen_update(g->d_lw, g->d_hist, g->d_off, m, 1, g->nbins[k], st));
+                HIPCHK(launch_gen_distribute(g->d_lw, g->d_off, m, n, g->nbins[k], st));
+            }
+            if ((s = sync_check(b, who)) != MCX_OK)
+                return s;
+            pr.sweeps += sc->therm_sweeps + sc->sweeps;
+            ++pr.updates;
+            continue;
+        }
+        // Wang-Landau: sweep the active groups, share the weights, test flatness
+        if ((s = gen_run(g, sc->sweeps, 0, &active, who, &pr.device_ms)) != MCX_OK)
+            return s;
+        pr.sweeps += sc->sweeps;
+        ++pr.updates;
+        for (int k = 0; k < g->n_groups; ++k) {
+            if (!active[k])
+                continue;
+            const int32_t* m = g->d_members + g->member_off[k];
+            const int32_t n = (int32_t)g->members[k].size();
+            HIPCHK(launch_gen_average(g->d_lw, g->d_off, m, n, g->nbins[k], st));
+            HIPCHK(launch_gen_distribute(g->d_lw, g->d_off, m, n, g->nbins[k], st));
+            HIPCHK(group_stats_async(g, k, n, st));
+        }
+        HIPCHK(hipMemcpyAsync(stats.data(), g->d_stats, sizeof(double) * stats.size(),
+                              hipMemcpyDeviceToHost, st));
+        if ((s = sync_check(b, who)) != MCX_OK)
+            return s;
+        bool logf_changed = false;
+        for (int k = 0; k < g->n_groups; ++k) {
+            if (!active[k])
+                continue;
+            const double mn = stats[3 * k], mean = stats[3 * k + 1], cnt = stats[3 * k + 2];
+            if (!(cnt > 0 && mn >= sc->flatness * mean))
+                continue;
+            const double lf = g->logf[g->members[k][0]] * sc->power;
+            for (int c : g->members[k])
+                g->logf[c] = lf;
+            logf_changed = true;
+            ++pr.stages;
+            HIPCHK(launch_gen_zero_hist(g->d_hist, g->d_off, g->d_members + g->member_off[k],
+                                        (int32_t)g->members[k].size(), g->nbins[k], st));
+        }
+        if (logf_changed)
+            HIPCHK(hipMemcpyAsync(g->d_logf, g->logf.data(), sizeof(double) * b->n,
+                                  hipMemcpyHostToDevice, st));
+        if ((s = sync_check(b, who)) != MCX_OK)
+            return s;
+    }
+    pr.active = count_active();
+    if (out)
+        *out = pr;
+    return MCX_OK;
 }
 
 extern "C" mcx_status mcx_gen_get_counters(mcx_gen* g, int64_t* steps, int64_t* accepted)
@@ -672,16 +826,21 @@ extern "C" mcx_status mcx_muca_update_values(mcx_ctx* ctx, double* lw, const dou
 
 // ---------------------------------------------------------------------------
 // device-state snapshot (checkpoint / resume): log-weights, histograms, ln f
-// and counters of every walker; restores into walkers of the same layout.
+// and counters of every walker and the groups' reached-bin masks; restores
+// into walkers of the same layout.
 // ---------------------------------------------------------------------------
 
 namespace {
+
+// 2: the groups' reached-bin masks follow the counters
+constexpr uint32_t kGenStateVersion = 2;
 
 size_t gen_state_bytes(const mcx_gen* g)
 {
     const size_t n = (size_t)g->b->n, t = (size_t)g->tot;
     return sizeof(StateHeader) + sizeof(int64_t) * n + sizeof(double) * t +
-           sizeof(unsigned long long) * t + sizeof(double) * n + sizeof(unsigned long long) * 2 * n;
+           sizeof(unsigned long long) * t + sizeof(double) * n + sizeof(unsigned long long) * 2 * n +
+           (size_t)g->rtot;
 }
 
 } // namespace
@@ -705,6 +864,7 @@ extern "C" mcx_status mcx_gen_save_state(mcx_gen* g, void* buf, size_t bytes)
     HIPCHK(hipSetDevice(b->ctx->device));
     const size_t n = (size_t)b->n, t = (size_t)g->tot;
     StateHeader h = make_state_header(3, 0);
+    h.version = kGenStateVersion;
     h.shape[0] = g->kind;
     h.shape[1] = b->n;
     h.shape[2] = g->n_groups;
@@ -721,6 +881,7 @@ extern "C" mcx_status mcx_gen_save_state(mcx_gen* g, void* buf, size_t bytes)
     HIPCHK(d2h(g->d_logf, sizeof(double) * n));
     HIPCHK(d2h(g->d_steps, sizeof(unsigned long long) * n));
     HIPCHK(d2h(g->d_acc, sizeof(unsigned long long) * n));
+    HIPCHK(d2h(g->d_reached, (size_t)g->rtot));
     return sync_check(b, "mcx_gen_save_state");
 }
 
@@ -730,7 +891,8 @@ extern "C" mcx_status mcx_gen_load_state(mcx_gen* g, const void* buf, size_t byt
         return fail(MCX_EINVAL, "mcx_gen_load_state: NULL argument");
     BlobReader r(buf, bytes);
     StateHeader h;
-    if (!r.get(&h, sizeof h) || std::memcmp(h.magic, "MCXS", 4) != 0 || h.version != 1 || h.kind != 3)
+    if (!r.get(&h, sizeof h) || std::memcmp(h.magic, "MCXS", 4) != 0 ||
+        h.version != kGenStateVersion || h.kind != 3)
         return fail(MCX_EINVAL, "mcx_gen_load_state: not a walker snapshot of this library");
     mcx_batch* b = g->b;
     if (h.shape[0] != g->kind || h.shape[1] != b->n || h.shape[2] != g->n_groups ||
@@ -753,5 +915,6 @@ extern "C" mcx_status mcx_gen_load_state(mcx_gen* g, const void* buf, size_t byt
     HIPCHK(hipMemcpyAsync(g->d_logf, g->logf.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
     HIPCHK(h2d(g->d_steps, sizeof(unsigned long long) * n));
     HIPCHK(h2d(g->d_acc, sizeof(unsigned long long) * n));
+    HIPCHK(h2d(g->d_reached, (size_t)g->rtot));
     return sync_check(b, "mcx_gen_load_state");
 }

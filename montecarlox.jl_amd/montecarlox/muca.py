@@ -32,7 +32,18 @@ class GenDesc(C.Structure):
     _fields_ = [("kind", C.c_int32), ("n_groups", C.c_int32),
                 ("group_of_chain", C.POINTER(C.c_int32)), ("bin_start", C.POINTER(C.c_double)),
                 ("bin_step", C.POINTER(C.c_double)), ("n_bins", C.POINTER(C.c_int64)),
-                ("pair_beta", C.POINTER(C.c_double))]
+                ("pair_beta", C.POINTER(C.c_double)), ("bin_integer", C.POINTER(C.c_int32))]
+
+
+class GenSchedule(C.Structure):
+    _fields_ = [("sweeps", C.c_int64), ("therm_sweeps", C.c_int64), ("max_updates", C.c_int64),
+                ("flatness", C.c_double), ("logf_final", C.c_double), ("power", C.c_double),
+                ("flags", C.c_uint32), ("pad", C.c_int32)]
+
+
+class GenProgress(C.Structure):
+    _fields_ = [("updates", C.c_int64), ("stages", C.c_int64), ("sweeps", C.c_int64),
+                ("active", C.c_int32), ("pad", C.c_int32), ("device_ms", C.c_double)]
 
 
 _P = C.c_void_p
@@ -61,6 +72,7 @@ for _n, _a in {
     "mcx_gen_state_size": [_P, C.POINTER(C.c_size_t)],
     "mcx_gen_save_state": [_P, _P, C.c_size_t],
     "mcx_gen_load_state": [_P, _P, C.c_size_t],
+    "mcx_gen_iterate": [_P, C.POINTER(GenSchedule), C.POINTER(GenProgress)],
 }.items():
     _lib.declare(_n, _a)
 
@@ -92,10 +104,11 @@ def _binned(a):
 
 
 def _binning(ens):
+    """(start, step, num, integer-typed) of the ensemble's DiscreteBinning."""
     b = ens.logweight.bins
     if not isinstance(b, DiscreteBinning):
         raise MCXArgumentError("device walkers need discrete (start, step, num) binning")
-    return (float(b.start), float(b.step), int(b.num))
+    return (float(b.start), float(b.step), int(b.num), bool(b.integer))
 
 
 class GeneralizedWalkers:
@@ -140,6 +153,7 @@ class GeneralizedWalkers:
         start = np.array([k[0] for k in keys])
         step = np.array([k[1] for k in keys])
         nb = np.array([k[2] for k in keys], np.int64)
+        integer = np.array([1 if k[3] else 0 for k in keys], np.int32)
         gof = np.array(self.group_of, np.int32)
         systems.bind_streams([a.rng for a in algs])
         pb = None
@@ -147,7 +161,7 @@ class GeneralizedWalkers:
             pb = np.array([float(a.ensemble.pair.beta) for a in algs], np.float64)
         d = GenDesc(self.kind, len(keys), _ptr(gof, C.c_int32), _ptr(start, C.c_double),
                     _ptr(step, C.c_double), _ptr(nb, C.c_int64),
-                    _ptr(pb, C.c_double) if pb is not None else None)
+                    _ptr(pb, C.c_double) if pb is not None else None, _ptr(integer, C.c_int32))
         self._h = C.c_void_p()
         call("mcx_gen_create", systems.batch.handle, C.byref(d), C.byref(self._h))
         self._last_lw = [None] * len(algs)
@@ -246,11 +260,40 @@ class GeneralizedWalkers:
         self.sync_out()
 
     def flatness(self, group: int = 0):
-        """(min, mean, bins) of the group root's histogram over reached bins."""
+        """(min, mean, bins) of the group root's histogram over the group's
+        reached bins (every bin any flatness evaluation has seen non-zero)."""
         self.sync_in()
         out = np.empty(3, np.float64)
         call("mcx_gen_flatness", self._h, int(group), _ptr(out, C.c_double))
         return float(out[0]), float(out[1]), int(out[2])
+
+    def iterate_(self, sweeps: int, max_updates: int, therm_sweeps: int = 0,
+                 flatness: float = 0.8, logf_final: float = 1e-6, power: float = 0.5) -> dict:
+        """The weight iteration on the device (mcx_gen_iterate), per binning
+        group.  Multicanonical walkers: per update, therm_sweeps sweeps,
+        reset!, `sweeps` sweeps, merge_histograms!, update! of the root,
+        distribute_logweight! (the ParallelMulticanonical loop of
+        parallel_chains_threads.jl:180-201).  Wang-Landau walkers: per update,
+        `sweeps` sweeps, the group's mean log-weight to every walker, and when
+        the summed histogram is flat (min >= flatness * mean over the reached
+        bins) ln f <- power * ln f and the histograms start again; a group stops
+        below logf_final.  Returns the progress counters."""
+        self.sync_in()
+        sc = GenSchedule(int(sweeps), int(therm_sweeps), int(max_updates), float(flatness),
+                         float(logf_final), float(power), 0, 0)
+        pr = GenProgress()
+        try:
+            call("mcx_gen_iterate", self._h, C.byref(sc), C.byref(pr))
+        finally:
+            self.sync_out()
+            if self.kind == _lib.WANGLANDAU:
+                lf = np.empty(len(self.algs), np.float64)
+                call("mcx_gen_get_logf", self._h, _ptr(lf, C.c_double))
+                for k, a in enumerate(self.algs):
+                    a.ensemble.logf = float(lf[k])
+                    self._last_logf[k] = a.ensemble.logf
+        return {"updates": pr.updates, "stages": pr.stages, "sweeps": pr.sweeps,
+                "active_groups": pr.active, "device_ms": pr.device_ms}
 
     def save_state(self) -> np.ndarray:
         """Log-weights, histograms, ln f and counters of every walker (uint8);

@@ -13,6 +13,11 @@ Pinning status of each function (what checks it besides the GPU parity tests):
   generalized_sweeps   reference muca/WL KATs (flat weights accept all, histogram
                        total == steps, WL decrement), flat-histogram convergence
                        to the Beale DOS at L=8
+  wl_iterate_group     (the Wang-Landau flatness schedule has no reference
+                       counterpart) convergence to the exact density of states:
+                       brute-force Ising 4x4 and Blume-Capel 4x4, Beale Ising 8x8
+  muca_iterate_update  the reference's merge / update! / distribute KATs and the
+                       convergence of the iterated weights to the exact 4x4 DOS
   pt_round             reference exchange KATs (lr = 2.5, u=0 accepts, u=1 rejects)
   refpath_ising2d      timing only (CPU baseline); detailed-balance statistics
 """
@@ -89,7 +94,8 @@ def _declare(L):
                                        C.c_uint64, C.c_uint32, C.c_uint64, C.c_int64, _i8p,
                                        n32, n32, n32, C.POINTER(C.c_int64)]
     L.orc_generalized_sweeps.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
-                                         C.c_double, C.c_double, C.c_int64, _f64p, _i64p, C.c_double,
+                                         C.c_double, C.c_double, C.c_int64, C.c_int, _f64p, _i64p,
+                                         C.c_double,
                                          C.c_uint64, C.c_uint32, C.c_uint64, C.c_int64, _i8p,
                                          C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                          n32, n32, n32, C.POINTER(C.c_int64)]
@@ -191,15 +197,20 @@ class BoundsError(IndexError):
 
 
 def generalized_sweeps(model, wl, dim, L, lw, hist, spins, seed, chain_key, sweep0, nsweeps,
-                       bin_start, bin_step, logf=0.0, D=0.0):
+                       bin_start, bin_step, logf=0.0, D=0.0, int_bins=None):
     """Sequential multicanonical (wl=0) / Wang-Landau (wl=1) sweeps, in place on
-    spins, lw (float64) and hist (int64).  Returns (E_pair, M, Q, steps, accepted)."""
+    spins, lw (float64) and hist (int64).  int_bins: Ising energies on an
+    integer-typed binning (truncating div; default for Ising), else rounded.
+    Returns (E_pair, M, Q, steps, accepted)."""
+    if int_bins is None:
+        int_bins = model == 0
     E = np.zeros(nsweeps, np.int32)
     M = np.zeros(nsweeps, np.int32)
     Q = np.zeros(nsweeps, np.int32)
     steps, acc, err = C.c_int64(0), C.c_int64(0), C.c_int64(-1)
     rc = lib().orc_generalized_sweeps(model, wl, dim, L, D, float(bin_start), float(bin_step),
-                                      len(lw), lw, hist, logf, seed, chain_key, sweep0, nsweeps,
+                                      len(lw), 1 if int_bins else 0, lw, hist, logf, seed,
+                                      chain_key, sweep0, nsweeps,
                                       spins, C.byref(steps), C.byref(acc), E, M, Q, C.byref(err))
     if rc == 2:
         raise BoundsError(f"energy bin out of range at site {err.value}")
@@ -309,3 +320,89 @@ def tau_int_times(traces, ys=None, D=None, absx=False, min_points=2, max_lag=Non
         out[k] = tau_int(x, None if ys is None else ys[k], 0.0 if D is None else D[k], absx,
                          cap, c)
     return out
+
+
+def _gen_group_stats(hists, reached):
+    """{min, mean, bins} of the summed histograms over the reached bins, the
+    mask OR-ed with sum > 0 first (the flatness rule of the walker schedule)."""
+    h = np.zeros_like(hists[0])
+    for x in hists:
+        h = h + x
+    reached |= h > 0
+    if not reached.any():
+        return 0.0, 0.0, 0
+    hr = h[reached]
+    return float(hr.min()), float(hr.sum()) / float(hr.size), int(hr.size)
+
+
+def wl_iterate_group(model, dim, L, spins, lw, hists, logf, reached, seed, keys, sweep0,
+                     bin_start, bin_step, sweeps, max_updates, flatness, logf_final, power,
+                     D=0.0, int_bins=None):
+    """The Wang-Landau schedule of one binning group of walkers, restated
+    sequentially (test infrastructure): per update, `sweeps` sweeps of every
+    walker (orc_generalized_sweeps, wl = 1, ln f = logf), the walkers' mean
+    log-weight summed in walker order and copied to every walker, and, when the
+    summed histogram is flat over the reached bins (min >= flatness * mean),
+    logf *= power and the histograms are zeroed; stops once logf < logf_final
+    or after max_updates updates.  update!(::WangLandauEnsemble; power) is
+    ensembles/wang_landau.jl:23; the accept rule algorithms/wang_landau.jl:29-37.
+    spins / lw / hists (lists, one per walker) and reached are updated in
+    place.  Returns (logf, sweep counter, updates, stages, steps, accepted)."""
+    n = len(spins)
+    updates = stages = 0
+    steps = [0] * n
+    acc = [0] * n
+    sweep = sweep0
+    while updates < max_updates and logf >= logf_final:
+        for k in range(n):
+            _, _, _, st, ac = generalized_sweeps(model, 1, dim, L, lw[k], hists[k], spins[k], seed,
+                                                 keys[k], sweep, sweeps, bin_start, bin_step,
+                                                 logf=logf, D=D, int_bins=int_bins)
+            steps[k] += st
+            acc[k] += ac
+        sweep += sweeps
+        updates += 1
+        mean = np.zeros_like(lw[0])
+        for k in range(n):
+            mean = mean + lw[k]
+        mean = mean / float(n)
+        for k in range(n):
+            lw[k][:] = mean
+        mn, avg, cnt = _gen_group_stats(hists, reached)
+        if cnt > 0 and mn >= flatness * avg:
+            logf = logf * power
+            stages += 1
+            for h in hists:
+                h[:] = 0
+    return logf, sweep, updates, stages, steps, acc
+
+
+def muca_iterate_update(model, dim, L, spins, lw, hists, seed, keys, sweep0, bin_start, bin_step,
+                        sweeps, therm_sweeps, D=0.0, int_bins=None):
+    """One update of the parallel multicanonical iteration of one group,
+    restated sequentially (test infrastructure): therm_sweeps sweeps, reset!
+    (histograms and counters, algorithms/multicanonical.jl:27-33), `sweeps`
+    sweeps, merge_histograms! into the root (parallel_multicanonical.jl:38-47),
+    update! of the root (ensembles/multicanonical.jl:32-44, libm log), and
+    distribute_logweight! (:59-68).  In place; returns (sweep counter, steps,
+    accepted) of the recording sweeps."""
+    n = len(spins)
+    sweep = sweep0
+    if therm_sweeps > 0:
+        for k in range(n):
+            generalized_sweeps(model, 0, dim, L, lw[k], hists[k], spins[k], seed, keys[k], sweep,
+                               therm_sweeps, bin_start, bin_step, D=D, int_bins=int_bins)
+        sweep += therm_sweeps
+    steps, acc = [0] * n, [0] * n
+    for k in range(n):
+        hists[k][:] = 0
+        _, _, _, steps[k], acc[k] = generalized_sweeps(model, 0, dim, L, lw[k], hists[k], spins[k],
+                                                       seed, keys[k], sweep, sweeps, bin_start,
+                                                       bin_step, D=D, int_bins=int_bins)
+    sweep += sweeps
+    for k in range(1, n):
+        hists[0] += hists[k]
+    muca_update(lw[0], hists[0].astype(np.float64))
+    for k in range(1, n):
+        lw[k][:] = lw[0]
+    return sweep, steps, acc

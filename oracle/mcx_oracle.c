@@ -405,7 +405,7 @@ static int bin_float(double x, double start, double step, int64_t num, int64_t* 
 }
 
 int orc_generalized_sweeps(int model, int wl, int dim, int L, double D,
-                           double bin_start, double bin_step, int64_t nbins,
+                           double bin_start, double bin_step, int64_t nbins, int int_bins,
                            double* lw, int64_t* hist, double logf,
                            uint64_t seed, uint32_t chain_key, uint64_t sweep0, int64_t nsweeps,
                            int8_t* spins, int64_t* steps, int64_t* accepted,
@@ -436,8 +436,15 @@ int orc_generalized_sweeps(int model, int wl, int dim, int L, double D,
                     int64_t E_old = -pair;
                     int64_t dE = 2 * (int64_t)si * nb;
                     int64_t E_new = E_old + dE;
-                    ok_new = bin_int(E_new, (int64_t)bin_start, (int64_t)bin_step, nbins, &b_new);
-                    ok_old = bin_int(E_old, (int64_t)bin_start, (int64_t)bin_step, nbins, &b_old);
+                    /* _binindex(::DiscreteBinning{Int}, ::Int) divides; a Float64
+                     * binning takes the generic rounding method */
+                    if (int_bins) {
+                        ok_new = bin_int(E_new, (int64_t)bin_start, (int64_t)bin_step, nbins, &b_new);
+                        ok_old = bin_int(E_old, (int64_t)bin_start, (int64_t)bin_step, nbins, &b_old);
+                    } else {
+                        ok_new = bin_float((double)E_new, bin_start, bin_step, nbins, &b_new);
+                        ok_old = bin_float((double)E_old, bin_start, bin_step, nbins, &b_old);
+                    }
                     s_new = (int8_t)-si;
                     dpair_i = -2 * (int64_t)si * nb;
                     u = ldexp((double)w, -32);

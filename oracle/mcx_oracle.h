@@ -78,7 +78,7 @@ void orc_canonical_sweeps(int model, int alg, int dim, int L, double beta, doubl
  * wl = 1: Wang-Landau (lw[visited] -= logf per attempt; hist still counted).
  * Returns 0, or 2 (bounds error: *err_site set; state as at the failing attempt). */
 int orc_generalized_sweeps(int model, int wl, int dim, int L, double D,
-                           double bin_start, double bin_step, int64_t nbins,
+                           double bin_start, double bin_step, int64_t nbins, int int_bins,
                            double* lw, int64_t* hist, double logf,
                            uint64_t seed, uint32_t chain_key, uint64_t sweep0, int64_t nsweeps,
                            int8_t* spins, int64_t* steps, int64_t* accepted,
