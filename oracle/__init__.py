@@ -112,7 +112,8 @@ def _declare(L):
     L.orc_refpath_ising2d.restype = C.c_int64
     nf64 = _NullablePtr(_f64p)
     L.orc_refpath_run.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _i32p, _f64p, nf64, nf64,
-                                  C.c_int64, C.c_double, C.c_double, C.c_uint64, C.c_int64, C.c_int]
+                                  C.c_int64, C.c_double, C.c_double, C.c_uint64, C.c_int64, C.c_int,
+                                  C.c_double]
     L.orc_refpath_run.restype = C.c_int64
     L.orc_pair_spin2_sweeps.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                         C.c_int64, _f64p, _i64p, C.c_uint64, C.c_uint32,
@@ -273,19 +274,21 @@ def refpath_ising2d(L, betas, seed, nsweeps, nthreads, keep_spins=False):
 
 
 def refpath_run(model, dim, Ls, betas, seed, nsweeps, nthreads, D=None, lw=None,
-                bin_start=0.0, bin_step=1.0):
+                bin_start=0.0, bin_step=1.0, logf=None):
     """General reference CPU path (random site, Xoshiro256++): Ising or
     Blume-Capel (model 0 / 1), 2D or 3D, per-chain L / beta / D; canonical
     Metropolis, or multicanonical with weights lw over DiscreteBinning(bin_start,
-    bin_step, len(lw)).  Returns the attempts made (timing baseline only)."""
+    bin_step, len(lw)), or (logf given) Wang-Landau walkers starting from lw.
+    Returns the attempts made (timing baseline only)."""
     Ls = np.ascontiguousarray(Ls, np.int32)
     betas = np.ascontiguousarray(np.broadcast_to(np.asarray(betas, np.float64), Ls.shape))
     Dv = None if D is None else np.ascontiguousarray(np.broadcast_to(np.asarray(D, np.float64),
                                                                       Ls.shape))
     lwv = None if lw is None else np.ascontiguousarray(lw, np.float64)
-    return lib().orc_refpath_run(model, dim, 0 if lwv is None else 1, len(Ls), Ls, betas, Dv, lwv,
+    mode = 0 if lwv is None else (2 if logf is not None else 1)
+    return lib().orc_refpath_run(model, dim, mode, len(Ls), Ls, betas, Dv, lwv,
                                  0 if lwv is None else len(lwv), float(bin_start), float(bin_step),
-                                 seed, nsweeps, nthreads)
+                                 seed, nsweeps, nthreads, 0.0 if logf is None else float(logf))
 
 
 def _lag_cap(n, max_lag):

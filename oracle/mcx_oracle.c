@@ -781,7 +781,8 @@ int64_t orc_refpath_ising2d(int L, int nchains, const double* betas, uint64_t se
  *                          as a rejected move (the caller sizes the bins).
  * Used only as the timed CPU baseline. */
 typedef struct {
-    int model, dim, muca, c0, c1;
+    int model, dim, muca, c0, c1; /* muca: 0 canonical, 1 multicanonical, 2 Wang-Landau */
+    double logf;
     const int32_t* L;
     const double* betas;
     const double* D;
@@ -803,6 +804,15 @@ static void* refrun_worker(void* arg)
         int8_t* s = (int8_t*)malloc((size_t)N);
         int32_t* nbr = (int32_t*)malloc(sizeof(int32_t) * (size_t)(N * Z));
         double* hist = job->muca ? (double*)calloc((size_t)job->nbins, sizeof(double)) : NULL;
+        /* Wang-Landau: the walker's own weights, lowered by logf at every visit
+         * (algorithms/wang_landau.jl:29-37) */
+        double* lww = NULL;
+        if (job->muca == 2) {
+            lww = (double*)malloc(sizeof(double) * (size_t)job->nbins);
+            for (int64_t b = 0; b < job->nbins; ++b)
+                lww[b] = job->lw[b];
+        }
+        const double* lw = lww ? lww : job->lw;
         for (int64_t i = 0; i < N; ++i) {
             const int x = (int)(i % L), y = (int)((i / L) % L), z = (int)(i / ((int64_t)L * L));
             int32_t* nb = nbr + i * Z;
@@ -853,7 +863,7 @@ static void* refrun_worker(void* arg)
             if (job->muca) {
                 const int in_old = bin_float(E, job->bin_start, job->bin_step, job->nbins, &b_old);
                 const int in_new = bin_float(E + dE, job->bin_start, job->bin_step, job->nbins, &b_new);
-                lr = in_old && in_new ? job->lw[b_new] - job->lw[b_old] : -INFINITY;
+                lr = in_old && in_new ? lw[b_new] - lw[b_old] : -INFINITY;
             } else {
                 lr = -beta * dE;
             }
@@ -866,12 +876,16 @@ static void* refrun_worker(void* arg)
             }
             if (job->muca) {
                 int64_t b;
-                if (bin_float(E, job->bin_start, job->bin_step, job->nbins, &b))
+                if (bin_float(E, job->bin_start, job->bin_step, job->nbins, &b)) {
                     hist[b] += 1.0;
+                    if (lww)
+                        lww[b] -= job->logf;
+                }
             }
         }
         job->attempts += steps;
         (void)accepted;
+        free(lww);
         free(hist);
         free(nbr);
         free(s);
@@ -882,7 +896,7 @@ static void* refrun_worker(void* arg)
 int64_t orc_refpath_run(int model, int dim, int muca, int nchains, const int32_t* L,
                         const double* betas, const double* D, const double* lw, int64_t nbins,
                         double bin_start, double bin_step, uint64_t seed, int64_t nsweeps,
-                        int nthreads)
+                        int nthreads, double logf)
 {
     if (nthreads < 1)
         nthreads = 1;
@@ -895,6 +909,7 @@ int64_t orc_refpath_run(int model, int dim, int muca, int nchains, const int32_t
         j->model = model;
         j->dim = dim;
         j->muca = muca;
+        j->logf = logf;
         j->c0 = (int)((int64_t)nchains * t / nthreads);
         j->c1 = (int)((int64_t)nchains * (t + 1) / nthreads);
         j->L = L;

@@ -127,12 +127,13 @@ int64_t orc_refpath_ising2d(int L, int nchains, const double* betas, uint64_t se
 /* General reference CPU path (timed baseline only): nchains independent chains
  * of model (0 Ising, 1 Blume-Capel) in dim 2 or 3, per-chain L, beta and D
  * (D may be NULL), random-site Metropolis, or multicanonical weights lw over
- * DiscreteBinning(bin_start, bin_step, nbins) when muca != 0.  Returns the
- * attempts made. */
+ * DiscreteBinning(bin_start, bin_step, nbins) when muca == 1, or Wang-Landau
+ * walkers starting from lw (each lowers its own copy by logf per visit) when
+ * muca == 2.  Returns the attempts made. */
 int64_t orc_refpath_run(int model, int dim, int muca, int nchains, const int32_t* L,
                         const double* betas, const double* D, const double* lw, int64_t nbins,
                         double bin_start, double bin_step, uint64_t seed, int64_t nsweeps,
-                        int nthreads);
+                        int nthreads, double logf);
 
 /* Integrated autocorrelation time (src/measurements/autocorrelations.jl:28-73)
  * of one integer series: x (|x| if absx), or the energy x + D*y when y is not
