@@ -164,6 +164,7 @@ def headline(a, mcx, np, torch, out):
     b = mcx.LatticeBatch("ising", 2, L, n_chains=n, seed=7, series_capacity=cap)
     b.set_betas(betas)
     b.init("random")
+    shape = b.tuned_shapes.get("metropolis", "built-in")
     torch.cuda.synchronize()
     for _ in range(a.warmup):
         b.run("metropolis", S, record=True)
@@ -190,6 +191,7 @@ def headline(a, mcx, np, torch, out):
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": "k_ising_sweeps<2,0>", "kernel_ms": round(kern_ms, 3),
+            "launch_shape": shape,
             "bytes_per_attempt": BYTES_PER_ATTEMPT,
             "algorithmic_bytes_per_launch": BYTES_PER_ATTEMPT * attempts_per_step}
     if traffic is not None and (n, L, S) == (4096, 64, 1000):
@@ -323,6 +325,7 @@ def pt_line(a, mcx, np, oracle, host, cpu):
                         "), set_betas(64, 0.20, 0.24, geometric), checkerboard Metropolis, "
                         f"{rounds} rounds of 10 sweeps + one exchange",
             "value": round(att / (ms * 1e6), 3), "unit": "attempts/ns", "ms": round(ms, 2),
+            "launch_shape": sys_.batch.tuned_shapes.get("metropolis", "built-in"),
             "exchange_acceptance": round(acc, 4), "roofline": _roof(att, ms),
             "tau_int": {"series": int(taus.size), "samples": rounds * 10, "max_lag": 200,
                         "ms": round(tau_ms, 2), "median": round(float(np.nanmedian(taus)), 3),
@@ -349,6 +352,7 @@ def bc_line(mcx, np, oracle, host, cpu):
     line = {"workload": f"{n} Blume-Capel chains, L={L}, D=1, T uniform [1.5, 3.5], checkerboard "
                         f"Metropolis, {sw} sweeps with per-sweep E/M/Q series + FP64 moments",
             "value": round(att / (ms * 1e6), 3), "unit": "attempts/ns", "ms": round(ms, 2),
+            "launch_shape": b.tuned_shapes.get("metropolis", "built-in"),
             "roofline": _roof(att, ms)}
     b.close()
     if cpu:
@@ -455,6 +459,7 @@ def serving_stream(mcx, n_jobs, budget):
         w.run([mcx.Job(m, L, 0.5, 2, alg, D=0.5) for m in ("ising", "blume_capel")
                for L in (16, 64, 256, 512) for alg in ("metropolis",)])
     srv = mcx.Server(hbm_budget=budget, max_jobs=1024)
+    settings = dict(srv.settings) or "built-in"
     t0 = time.perf_counter()
     results = []
     for j in jobs:
@@ -483,6 +488,7 @@ def serving_stream(mcx, n_jobs, budget):
         "busy_attempts_per_ns": round(st["busy_attempts_per_ns"], 3),
         "offered_attempts_per_ns": round(work / (jobs[-1].arrival_s * 1e9), 3),
         "wall_s": round(wall, 3), "quanta": st["quanta"], "hbm_budget": budget,
+        "server_settings": settings,
     }
 
 

@@ -176,6 +176,25 @@ mcx_status mcx_batch_run(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t fl
  * context stream around the kernels of that call. */
 mcx_status mcx_batch_last_run_ms(mcx_batch* b, double* ms);
 
+/* No reference counterpart (launch tuning).  The launch shape of the batched
+ * sweep kernels for one algorithm (alg < 0: every algorithm), as an autotuner
+ * picks it per (model, dim, L, algorithm); any shape gives the same results,
+ * bit for bit -- it changes only how the chains are laid over workgroups:
+ *   threads           workgroup size (a multiple of 64, <= 1024; 0 = built-in:
+ *                     one lane per plane word, 64..256)
+ *   chains_per_group  chains one workgroup runs one after another (>= 1; 0 = 1)
+ *   lanes_per_word    Blume-Capel: lanes sharing one 32-site plane word
+ *                     (1, 2, 4, 8; 0 = 1; -1 = the most, up to 8, that fit the
+ *                     workgroup); must be 0 for Ising */
+typedef struct {
+    int32_t threads;
+    int32_t chains_per_group;
+    int32_t lanes_per_word;
+    int32_t pad;
+} mcx_launch_shape;
+mcx_status mcx_batch_set_launch_shape(mcx_batch* b, int32_t alg, const mcx_launch_shape* shape);
+mcx_status mcx_batch_get_launch_shape(mcx_batch* b, int32_t alg, mcx_launch_shape* shape);
+
 /* Replaces energy(sys) / magnetization(sys) (SpinSystems/src/ising.jl:17-18,
  * blume_capel.jl:18-19): per chain E_pair = -sum_<ij> s_i s_j, M = sum s_i,
  * Q = sum s_i^2 (any output may be NULL).  Energy = E_pair + D*Q. */

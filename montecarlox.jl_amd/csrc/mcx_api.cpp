@@ -710,13 +710,19 @@ mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t
             a.last_M = b->d_last + b->n;
             a.last_Q = b->d_last + 2 * b->n;
         }
+        // the tuned launch shape of this algorithm, if any (results do not depend on it)
+        const mcx_launch_shape& sh = b->shape[alg];
+        a.lanes_per_word = sh.lanes_per_word;
         for (const auto& ln : b->launches) {
             a.items = b->d_items + ln.first;
             a.n_items = ln.count;
+            const int block = sh.threads > 0 ? sh.threads : ln.block;
+            const int cpg = sh.chains_per_group > 1 ? sh.chains_per_group : 1;
+            const int grid = (ln.count + cpg - 1) / cpg;
             if (b->model == MCX_ISING)
-                HIPCHK(launch_ising_sweeps(b->dim, alg, a, ln.block, ln.count, ln.lds, st));
+                HIPCHK(launch_ising_sweeps(b->dim, alg, a, block, grid, ln.lds, st));
             else
-                HIPCHK(launch_bc_sweeps(b->dim, alg, a, ln.block, ln.count, ln.lds, st));
+                HIPCHK(launch_bc_sweeps(b->dim, alg, a, block, grid, ln.lds, st));
         }
     }
     for (int ch = 0; ch < b->n; ++ch) {
@@ -769,6 +775,42 @@ extern "C" mcx_status mcx_batch_run(mcx_batch* b, int32_t alg, int64_t nsweeps, 
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1));
     b->last_ms = ms;
+    return MCX_OK;
+}
+
+extern "C" mcx_status mcx_batch_set_launch_shape(mcx_batch* b, int32_t alg,
+                                                 const mcx_launch_shape* s)
+{
+    if (!b || !s)
+        return fail(MCX_EINVAL, "mcx_batch_set_launch_shape: NULL argument");
+    if (alg >= 0 && alg != MCX_METROPOLIS && alg != MCX_HEATBATH && alg != MCX_GLAUBER)
+        return fail(MCX_EINVAL, "mcx_batch_set_launch_shape: alg must be MCX_METROPOLIS, "
+                                "MCX_HEATBATH, MCX_GLAUBER or < 0 (got %d)", alg);
+    if (s->threads < 0 || s->threads > 1024 || s->threads % 64 != 0)
+        return fail(MCX_EINVAL, "mcx_batch_set_launch_shape: threads must be 0 or a multiple of 64 "
+                                "up to 1024 (got %d)", s->threads);
+    if (s->chains_per_group < 0)
+        return fail(MCX_EINVAL, "mcx_batch_set_launch_shape: chains_per_group must be >= 0");
+    const int l = s->lanes_per_word;
+    if (!(l == -1 || l == 0 || l == 1 || l == 2 || l == 4 || l == 8))
+        return fail(MCX_EINVAL, "mcx_batch_set_launch_shape: lanes_per_word must be -1, 0, 1, 2, 4 "
+                                "or 8 (got %d)", l);
+    if (l != 0 && b->model == MCX_ISING)
+        return fail(MCX_EINVAL, "mcx_batch_set_launch_shape: lanes_per_word is a Blume-Capel "
+                                "setting");
+    for (int k = 0; k < 5; ++k)
+        if (alg < 0 ? (k == MCX_METROPOLIS || k == MCX_HEATBATH || k == MCX_GLAUBER) : k == alg)
+            b->shape[k] = *s;
+    return MCX_OK;
+}
+
+extern "C" mcx_status mcx_batch_get_launch_shape(mcx_batch* b, int32_t alg, mcx_launch_shape* s)
+{
+    if (!b || !s)
+        return fail(MCX_EINVAL, "mcx_batch_get_launch_shape: NULL argument");
+    if (alg != MCX_METROPOLIS && alg != MCX_HEATBATH && alg != MCX_GLAUBER)
+        return fail(MCX_EINVAL, "mcx_batch_get_launch_shape: unknown alg %d", alg);
+    *s = b->shape[alg];
     return MCX_OK;
 }
 

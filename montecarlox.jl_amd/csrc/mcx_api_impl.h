@@ -64,6 +64,7 @@ struct mcx_batch {
     uint32_t* d_planes = nullptr;
     int64_t* d_plane_off = nullptr; // [n] word offset of chain k's planes
     mcx_pt* pt = nullptr;      // replica-exchange object driving the tables, if any
+    mcx_launch_shape shape[5] = {}; // per algorithm (MCX_METROPOLIS .. MCX_GLAUBER): tuned launch
     int n_gen = 0;             // multicanonical / Wang-Landau objects built on this batch
 };
 

@@ -7,6 +7,7 @@ underscore (`init!` -> `init_`, `update!` -> `update_`).  All sweeps, replica
 exchanges and multicanonical / Wang-Landau updates run in libmcx_hip.so; this
 package is the host-side object model over its C ABI.
 """
+from . import autotune
 from ._lib import (Context, MCXArgumentError, MCXAssertionError, MCXBoundsError, MCXError,
                    MCXStateError, default_context)
 from .algorithms import (AbstractHeatBath, AbstractImportanceSampling, AbstractMetropolis,

@@ -55,6 +55,11 @@ class BatchDesc(C.Structure):
                 ("D", C.POINTER(C.c_double)), ("series_capacity", C.c_int64)]
 
 
+class LaunchShape(C.Structure):
+    _fields_ = [("threads", C.c_int32), ("chains_per_group", C.c_int32),
+                ("lanes_per_word", C.c_int32), ("pad", C.c_int32)]
+
+
 class ServerDesc(C.Structure):
     _fields_ = [("hbm_budget", C.c_int64), ("max_jobs", C.c_int32), ("n_streams", C.c_int32),
                 ("quantum_ms", C.c_double), ("tile_sites", C.c_int64),
@@ -105,6 +110,8 @@ SIGNATURES = {
     "mcx_batch_get_spins": [_P, _i32, _pi8, C.c_size_t],
     "mcx_batch_run": [_P, _i32, _i64, _u32],
     "mcx_batch_last_run_ms": [_P, _pf64],
+    "mcx_batch_set_launch_shape": [_P, _i32, C.POINTER(LaunchShape)],
+    "mcx_batch_get_launch_shape": [_P, _i32, C.POINTER(LaunchShape)],
     "mcx_batch_get_observables": [_P, _pi32, _pi32, _pi32],
     "mcx_batch_series_length": [_P, _pi64],
     "mcx_batch_get_series": [_P, _i32, _i64, _i64, _pi32, _pi32, _pi32],

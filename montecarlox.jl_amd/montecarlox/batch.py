@@ -43,7 +43,7 @@ class LatticeBatch:
                  L: Union[int, Sequence[int]] = 16, n_chains: Optional[int] = None,
                  seed: int = 0, keys: Optional[Sequence[int]] = None,
                  D: Union[float, Sequence[float]] = 0.0, series_capacity: int = 0,
-                 ctx: Optional[_lib.Context] = None):
+                 ctx: Optional[_lib.Context] = None, tuned: bool = True):
         self.ctx = ctx or _lib.default_context()
         self.model = _MODELS[model.lower()] if isinstance(model, str) else int(model)
         self.dim = int(dim)
@@ -74,6 +74,14 @@ class LatticeBatch:
         self._h = C.c_void_p()
         call("mcx_batch_create", self.ctx.handle, C.byref(d), C.byref(self._h))
         self.nsites = (self.L.astype(np.int64) ** self.dim)
+        # launch shapes an autotuner measured for this (model, dim, L) (results
+        # are the same under any shape; only the speed changes)
+        self.tuned_shapes = {}
+        if tuned:
+            from . import autotune
+            for alg, shape in autotune.lookup(self.model, self.dim, self.L).items():
+                self.set_launch_shape(alg, **shape)
+                self.tuned_shapes[alg] = dict(shape)
 
     # -- lifetime ------------------------------------------------------------
     @property
@@ -134,6 +142,21 @@ class LatticeBatch:
         flags = (_lib.RUN_RECORD if record else 0) | (_lib.RUN_PLAIN if plain else 0)
         call("mcx_batch_run", self._h, alg_code(alg), int(nsweeps), flags)
         return self.last_run_ms()
+
+    def set_launch_shape(self, alg=None, threads: int = 0, chains_per_group: int = 0,
+                         lanes_per_word: int = 0):
+        """Launch shape of the batched kernels for `alg` (None: every algorithm):
+        workgroup size, chains per workgroup, Blume-Capel lanes per word (0 =
+        built-in).  Results are bit-identical under every shape."""
+        s = _lib.LaunchShape(int(threads), int(chains_per_group), int(lanes_per_word), 0)
+        call("mcx_batch_set_launch_shape", self._h, -1 if alg is None else alg_code(alg),
+             C.byref(s))
+
+    def launch_shape(self, alg) -> dict:
+        s = _lib.LaunchShape()
+        call("mcx_batch_get_launch_shape", self._h, alg_code(alg), C.byref(s))
+        return {"threads": s.threads, "chains_per_group": s.chains_per_group,
+                "lanes_per_word": s.lanes_per_word}
 
     def last_run_ms(self) -> float:
         ms = C.c_double()

@@ -82,12 +82,21 @@ class Server:
     tile_sites: lattices at least this big run spread over many workgroups
     (0 = 2^15 Ising / 2^14 Blume-Capel sites, negative = never);
     band_sweeps: sweeps per band launch of those lattices (0 = built-in shapes,
-    k > 0 = k, negative = per-colour launches without bands)."""
+    k > 0 = k, negative = per-colour launches without bands).  With tuned=True
+    the autotuner's stream count, quantum and band depth replace the built-in
+    ones that the caller leaves at 0; results do not depend on them."""
 
     def __init__(self, hbm_budget: int, max_jobs: int = 1024, quantum_ms: float = 0.0,
                  n_streams: int = 0, tile_sites: int = 0, band_sweeps: int = 0,
-                 ctx: Optional[_lib.Context] = None):
+                 ctx: Optional[_lib.Context] = None, tuned: bool = True):
         self.ctx = ctx or _lib.default_context()
+        self.settings = {}
+        if tuned:
+            from .autotune import server_settings
+            self.settings = server_settings()
+            n_streams = n_streams or int(self.settings.get("n_streams", 0))
+            quantum_ms = quantum_ms or float(self.settings.get("quantum_ms", 0.0))
+            band_sweeps = band_sweeps or int(self.settings.get("band_sweeps", 0))
         d = ServerDesc(hbm_budget=int(hbm_budget), max_jobs=int(max_jobs),
                        n_streams=int(n_streams), quantum_ms=float(quantum_ms),
                        tile_sites=int(tile_sites), band_sweeps=int(band_sweeps))
