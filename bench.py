@@ -319,6 +319,15 @@ def pt_line(a, mcx, np, oracle, host, cpu):
     sw = np.full((S, R), 10, np.int64)
     mcx.optimize_exchange_interval_(pt, None, sw, base_sweeps=10, min_sweeps=1, max_sweeps=400,
                                     min_points=400, max_lag=200)
+    # the same 100-round block eagerly (one launch per round and step) and as one
+    # hipGraph: the first graph call captures and instantiates it, later ones replay it
+    gr = {}
+    for name, graph in (("eager", False), ("graph_first", True), ("graph_replay", True),
+                        ("graph_replay2", True)):
+        t0 = time.perf_counter()
+        dev = pt.run_(100, 10, graph=graph)
+        gr[name] = {"wall_ms": round((time.perf_counter() - t0) * 1e3, 2),
+                    "device_ms": round(dev, 2)}
     line = {"workload": f"{S} replica-exchange sets x {R} replicas, 3D Ising, L per set from "
                         "{16,24,32,48} (PCG64 seed 11: " +
                         ",".join(f"{L}:{int((set_L == L).sum())}" for L in (16, 24, 32, 48)) +
@@ -330,7 +339,8 @@ def pt_line(a, mcx, np, oracle, host, cpu):
             "tau_int": {"series": int(taus.size), "samples": rounds * 10, "max_lag": 200,
                         "ms": round(tau_ms, 2), "median": round(float(np.nanmedian(taus)), 3),
                         "max": round(float(np.nanmax(taus)), 3),
-                        "retuned_sweeps_min_max": [int(sw.min()), int(sw.max())]}}
+                        "retuned_sweeps_min_max": [int(sw.min()), int(sw.max())]},
+            "hipgraph_vs_eager_100_rounds": gr}
     pt.close()
     sys_.batch.close()
     if cpu:
@@ -460,6 +470,9 @@ def serving_stream(mcx, n_jobs, budget):
                for L in (16, 64, 256, 512) for alg in ("metropolis",)])
     srv = mcx.Server(hbm_budget=budget, max_jobs=1024)
     settings = dict(srv.settings) or "built-in"
+    # latency SLO: every job within 50 ms plus its attempts at 20 attempts/ns
+    slo_ms, slo_rate = 50.0, 20.0
+    srv.set_slo(slo_ms, slo_rate)
     t0 = time.perf_counter()
     results = []
     for j in jobs:
@@ -472,6 +485,7 @@ def serving_stream(mcx, n_jobs, budget):
     results.extend(srv.poll(1024))
     wall = time.perf_counter() - t0
     st = srv.stats()
+    slo = srv.slo_report()
     srv.close()
     assert len(results) == n_jobs and all(r["status"] == 0 for r in results)
     work = float(sum(j.attempts for j in jobs))
@@ -489,6 +503,13 @@ def serving_stream(mcx, n_jobs, budget):
         "offered_attempts_per_ns": round(work / (jobs[-1].arrival_s * 1e9), 3),
         "wall_s": round(wall, 3), "quanta": st["quanta"], "hbm_budget": budget,
         "server_settings": settings,
+        "slo": {"rule": f"latency <= {slo_ms:g} ms + attempts / ({slo_rate:g} attempts/ns)",
+                "jobs": slo["jobs"], "met": slo["met"], "attainment": round(slo["attainment"], 4),
+                "met_by_model": slo["met_by_model"], "jobs_by_model": slo["jobs_by_model"],
+                "worst_lateness_ms": round(slo["worst_lateness_ms"], 2),
+                "mean_latency_over_deadline": round(slo["mean_deadline_frac"], 4),
+                "p90_ms": round(slo["p90_ms"], 2), "p999_ms": round(slo["p999_ms"], 2),
+                "mean_ms": round(slo["mean_ms"], 2)},
     }
 
 

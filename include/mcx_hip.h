@@ -496,7 +496,8 @@ typedef struct {
     int32_t status; /* MCX_OK */
     int32_t L;
     int64_t nsweeps, accepted;
-    int32_t E_pair, M, Q, pad; /* observables after the last sweep */
+    int32_t E_pair, M, Q;      /* observables after the last sweep */
+    int32_t slo_met;           /* 1 met its deadline, 0 missed it, -1 no SLO (mcx_server_set_slo) */
     double moments[MCX_N_MOMENTS]; /* as mcx_batch_get_moments, over every sweep */
     double latency_ms;  /* submit -> result ready                        */
     double queue_ms;    /* submit -> admission                           */
@@ -524,6 +525,24 @@ mcx_status mcx_server_poll(mcx_server* s, mcx_job_result* out, int32_t max, int3
  * MCX_ESTATE on timeout. */
 mcx_status mcx_server_drain(mcx_server* s, double timeout_ms);
 mcx_status mcx_server_stats(mcx_server* s, mcx_server_stats_t* out);
+
+/* Latency SLO (no reference counterpart).  Jobs submitted after the call get
+ * the deadline slo_ms + attempts / (attempts_per_ns * 1e6) ms from submission
+ * (attempts = L^dim * nsweeps; attempts_per_ns <= 0: slo_ms alone; slo_ms <= 0
+ * and attempts_per_ns <= 0: no SLO).  The report covers the completed jobs
+ * that carry a deadline: how many met it, the worst lateness, the latency
+ * quantiles (all completed jobs) and the mean latency / deadline ratio. */
+typedef struct {
+    double slo_ms, slo_attempts_per_ns; /* the rule in force                      */
+    int64_t jobs, met, missed;          /* completed jobs with a deadline         */
+    int64_t jobs_by_model[2], met_by_model[2]; /* [MCX_ISING], [MCX_BLUME_CAPEL]  */
+    double attainment;                  /* met / jobs (1 when jobs == 0)          */
+    double worst_lateness_ms;           /* max(latency - deadline) over them      */
+    double mean_deadline_frac;          /* mean of latency / deadline             */
+    double p50_ms, p90_ms, p99_ms, p999_ms, max_ms, mean_ms; /* every completed job */
+} mcx_slo_report;
+mcx_status mcx_server_set_slo(mcx_server* s, double slo_ms, double slo_attempts_per_ns);
+mcx_status mcx_server_slo_report(mcx_server* s, mcx_slo_report* out);
 
 #ifdef __cplusplus
 }

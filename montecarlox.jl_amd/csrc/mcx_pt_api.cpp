@@ -16,12 +16,33 @@ struct mcx_pt {
     uint64_t* d_round = nullptr;
     double* d_ladder = nullptr;
     double* d_x = nullptr;
+    // the instantiated graph of the last MCX_RUN_GRAPH run, and what its launch
+    // arguments depend on: a run with the same key replays it as it is (the
+    // kernels read the streams, ladders and tables from device memory); one that
+    // differs only in the series position updates it in place
+    struct GraphKey {
+        int alg = -1, by_table = 0, keep_planes = 0;
+        int64_t rounds = 0, sweeps = 0, rec_len = 0;
+        uint32_t rflags = 0;
+        mcx_launch_shape shape{};
+        bool same_shape(const GraphKey& o) const
+        {
+            return alg == o.alg && by_table == o.by_table && keep_planes == o.keep_planes &&
+                   rounds == o.rounds && sweeps == o.sweeps && rflags == o.rflags &&
+                   std::memcmp(&shape, &o.shape, sizeof shape) == 0;
+        }
+    };
+    hipGraphExec_t gexec = nullptr;
+    GraphKey gkey;
 };
 
 namespace {
 
 void pt_free(mcx_pt* p)
 {
+    if (p->gexec)
+        hipGraphExecDestroy(p->gexec);
+    p->gexec = nullptr;
     hipFree(p->d_idx);
     hipFree(p->d_pos);
     hipFree(p->d_steps);
@@ -218,30 +239,68 @@ extern "C" mcx_status mcx_pt_run(mcx_pt* p, int32_t alg, int64_t n_rounds, int64
     HIPCHK(hipEventRecord(b->ctx->ev0, st));
     if (graph && n_rounds > 0) {
         // one graph holding every round: sweep launch(es) then the exchange
-        hipGraph_t g = nullptr;
-        hipGraphExec_t ge = nullptr;
-        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        mcx_status rs = MCX_OK;
-        for (int64_t k = 0; k < n_rounds && rs == MCX_OK; ++k) {
-            rs = run_sweeps_async(b, alg, sweeps_between, rflags, true, planes_of(k));
-            if (rs == MCX_OK && launch_pt_exchange(a, st) != hipSuccess)
-                rs = fail(MCX_EHIP, "mcx_pt_run: exchange launch failed during capture");
+        mcx_pt::GraphKey key;
+        key.alg = alg;
+        key.by_table = b->by_table ? 1 : 0;
+        key.keep_planes = keep_planes ? 1 : 0;
+        key.rounds = n_rounds;
+        key.sweeps = sweeps_between;
+        key.rec_len = rflags ? b->rec_len : 0;
+        key.rflags = rflags;
+        key.shape = b->shape[alg];
+        if (p->gexec && key.same_shape(p->gkey) && key.rec_len == p->gkey.rec_len) {
+            // the same launch arguments as the instantiated graph: replay it, and
+            // advance the host mirror of the stream positions as the launches would
+            for (int64_t k = 0; k < n_rounds; ++k)
+                for (int ch = 0; ch < b->n; ++ch) {
+                    b->rec[ch].sweep += (uint64_t)sweeps_between;
+                    b->steps[ch] += b->nsites[ch] * sweeps_between;
+                }
+        } else {
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            mcx_status rs = MCX_OK;
+            for (int64_t k = 0; k < n_rounds && rs == MCX_OK; ++k) {
+                rs = run_sweeps_async(b, alg, sweeps_between, rflags, true, planes_of(k));
+                if (rs == MCX_OK && launch_pt_exchange(a, st) != hipSuccess)
+                    rs = fail(MCX_EHIP, "mcx_pt_run: exchange launch failed during capture");
+            }
+            hipError_t ce = hipStreamEndCapture(st, &g);
+            if (rs != MCX_OK || ce != hipSuccess) {
+                if (g)
+                    hipGraphDestroy(g);
+                return rs != MCX_OK ? rs
+                                    : fail(MCX_EHIP, "mcx_pt_run: graph capture failed: %s",
+                                           hipGetErrorString(ce));
+            }
+            hipError_t ie = hipSuccess;
+            if (p->gexec && key.same_shape(p->gkey)) {
+                // same topology, other series position: update the instantiated graph
+                hipGraphNode_t err_node = nullptr;
+                hipGraphExecUpdateResult res = hipGraphExecUpdateSuccess;
+                if (hipGraphExecUpdate(p->gexec, g, &err_node, &res) != hipSuccess ||
+                    res != hipGraphExecUpdateSuccess) {
+                    (void)hipGetLastError();
+                    hipGraphExecDestroy(p->gexec);
+                    p->gexec = nullptr;
+                }
+            } else if (p->gexec) {
+                hipGraphExecDestroy(p->gexec);
+                p->gexec = nullptr;
+            }
+            if (!p->gexec)
+                ie = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (ie != hipSuccess) {
+                p->gexec = nullptr;
+                return fail(MCX_EHIP, "mcx_pt_run: graph instantiation failed: %s",
+                            hipGetErrorString(ie));
+            }
+            p->gkey = key;
         }
-        hipError_t ce = hipStreamEndCapture(st, &g);
-        if (rs)
-            return rs;
-        if (ce != hipSuccess)
-            return fail(MCX_EHIP, "mcx_pt_run: graph capture failed: %s", hipGetErrorString(ce));
-        hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-        if (ie == hipSuccess)
-            ie = hipGraphLaunch(ge, st);
-        if (ie == hipSuccess)
-            ie = hipStreamSynchronize(st);
-        if (ge)
-            hipGraphExecDestroy(ge);
-        hipGraphDestroy(g);
-        if (ie != hipSuccess)
-            return fail(MCX_EHIP, "mcx_pt_run: graph execution failed: %s", hipGetErrorString(ie));
+        hipError_t le = hipGraphLaunch(p->gexec, st);
+        if (le != hipSuccess)
+            return fail(MCX_EHIP, "mcx_pt_run: graph launch failed: %s", hipGetErrorString(le));
     } else {
         for (int64_t k = 0; k < n_rounds; ++k) {
             s = run_sweeps_async(b, alg, sweeps_between, rflags, true, planes_of(k));
@@ -387,6 +446,27 @@ extern "C" mcx_status mcx_pt_load_state(mcx_pt* p, const void* buf, size_t bytes
     for (size_t i = 0; i < n; ++i)
         if (ladder[i] != p->ladder[i])
             return fail(MCX_EINVAL, "mcx_pt_load_state: the snapshot's ladder differs at %zu", i);
+    {
+        // ladder positions: per set, idx a permutation of 1..R and pos its
+        // inverse (the exchange kernel indexes replicas and chains by them);
+        // stages 0 or 1
+        BlobReader peek = r;
+        std::vector<int32_t> idx(n), pos(n), stage((size_t)p->n_sets);
+        peek.get(idx.data(), sizeof(int32_t) * n);
+        peek.get(pos.data(), sizeof(int32_t) * n);
+        peek.take(sizeof(int32_t) * 2 * e);
+        peek.get(stage.data(), sizeof(int32_t) * (size_t)p->n_sets);
+        for (int s = 0; s < p->n_sets; ++s) {
+            const int32_t* ix = idx.data() + (size_t)s * p->R;
+            const int32_t* ps = pos.data() + (size_t)s * p->R;
+            for (int k = 0; k < p->R; ++k)
+                if (ps[k] < 0 || ps[k] >= p->R || ix[ps[k]] != k + 1)
+                    return fail(MCX_EINVAL, "mcx_pt_load_state: set %d: the ladder positions are "
+                                            "not a permutation and its inverse", s);
+            if (stage[(size_t)s] != 0 && stage[(size_t)s] != 1)
+                return fail(MCX_EINVAL, "mcx_pt_load_state: set %d: stage %d", s, stage[(size_t)s]);
+        }
+    }
     mcx_batch* b = p->b;
     HIPCHK(hipSetDevice(b->ctx->device));
     hipStream_t st = b->ctx->stream;

@@ -127,6 +127,7 @@ struct Job {
     int64_t bytes = 0;
     int slot = -1;         // index into the per-job device arrays
     double t_submit = 0, t_admit = 0, device_ms = 0;
+    double deadline_ms = -1; // latency target from submission (< 0: none)
 };
 
 // First-fit allocator over [0, size), with coalescing on release.
@@ -231,6 +232,10 @@ struct mcx_server {
     std::deque<Job> pending;
     std::deque<mcx_job_result> done;
     std::vector<double> latencies;
+    // latency SLO: the rule for new jobs, and per finished job with a deadline
+    // its model and latency / deadline
+    double slo_ms = 0, slo_rate = 0;
+    std::vector<std::tuple<int, double, double>> slo_done; // model, latency, deadline
     uint64_t next_id = 1;
     int64_t submitted = 0, completed = 0, n_active = 0, quanta = 0;
     double attempts = 0, busy_ms = 0, t_first = -1, t_last = 0;
@@ -759,6 +764,9 @@ void worker_main(mcx_server* s)
             r.latency_ms = t - j.t_submit;
             r.queue_ms = j.t_admit - j.t_submit;
             r.device_ms = j.device_ms;
+            r.slo_met = j.deadline_ms < 0 ? -1 : (r.latency_ms <= j.deadline_ms ? 1 : 0);
+            if (j.deadline_ms >= 0)
+                s->slo_done.emplace_back(j.d.model, r.latency_ms, j.deadline_ms);
             s->done.push_back(r);
             s->latencies.push_back(r.latency_ms);
             ++s->completed;
@@ -930,6 +938,11 @@ extern "C" mcx_status mcx_server_submit(mcx_server* s, const mcx_job_desc* d, ui
                         s->error.c_str());
         j.id = s->next_id++;
         j.t_submit = now_ms();
+        if (s->slo_ms > 0 || s->slo_rate > 0)
+            j.deadline_ms = std::max(0.0, s->slo_ms) +
+                            (s->slo_rate > 0 ? (double)j.nsites * (double)d->nsweeps /
+                                                   (s->slo_rate * 1e6)
+                                             : 0.0);
         ++s->submitted;
         s->pending.push_back(j);
     }
@@ -1000,6 +1013,63 @@ extern "C" mcx_status mcx_server_stats(mcx_server* s, mcx_server_stats_t* out)
     out->wall_ms = s->t_first >= 0 && s->completed > 0 ? s->t_last - s->t_first : 0.0;
     out->attempts_per_ns = out->wall_ms > 0 ? s->attempts / (out->wall_ms * 1e6) : 0.0;
     out->busy_attempts_per_ns = s->busy_ms > 0 ? s->attempts / (s->busy_ms * 1e6) : 0.0;
+    return MCX_OK;
+}
+
+extern "C" mcx_status mcx_server_set_slo(mcx_server* s, double slo_ms, double slo_attempts_per_ns)
+{
+    if (!s)
+        return fail(MCX_EINVAL, "mcx_server_set_slo: NULL handle");
+    if (!std::isfinite(slo_ms) || !std::isfinite(slo_attempts_per_ns))
+        return fail(MCX_EINVAL, "mcx_server_set_slo: the SLO must be finite");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->slo_ms = slo_ms > 0 ? slo_ms : 0.0;
+    s->slo_rate = slo_attempts_per_ns > 0 ? slo_attempts_per_ns : 0.0;
+    return MCX_OK;
+}
+
+extern "C" mcx_status mcx_server_slo_report(mcx_server* s, mcx_slo_report* out)
+{
+    if (!s || !out)
+        return fail(MCX_EINVAL, "mcx_server_slo_report: NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::memset(out, 0, sizeof *out);
+    out->slo_ms = s->slo_ms;
+    out->slo_attempts_per_ns = s->slo_rate;
+    double frac = 0, worst = -1e300;
+    for (const auto& e : s->slo_done) {
+        const int m = std::get<0>(e);
+        const double lat = std::get<1>(e), dl = std::get<2>(e);
+        const bool met = lat <= dl;
+        ++out->jobs;
+        out->met += met ? 1 : 0;
+        ++out->jobs_by_model[m];
+        out->met_by_model[m] += met ? 1 : 0;
+        worst = std::max(worst, lat - dl);
+        frac += dl > 0 ? lat / dl : 0.0;
+    }
+    out->missed = out->jobs - out->met;
+    out->attainment = out->jobs ? (double)out->met / (double)out->jobs : 1.0;
+    out->worst_lateness_ms = out->jobs ? worst : 0.0;
+    out->mean_deadline_frac = out->jobs ? frac / (double)out->jobs : 0.0;
+    if (!s->latencies.empty()) {
+        std::vector<double> v = s->latencies;
+        std::sort(v.begin(), v.end());
+        auto q = [&](double p) {
+            const double x = p * (double)(v.size() - 1);
+            const size_t i = (size_t)x;
+            return i + 1 < v.size() ? v[i] + (x - (double)i) * (v[i + 1] - v[i]) : v[i];
+        };
+        out->p50_ms = q(0.50);
+        out->p90_ms = q(0.90);
+        out->p99_ms = q(0.99);
+        out->p999_ms = q(0.999);
+        out->max_ms = v.back();
+        double sum = 0;
+        for (double x : v)
+            sum += x;
+        out->mean_ms = sum / (double)v.size();
+    }
     return MCX_OK;
 }
 

@@ -6,6 +6,8 @@
 // n sweeps, a snapshot and m more sweeps equals a run of n + m sweeps.
 #include "mcx_api_impl.h"
 
+#include <cmath>
+
 using namespace mcx;
 
 namespace {
@@ -115,10 +117,19 @@ extern "C" mcx_status mcx_batch_load_state(mcx_batch* b, const void* buf, size_t
     const size_t n = (size_t)b->n;
     std::vector<ChainRec> rec(n);
     r.get(rec.data(), sizeof(ChainRec) * n);
-    for (size_t i = 0; i < n; ++i)
+    for (size_t i = 0; i < n; ++i) {
         if (rec[i].L != b->rec[i].L || rec[i].dim != b->rec[i].dim ||
             rec[i].spin_off != b->rec[i].spin_off || rec[i].series_off != b->rec[i].series_off)
             return fail(MCX_EINVAL, "mcx_batch_load_state: chain %zu differs in size or layout", i);
+        // the kernels index the tables (and, in temperature order, the series
+        // slots) by the record's table, and the Philox counter holds 48 bits
+        // of the sweep count
+        if (rec[i].table < 0 || rec[i].table >= b->n || rec[i].sweep >= (1ull << 48) ||
+            !std::isfinite(rec[i].D))
+            return fail(MCX_EINVAL, "mcx_batch_load_state: chain %zu: corrupted record (table %d, "
+                                    "sweep %llu)", i, rec[i].table,
+                        (unsigned long long)rec[i].sweep);
+    }
     std::vector<int64_t> steps(n);
     std::vector<double> betas(n);
     r.get(steps.data(), sizeof(int64_t) * n);
@@ -130,6 +141,19 @@ extern "C" mcx_status mcx_batch_load_state(mcx_batch* b, const void* buf, size_t
     const size_t ser = series ? sizeof(int32_t) * (size_t)rec_len * n * (b->d_Q ? 3 : 2) : 0;
     if (r.left() < fixed + ser || rec_len < 0 || rec_len > b->cap)
         return fail(MCX_EINVAL, "mcx_batch_load_state: truncated or inconsistent snapshot");
+    {
+        // the spins must lie in the model's domain, as mcx_batch_set_spins requires
+        BlobReader peek = r;
+        peek.take(fixed - (size_t)b->pool_bytes);
+        const int8_t* pool = reinterpret_cast<const int8_t*>(peek.take((size_t)b->pool_bytes));
+        for (size_t i = 0; i < n; ++i) {
+            const int8_t* s = pool + rec[i].spin_off;
+            for (int64_t k = 0; k < b->nsites[i]; ++k)
+                if (s[k] < -1 || s[k] > 1 || (b->model == MCX_ISING && s[k] == 0))
+                    return fail(MCX_EINVAL, "mcx_batch_load_state: chain %zu: spin %lld = %d is "
+                                            "outside the model's states", i, (long long)k, (int)s[k]);
+        }
+    }
     HIPCHK(hipSetDevice(b->ctx->device));
     hipStream_t st = b->ctx->stream;
     auto h2d = [&](void* dst, size_t len) -> hipError_t {

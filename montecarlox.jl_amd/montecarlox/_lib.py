@@ -75,7 +75,8 @@ class JobDesc(C.Structure):
 class JobResult(C.Structure):
     _fields_ = [("job_id", C.c_uint64), ("tag", C.c_uint64), ("status", C.c_int32),
                 ("L", C.c_int32), ("nsweeps", C.c_int64), ("accepted", C.c_int64),
-                ("E_pair", C.c_int32), ("M", C.c_int32), ("Q", C.c_int32), ("pad", C.c_int32),
+                ("E_pair", C.c_int32), ("M", C.c_int32), ("Q", C.c_int32),
+                ("slo_met", C.c_int32),
                 ("moments", C.c_double * N_MOMENTS), ("latency_ms", C.c_double),
                 ("queue_ms", C.c_double), ("device_ms", C.c_double)]
 

@@ -20,6 +20,16 @@ from . import _lib
 from ._lib import (BLUME_CAPEL, GLAUBER, HEATBATH, ISING, METROPOLIS, JobDesc, JobResult,
                    MCXArgumentError, ServerDesc, ServerStats, call)
 
+class SLOReport(C.Structure):
+    _fields_ = [("slo_ms", C.c_double), ("slo_attempts_per_ns", C.c_double),
+                ("jobs", C.c_int64), ("met", C.c_int64), ("missed", C.c_int64),
+                ("jobs_by_model", C.c_int64 * 2), ("met_by_model", C.c_int64 * 2),
+                ("attainment", C.c_double), ("worst_lateness_ms", C.c_double),
+                ("mean_deadline_frac", C.c_double), ("p50_ms", C.c_double),
+                ("p90_ms", C.c_double), ("p99_ms", C.c_double), ("p999_ms", C.c_double),
+                ("max_ms", C.c_double), ("mean_ms", C.c_double)]
+
+
 _P = C.c_void_p
 for _n, _a in {
     "mcx_server_create": [_P, C.POINTER(ServerDesc), C.POINTER(_P)],
@@ -28,6 +38,8 @@ for _n, _a in {
     "mcx_server_poll": [_P, C.POINTER(JobResult), C.c_int32, C.POINTER(C.c_int32)],
     "mcx_server_drain": [_P, C.c_double],
     "mcx_server_stats": [_P, C.POINTER(ServerStats)],
+    "mcx_server_set_slo": [_P, C.c_double, C.c_double],
+    "mcx_server_slo_report": [_P, C.POINTER(SLOReport)],
 }.items():
     _lib.declare(_n, _a)
 
@@ -70,7 +82,8 @@ def _result_dict(r: JobResult) -> dict:
     return {"job_id": r.job_id, "tag": r.tag, "status": r.status, "L": r.L,
             "nsweeps": r.nsweeps, "accepted": r.accepted, "E_pair": r.E_pair, "M": r.M,
             "Q": r.Q, "moments": np.array(r.moments[:], dtype=np.float64),
-            "latency_ms": r.latency_ms, "queue_ms": r.queue_ms, "device_ms": r.device_ms}
+            "latency_ms": r.latency_ms, "queue_ms": r.queue_ms, "device_ms": r.device_ms,
+            "slo_met": r.slo_met}
 
 
 class Server:
@@ -136,6 +149,20 @@ class Server:
         s = ServerStats()
         call("mcx_server_stats", self._h, C.byref(s))
         return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def set_slo(self, slo_ms: float, attempts_per_ns: float = 0.0) -> None:
+        """Deadline of every job submitted from now on: slo_ms plus its attempts
+        at attempts_per_ns (0: slo_ms alone)."""
+        call("mcx_server_set_slo", self._h, float(slo_ms), float(attempts_per_ns))
+
+    def slo_report(self) -> dict:
+        """Deadlines met / missed (per model too), worst lateness, latency quantiles."""
+        r = SLOReport()
+        call("mcx_server_slo_report", self._h, C.byref(r))
+        out = {f: getattr(r, f) for f, _ in SLOReport._fields_}
+        for f in ("jobs_by_model", "met_by_model"):
+            out[f] = {"ising": int(out[f][0]), "blume_capel": int(out[f][1])}
+        return out
 
     def close(self):
         if self._h:

@@ -157,3 +157,95 @@ def test_snapshot_shape_mismatch(mcx):
     with pytest.raises(mcx.MCXArgumentError):
         a.load_state(np.zeros(64, np.uint8))
     a.load_state(blob)  # its own snapshot loads
+
+
+# byte layout of a snapshot: an 80-byte header, then (batch) the 56-byte chain
+# records, whose table field sits at +40 and sweep counter at +16
+_HDR, _REC, _REC_TABLE, _REC_SWEEP = 80, 56, 40, 16
+
+
+def _refused(mcx, load, blob):
+    with pytest.raises(mcx.MCXArgumentError):
+        load(blob)
+
+
+def test_corrupted_batch_snapshots_are_refused(mcx):
+    """A snapshot whose chain records point outside the tables, whose sweep
+    counter overflows the Philox counter, or whose spins leave the model's
+    states is refused before anything reaches the device; the batch keeps
+    working and still loads the intact snapshot."""
+    print("corrupted batch snapshots", flush=True)
+    for model, bad_spin in (("ising", [5, 0]), ("blume_capel", [2, -3])):
+        b = mcx.LatticeBatch(model, 2, 8, n_chains=4, seed=3, D=0.5)
+        b.set_betas(np.linspace(0.3, 0.5, 4))
+        b.init("random")
+        b.run("metropolis", 5)
+        good = b.save_state(series=False)
+        for chain in (0, 3):
+            blob = good.copy()
+            at = _HDR + chain * _REC + _REC_TABLE
+            blob[at:at + 4] = np.frombuffer(np.int32(4).tobytes(), np.uint8)  # table == n_chains
+            _refused(mcx, b.load_state, blob)
+            blob = good.copy()
+            at = _HDR + chain * _REC + _REC_TABLE
+            blob[at:at + 4] = np.frombuffer(np.int32(-1).tobytes(), np.uint8)
+            _refused(mcx, b.load_state, blob)
+            blob = good.copy()
+            at = _HDR + chain * _REC + _REC_SWEEP
+            blob[at:at + 8] = np.frombuffer(np.uint64(1 << 48).tobytes(), np.uint8)
+            _refused(mcx, b.load_state, blob)
+        pool = 4 * 256  # four 8x8 lattices, 256-byte aligned, at the end of the blob
+        for v in bad_spin:
+            blob = good.copy()
+            blob[len(blob) - pool + 256 + 17] = np.uint8(v & 0xFF)  # chain 1, site 17
+            _refused(mcx, b.load_state, blob)
+        _refused(mcx, b.load_state, good[:-1])
+        b.run("metropolis", 3)
+        b.load_state(good)
+        s0 = b.get_spins(1)
+        b.run("metropolis", 2)
+        b.load_state(good)
+        assert np.array_equal(b.get_spins(1), s0)
+        b.close()
+
+
+def test_corrupted_ladder_snapshots_are_refused(mcx):
+    from montecarlox.spinsystems import SpinSystemBatch
+    print("corrupted replica-exchange snapshots", flush=True)
+    S, R = 2, 4
+    sys_ = SpinSystemBatch("ising", [8, 8], n=S * R)
+    pt = mcx.ParallelTempering(sys_, np.tile(mcx.set_betas(R, 0.3, 0.5, "uniform"), S),
+                               n_sets=S, seed=5)
+    sys_.init_("random", [a.rng for a in pt.algs])
+    pt.run_(6, 2)
+    good = pt.save_state()
+    n = S * R
+    idx_at, pos_at = _HDR + 8 * n, _HDR + 12 * n
+    stage_at = pos_at + 4 * n + 8 * S * (R - 1)
+    ints = lambda a, k: np.frombuffer(a[k:k + 4 * n].tobytes(), np.int32)
+    # pos no longer the inverse of idx (two ladder positions swapped)
+    blob = good.copy()
+    p = ints(blob, pos_at).copy()
+    p[[0, 1]] = p[[1, 0]]
+    blob[pos_at:pos_at + 4 * n] = np.frombuffer(p.tobytes(), np.uint8)
+    _refused(mcx, pt.load_state, blob)
+    # a ladder index outside 1..R
+    blob = good.copy()
+    i = ints(blob, idx_at).copy()
+    i[5] = R + 1
+    blob[idx_at:idx_at + 4 * n] = np.frombuffer(i.tobytes(), np.uint8)
+    _refused(mcx, pt.load_state, blob)
+    # a position outside the set
+    blob = good.copy()
+    p = ints(blob, pos_at).copy()
+    p[2] = -7
+    blob[pos_at:pos_at + 4 * n] = np.frombuffer(p.tobytes(), np.uint8)
+    _refused(mcx, pt.load_state, blob)
+    # a stage that is neither 0 nor 1
+    blob = good.copy()
+    blob[stage_at:stage_at + 4] = np.frombuffer(np.int32(2).tobytes(), np.uint8)
+    _refused(mcx, pt.load_state, blob)
+    pt.load_state(good)
+    pt.run_(2, 2)
+    pt.close()
+    sys_.batch.close()

@@ -122,3 +122,42 @@ def test_update_with_given_energies(mcx):
     pt2 = mcx.ParallelTempering(SpinSystemBatch("ising", [8, 8], n=2), [1.0, 0.5], seed=3)
     pt2.update_([0.0, -5.0])  # lr = 2.5 > 0: accepted
     assert pt2.algs[0].ensemble.beta == 0.5 and pt2.algs[1].ensemble.beta == 1.0
+
+
+def test_graph_replays_equal_eager_runs(mcx, orc):
+    """The instantiated graph is kept: later runs of the same shape replay it
+    (and recording runs update it in place).  Every replay, with eager work
+    and synchronisations between them, equals the eager run and the oracle."""
+    from montecarlox.spinsystems import SpinSystemBatch
+    n_sets, R, L, seed, rounds, sb = 2, 4, 8, 77, 3, 2
+    ladder = np.concatenate([mcx.set_betas(R, 0.3, 0.5, "uniform"),
+                             mcx.set_betas(R, 0.25, 0.45, "geometric")])
+    Ls = [L] * (n_sets * R)
+    runs = {}
+    for graph in (False, True):
+        sys_ = SpinSystemBatch("ising", [L, L], n=len(Ls), L=Ls, series_capacity=4 * rounds * sb)
+        pt = mcx.ParallelTempering(sys_, ladder, n_sets=n_sets, seed=seed)
+        sys_.init_("random", [a.rng for a in pt.algs])
+        init = [sys_.batch.get_spins(k) for k in range(len(Ls))]
+        snaps = []
+        for rep in range(4):
+            print(f"graph={graph} replay {rep}", flush=True)
+            pt.run_(rounds, sb, graph=graph, record=rep >= 2)  # reps 2, 3: update path
+            snaps.append((pt.indices.copy(), pt.accepted.copy(), pt.stage.copy(),
+                          [sys_.batch.get_spins(k) for k in range(len(Ls))]))
+            sys_.batch.observables()  # eager device work between the replays
+        E = [sys_.batch.series(k)[0] for k in range(len(Ls))]
+        runs[graph] = (init, snaps, E)
+        pt.close()
+        sys_.batch.close()
+    (init, eager, Ee), (_, replays, Eg) = runs[False], runs[True]
+    for rep, (a, b) in enumerate(zip(eager, replays)):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), rep
+        assert np.array_equal(a[2], b[2]), rep
+        assert all(np.array_equal(x, y) for x, y in zip(a[3], b[3])), rep
+    assert all(np.array_equal(x, y) for x, y in zip(Ee, Eg))
+    keys = np.arange(1, n_sets * R + 1)
+    spins, idx, steps, acc, stage = _oracle_pt(orc, 2, Ls, n_sets, R, ladder, seed, keys,
+                                               4 * rounds, sb, init_spins=init)
+    assert np.array_equal(replays[-1][0], idx) and np.array_equal(replays[-1][1], acc)
+    assert all(np.array_equal(x, y) for x, y in zip(replays[-1][3], spins))

@@ -202,3 +202,35 @@ def test_server_destroy_with_work_in_flight(mcx):
     for k in range(6):
         srv.submit(mcx.Job("ising", 32, 0.44, 400, seed=1, key=k))
     srv.close()  # stops after the current quantum; must not hang
+
+
+def test_slo_report(mcx):
+    """Deadlines of slo_ms + attempts / rate: a loose SLO is met by every job, an
+    impossible one by none; per-job flags, per-model counts and the latency
+    quantiles agree with the results and with the server's own statistics."""
+    jobs = _jobs(mcx)
+    print("SLO report", flush=True)
+    with mcx.Server(hbm_budget=1 << 26, max_jobs=8) as srv:
+        r0 = srv.slo_report()
+        assert r0["jobs"] == 0 and r0["attainment"] == 1.0
+        res = srv.run(jobs[:4])              # before any SLO: no deadline
+        assert all(r["slo_met"] == -1 for r in res)
+        srv.set_slo(60_000.0)                # one minute: always met
+        res = srv.run(jobs)
+        assert all(r["slo_met"] == 1 for r in res)
+        r1 = srv.slo_report()
+        assert (r1["jobs"], r1["met"], r1["missed"]) == (len(jobs), len(jobs), 0)
+        assert r1["worst_lateness_ms"] < 0 and 0 < r1["mean_deadline_frac"] < 1
+        n_bc = sum(j.model == "blume_capel" for j in jobs)
+        assert r1["jobs_by_model"] == {"ising": len(jobs) - n_bc, "blume_capel": n_bc}
+        srv.set_slo(0.0, 1e9)                # a billion attempts per ns: never met
+        res = srv.run(jobs[:5])
+        assert all(r["slo_met"] == 0 for r in res)
+        r2 = srv.slo_report()
+        assert r2["jobs"] == len(jobs) + 5 and r2["missed"] == 5 and r2["worst_lateness_ms"] > 0
+        assert r2["attainment"] == len(jobs) / (len(jobs) + 5)
+        st = srv.stats()
+        assert abs(r2["p50_ms"] - st["p50_ms"]) < 1e-9 and abs(r2["p99_ms"] - st["p99_ms"]) < 1e-9
+        assert r2["p50_ms"] <= r2["p90_ms"] <= r2["p99_ms"] <= r2["p999_ms"] <= r2["max_ms"]
+        with pytest.raises(mcx.MCXArgumentError):
+            srv.set_slo(float("nan"))

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _mirrors():
-    from montecarlox import _lib, muca, pt
+    from montecarlox import _lib, muca, pt, serving
     return [
         ("mcx_batch_desc", _lib.BatchDesc),
         ("mcx_pt_desc", pt.PTDesc),
@@ -23,6 +23,10 @@ def _mirrors():
         ("mcx_job_desc", _lib.JobDesc),
         ("mcx_job_result", _lib.JobResult),
         ("mcx_server_stats_t", _lib.ServerStats),
+        ("mcx_launch_shape", _lib.LaunchShape),
+        ("mcx_gen_schedule", muca.GenSchedule),
+        ("mcx_gen_progress", muca.GenProgress),
+        ("mcx_slo_report", serving.SLOReport),
     ]
 
 
@@ -68,8 +72,10 @@ def _julia_structs(text):
             elif t.startswith("NTuple"):
                 k, e = re.match(r"NTuple\{(\d+),(\w+)\}", t).groups()
                 n = int(k) * _JL_SIZE[e]
-            else:
+            elif t in _JL_SIZE:
                 n = _JL_SIZE[t]
+            else:
+                n = None  # a host-side Julia struct (not passed to C)
             fields.append((f, n))
         out[m.group(1)] = fields
     return out
@@ -78,12 +84,15 @@ def _julia_structs(text):
 def test_julia_bindings_match_the_ctypes_mirrors():
     """The Julia structs INTEGRATION.md gives a MonteCarloX.jl user have the
     fields, in order and of the sizes, of the mirrors checked above."""
-    from montecarlox import _lib, pt
+    from montecarlox import _lib, muca, pt, serving
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     jl = _julia_structs(text)
     for name, cls in [("BatchDesc", _lib.BatchDesc), ("PTDesc", pt.PTDesc),
                       ("ServerDesc", _lib.ServerDesc), ("JobDesc", _lib.JobDesc),
-                      ("JobResult", _lib.JobResult)]:
+                      ("JobResult", _lib.JobResult), ("LaunchShape", _lib.LaunchShape),
+                      ("GenDesc", muca.GenDesc), ("GenSchedule", muca.GenSchedule),
+                      ("GenProgress", muca.GenProgress), ("ServerStats", _lib.ServerStats),
+                      ("SLOReport", serving.SLOReport)]:
         assert name in jl, name
         want = [(f, C.sizeof(t)) for f, t in cls._fields_]
         assert jl[name] == want, (name, jl[name], want)
