@@ -184,8 +184,9 @@ mcx_status mcx_batch_last_run_ms(mcx_batch* b, double* ms);
  *                     one lane per plane word, 64..256)
  *   chains_per_group  chains one workgroup runs one after another (>= 1; 0 = 1)
  *   lanes_per_word    Blume-Capel: lanes sharing one 32-site plane word
- *                     (1, 2, 4, 8; 0 = 1; -1 = the most, up to 8, that fit the
- *                     workgroup); must be 0 for Ising */
+ *                     (1, 2, 4, 8; 0 = 1; -1 = the most, up to 4, that fit the
+ *                     workgroup; a word draws 4 prefix blocks, so lanes past
+ *                     the fourth idle); must be 0 for Ising */
 typedef struct {
     int32_t threads;
     int32_t chains_per_group;

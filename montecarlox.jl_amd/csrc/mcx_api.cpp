@@ -215,11 +215,11 @@ size_t lattice_lds_bytes(int model, int dim, int L)
     const int nw = (half + 31) / 32;
     const int64_t rows = dim == 3 ? (int64_t)L * L : L;
     // Ising: 192-word header (partials, 64-sweep E/M ring) + 2 planes;
-    // Blume-Capel: 496-word header (partials, tables, E/M/Q ring) + occupancy and
-    // up planes per colour
+    // Blume-Capel: static class tables + kBCLdsHeaderWords-word header
+    // (partials, E/M/Q ring) + occupancy and up planes per colour
     if (model == MCX_ISING)
         return 768 + 2 * (size_t)rows * (size_t)nw * 4;
-    return 1984 + 4 * (size_t)rows * (size_t)nw * 4;
+    return (size_t)kBCStaticLds + 4 * (size_t)kBCLdsHeaderWords + 4 * (size_t)rows * (size_t)nw * 4;
 }
 
 int lattice_block(int dim, int L)

@@ -9,14 +9,26 @@
 //   heat bath (blume_capel.jl:61-85):
 //     w1 = exp(-beta*e1), w2 = exp(-beta*0), w3 = exp(-beta*e3), z = w1+w2+w3,
 //     r = u * z; s_new = r < w1 ? -1 : (r < w1 + w2 ? 0 : 1)
-// Random words: site word w of (group, bit) = word (bit & 3) of Philox block
-// (8*group + bit/4, sweep, purpose).  Metropolis: b = w & 1, u = (w >> 1) / 2^31;
-// heat bath: u = w / 2^32.
+// Random words (purpose P_BC_METRO for Metropolis and Glauber, P_BC_HB for
+// heat bath), for the 32-site word `group` of one sweep:
+//   prefix blocks: Philox block i (i = 0..3) at counter (8*group + i, sweep);
+//     its word m holds the 16-bit fields of sites k = 4i + m (low half) and
+//     k + 16 (high half); a site's field f gives u16 = f ^ 0x8000;
+//   tail: the site's 32-bit uniform continues with t16 = low 16 bits of word 0
+//     of the block at counter (8*group + 5, sweep, chunk 1 + bit).
+// Metropolis / Glauber: the proposal bit b = u16 & 1 and the 31-bit uniform
+// U31 = (u16 >> 1) * 2^16 + t16 (accept iff U31 < ceil(p * 2^31), i.e.
+// U31 / 2^31 < p); heat bath: U = u16 * 2^16 + t16 and r = (U / 2^32) * z.
+// The tail matters only when the prefix equals the threshold's leading bits
+// (probability 2^-15 or 2^-16 per comparison): the kernels draw it then only.
 //
 // Batched kernel: one workgroup per chain; the lattice lives in LDS as four bit
 // planes (occupied, up) x (colour 0, colour 1) in the word layout of the Ising
-// kernels; one thread updates the 32 sites of a word, drawing one Philox block
-// per 4 sites.  The neighbour sum is formed bit-sliced for all 32 sites at once.
+// kernels; one thread updates the 32 sites of a word: the neighbour sum is
+// formed bit-sliced for all 32 sites at once, an 8 x 8 bit-matrix transpose
+// turns the planes into one class byte per site, and each pair of sites is
+// decided by one packed 16-bit saturating subtract of its two prefixes against
+// its two thresholds (looked up by class in LDS).
 #include "mcx_api_impl.h"
 #include "mcx_device.h"
 
@@ -108,15 +120,33 @@ void fill_bc_gl_from_p(BCTable& t, int dim, const double* p)
     fill_bc_thr31(t.gl_thr, t.gl_always, dim, p);
 }
 
+// The number of 32-bit uniforms U with fl(fl(U * 2^-32) * z) < w: the first U
+// for which the product is not below w (2^32 if there is none).  For finite
+// z >= 0 the product is monotone in U, so these U form a prefix of [0, 2^32);
+// with z = inf every product is inf or NaN (U = 0), never below w, and the
+// count is 0.  The same IEEE double operations as the per-site formula.
+static uint64_t hb_threshold(double w, double z)
+{
+    uint64_t lo = 0, hi = (uint64_t)1 << 32;
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        const double r = ((double)mid * 0x1.0p-32) * z;
+        if (r < w)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 void fill_bc_hb_from_p(BCTable& t, int dim, const double* p)
 {
     const int NS = 4 * dim + 1;
     for (int k = 0; k < 13; ++k)
-        t.hb[k][0] = t.hb[k][1] = t.hb[k][2] = 0;
+        t.hb_w[k][0] = t.hb_w[k][1] = 0;
     for (int k = 0; k < NS; ++k) {
-        t.hb[k][0] = p[3 * k];
-        t.hb[k][1] = p[3 * k + 1];
-        t.hb[k][2] = p[3 * k + 2];
+        t.hb_w[k][0] = hb_threshold(p[3 * k], p[3 * k + 2]);
+        t.hb_w[k][1] = hb_threshold(p[3 * k + 1], p[3 * k + 2]);
     }
 }
 
@@ -145,14 +175,18 @@ struct BCNbr {
 
 // Neighbour occupancy / up words of word q of plane row iA, given the plane
 // rows of its y-1, y+1 (and z-1, z+1) neighbours in the other colour's planes.
-template <int DIM>
+// SMALL: every row index times nw fits 24 bits (LDS-resident planes).
+template <int DIM, bool SMALL = false>
 __device__ __forceinline__ void bc_nbrs_rows(const uint32_t* __restrict__ Oo,
                                              const uint32_t* __restrict__ So, int nw, int half,
                                              int iA0, int iU0, int iD0, int iZu0, int iZd0, int q,
                                              int p, int W, uint32_t valid, BCNbr& n)
 {
+    auto mul = [&](int a) {
+        return SMALL ? (int)__umul24((uint32_t)a, (uint32_t)nw) : a * nw;
+    };
     const int r = iA0;
-    const int iU = iU0 * nw + q, iD = iD0 * nw + q, iA = r * nw + q;
+    const int iU = mul(iU0) + q, iD = mul(iD0) + q, iA = mul(r) + q;
     n.occ[0] = Oo[iU];
     n.up[0] = So[iU];
     n.occ[1] = Oo[iD];
@@ -160,7 +194,7 @@ __device__ __forceinline__ void bc_nbrs_rows(const uint32_t* __restrict__ Oo,
     n.occ[2] = Oo[iA];
     n.up[2] = So[iA];
     if (DIM == 3) {
-        const int iZu = iZu0 * nw + q, iZd = iZd0 * nw + q;
+        const int iZu = mul(iZu0) + q, iZd = mul(iZd0) + q;
         n.occ[4] = Oo[iZu];
         n.up[4] = So[iZu];
         n.occ[5] = Oo[iZd];
@@ -179,17 +213,17 @@ __device__ __forceinline__ void bc_nbrs_rows(const uint32_t* __restrict__ Oo,
         }
     } else if (p) {
         const int qn = q + 1 == nw ? 0 : q + 1;
-        n.occ[3] = (A0 >> 1) | ((Oo[r * nw + qn] & 1u) << (W - 1));
-        n.up[3] = (A1 >> 1) | ((So[r * nw + qn] & 1u) << (W - 1));
+        n.occ[3] = (A0 >> 1) | ((Oo[mul(r) + qn] & 1u) << (W - 1));
+        n.up[3] = (A1 >> 1) | ((So[mul(r) + qn] & 1u) << (W - 1));
     } else {
         const int qp = q == 0 ? nw - 1 : q - 1;
         const int Wp = word_width(qp, half);
-        n.occ[3] = ((A0 << 1) | ((Oo[r * nw + qp] >> (Wp - 1)) & 1u)) & valid;
-        n.up[3] = ((A1 << 1) | ((So[r * nw + qp] >> (Wp - 1)) & 1u)) & valid;
+        n.occ[3] = ((A0 << 1) | ((Oo[mul(r) + qp] >> (Wp - 1)) & 1u)) & valid;
+        n.up[3] = ((A1 << 1) | ((So[mul(r) + qp] >> (Wp - 1)) & 1u)) & valid;
     }
 }
 
-template <int DIM>
+template <int DIM, bool SMALL = false>
 __device__ __forceinline__ void bc_load_nbrs(const uint32_t* __restrict__ Oo,
                                              const uint32_t* __restrict__ So, int L, int nw,
                                              int half, int r, int q, int y, int z, int p, int W,
@@ -200,8 +234,8 @@ __device__ __forceinline__ void bc_load_nbrs(const uint32_t* __restrict__ Oo,
     const int yd = y == L - 1 ? 0 : y + 1;
     const int zu = DIM == 3 ? (z == 0 ? L - 1 : z - 1) : 0;
     const int zd = DIM == 3 ? (z == L - 1 ? 0 : z + 1) : 0;
-    bc_nbrs_rows<DIM>(Oo, So, nw, half, r, base + yu, base + yd, zu * L + y, zd * L + y, q, p, W,
-                      valid, n);
+    bc_nbrs_rows<DIM, SMALL>(Oo, So, nw, half, r, base + yu, base + yd, zu * L + y, zd * L + y, q,
+                             p, W, valid, n);
 }
 
 // Bit-sliced (sum_j s_j + 2*dim) for 32 sites: each neighbour adds 2 if up,
@@ -281,29 +315,12 @@ __device__ __forceinline__ void bc_unpack_word(int8_t* __restrict__ sp,
     }
 }
 
-// Threshold table of the device kernels, indexed by a site's class
-// i = sidx | o << 4 | u << 5 | alt << 6 (sidx = sum of the neighbour spins +
-// 2*DIM, o / u the site's occupancy and up bits, alt the proposal bit), from
-// the host layout k = ((s_old + 1) * 2 + alt) * NS + sidx.  Classes that no
-// site can be in (sidx >= NS, or up without occupancy) get 0.
-constexpr int kBCClasses = 128; // class index: 4 neighbour-sum bits, o, u, alt
-
-// Bit 4p + b of x moves to bit 8b + p (p < 8, b < 4): a 8 x 4 bit-matrix
-// transpose, the bit index rotated right by 2, as four swaps of index bits
-// (0 1), (0 2), (1 3), (2 4).
-__device__ __forceinline__ uint32_t bits_8x4_transpose(uint32_t x)
-{
-    uint32_t t;
-    t = ((x >> 1) ^ x) & 0x22222222u;
-    x ^= t ^ (t << 1);
-    t = ((x >> 3) ^ x) & 0x0a0a0a0au;
-    x ^= t ^ (t << 3);
-    t = ((x >> 6) ^ x) & 0x00cc00ccu;
-    x ^= t ^ (t << 6);
-    t = ((x >> 12) ^ x) & 0x0000f0f0u;
-    x ^= t ^ (t << 12);
-    return x;
-}
+// Class of a site, i = sidx | o << 4 | u << 5 | alt << 6 (sidx = sum of the
+// neighbour spins + 2*DIM, o / u the site's occupancy and up bits, alt the
+// proposal bit), and its threshold from the host layout
+// k = ((s_old + 1) * 2 + alt) * NS + sidx.  Classes that no site can be in
+// (sidx >= NS, or up without occupancy) get 0.
+constexpr int kBCClasses = 128;
 
 template <int DIM>
 __device__ __forceinline__ uint32_t bc_class_thr(const uint32_t* thr, int i)
@@ -316,105 +333,306 @@ __device__ __forceinline__ uint32_t bc_class_thr(const uint32_t* thr, int i)
     return thr[((s_old + 1) * 2 + alt) * NS + sidx];
 }
 
+// LDS tables of one chain for the word update.  A site's 16-bit field f read
+// as int16 is u16 - 32768 (u16 = f ^ 0x8000), so that a saturating int16
+// subtract of a stored (A - 32768) is negative exactly when u16 < A.
+//   Metropolis / Glauber, by class: a16 = min(2 * (thr >> 16), 65535) - 32768:
+//     u16 < 2 * (thr >> 16) exactly when the uniform's leading 15 bits
+//     u16 >> 1 are below thr's (accept); a difference of 0 or 1 is a tie
+//     (u16 >> 1 == thr >> 16, decided on the full uniform against thr).
+//   Heat bath, by sidx: a32 = (A(W1) - 32768) | (A(W12) - 32768) << 16 with
+//     A(W) = min(W >> 16, 65535); a difference of 0 is a tie (decided against
+//     hbw = (W1, W12)).
+struct BCTabs {
+    int16_t a16[kBCClasses];
+    uint32_t a32[16];
+    uint32_t thr[kBCClasses];
+    uint64_t hbw[16][2];
+};
+constexpr int kBCTabWords = (int)(sizeof(BCTabs) / 4);
+
+template <int DIM, int ALG>
+__device__ __forceinline__ void bc_fill_tabs(BCTabs& t, const BCTable* tb, int glauber, int tid,
+                                             int nthr)
+{
+    if (ALG == 0) {
+        const uint32_t* src = glauber ? tb->gl_thr : tb->metro_thr;
+        for (int k = tid; k < kBCClasses; k += nthr) {
+            const uint32_t thr = bc_class_thr<DIM>(src, k);
+            const int v = 2 * (int)(thr >> 16) - 32768;
+            t.a16[k] = (int16_t)(v > 32767 ? 32767 : v);
+            t.thr[k] = thr;
+        }
+    } else {
+        constexpr int NS = 4 * DIM + 1;
+        for (int k = tid; k < 16; k += nthr) {
+            const uint64_t w1 = k < NS ? tb->hb_w[k][0] : 0, w12 = k < NS ? tb->hb_w[k][1] : 0;
+            const uint32_t a1 = (uint32_t)((w1 >> 16) < 65535 ? (w1 >> 16) : 65535) - 32768u;
+            const uint32_t a2 = (uint32_t)((w12 >> 16) < 65535 ? (w12 >> 16) : 65535) - 32768u;
+            t.a32[k] = (a1 & 0xffffu) | (a2 << 16);
+            t.hbw[k][0] = w1;
+            t.hbw[k][1] = w12;
+        }
+    }
+}
+
+// Rows p, p + S of an 8 x 8 bit-matrix stored by byte column: exchange the
+// columns whose index has bit S set in row p with those where it is clear in
+// row p + S (m = the columns with it clear).
+template <int S>
+__device__ __forceinline__ void bc_swap_rows(uint32_t& a, uint32_t& b, uint32_t m)
+{
+    const uint32_t na = (a & m) | ((b << S) & ~m);
+    const uint32_t nb = ((a >> S) & m) | (b & ~m);
+    a = na;
+    b = nb;
+}
+
+// 8 x 8 bit-matrix transpose within each byte column of x[0..7]: afterwards
+// bit p of byte b of x[r] is what bit r of byte b of x[p] was.
+__device__ __forceinline__ void bc_transpose8(uint32_t* x)
+{
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+        bc_swap_rows<4>(x[p], x[p + 4], 0x0f0f0f0fu);
+#pragma unroll
+    for (int p = 0; p < 8; p += (p & 1) ? 3 : 1) // 0, 1, 4, 5
+        bc_swap_rows<2>(x[p], x[p + 2], 0x33333333u);
+#pragma unroll
+    for (int p = 0; p < 8; p += 2)
+        bc_swap_rows<1>(x[p], x[p + 1], 0x55555555u);
+}
+
+__device__ __forceinline__ uint32_t rotr32(uint32_t x, uint32_t s)
+{
+    return __builtin_amdgcn_alignbit(x, x, s);
+}
+
+// 16-bit field of site `bit` of word `group` (see the random-word layout above)
+// and the 16 bits that continue its uniform.  w2: counter word 2 of chunk 0.
+__device__ __forceinline__ uint32_t bc_site_field(uint32_t group, int bit, uint32_t sw_lo,
+                                                  uint32_t w2, uint32_t c3, uint32_t k0,
+                                                  uint32_t k1)
+{
+    const int k = bit & 15;
+    const uint4 R = philox4x32_10(8u * group + (uint32_t)(k >> 2), sw_lo, w2, c3, k0, k1);
+    const int m = k & 3;
+    const uint32_t w = m == 0 ? R.x : (m == 1 ? R.y : (m == 2 ? R.z : R.w));
+    return bit >= 16 ? w >> 16 : w & 0xffffu;
+}
+
+__device__ __forceinline__ uint32_t bc_site_tail(uint32_t group, int bit, uint32_t sw_lo,
+                                                 uint32_t w2, uint32_t c3, uint32_t k0,
+                                                 uint32_t k1)
+{
+    return philox4x32_10(8u * group + 5u, sw_lo, w2 | ((uint32_t)(1 + bit) << 16), c3, k0, k1).x &
+           0xffffu;
+}
+
+typedef short bc_s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short bc_u16x2 __attribute__((ext_vector_type(2)));
+
 // One 32-site word of colour c: updates its occupancy / up bits in place from
-// the bit-sliced shifted neighbour sums v0..v3 (the per-site rule above, one
-// Philox block per 4 sites) and returns the number of changed sites this lane
-// decided.  The sites of a word are independent (one colour), so a word may
-// be shared by lpw adjacent lanes (lpw = 1, 2, 4 or 8, lane part = lane % lpw):
-// each takes the Philox blocks part, part + lpw, ... and the lanes then OR
-// their bit changes together, so every lane of the group ends with the whole
-// word.  tabw: the kBCClasses Metropolis (or Glauber) thresholds by class
-// (bc_class_thr; 2^31 for an always-accepted class, above every w >> 1),
-// tabd: 13 x (w1, w1 + w2, z).  Metropolis collects the accept and proposal
-// bits of the sites and moves all accepted sites of the word at once: the
-// proposal (the lower other state if alt, else the upper) changes the
-// occupancy where ~o | (u ^ alt) and the up bit where ~alt | u.  Shared by the
-// LDS-resident batched kernel and the multi-workgroup tile and band kernels.
+// the bit-sliced shifted neighbour sums v0..v3 (the per-site rule and random
+// words above) and returns the number of changed sites this lane decided.
+// The class bytes of all 32 sites come from one 8 x 8 bit-matrix transpose of
+// the planes (byte = 2 * class for the int16 Metropolis table, 4 * sidx for
+// the heat-bath pairs); pair k of prefix block i (k = 4i + m) holds sites k and
+// k + 16, whose two thresholds are subtracted from the two fields at once.
+// The sites of a word are independent (one colour), so a word may be shared
+// by lpw adjacent lanes (lpw = 1, 2, 4 or 8, lane part = lane % lpw): each
+// takes the prefix blocks part, part + lpw, ... and the lanes then OR their bit
+// changes together, so every lane of the group ends with the whole word.
+// Metropolis collects the accept and proposal bits of the sites and moves all
+// accepted sites of the word at once: the proposal (the lower other state if
+// alt, else the upper) changes the occupancy where ~o | (u ^ alt) and the up
+// bit where ~alt | u.  Shared by the LDS-resident batched kernel and the
+// multi-workgroup tile and band kernels.
 template <int DIM, int ALG>
 __device__ __forceinline__ uint32_t bc_word(uint32_t& occ, uint32_t& up, uint32_t v0, uint32_t v1,
-                                            uint32_t v2, uint32_t v3, int W,
-                                            const uint32_t* __restrict__ tabw,
-                                            const double* __restrict__ tabd, uint32_t group,
-                                            uint32_t sw_lo, uint32_t w2, uint32_t c3, uint32_t k0,
-                                            uint32_t k1, int part, int lpw)
+                                            uint32_t v2, uint32_t v3, int W, const BCTabs& T,
+                                            uint32_t group, uint32_t sw_lo, uint32_t w2,
+                                            uint32_t c3, uint32_t k0, uint32_t k1, int part,
+                                            int lpw)
 {
-    uint32_t xo = 0, xu = 0; // bits of the sites this lane changes
-    uint32_t n_changed;
+    const uint32_t valid = width_mask(W);
+    // Rotating the planes right by 4 * part puts the sites of prefix block
+    // part + i where those of block i are, so that every lane runs the loop
+    // below over the same compile-time pair positions.
+    const uint32_t rot = 4u * (uint32_t)part;
+    const bool shared = lpw > 1; // uniform: no rotation for one lane per word
+    auto rin = [&](uint32_t v) { return shared ? rotr32(v, rot) : v; };
+    uint32_t x[8];
     if (ALG == 0) {
-        uint32_t okm = 0, altm = 0;
-        for (int j0 = 4 * part; j0 < W; j0 += 4 * lpw) {
-            const uint4 R = philox4x32_10(8u * group + (uint32_t)(j0 >> 2), sw_lo, w2, c3, k0, k1);
-            const uint32_t Rb[4] = {R.x, R.y, R.z, R.w};
-            if (j0 + 4 <= W) {
-                // the block's 4 sites at once: their bits of the 6 planes and the
-                // proposal bits, plane p in bits 4p..4p+3, transposed so that
-                // site b's 7-bit class is bits 8b..8b+6
-                const uint32_t u0 = (uint32_t)j0;
-                const uint32_t alt4 = (Rb[0] & 1u) | ((Rb[1] & 1u) << 1) | ((Rb[2] & 1u) << 2) |
-                                      ((Rb[3] & 1u) << 3);
-                uint32_t x = __builtin_amdgcn_ubfe(v0, u0, 4u);
-                x |= __builtin_amdgcn_ubfe(v1, u0, 4u) << 4;
-                x |= __builtin_amdgcn_ubfe(v2, u0, 4u) << 8;
-                x |= __builtin_amdgcn_ubfe(v3, u0, 4u) << 12;
-                x |= __builtin_amdgcn_ubfe(occ, u0, 4u) << 16;
-                x |= __builtin_amdgcn_ubfe(up, u0, 4u) << 20;
-                x |= alt4 << 24;
-                x = bits_8x4_transpose(x);
-                uint32_t ok4 = 0;
+        // class byte: bit 0 clear, bits 1-4 sidx, 5 occupancy, 6 up, 7 proposal
+        x[0] = 0;
+        x[1] = rin(v0);
+        x[2] = rin(v1);
+        x[3] = rin(v2);
+        x[4] = rin(v3);
+        x[5] = rin(occ);
+        x[6] = rin(up);
+        x[7] = 0;
+    } else {
+        // byte = 4 * sidx
+        x[0] = x[1] = 0;
+        x[2] = rin(v0);
+        x[3] = rin(v1);
+        x[4] = rin(v2);
+        x[5] = rin(v3);
+        x[6] = x[7] = 0;
+    }
+    bc_transpose8(x);
+    // byte b of x[r]: site 8b + r (rotated positions)
+    uint32_t lt1 = 0, lt2 = 0, altm = 0, own = 0;
+    bc_u16x2 tz = {0xffff, 0xffff};
+    const int nblk = W > 16 ? 4 : (W + 3) >> 2; // prefix blocks holding a valid site
+    const uint32_t gb = 8u * group + (uint32_t)part;
 #pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const uint32_t cls = __builtin_amdgcn_ubfe(x, 8u * b, 7u);
-                    // (w >> 1) - thr < 0 exactly when w >> 1 < thr (both at most 2^31)
-                    ok4 |= (((Rb[b] >> 1) - tabw[cls]) >> 31) << b;
-                }
-                okm |= ok4 << u0;
-                altm |= alt4 << u0;
-                continue;
-            }
+    for (int i = 0; i < 4; ++i) {
+        if ((i & (lpw - 1)) != 0 || part + i >= nblk)
+            continue;
+        const uint4 R = philox4x32_10(gb + (uint32_t)i, sw_lo, w2, c3, k0, k1);
+        const uint32_t Rw[4] = {R.x, R.y, R.z, R.w};
+        own |= 0x000f000fu << (4 * i);
+        // the block's 8 table loads first, then the 4 packed compares
+        uint32_t tlo[4], thi[4];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int j = j0 + b;
-                if (j >= W)
-                    break;
-                const uint32_t rw = Rb[b];
-                const uint32_t uj = (uint32_t)j;
-                uint32_t cls = (rw & 1u) << 6;
-                cls |= __builtin_amdgcn_ubfe(up, uj, 1u) << 5;
-                cls |= __builtin_amdgcn_ubfe(occ, uj, 1u) << 4;
-                cls |= __builtin_amdgcn_ubfe(v3, uj, 1u) << 3;
-                cls |= __builtin_amdgcn_ubfe(v2, uj, 1u) << 2;
-                cls |= __builtin_amdgcn_ubfe(v1, uj, 1u) << 1;
-                cls |= __builtin_amdgcn_ubfe(v0, uj, 1u);
-                // (w >> 1) - thr < 0 exactly when w >> 1 < thr (both below 2^31 + 1)
-                okm |= (((rw >> 1) - tabw[cls]) >> 31) << j;
-                altm |= (rw & 1u) << j;
+        for (int m = 0; m < 4; ++m) {
+            const int k = 4 * i + m;
+            // class bytes of sites k, k + 16 in bytes 0 and 2
+            uint32_t c = k < 8 ? x[k] : x[k - 8] >> 8;
+            if (ALG == 0) {
+                c |= (Rw[m] << 7) & 0x00800080u; // proposal bits -> bit 7
+                tlo[m] = *reinterpret_cast<const uint16_t*>(
+                    reinterpret_cast<const char*>(T.a16) + (c & 0xffu));
+                thi[m] = *reinterpret_cast<const uint16_t*>(
+                    reinterpret_cast<const char*>(T.a16) + __builtin_amdgcn_ubfe(c, 16u, 8u));
+            } else {
+                tlo[m] = *reinterpret_cast<const uint32_t*>(
+                    reinterpret_cast<const char*>(T.a32) + (c & 0xffu));
+                thi[m] = *reinterpret_cast<const uint32_t*>(
+                    reinterpret_cast<const char*>(T.a32) + __builtin_amdgcn_ubfe(c, 16u, 8u));
             }
         }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int k = 4 * i + m;
+            const uint32_t f = Rw[m];
+            const bc_s16x2 fs = __builtin_bit_cast(bc_s16x2, f);
+            if (ALG == 0) {
+                altm |= (f << k) & (0x00010001u << k);
+                const bc_s16x2 th =
+                    __builtin_bit_cast(bc_s16x2, __builtin_amdgcn_perm(thi[m], tlo[m], 0x05040100u));
+                const bc_s16x2 d = __builtin_elementwise_sub_sat(fs, th);
+                const uint32_t du = __builtin_bit_cast(uint32_t, d);
+                lt1 |= (du >> (15 - k)) & (0x00010001u << k);
+                tz = __builtin_elementwise_min(tz, __builtin_bit_cast(bc_u16x2, d));
+            } else {
+                const bc_s16x2 th1 = __builtin_bit_cast(
+                    bc_s16x2, __builtin_amdgcn_perm(thi[m], tlo[m], 0x05040100u));
+                const bc_s16x2 th2 = __builtin_bit_cast(
+                    bc_s16x2, __builtin_amdgcn_perm(thi[m], tlo[m], 0x07060302u));
+                const bc_s16x2 d1 = __builtin_elementwise_sub_sat(fs, th1);
+                const bc_s16x2 d2 = __builtin_elementwise_sub_sat(fs, th2);
+                const uint32_t du1 = __builtin_bit_cast(uint32_t, d1);
+                const uint32_t du2 = __builtin_bit_cast(uint32_t, d2);
+                lt1 |= (du1 >> (15 - k)) & (0x00010001u << k);
+                lt2 |= (du2 >> (15 - k)) & (0x00010001u << k);
+                tz = __builtin_elementwise_min(
+                    tz, __builtin_elementwise_min(__builtin_bit_cast(bc_u16x2, d1),
+                                                  __builtin_bit_cast(bc_u16x2, d2)));
+            }
+        }
+    }
+    const uint32_t tzw = __builtin_bit_cast(uint32_t, tz);
+    const uint32_t tie_bits = ALG == 0 ? 0xfffefffeu : 0xffffffffu;
+    if ((tzw & tie_bits & 0xffffu) == 0 || (tzw & tie_bits & 0xffff0000u) == 0) {
+        // rare: a site whose prefix equals its threshold's leading bits; mark
+        // those sites, then decide each on its full uniform
+        uint32_t tmask = 0;
+        const bc_u16x2 one = {1, 1};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if ((i & (lpw - 1)) != 0 || part + i >= nblk)
+                continue;
+            const uint4 R = philox4x32_10(gb + (uint32_t)i, sw_lo, w2, c3, k0, k1);
+            const uint32_t Rw[4] = {R.x, R.y, R.z, R.w};
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int k = 4 * i + m;
+                const uint32_t f = Rw[m];
+                uint32_t c = k < 8 ? x[k] : x[k - 8] >> 8;
+                const bc_s16x2 fs = __builtin_bit_cast(bc_s16x2, f);
+                uint32_t z; // bit 15 / 31: the low / high site ties
+                if (ALG == 0) {
+                    c |= (f << 7) & 0x00800080u;
+                    bc_s16x2 th;
+                    th.x = *reinterpret_cast<const int16_t*>(
+                        reinterpret_cast<const char*>(T.a16) + (c & 0xffu));
+                    th.y = *reinterpret_cast<const int16_t*>(
+                        reinterpret_cast<const char*>(T.a16) + ((c >> 16) & 0xffu));
+                    const bc_u16x2 e = __builtin_bit_cast(
+                        bc_u16x2, __builtin_bit_cast(uint32_t, __builtin_elementwise_sub_sat(fs, th)) &
+                                      0xfffefffeu);
+                    // a zero half of e: (e - 1) has bit 15 set and e has not
+                    z = __builtin_bit_cast(uint32_t, e - one) & ~__builtin_bit_cast(uint32_t, e);
+                } else {
+                    const uint32_t elo = *reinterpret_cast<const uint32_t*>(
+                        reinterpret_cast<const char*>(T.a32) + (c & 0xffu));
+                    const uint32_t ehi = *reinterpret_cast<const uint32_t*>(
+                        reinterpret_cast<const char*>(T.a32) + ((c >> 16) & 0xffu));
+                    const uint32_t t1 = __builtin_amdgcn_perm(ehi, elo, 0x05040100u);
+                    const uint32_t t2 = __builtin_amdgcn_perm(ehi, elo, 0x07060302u);
+                    const bc_u16x2 e1 = __builtin_bit_cast(bc_u16x2, f ^ t1);
+                    const bc_u16x2 e2 = __builtin_bit_cast(bc_u16x2, f ^ t2);
+                    z = (__builtin_bit_cast(uint32_t, e1 - one) & ~__builtin_bit_cast(uint32_t, e1)) |
+                        (__builtin_bit_cast(uint32_t, e2 - one) & ~__builtin_bit_cast(uint32_t, e2));
+                }
+                tmask |= (z >> (15 - k)) & (0x00010001u << k);
+            }
+        }
+        while (tmask) {
+            const int kv = __builtin_ctz(tmask); // rotated position
+            tmask &= tmask - 1u;
+            const int j = (kv + 4 * part) & 31; // the site's bit
+            if (j >= W)
+                continue;
+            const uint32_t f = bc_site_field(group, j, sw_lo, w2, c3, k0, k1);
+            const uint32_t t16 = bc_site_tail(group, j, sw_lo, w2, c3, k0, k1);
+            const int sidx = (int)(((v0 >> j) & 1u) | (((v1 >> j) & 1u) << 1) |
+                                   (((v2 >> j) & 1u) << 2) | (((v3 >> j) & 1u) << 3));
+            if (ALG == 0) {
+                const int cls = sidx | (int)(((occ >> j) & 1u) << 4) |
+                                (int)(((up >> j) & 1u) << 5) | (int)((f & 1u) << 6);
+                const uint32_t u31 = (((f ^ 0x8000u) >> 1) << 16) | t16;
+                lt1 = (lt1 & ~(1u << kv)) | ((u31 < T.thr[cls] ? 1u : 0u) << kv);
+            } else {
+                const uint64_t U = ((uint64_t)(f ^ 0x8000u) << 16) | t16;
+                lt1 = (lt1 & ~(1u << kv)) | ((U < T.hbw[sidx][0] ? 1u : 0u) << kv);
+                lt2 = (lt2 & ~(1u << kv)) | ((U < T.hbw[sidx][1] ? 1u : 0u) << kv);
+            }
+        }
+    }
+    // back to the sites' own positions
+    const uint32_t back = (32u - rot) & 31u;
+    auto rout = [&](uint32_t v) { return shared ? rotr32(v, back) : v; };
+    own = rout(own) & valid;
+    lt1 = rout(lt1);
+    uint32_t xo, xu, n_changed;
+    if (ALG == 0) {
+        altm = rout(altm);
+        const uint32_t okm = lt1 & own;
         xo = okm & (~occ | (up ^ altm));
         xu = okm & (~altm | up);
         n_changed = (uint32_t)__popc(okm);
     } else {
-        for (int j0 = 4 * part; j0 < W; j0 += 4 * lpw) {
-            const uint4 R = philox4x32_10(8u * group + (uint32_t)(j0 >> 2), sw_lo, w2, c3, k0, k1);
-            const uint32_t Rb[4] = {R.x, R.y, R.z, R.w};
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int j = j0 + b;
-                if (j >= W)
-                    break;
-                const int sidx = (int)(((v0 >> j) & 1u) | (((v1 >> j) & 1u) << 1) |
-                                       (((v2 >> j) & 1u) << 2) | (((v3 >> j) & 1u) << 3));
-                // s_new = rr < w1 ? -1 : (rr < w1 + w2 ? 0 : +1), and rr < w1
-                // implies rr < w1 + w2
-                const double rr = ((double)Rb[b] * 0x1.0p-32) * tabd[sidx * 3 + 2];
-                const bool lo = rr < tabd[sidx * 3];
-                const bool mid = rr < tabd[sidx * 3 + 1];
-                const uint32_t nz = (lo || !mid) ? 1u : 0u; // s_new != 0
-                const uint32_t nu = mid ? 0u : 1u;          // s_new = +1
-                xo |= (nz ^ ((occ >> j) & 1u)) << j;
-                xu |= (nu ^ ((up >> j) & 1u)) << j;
-            }
-        }
+        lt2 = rout(lt2);
+        // s_new = -1 if U < W1, else 0 if U < W12, else +1
+        const uint32_t nz = lt1 | ~lt2; // s_new != 0
+        const uint32_t nu = ~(lt1 | lt2); // s_new = +1
+        xo = (nz ^ occ) & own;
+        xu = (nu ^ up) & own;
         n_changed = (uint32_t)__popc(xo | xu);
     }
     if (lpw > 1) {
@@ -432,11 +650,16 @@ template <int DIM, int ALG>
 __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
 {
     extern __shared__ uint32_t lds[];
+    // the class tables at a fixed LDS address (their loads take it as an
+    // immediate offset), the rest in the dynamic allocation
+    __shared__ BCTabs tabs;
+    static_assert(96 + 192 + 10 <= kBCLdsHeaderWords, "LDS header");
+    static_assert(sizeof(BCTabs) == kBCStaticLds, "static LDS");
     int32_t* part = reinterpret_cast<int32_t*>(lds); // [2][16][3]
-    uint32_t* tabw = lds + 96;                       // kBCClasses thresholds by class
-    double* tabd = reinterpret_cast<double*>(lds + 224); // 39 doubles (8-byte aligned)
-    int32_t* ring = reinterpret_cast<int32_t*>(lds + 304); // E[64], M[64], Q[64] of the last sweeps
-    uint32_t* P = lds + 496;                          // [occ c0, occ c1, up c0, up c1]
+    int32_t* ring = reinterpret_cast<int32_t*>(lds + 96); // E[64], M[64], Q[64] of the last sweeps
+    // thread 0's running FP64 sums (sum E, E^2, |M|, M^2, M^4), 8-byte aligned
+    double* msum = reinterpret_cast<double*>(lds + 288);
+    uint32_t* P = lds + kBCLdsHeaderWords;            // [occ c0, occ c1, up c0, up c1]
     const int tid = threadIdx.x;
     const int B = blockDim.x;
     const int lane = tid & 63;
@@ -458,21 +681,19 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
         const int nw = (half + 31) >> 5;
         const int rows = DIM == 3 ? L * L : L;
         const int words = rows * nw;
-        // lanes per word: as asked, or (negative) the most, up to 8, that keep
-        // words x lanes within the workgroup; uniform over the workgroup
+        // lanes per word: as asked, or (negative) the most, up to 4 (one prefix
+        // block each), that keep words x lanes within the workgroup; uniform
+        // over the workgroup
         int lpw = a.lanes_per_word > 1 ? a.lanes_per_word : 1;
         if (a.lanes_per_word < 0) {
-            lpw = 8;
+            lpw = 4;
             while (lpw > 1 && words * lpw > B)
                 lpw >>= 1;
         }
         const int lpart = tid & (lpw - 1);
         int8_t* __restrict__ sp = a.spins + cr.spin_off;
         const BCTable* tb = a.btab + cr.table;
-        for (int k = tid; k < kBCClasses; k += B)
-            tabw[k] = bc_class_thr<DIM>(a.glauber ? tb->gl_thr : tb->metro_thr, k);
-        for (int k = tid; k < 39; k += B)
-            tabd[k] = tb->hb[k / 3][k % 3];
+        bc_fill_tabs<DIM, ALG>(tabs, tb, a.glauber, tid, B);
 
         // int8 lattice (or the planes a previous launch left) -> planes
         if (a.planes_in) {
@@ -488,14 +709,10 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
         unsigned long long acc = 0;
         // running sums continue from the stored ones, sweep by sweep, so the
         // FP64 moments do not depend on how a run is split into launches
-        double mE = 0, mE2 = 0, mAM = 0, mM2 = 0, mM4 = 0;
         if (a.record && tid == 0) {
             const double* mo = a.moments + oslot * 6;
-            mE = mo[1];
-            mE2 = mo[2];
-            mAM = mo[3];
-            mM2 = mo[4];
-            mM4 = mo[5];
+            for (int k = 0; k < 5; ++k)
+                msum[k] = mo[1 + k];
         }
         const int64_t nsw = a.item_nsweeps ? a.item_nsweeps[item] : a.nsweeps;
         const bool series = a.record && a.ser_E != nullptr;
@@ -511,8 +728,9 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
                 const uint32_t* __restrict__ Oo = P + (c ^ 1) * words;
                 const uint32_t* __restrict__ So = P + (2 + (c ^ 1)) * words;
                 for (int w = tid / lpw; w < words; w += B / lpw) {
+                    // (LDS-resident: every index fits 24 bits)
                     const int r = nw == 1 ? w : w / nw;
-                    const int q = w - r * nw;
+                    const int q = w - (int)__umul24((uint32_t)r, (uint32_t)nw);
                     const int y = DIM == 3 ? r % L : r;
                     const int z = DIM == 3 ? r / L : 0;
                     const int p = (c + y + z) & 1;
@@ -520,11 +738,11 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
                     const uint32_t valid = width_mask(W);
                     uint32_t occ = Oc[w], up = Sc[w];
                     BCNbr n;
-                    bc_load_nbrs<DIM>(Oo, So, L, nw, half, r, q, y, z, p, W, valid, n);
+                    bc_load_nbrs<DIM, true>(Oo, So, L, nw, half, r, q, y, z, p, W, valid, n);
                     uint32_t v0, v1, v2, v3;
                     bc_shifted_sum<DIM>(n, v0, v1, v2, v3);
-                    const uint32_t group = (uint32_t)((r * 2 + c) * nw + q);
-                    acc += bc_word<DIM, ALG>(occ, up, v0, v1, v2, v3, W, tabw, tabd, group,
+                    const uint32_t group = __umul24((uint32_t)(r * 2 + c), (uint32_t)nw) + (uint32_t)q;
+                    acc += bc_word<DIM, ALG>(occ, up, v0, v1, v2, v3, W, tabs, group,
                                              sw_lo, w2, c3, k0, cr.key, lpart, lpw);
                     if (lpart == 0) {
                         Oc[w] = occ;
@@ -569,11 +787,11 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
                 if (a.record) {
                     const double e = (double)(int32_t)(-tp) + cr.D * (double)(int32_t)tq;
                     const double m = (double)tm;
-                    mE += e;
-                    mE2 += e * e;
-                    mAM += fabs(m);
-                    mM2 += m * m;
-                    mM4 += (m * m) * (m * m);
+                    msum[0] += e;
+                    msum[1] += e * e;
+                    msum[2] += fabs(m);
+                    msum[3] += m * m;
+                    msum[4] += (m * m) * (m * m);
                 }
             }
             // the series leaves the ring every 64 sweeps, coalesced
@@ -610,11 +828,8 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
             if (a.record) {
                 double* mo = a.moments + oslot * 6;
                 mo[0] += (double)nsw;
-                mo[1] = mE;
-                mo[2] = mE2;
-                mo[3] = mAM;
-                mo[4] = mM2;
-                mo[5] = mM4;
+                for (int k = 0; k < 5; ++k)
+                    mo[1 + k] = msum[k];
             }
             a.chains[ch].sweep = cr.sweep + (uint64_t)nsw;
         }
@@ -662,20 +877,31 @@ __global__ void __launch_bounds__(256) k_bc_plain(PlainArgs a, int n_col)
         const int s_old = sp[rowb + x];
         const BCTable& tb = a.btab[cr.table];
         const uint32_t group = (uint32_t)((r * 2 + c) * nw + (j >> 5));
-        const uint32_t rw = site_word(cr.seed, cr.key, group, j & 31, a.sweep,
-                                      ALG == 0 ? P_BC_METRO : P_BC_HB);
+        const uint32_t sw_lo = (uint32_t)a.sweep;
+        const uint32_t w2 = ctr_word2(a.sweep, 0, ALG == 0 ? P_BC_METRO : P_BC_HB);
+        const uint32_t c3 = (uint32_t)(cr.seed >> 32), k0 = (uint32_t)cr.seed;
+        const uint32_t f = bc_site_field(group, j & 31, sw_lo, w2, c3, k0, cr.key);
+        const uint32_t u16 = f ^ 0x8000u;
         const int sidx = nb + 2 * DIM;
         int s_new;
         if (ALG == 0) {
-            const int alt = (int)(rw & 1u);
+            // U31 = (u16 >> 1) * 2^16 + tail < thr; the tail only on a tie
+            const int alt = (int)(f & 1u);
             const int k = ((s_old + 1) * 2 + alt) * NS + sidx;
-            const uint32_t* thr = a.glauber ? tb.gl_thr : tb.metro_thr;
-            const uint32_t* alw = a.glauber ? tb.gl_always : tb.metro_always;
-            const bool ok = ((alw[k >> 5] >> (k & 31)) & 1u) || (rw >> 1) < thr[k];
+            const uint32_t thr = a.glauber ? tb.gl_thr[k] : tb.metro_thr[k];
+            const uint32_t pre = u16 >> 1, A = thr >> 16;
+            bool ok = pre < A;
+            if (pre == A)
+                ok = ((pre << 16) | bc_site_tail(group, j & 31, sw_lo, w2, c3, k0, cr.key)) < thr;
             s_new = ok ? bc_propose(s_old, alt) : s_old;
         } else {
-            const double rr = ((double)rw * 0x1.0p-32) * tb.hb[sidx][2];
-            s_new = rr < tb.hb[sidx][0] ? -1 : (rr < tb.hb[sidx][1] ? 0 : 1);
+            // U = u16 * 2^16 + tail against W1, W12
+            const uint64_t W1 = tb.hb_w[sidx][0], W12 = tb.hb_w[sidx][1];
+            const uint64_t A1 = W1 >> 16, A12 = W12 >> 16;
+            uint64_t U = (uint64_t)u16 << 16;
+            if (u16 == A1 || u16 == A12)
+                U |= bc_site_tail(group, j & 31, sw_lo, w2, c3, k0, cr.key);
+            s_new = U < W1 ? -1 : (U < W12 ? 0 : 1);
         }
         if (s_new != s_old) {
             sp[rowb + x] = (int8_t)s_new;
@@ -712,8 +938,7 @@ __global__ void __launch_bounds__(256) k_bc_plain(PlainArgs a, int n_col)
 template <int DIM, int ALG>
 __global__ void __launch_bounds__(256) k_bc_tiles(TileArgs a)
 {
-    __shared__ uint32_t tabw[kBCClasses];
-    __shared__ double tabd[39];
+    __shared__ BCTabs tabs;
     const int item = blockIdx.y;
     if (a.s >= a.item_nsw[item])
         return;
@@ -722,10 +947,7 @@ __global__ void __launch_bounds__(256) k_bc_tiles(TileArgs a)
     const int ch = a.items[item];
     const ChainRec cr = a.chains[ch];
     const BCTable* tb = a.btab + cr.table;
-    for (int k = tid; k < kBCClasses; k += blockDim.x)
-        tabw[k] = bc_class_thr<DIM>(a.glauber ? tb->gl_thr : tb->metro_thr, k);
-    for (int k = tid; k < 39; k += blockDim.x)
-        tabd[k] = tb->hb[k / 3][k % 3];
+    bc_fill_tabs<DIM, ALG>(tabs, tb, a.glauber, tid, blockDim.x);
     __syncthreads();
     const int L = cr.L;
     const int half = L >> 1;
@@ -758,7 +980,7 @@ __global__ void __launch_bounds__(256) k_bc_tiles(TileArgs a)
         uint32_t v0, v1, v2, v3;
         bc_shifted_sum<DIM>(n, v0, v1, v2, v3);
         const uint32_t group = (uint32_t)((r * 2 + c) * nw + q);
-        flips = bc_word<DIM, ALG>(occ, up, v0, v1, v2, v3, W, tabw, tabd, group, (uint32_t)sweep,
+        flips = bc_word<DIM, ALG>(occ, up, v0, v1, v2, v3, W, tabs, group, (uint32_t)sweep,
                                   ctr_word2(sweep, 0, purpose), (uint32_t)(cr.seed >> 32),
                                   (uint32_t)cr.seed, cr.key, part, lpw);
         if (part == 0) {
@@ -811,8 +1033,7 @@ template <int DIM, int ALG>
 __global__ void __launch_bounds__(kBandThreads) k_bc_band(TileArgs a)
 {
     extern __shared__ uint32_t bq[];
-    __shared__ uint32_t tabw[kBCClasses];
-    __shared__ double tabd[39];
+    __shared__ BCTabs tabs;
     const int item = blockIdx.y;
     const int64_t nsw = a.item_nsw[item];
     if (a.s >= nsw)
@@ -828,10 +1049,7 @@ __global__ void __launch_bounds__(kBandThreads) k_bc_band(TileArgs a)
     if (s0 >= L)
         return;
     const BCTable* tb = a.btab + cr.table;
-    for (int k = tid; k < kBCClasses; k += nthr)
-        tabw[k] = bc_class_thr<DIM>(a.glauber ? tb->gl_thr : tb->metro_thr, k);
-    for (int k = tid; k < 39; k += nthr)
-        tabd[k] = tb->hb[k / 3][k % 3];
+    bc_fill_tabs<DIM, ALG>(tabs, tb, a.glauber, tid, nthr);
     const int half = L >> 1;
     const int nw = (half + 31) >> 5;
     const int spr = DIM == 3 ? L : 1;
@@ -899,7 +1117,7 @@ __global__ void __launch_bounds__(kBandThreads) k_bc_band(TileArgs a)
                 bc_shifted_sum<DIM>(n, v0, v1, v2, v3);
                 const uint32_t group = (uint32_t)((r * 2 + c) * nw + q);
                 const uint32_t nchanged =
-                    bc_word<DIM, ALG>(occ, up, v0, v1, v2, v3, W, tabw, tabd, group, (uint32_t)sweep,
+                    bc_word<DIM, ALG>(occ, up, v0, v1, v2, v3, W, tabs, group, (uint32_t)sweep,
                                       w2, c3, k0, cr.key, part, lpw);
                 if (part == 0) {
                     Oc[lr * nw + q] = occ;
@@ -988,6 +1206,8 @@ hipError_t launch_bc_sweeps(int dim, int alg, const SweepArgs& a0, int block, in
     SweepArgs a = a0;
     a.glauber = alg == MCX_GLAUBER ? 1 : 0;
     const bool hb = alg == MCX_HEATBATH;
+    // lds_bytes is the whole footprint; the class tables are static
+    lds_bytes = lds_bytes > (size_t)kBCStaticLds ? lds_bytes - (size_t)kBCStaticLds : 0;
     if (dim == 2 && !hb)
         hipLaunchKernelGGL((k_bc_sweeps<2, 0>), dim3(grid), dim3(block), lds_bytes, st, a);
     else if (dim == 2)

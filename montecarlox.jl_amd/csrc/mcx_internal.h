@@ -12,8 +12,8 @@ enum Purpose : uint32_t {
     P_METRO = 1,    // Ising Metropolis acceptance (bit-sliced uniform)
     P_HB = 2,       // Ising heat-bath draw (bit-sliced uniform)
     P_INIT = 3,     // random initial state (site word)
-    P_BC_METRO = 4, // Blume-Capel Metropolis: bit 0 = proposal, bits 31..1 = uniform
-    P_BC_HB = 5,    // Blume-Capel heat bath (site word)
+    P_BC_METRO = 4, // Blume-Capel Metropolis / Glauber: 16-bit site fields (proposal + prefix)
+    P_BC_HB = 5,    // Blume-Capel heat bath: 16-bit site fields (prefix)
     P_GEN = 6,      // multicanonical / Wang-Landau acceptance (site word)
     P_PT = 8        // replica-exchange shared uniform
 };
@@ -46,15 +46,25 @@ struct IsingTable {
 };
 
 // Blume-Capel acceptance for one (beta, D).  Metropolis: per (s_old, alt,
-// sum_nbr) the threshold ceil(p * 2^31) on the 31-bit uniform w >> 1, with
+// sum_nbr) the threshold ceil(p * 2^31) on a site's 31-bit uniform, with
 // index ((s_old+1)*2 + alt) * NS + (sum_nbr + 2*dim), NS = 4*dim + 1; bit k of
-// metro_always marks p >= 1.  Heat bath: per sum_nbr the FP64 triple
-// (w1, w1 + w2, z) of SpinSystems/src/blume_capel.jl:66-76.
+// metro_always marks p >= 1.  Heat bath: per sum_nbr the integer thresholds
+// (W1, W12) on a site's 32-bit uniform U: U < W1 exactly when
+// fl(fl(U * 2^-32) * z) < w1, U < W12 exactly when it is < fl(w1 + w2), for the
+// FP64 (w1, w1 + w2, z) of SpinSystems/src/blume_capel.jl:66-76 (fl(x * z) is
+// monotone in x, so each set of such U is a prefix of [0, 2^32)).
+// Words of the Blume-Capel batched kernel's dynamic LDS ahead of its bit
+// planes: partial sums (96), the E/M/Q ring (192) and five FP64 running sums
+// (10, and 2 of padding); its class tables are a static LDS block of
+// kBCStaticLds bytes.
+constexpr int kBCLdsHeaderWords = 300;
+constexpr int kBCStaticLds = 1088;
+
 struct BCTable {
     uint32_t metro_thr[80];
     uint32_t metro_always[3];
     uint32_t pad;
-    double hb[13][3];
+    uint64_t hb_w[13][2];   // (W1, W12), each in [0, 2^32]
     uint32_t gl_thr[80];    // Glauber: as metro_thr with p = logistic(-beta * dE)
     uint32_t gl_always[3];
     uint32_t gl_pad;

@@ -51,7 +51,7 @@ constexpr int kMaxTileClasses = 16; // tiled classes (model, dim, algorithm) of 
 // the neighbour gathers once the chip is covered.
 int tile_lanes(int max_words)
 {
-    int l = 8;
+    int l = 4;
     while (l > 1 && (int64_t)max_words * l > 16384)
         l /= 2;
     return l;
@@ -78,12 +78,12 @@ Band band_shape(int model, int dim, int L, size_t lds_max, int band_sweeps)
     const size_t budget = lds_max > 4096 ? lds_max - 4096 : 0;
     const int rw = (dim == 3 ? L : 1) * ((L / 2 + 31) / 32); // plane words per slab and colour
     // Ising: 32 rows (8 planes in 3D) and 8 (4) sweeps per launch, one lane per
-    // word; Blume-Capel, whose word update costs many times more: 16 rows
-    // (8 planes) and 4 (2) sweeps per launch, 8 lanes per word.  Chosen on the
-    // Poisson serving stream against other band heights, depths and lane
-    // counts and against the per-colour path.
+    // word; Blume-Capel, whose word update costs several times more: 16 rows
+    // (8 planes) and 4 (2) sweeps per launch, 4 lanes per word (one prefix
+    // block each).  Chosen on the Poisson serving stream against other band
+    // heights, depths and lane counts and against the per-colour path.
     Band b = model == MCX_ISING ? Band{dim == 3 ? 8 : 32, dim == 3 ? 4 : 8, 1, 0}
-                                : Band{dim == 3 ? 8 : 16, dim == 3 ? 2 : 4, 8, 0};
+                                : Band{dim == 3 ? 8 : 16, dim == 3 ? 2 : 4, 4, 0};
     if (band_sweeps < 0) {
         b.T = 0; // asked for the per-colour kernels
         return b;
@@ -277,7 +277,7 @@ int64_t job_bytes(const mcx_server* s, const mcx_job_desc& d)
 }
 
 // Workgroup shape for one LDS-resident lattice: lanes per 32-site plane word
-// (Blume-Capel up to 8, one Philox block each, while the workgroup stays
+// (Blume-Capel up to 4, one prefix block each, while the workgroup stays
 // within 1024 threads; Ising 1, its bit-serial draws serve all 32 sites at
 // once) and threads = words * lanes, at most 1024 (a whole CU's worth of waves
 // for a big lattice on its own), rounded up to a power of two so that few
@@ -286,7 +286,7 @@ void server_shape(int model, int dim, int L, int& block, int& lpw)
 {
     const int64_t nw = (L / 2 + 31) / 32;
     const int64_t words = (dim == 3 ? (int64_t)L * L : L) * nw;
-    lpw = model == MCX_ISING ? 1 : 8;
+    lpw = model == MCX_ISING ? 1 : 4;
     while (lpw > 1 && words * lpw > 1024)
         lpw /= 2;
     block = 64;

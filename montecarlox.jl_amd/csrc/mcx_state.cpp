@@ -12,6 +12,11 @@ using namespace mcx;
 
 namespace {
 
+// 2: Blume-Capel tables hold integer heat-bath thresholds, and the Blume-Capel
+// site uniforms come from 16-bit fields (a version-1 snapshot would resume on
+// other random words)
+constexpr uint32_t kBatchStateVersion = 2;
+
 size_t batch_state_bytes(const mcx_batch* b, uint32_t flags)
 {
     const size_t n = (size_t)b->n;
@@ -54,6 +59,7 @@ extern "C" mcx_status mcx_batch_save_state(mcx_batch* b, uint32_t flags, void* b
         return s;
     const size_t n = (size_t)b->n;
     StateHeader h = make_state_header(1, flags);
+    h.version = kBatchStateVersion;
     h.shape[0] = b->model;
     h.shape[1] = b->dim;
     h.shape[2] = b->n;
@@ -100,7 +106,7 @@ extern "C" mcx_status mcx_batch_load_state(mcx_batch* b, const void* buf, size_t
         return fail(MCX_EINVAL, "mcx_batch_load_state: NULL argument");
     BlobReader r(buf, bytes);
     StateHeader h;
-    if (!r.get(&h, sizeof h) || std::memcmp(h.magic, "MCXS", 4) != 0 || h.version != 1 || h.kind != 1)
+    if (!r.get(&h, sizeof h) || std::memcmp(h.magic, "MCXS", 4) != 0 || h.version != kBatchStateVersion || h.kind != 1)
         return fail(MCX_EINVAL, "mcx_batch_load_state: not a lattice-batch snapshot of this library");
     const bool series = (h.flags & MCX_STATE_SERIES) != 0;
     if (h.shape[0] != b->model || h.shape[1] != b->dim || h.shape[2] != b->n ||

@@ -87,7 +87,7 @@ def candidates(model, dim: int, Ls) -> List[dict]:
             if _model_name(model) == "ising":
                 out.append({"threads": t, "chains_per_group": cpg, "lanes_per_word": 0})
             else:
-                for lpw in (1, 2, 4, 8):
+                for lpw in (1, 2, 4):
                     if words * lpw <= 2 * t:
                         out.append({"threads": t, "chains_per_group": cpg, "lanes_per_word": lpw})
     return out

@@ -81,6 +81,9 @@ def _declare(L):
     L.orc_site_word.restype = C.c_uint32
     L.orc_pt_word.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64]
     L.orc_pt_word.restype = C.c_uint32
+    L.orc_bc_site_uniform.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_uint64,
+                                      C.c_int, C.POINTER(C.c_int)]
+    L.orc_bc_site_uniform.restype = C.c_uint32
     L.orc_logistic.argtypes = [C.c_double]
     L.orc_logistic.restype = C.c_double
     L.orc_threshold32.argtypes = [C.c_double]
@@ -149,6 +152,13 @@ def bitsliced_word(seed, chain_key, group, bit, sweep, purpose):
 
 def site_word(seed, chain_key, group, bit, sweep, purpose):
     return lib().orc_site_word(seed, chain_key, group, bit, sweep, purpose)
+
+
+def bc_site_uniform(seed, chain_key, group, bit, sweep, purpose):
+    """(U, alt) of a Blume-Capel site: U = u16 * 2^16 + t16, alt = the proposal bit."""
+    alt = C.c_int()
+    U = lib().orc_bc_site_uniform(seed, chain_key, group, bit, sweep, purpose, C.byref(alt))
+    return U, alt.value
 
 
 def pt_word(seed, chain_key, rnd):

@@ -150,6 +150,23 @@ uint32_t orc_site_word(uint64_t seed, uint32_t chain_key, uint32_t group, int bi
     return out[bit & 3];
 }
 
+uint32_t orc_bc_site_uniform(uint64_t seed, uint32_t chain_key, uint32_t group, int bit,
+                             uint64_t sweep, int purpose, int* alt)
+{
+    uint32_t key[2] = {(uint32_t)seed, chain_key};
+    uint32_t ctr[4], out[4];
+    int k = bit & 15;
+    stream_ctr(ctr, 8u * group + (uint32_t)(k >> 2), sweep, 0, purpose, seed);
+    orc_philox4x32_10(ctr, key, out);
+    uint32_t f = (bit >= 16 ? out[k & 3] >> 16 : out[k & 3]) & 0xffffu;
+    stream_ctr(ctr, 8u * group + 5u, sweep, 1 + bit, purpose, seed);
+    orc_philox4x32_10(ctr, key, out);
+    uint32_t t16 = out[0] & 0xffffu;
+    if (alt)
+        *alt = (int)(f & 1u);
+    return ((f ^ 0x8000u) << 16) | t16;
+}
+
 uint32_t orc_pt_word(uint64_t seed, uint32_t chain_key, uint64_t round)
 {
     uint32_t key[2] = {(uint32_t)seed, chain_key};
@@ -342,22 +359,26 @@ void orc_canonical_sweeps(int model, int alg, int dim, int L, double beta, doubl
                 } else if (alg == 0 || alg == 2) {
                     /* Metropolis (alg 0) or Glauber (alg 2): same proposal, acceptance
                      * exp(lr) or logistic(lr) */
-                    uint32_t w = orc_site_word(seed, chain_key, g, t, sweep, ORC_P_BC_METRO);
-                    int8_t s_new = bc_propose(si, (int)(w & 1u));
+                    int alt;
+                    uint32_t U = orc_bc_site_uniform(seed, chain_key, g, t, sweep, ORC_P_BC_METRO,
+                                                     &alt);
+                    uint32_t u31 = ((U >> 17) << 16) | (U & 0xffffu);
+                    int8_t s_new = bc_propose(si, alt);
                     int ds = s_new - si;
                     int ds2 = s_new * s_new - si * si;
                     double coupling = 1.0 * (double)nb;          /* local_coupling, J = 1 */
                     double dpair = (double)ds * coupling;
                     double dE = (-dpair - 0.0) + D * (double)ds2; /* delta_energy, h = 0 */
                     double lr = -beta * dE;
-                    double u = ldexp((double)(w >> 1), -31);
+                    double u = ldexp((double)u31, -31);
                     int ok = alg == 2 ? u < orc_logistic(lr) : (lr > 0 || u < exp(lr));
                     if (ok) {
                         spins[i] = s_new;
                         ++acc;
                     }
                 } else {
-                    uint32_t w = orc_site_word(seed, chain_key, g, t, sweep, ORC_P_BC_HB);
+                    uint32_t U = orc_bc_site_uniform(seed, chain_key, g, t, sweep, ORC_P_BC_HB,
+                                                     NULL);
                     double coupling = 1.0 * (double)nb;
                     double h_i = 0.0;
                     double e1 = 1.0 * coupling - h_i * (-1.0) + D;
@@ -367,7 +388,7 @@ void orc_canonical_sweeps(int model, int alg, int dim, int L, double beta, doubl
                     double w2 = exp(-beta * e2);
                     double w3 = exp(-beta * e3);
                     double z = w1 + w2 + w3;
-                    double r = ldexp((double)w, -32) * z;
+                    double r = ldexp((double)U, -32) * z;
                     int8_t s_new = r < w1 ? -1 : (r < (w1 + w2) ? 0 : 1);
                     if (s_new != si) {
                         spins[i] = s_new;

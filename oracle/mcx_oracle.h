@@ -26,8 +26,8 @@ enum {
     ORC_P_METRO = 1,   /* Ising Metropolis acceptance, bit-sliced uniform   */
     ORC_P_HB = 2,      /* Ising heat-bath draw, bit-sliced uniform          */
     ORC_P_INIT = 3,    /* random initial state, site word                   */
-    ORC_P_BC_METRO = 4,/* Blume-Capel Metropolis: proposal bit + 31-bit u   */
-    ORC_P_BC_HB = 5,   /* Blume-Capel heat bath: 32-bit u                   */
+    ORC_P_BC_METRO = 4,/* Blume-Capel Metropolis / Glauber: proposal + U31  */
+    ORC_P_BC_HB = 5,   /* Blume-Capel heat bath: 32-bit U                   */
     ORC_P_GEN = 6,     /* multicanonical / Wang-Landau acceptance, site word */
     ORC_P_PT = 8       /* replica-exchange shared uniform                   */
 };
@@ -44,6 +44,14 @@ uint32_t orc_bitsliced_word(uint64_t seed, uint32_t chain_key, uint32_t group, i
 uint32_t orc_site_word(uint64_t seed, uint32_t chain_key, uint32_t group, int bit,
                        uint64_t sweep, int purpose);
 uint32_t orc_pt_word(uint64_t seed, uint32_t chain_key, uint64_t round);
+/* Blume-Capel site uniform: the site's 16-bit field f (word (bit & 15) % 4,
+ * half bit >> 4, of the block at counter (8*group + (bit & 15) / 4, sweep)),
+ * u16 = f ^ 0x8000, and the tail t16 = low 16 bits of word 0 of the block at
+ * (8*group + 5, sweep, chunk 1 + bit).  Returns U = u16 * 2^16 + t16; *alt
+ * (if not NULL) = f & 1, the Metropolis proposal bit, whose uniform is
+ * U31 = (u16 >> 1) * 2^16 + t16. */
+uint32_t orc_bc_site_uniform(uint64_t seed, uint32_t chain_key, uint32_t group, int bit,
+                             uint64_t sweep, int purpose, int* alt);
 
 /* Helpers of the reference arithmetic. */
 double orc_logistic(double x);

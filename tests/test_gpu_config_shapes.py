@@ -8,6 +8,12 @@
   ladder for each L in {16, 24, 32, 48}, all four sets in one launch, several
   rounds of 10 Metropolis sweeps + an exchange: ladders, counters, stages,
   spins bit for bit.
+* 4096 independent 2D Blume-Capel chains, L=64, D=1, T_k = 1.5 + 2k/4095,
+  Philox key (5, k): Metropolis, heat bath and Glauber, every chain bit for bit
+  against the oracle over a few sweeps.  At 2^24 site updates per sweep the
+  decisions that need a site's 16-bit tail (its prefix equal to the
+  threshold's leading bits, about 2^-15 per Metropolis site and 2^-16 per
+  heat-bath comparison) occur hundreds of times per sweep.
 * One 2D Ising chain, L=32, beta = 0.4406868, seed 42 / key (42, 0): the plain
   one-chain-per-launch path against the oracle, and the batched path against
   the plain path over the full 10^4 + 10^5 sweeps.
@@ -69,6 +75,34 @@ def test_4096_chains_l64_bit_exact(mcx, orc):
         Eg, Mg, _ = b.series(k, first, more)
         assert np.array_equal(E, Eg) and np.array_equal(M, Mg), f"chain {k} series"
         assert res[k][3] + a == acc_end[k], f"chain {k} accepted"
+    b.close()
+
+
+@pytest.mark.parametrize("alg", ["metropolis", "heatbath", "glauber"])
+def test_bc_4096_chains_l64_bit_exact(mcx, orc, alg):
+    n, L, seed, D, sweeps = 4096, 64, 5, 1.0, 3
+    code = {"metropolis": 0, "heatbath": 1, "glauber": 2}[alg]
+    betas = 1.0 / (1.5 + 2.0 * np.arange(n) / (n - 1))
+    print(f"{n} Blume-Capel chains L={L} {alg}: {sweeps} sweeps", flush=True)
+    b = mcx.LatticeBatch("blume_capel", 2, L, n_chains=n, seed=seed, D=D, series_capacity=sweeps)
+    b.set_betas(betas)
+    b.init("random")
+    init = [b.get_spins(k) for k in range(n)]
+    b.run(alg, sweeps, record=True)
+    _, acc = b.counters()
+
+    def oracle_chain(k):
+        s = init[k].copy()
+        E, M, Q, a = orc.canonical_sweeps(1, code, 2, L, float(betas[k]), s, seed, k, 0, sweeps,
+                                          D=D)
+        return s, E, M, Q, a
+
+    for k, (s, E, M, Q, a) in enumerate(_pool(oracle_chain, range(n))):
+        assert np.array_equal(s, b.get_spins(k)), f"chain {k} spins"
+        Eg, Mg, Qg = b.series(k)
+        assert np.array_equal(E, Eg) and np.array_equal(M, Mg) and np.array_equal(Q, Qg), \
+            f"chain {k} series"
+        assert a == acc[k], f"chain {k} accepted"
     b.close()
 
 

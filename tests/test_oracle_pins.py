@@ -74,6 +74,43 @@ def test_words_are_uniform(orc):
         assert np.all(np.abs(bits - 0.5) < 4 * 0.5 / math.sqrt(len(ws)))
 
 
+def test_bc_site_fields_use_each_prefix_bit_once(orc):
+    # The 32 sites of a word take the 16 words of prefix blocks 0..3 half by
+    # half: the multiset of their fields is exactly those 32 halves, so no
+    # random bit is shared or dropped; the tail comes from block 5, chunk 1 + bit.
+    seed, key, g, sweep = (3 << 32) | 21, 1234, 77, (5 << 32) | 99
+    for P in (orc.P_BC_METRO, orc.P_BC_HB):
+        halves = []
+        for i in range(4):
+            ctr = [8 * g + i, sweep & 0xFFFFFFFF, ((sweep >> 32) & 0xFFFF) | (P << 24), seed >> 32]
+            out = orc.philox4x32_10(ctr, [seed & 0xFFFFFFFF, key])
+            halves += [int(w) & 0xFFFF for w in out] + [int(w) >> 16 for w in out]
+        fields, tails = [], []
+        for t in range(32):
+            U, alt = orc.bc_site_uniform(seed, key, g, t, sweep, P)
+            f = (U >> 16) ^ 0x8000
+            assert alt == f & 1
+            fields.append(f)
+            ctr = [8 * g + 5, sweep & 0xFFFFFFFF, ((sweep >> 32) & 0xFFFF) | ((1 + t) << 16) | (P << 24),
+                   seed >> 32]
+            assert U & 0xFFFF == int(orc.philox4x32_10(ctr, [seed & 0xFFFFFFFF, key])[0]) & 0xFFFF
+            tails.append(U & 0xFFFF)
+        assert sorted(fields) == sorted(halves)
+        assert len(set(tails)) > 28  # 32 distinct draws (a collision is possible, not 4)
+
+
+def test_bc_site_uniform_is_uniform(orc):
+    rng = np.random.default_rng(3)
+    for P in (orc.P_BC_METRO, orc.P_BC_HB):
+        draws = [orc.bc_site_uniform(11, int(k), int(g), int(t), 5, P) for k, g, t in
+                 rng.integers(0, [100, 1000, 32], size=(4000, 3))]
+        ws = np.array([u for u, _ in draws], np.uint64)
+        bits = ((ws[:, None] >> np.arange(32, dtype=np.uint64)) & 1).mean(0)
+        assert np.all(np.abs(bits - 0.5) < 4 * 0.5 / math.sqrt(len(ws)))
+        alts = np.array([a for _, a in draws])
+        assert abs(alts.mean() - 0.5) < 4 * 0.5 / math.sqrt(len(alts))
+
+
 def test_threshold_is_exact_ceiling(orc):
     rng = np.random.default_rng(1)
     ps = list(rng.random(200)) + [math.exp(-8 * 0.4406868), 0.5, 1 - 2 ** -40, 2 ** -40, 0.0, 1.0]
