@@ -365,6 +365,17 @@ def bc_line(mcx, np, oracle, host, cpu):
             "launch_shape": b.tuned_shapes.get("metropolis", "built-in"),
             "roofline": _roof(att, ms)}
     b.close()
+    # measured traffic and VALU work per launch / attempt (committed counter summary)
+    prof = "k_bc_sweeps_2d_L64_4096chains_counters.txt"
+    line["roofline"]["kernel"] = "k_bc_sweeps<2,0>"
+    line["roofline"]["traffic"] = _profile_value(prof, "traffic_bytes_per_launch")
+    lane_ops = _profile_value(prof, "valu_lane_ops_per_attempt")
+    if lane_ops is not None:
+        rate = att / (ms * 1e-3)
+        line["roofline"]["valu"] = {"lane_ops_per_attempt": lane_ops,
+                                    "achieved_lane_ops_per_s": round(lane_ops * rate, -9),
+                                    "peak_lane_ops_per_s": VALU_PEAK_LANE_OPS,
+                                    "frac": round(lane_ops * rate / VALU_PEAK_LANE_OPS, 4)}
     if cpu:
         line["cpu_baseline"] = _cpu(oracle, host, "256 chains L=64 x 300 sweeps of random-site "
                                     "Metropolis on BlumeCapel([L,L]; D=1)",
