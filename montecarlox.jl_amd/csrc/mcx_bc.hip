@@ -488,61 +488,94 @@ __device__ __forceinline__ uint32_t bc_word(uint32_t& occ, uint32_t& up, uint32_
     bc_u16x2 tz = {0xffff, 0xffff};
     const int nblk = W > 16 ? 4 : (W + 3) >> 2; // prefix blocks holding a valid site
     const uint32_t gb = 8u * group + (uint32_t)part;
+    // Blocks i and i + 2 together: pair k = 4i + m and pair k + 8 hold sites
+    // k, k + 16 and k + 8, k + 24, so one v_perm packs the four sites' sign
+    // (or proposal) bits into one word whose shift by k puts them in place.
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        if ((i & (lpw - 1)) != 0 || part + i >= nblk)
-            continue;
-        const uint4 R = philox4x32_10(gb + (uint32_t)i, sw_lo, w2, c3, k0, k1);
-        const uint32_t Rw[4] = {R.x, R.y, R.z, R.w};
-        own |= 0x000f000fu << (4 * i);
-        // the block's 8 table loads first, then the 4 packed compares
-        uint32_t tlo[4], thi[4];
+    for (int ip = 0; ip < 2; ++ip) {
+        uint32_t d1[2][4], d2[2][4], fw[2][4];
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            const int k = 4 * i + m;
-            // class bytes of sites k, k + 16 in bytes 0 and 2
-            uint32_t c = k < 8 ? x[k] : x[k - 8] >> 8;
-            if (ALG == 0) {
-                c |= (Rw[m] << 7) & 0x00800080u; // proposal bits -> bit 7
-                tlo[m] = *reinterpret_cast<const uint16_t*>(
-                    reinterpret_cast<const char*>(T.a16) + (c & 0xffu));
-                thi[m] = *reinterpret_cast<const uint16_t*>(
-                    reinterpret_cast<const char*>(T.a16) + __builtin_amdgcn_ubfe(c, 16u, 8u));
-            } else {
-                tlo[m] = *reinterpret_cast<const uint32_t*>(
-                    reinterpret_cast<const char*>(T.a32) + (c & 0xffu));
-                thi[m] = *reinterpret_cast<const uint32_t*>(
-                    reinterpret_cast<const char*>(T.a32) + __builtin_amdgcn_ubfe(c, 16u, 8u));
+        for (int h = 0; h < 2; ++h) {
+            const int i = ip + 2 * h;
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                d1[h][m] = d2[h][m] = fw[h][m] = 0; // no decision: sign clear
+            if ((i & (lpw - 1)) != 0 || part + i >= nblk)
+                continue;
+            const uint4 R = philox4x32_10(gb + (uint32_t)i, sw_lo, w2, c3, k0, k1);
+            const uint32_t Rw[4] = {R.x, R.y, R.z, R.w};
+            own |= 0x000f000fu << (4 * i);
+            // the block's 8 table loads first, then the 4 packed compares
+            uint32_t tlo[4], thi[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int k = 4 * i + m;
+                // class bytes of sites k, k + 16: bytes 0 and 2 of x[k] (k < 8)
+                // or bytes 1 and 3 of x[k - 8]; Metropolis adds the proposal
+                // bits (field bits 0 and 16) as bit 7 of each byte
+                uint32_t lo, hi;
+                if (k < 8) {
+                    const uint32_t c = ALG == 0
+                        ? __builtin_amdgcn_bitop3_b32(Rw[m] << 7, 0x00800080u, x[k], 0xEA)
+                        : x[k];
+                    lo = c & 0xffu;
+                    hi = __builtin_amdgcn_ubfe(c, 16u, 8u);
+                } else {
+                    const uint32_t c = ALG == 0
+                        ? __builtin_amdgcn_bitop3_b32(Rw[m] << 15, 0x80008000u, x[k - 8], 0xEA)
+                        : x[k - 8];
+                    lo = __builtin_amdgcn_ubfe(c, 8u, 8u);
+                    hi = c >> 24;
+                }
+                if (ALG == 0) {
+                    tlo[m] = *reinterpret_cast<const uint16_t*>(
+                        reinterpret_cast<const char*>(T.a16) + lo);
+                    thi[m] = *reinterpret_cast<const uint16_t*>(
+                        reinterpret_cast<const char*>(T.a16) + hi);
+                } else {
+                    tlo[m] = *reinterpret_cast<const uint32_t*>(
+                        reinterpret_cast<const char*>(T.a32) + lo);
+                    thi[m] = *reinterpret_cast<const uint32_t*>(
+                        reinterpret_cast<const char*>(T.a32) + hi);
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const uint32_t f = Rw[m];
+                const bc_s16x2 fs = __builtin_bit_cast(bc_s16x2, f);
+                fw[h][m] = f;
+                if (ALG == 0) {
+                    const bc_s16x2 th = __builtin_bit_cast(
+                        bc_s16x2, __builtin_amdgcn_perm(thi[m], tlo[m], 0x05040100u));
+                    const bc_s16x2 d = __builtin_elementwise_sub_sat(fs, th);
+                    d1[h][m] = __builtin_bit_cast(uint32_t, d);
+                    tz = __builtin_elementwise_min(tz, __builtin_bit_cast(bc_u16x2, d));
+                } else {
+                    const bc_s16x2 th1 = __builtin_bit_cast(
+                        bc_s16x2, __builtin_amdgcn_perm(thi[m], tlo[m], 0x05040100u));
+                    const bc_s16x2 th2 = __builtin_bit_cast(
+                        bc_s16x2, __builtin_amdgcn_perm(thi[m], tlo[m], 0x07060302u));
+                    const bc_s16x2 e1 = __builtin_elementwise_sub_sat(fs, th1);
+                    const bc_s16x2 e2 = __builtin_elementwise_sub_sat(fs, th2);
+                    d1[h][m] = __builtin_bit_cast(uint32_t, e1);
+                    d2[h][m] = __builtin_bit_cast(uint32_t, e2);
+                    tz = __builtin_elementwise_min(
+                        tz, __builtin_elementwise_min(__builtin_bit_cast(bc_u16x2, e1),
+                                                      __builtin_bit_cast(bc_u16x2, e2)));
+                }
             }
         }
+        // sign bits 15 / 31 of pairs k and k + 8 -> bits 7, 15, 23, 31 (sites
+        // k, k + 8, k + 16, k + 24); proposal bits 0 / 16 -> bits 0, 8, 16, 24
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
-            const int k = 4 * i + m;
-            const uint32_t f = Rw[m];
-            const bc_s16x2 fs = __builtin_bit_cast(bc_s16x2, f);
-            if (ALG == 0) {
-                altm |= (f << k) & (0x00010001u << k);
-                const bc_s16x2 th =
-                    __builtin_bit_cast(bc_s16x2, __builtin_amdgcn_perm(thi[m], tlo[m], 0x05040100u));
-                const bc_s16x2 d = __builtin_elementwise_sub_sat(fs, th);
-                const uint32_t du = __builtin_bit_cast(uint32_t, d);
-                lt1 |= (du >> (15 - k)) & (0x00010001u << k);
-                tz = __builtin_elementwise_min(tz, __builtin_bit_cast(bc_u16x2, d));
-            } else {
-                const bc_s16x2 th1 = __builtin_bit_cast(
-                    bc_s16x2, __builtin_amdgcn_perm(thi[m], tlo[m], 0x05040100u));
-                const bc_s16x2 th2 = __builtin_bit_cast(
-                    bc_s16x2, __builtin_amdgcn_perm(thi[m], tlo[m], 0x07060302u));
-                const bc_s16x2 d1 = __builtin_elementwise_sub_sat(fs, th1);
-                const bc_s16x2 d2 = __builtin_elementwise_sub_sat(fs, th2);
-                const uint32_t du1 = __builtin_bit_cast(uint32_t, d1);
-                const uint32_t du2 = __builtin_bit_cast(uint32_t, d2);
-                lt1 |= (du1 >> (15 - k)) & (0x00010001u << k);
-                lt2 |= (du2 >> (15 - k)) & (0x00010001u << k);
-                tz = __builtin_elementwise_min(
-                    tz, __builtin_elementwise_min(__builtin_bit_cast(bc_u16x2, d1),
-                                                  __builtin_bit_cast(bc_u16x2, d2)));
-            }
+            const int k = 4 * ip + m;
+            const uint32_t msk = 0x01010101u << k;
+            lt1 |= (__builtin_amdgcn_perm(d1[1][m], d1[0][m], 0x07030501u) >> (7 - k)) & msk;
+            if (ALG == 0)
+                altm |= (__builtin_amdgcn_perm(fw[1][m], fw[0][m], 0x06020400u) << k) & msk;
+            else
+                lt2 |= (__builtin_amdgcn_perm(d2[1][m], d2[0][m], 0x07030501u) >> (7 - k)) & msk;
         }
     }
     const uint32_t tzw = __builtin_bit_cast(uint32_t, tz);
