@@ -204,13 +204,12 @@ __device__ __forceinline__ void bc_nbrs_rows(const uint32_t* __restrict__ Oo,
     }
     const uint32_t A0 = n.occ[2], A1 = n.up[2];
     if (nw == 1) {
-        if (p) {
-            n.occ[3] = ((A0 >> 1) | (A0 << (W - 1))) & valid;
-            n.up[3] = ((A1 >> 1) | (A1 << (W - 1))) & valid;
-        } else {
-            n.occ[3] = ((A0 << 1) | (A0 >> (W - 1))) & valid;
-            n.up[3] = ((A1 << 1) | (A1 >> (W - 1))) & valid;
-        }
+        // the W-bit row rotated right by 1 (p) or left by 1 = right by W - 1,
+        // without a branch on the lane's row parity
+        const uint32_t sr = p ? 1u : (uint32_t)(W - 1);
+        const uint32_t sl = (uint32_t)W - sr;
+        n.occ[3] = ((A0 >> sr) | (A0 << sl)) & valid;
+        n.up[3] = ((A1 >> sr) | (A1 << sl)) & valid;
     } else if (p) {
         const int qn = q + 1 == nw ? 0 : q + 1;
         n.occ[3] = (A0 >> 1) | ((Oo[mul(r) + qn] & 1u) << (W - 1));
@@ -448,20 +447,24 @@ typedef unsigned short bc_u16x2 __attribute__((ext_vector_type(2)));
 // alt, else the upper) changes the occupancy where ~o | (u ^ alt) and the up
 // bit where ~alt | u.  Shared by the LDS-resident batched kernel and the
 // multi-workgroup tile and band kernels.
-template <int DIM, int ALG>
+// SHARED: the word's lanes are lpw > 1 (the caller branches on it, uniformly).
+template <int DIM, int ALG, bool SHARED>
 __device__ __forceinline__ uint32_t bc_word(uint32_t& occ, uint32_t& up, uint32_t v0, uint32_t v1,
                                             uint32_t v2, uint32_t v3, int W, const BCTabs& T,
                                             uint32_t group, uint32_t sw_lo, uint32_t w2,
                                             uint32_t c3, uint32_t k0, uint32_t k1, int part,
                                             int lpw)
 {
+    if (!SHARED) {
+        part = 0;
+        lpw = 1;
+    }
     const uint32_t valid = width_mask(W);
     // Rotating the planes right by 4 * part puts the sites of prefix block
     // part + i where those of block i are, so that every lane runs the loop
     // below over the same compile-time pair positions.
     const uint32_t rot = 4u * (uint32_t)part;
-    const bool shared = lpw > 1; // uniform: no rotation for one lane per word
-    auto rin = [&](uint32_t v) { return shared ? rotr32(v, rot) : v; };
+    auto rin = [&](uint32_t v) { return SHARED ? rotr32(v, rot) : v; };
     uint32_t x[8];
     if (ALG == 0) {
         // class byte: bit 0 clear, bits 1-4 sidx, 5 occupancy, 6 up, 7 proposal
@@ -649,7 +652,7 @@ __device__ __forceinline__ uint32_t bc_word(uint32_t& occ, uint32_t& up, uint32_
     }
     // back to the sites' own positions
     const uint32_t back = (32u - rot) & 31u;
-    auto rout = [&](uint32_t v) { return shared ? rotr32(v, back) : v; };
+    auto rout = [&](uint32_t v) { return SHARED ? rotr32(v, back) : v; };
     own = rout(own) & valid;
     lt1 = rout(lt1);
     uint32_t xo, xu, n_changed;
@@ -668,7 +671,7 @@ __device__ __forceinline__ uint32_t bc_word(uint32_t& occ, uint32_t& up, uint32_
         xu = (nu ^ up) & own;
         n_changed = (uint32_t)__popc(xo | xu);
     }
-    if (lpw > 1) {
+    if (SHARED) {
         for (int off = 1; off < lpw; off <<= 1) {
             xo |= __shfl_xor(xo, off, 64);
             xu |= __shfl_xor(xu, off, 64);
@@ -775,8 +778,12 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
                     uint32_t v0, v1, v2, v3;
                     bc_shifted_sum<DIM>(n, v0, v1, v2, v3);
                     const uint32_t group = __umul24((uint32_t)(r * 2 + c), (uint32_t)nw) + (uint32_t)q;
-                    acc += bc_word<DIM, ALG>(occ, up, v0, v1, v2, v3, W, tabs, group,
-                                             sw_lo, w2, c3, k0, cr.key, lpart, lpw);
+                    acc += lpw > 1 ? bc_word<DIM, ALG, true>(occ, up, v0, v1, v2, v3, W, tabs,
+                                                             group, sw_lo, w2, c3, k0, cr.key,
+                                                             lpart, lpw)
+                                   : bc_word<DIM, ALG, false>(occ, up, v0, v1, v2, v3, W, tabs,
+                                                              group, sw_lo, w2, c3, k0, cr.key,
+                                                              0, 1);
                     if (lpart == 0) {
                         Oc[w] = occ;
                         Sc[w] = up;
@@ -1013,9 +1020,10 @@ __global__ void __launch_bounds__(256) k_bc_tiles(TileArgs a)
         uint32_t v0, v1, v2, v3;
         bc_shifted_sum<DIM>(n, v0, v1, v2, v3);
         const uint32_t group = (uint32_t)((r * 2 + c) * nw + q);
-        flips = bc_word<DIM, ALG>(occ, up, v0, v1, v2, v3, W, tabs, group, (uint32_t)sweep,
-                                  ctr_word2(sweep, 0, purpose), (uint32_t)(cr.seed >> 32),
-                                  (uint32_t)cr.seed, cr.key, part, lpw);
+        flips = bc_word<DIM, ALG, true>(occ, up, v0, v1, v2, v3, W, tabs, group,
+                                        (uint32_t)sweep, ctr_word2(sweep, 0, purpose),
+                                        (uint32_t)(cr.seed >> 32), (uint32_t)cr.seed, cr.key,
+                                        part, lpw);
         if (part == 0) {
             Oc[w] = occ;
             Sc[w] = up;
@@ -1150,8 +1158,8 @@ __global__ void __launch_bounds__(kBandThreads) k_bc_band(TileArgs a)
                 bc_shifted_sum<DIM>(n, v0, v1, v2, v3);
                 const uint32_t group = (uint32_t)((r * 2 + c) * nw + q);
                 const uint32_t nchanged =
-                    bc_word<DIM, ALG>(occ, up, v0, v1, v2, v3, W, tabs, group, (uint32_t)sweep,
-                                      w2, c3, k0, cr.key, part, lpw);
+                    bc_word<DIM, ALG, true>(occ, up, v0, v1, v2, v3, W, tabs, group,
+                                            (uint32_t)sweep, w2, c3, k0, cr.key, part, lpw);
                 if (part == 0) {
                     Oc[lr * nw + q] = occ;
                     Sc[lr * nw + q] = up;
