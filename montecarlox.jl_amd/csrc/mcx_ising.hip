@@ -55,9 +55,10 @@ __device__ __forceinline__ void nbr_words_rows(const uint32_t* __restrict__ Po, 
         n.Zd = 0;
     }
     if (nw == 1) {
-        // whole half-row in one word: rotate within W bits
-        n.Bw = p ? (((n.A >> 1) | (n.A << (W - 1))) & valid)
-                 : (((n.A << 1) | (n.A >> (W - 1))) & valid);
+        // whole half-row in one word: rotate within W bits, right by 1 (p) or
+        // left by 1 = right by W - 1, the direction as a shift amount
+        const uint32_t sr = p ? 1u : (uint32_t)(W - 1);
+        n.Bw = ((n.A >> sr) | (n.A << ((uint32_t)W - sr))) & valid;
     } else if (p) {
         // x = 2j+1: right neighbour is other-colour bit j+1
         const int qn = q + 1 == nw ? 0 : q + 1;
