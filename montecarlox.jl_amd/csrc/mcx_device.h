@@ -17,7 +17,11 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
 // as 1, 2, 3", SC'11).  The 32x32->64 products map to v_mad_u64_u32 and each
 // round's two "hi ^ c ^ key" to one v_bitop3_b32 each, so a round is four VALU
 // instructions; round keys depend only on (k0, k1) and fold into scalar
-// registers when the key is uniform across the wavefront.
+// registers when the key is uniform across the wavefront.  In the first two
+// rounds the counter words of the sweep, the seed and the chunk are still
+// wave-uniform in every stream here, so those XORs are written plainly: the
+// compiler keeps their uniform parts on the scalar unit (v_bitop3 has no
+// scalar form).
 __device__ __forceinline__ uint4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                                uint32_t k0, uint32_t k1)
 {
@@ -25,8 +29,9 @@ __device__ __forceinline__ uint4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-        const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
-        const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
+        const uint32_t h1 = (uint32_t)(p1 >> 32), h0 = (uint32_t)(p0 >> 32);
+        const uint32_t n0 = r < 2 ? (h1 ^ (c1 ^ k0)) : xor3(h1, c1, k0);
+        const uint32_t n2 = r < 2 ? (h0 ^ (c3 ^ k1)) : xor3(h0, c3, k1);
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;

@@ -97,7 +97,10 @@ __device__ __forceinline__ uint32_t bitserial_less(uint32_t eq, const uint32_t (
     for (uint32_t chunk = 0; chunk < 8; ++chunk) {
         if (!__any(eq != 0u))
             break;
-        const uint4 R = philox4x32_10(group, sw_lo, w2 | (chunk << 16), c3, k0, k1);
+        // (the chunk's counter word is wave-uniform: its Philox products with
+        // the uniform words go to the scalar unit)
+        const uint32_t cw = __builtin_amdgcn_readfirstlane(w2 | (chunk << 16));
+        const uint4 R = philox4x32_10(group, sw_lo, cw, c3, k0, k1);
         const uint32_t Rb[4] = {R.x, R.y, R.z, R.w};
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
@@ -345,7 +348,12 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
             for (int c = 0; c < 2; ++c) {
                 uint32_t* __restrict__ Pc = P + c * words;
                 const uint32_t* __restrict__ Po = P + (c ^ 1) * words;
-                for (int w = tid; w < words; w += B) {
+                // (wave-uniform trip count: the chunk loop inside sees uniform
+                // control flow, so its per-chunk counter words stay scalar)
+                for (int wb = tid - lane; wb < words; wb += B) {
+                    const int w = wb + lane;
+                    if (w >= words)
+                        continue;
                     const int r = nw == 1 ? w : w / nw;
                     const int q = w - r * nw;
                     const int y = DIM == 3 ? r % L : r;
