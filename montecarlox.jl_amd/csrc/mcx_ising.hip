@@ -113,11 +113,12 @@ __device__ __forceinline__ uint32_t bitserial_less(uint32_t eq, const uint32_t (
                 // rather than one select per class
                 uint32_t tm = 0u - ((thr[k] >> sh) & 1u);
                 asm volatile("" : "+s"(tm));
-                Tb |= m[k] & tm;
+                Tb = k == 0 ? (m[k] & tm) : __builtin_amdgcn_bitop3_b32(m[k], tm, Tb, 0xEA);
             }
-            const uint32_t x = Rb[b] ^ Tb;
-            lt |= eq & Tb & x;
-            eq &= ~x;
+            // less where tied, the threshold bit is 1 and the random bit 0;
+            // still tied where the two bits agree
+            lt |= __builtin_amdgcn_bitop3_b32(eq, Tb, Rb[b], 0x40);
+            eq = __builtin_amdgcn_bitop3_b32(eq, Rb[b], Tb, 0x90);
         }
     }
     return lt;
