@@ -38,18 +38,23 @@ struct NbrWords {
 
 // Neighbour words of word q of plane row iA, given the plane rows of its
 // y-1, y+1 (and z-1, z+1) neighbours in the other colour's plane Po.
-template <int DIM>
+// SMALL: every row index times nw fits 24 bits (LDS-resident planes), so the
+// products use the full-rate 24-bit multiply.
+template <int DIM, bool SMALL = false>
 __device__ __forceinline__ void nbr_words_rows(const uint32_t* __restrict__ Po, int nw, int half,
                                                int iA, int iU, int iD, int iZu, int iZd, int q,
                                                int p, int W, uint32_t valid, NbrWords& n)
 {
+    auto mul = [&](int a) {
+        return SMALL ? (int)__umul24((uint32_t)a, (uint32_t)nw) : a * nw;
+    };
     const int r = iA;
-    n.A = Po[r * nw + q];
-    n.U = Po[iU * nw + q];
-    n.Dn = Po[iD * nw + q];
+    n.A = Po[mul(r) + q];
+    n.U = Po[mul(iU) + q];
+    n.Dn = Po[mul(iD) + q];
     if (DIM == 3) {
-        n.Zu = Po[iZu * nw + q];
-        n.Zd = Po[iZd * nw + q];
+        n.Zu = Po[mul(iZu) + q];
+        n.Zd = Po[mul(iZd) + q];
     } else {
         n.Zu = 0;
         n.Zd = 0;
@@ -62,16 +67,16 @@ __device__ __forceinline__ void nbr_words_rows(const uint32_t* __restrict__ Po, 
     } else if (p) {
         // x = 2j+1: right neighbour is other-colour bit j+1
         const int qn = q + 1 == nw ? 0 : q + 1;
-        n.Bw = (n.A >> 1) | ((Po[r * nw + qn] & 1u) << (W - 1));
+        n.Bw = (n.A >> 1) | ((Po[mul(r) + qn] & 1u) << (W - 1));
     } else {
         // x = 2j: left neighbour is other-colour bit j-1
         const int qp = q == 0 ? nw - 1 : q - 1;
         const int Wp = word_width(qp, half);
-        n.Bw = ((n.A << 1) | ((Po[r * nw + qp] >> (Wp - 1)) & 1u)) & valid;
+        n.Bw = ((n.A << 1) | ((Po[mul(r) + qp] >> (Wp - 1)) & 1u)) & valid;
     }
 }
 
-template <int DIM>
+template <int DIM, bool SMALL = false>
 __device__ __forceinline__ void load_nbr_words(const uint32_t* __restrict__ Po, int L, int nw,
                                                int half, int r, int q, int y, int z, int p,
                                                int W, uint32_t valid, NbrWords& n)
@@ -81,8 +86,8 @@ __device__ __forceinline__ void load_nbr_words(const uint32_t* __restrict__ Po, 
     const int yd = y == L - 1 ? 0 : y + 1;
     const int zu = DIM == 3 ? (z == 0 ? L - 1 : z - 1) : 0;
     const int zd = DIM == 3 ? (z == L - 1 ? 0 : z + 1) : 0;
-    nbr_words_rows<DIM>(Po, nw, half, r, base + yu, base + yd, zu * L + y, zd * L + y, q, p, W,
-                        valid, n);
+    nbr_words_rows<DIM, SMALL>(Po, nw, half, r, base + yu, base + yd, zu * L + y, zd * L + y, q,
+                               p, W, valid, n);
 }
 
 // Bit-serial "w < thr[class]" for the 32 sites of one word.  eq: sites still
@@ -356,7 +361,7 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                     if (w >= words)
                         continue;
                     const int r = nw == 1 ? w : w / nw;
-                    const int q = w - r * nw;
+                    const int q = w - (int)__umul24((uint32_t)r, (uint32_t)nw);
                     const int y = DIM == 3 ? r % L : r;
                     const int z = DIM == 3 ? r / L : 0;
                     const int p = (c + y + z) & 1;
@@ -364,8 +369,9 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                     const uint32_t valid = width_mask(W);
                     const uint32_t s0 = Pc[w];
                     NbrWords n;
-                    load_nbr_words<DIM>(Po, L, nw, half, r, q, y, z, p, W, valid, n);
-                    const uint32_t group = (uint32_t)((r * 2 + c) * nw + q);
+                    load_nbr_words<DIM, true>(Po, L, nw, half, r, q, y, z, p, W, valid, n);
+                    const uint32_t group =
+                        __umul24((uint32_t)(r * 2 + c), (uint32_t)nw) + (uint32_t)q;
                     const uint32_t snew = ising_word<DIM, ALG>(s0, n, valid, tb, group, sw_lo,
                                                                w2, c3, k0, cr.key);
                     const uint32_t flip = snew ^ s0;
