@@ -78,12 +78,12 @@ Band band_shape(int model, int dim, int L, size_t lds_max, int band_sweeps)
     const size_t budget = lds_max > 4096 ? lds_max - 4096 : 0;
     const int rw = (dim == 3 ? L : 1) * ((L / 2 + 31) / 32); // plane words per slab and colour
     // Ising: 32 rows (8 planes in 3D) and 8 (4) sweeps per launch, one lane per
-    // word; Blume-Capel, whose word update costs several times more: 16 rows
-    // (8 planes) and 4 (2) sweeps per launch, 4 lanes per word (one prefix
-    // block each).  Chosen on the Poisson serving stream against other band
+    // word; Blume-Capel, whose word update costs about twice as much: 16 rows
+    // (8 planes) and 6 (2) sweeps per launch, 2 lanes per word (two prefix
+    // blocks each).  Chosen on the Poisson serving stream against other band
     // heights, depths and lane counts and against the per-colour path.
     Band b = model == MCX_ISING ? Band{dim == 3 ? 8 : 32, dim == 3 ? 4 : 8, 1, 0}
-                                : Band{dim == 3 ? 8 : 16, dim == 3 ? 2 : 4, 4, 0};
+                                : Band{dim == 3 ? 8 : 16, dim == 3 ? 2 : 6, 2, 0};
     if (band_sweeps < 0) {
         b.T = 0; // asked for the per-colour kernels
         return b;
