@@ -78,7 +78,9 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
     constexpr bool WL = KIND == 1; // Wang-Landau
     constexpr bool PS = KIND == 2; // Boltzmann pair term + multicanonical sum s^2
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    double* lw = reinterpret_cast<double*>(lds_raw);
+    // the weight window, with max_dbin padding slots on each side for the
+    // linear walk's reads past the ends of the table
+    double* lw = reinterpret_cast<double*>(lds_raw) + a.max_dbin;
     // two chunks' staged attempts (stage [i & 1] holds chunk i of the colour)
     // and the lane-0 walk's results.  Entry 64 of st_dE / st_luh / st_lul is the
     // integer walk's read-ahead past the last attempt; its value is never used.
@@ -93,7 +95,7 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
     // the weights in LDS are the window [wbase, wbase + wsz) of the walker's
     // table (the whole table when it fits: wsz = nbins, wbase = 0)
     const int wsz = a.win;
-    uint32_t* hl = reinterpret_cast<uint32_t*>(lw + wsz);
+    uint32_t* hl = reinterpret_cast<uint32_t*>(lw + wsz + a.max_dbin);
     int8_t* sp = reinterpret_cast<int8_t*>(hl + (a.hist_in_lds ? a.nbins : 0));
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -198,6 +200,11 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
         const long long start_i = (long long)a.bin_start;
         const int start32 = (int)start_i;
         const int smask = step_pow2 ? (1 << a.step_shift) - 1 : 0;
+        // Linear bins: bin(E + dE) = bin(E) + dE / step exactly -- unit bins,
+        // or Ising energies (which change by multiples of 4) on a step of 1, 2
+        // or 4 that divides E - start.  The staging then stores each attempt's
+        // bin change and the walk carries only the bin.
+        const int lin_sh = MODEL == 0 ? a.step_shift : 0;
         const unsigned nbu = (unsigned)nb;
         auto ibin = [&](long long E, int& b) -> bool {
             if (MODEL == 0)
@@ -216,6 +223,7 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
         // only when a move is accepted
         double Eo = MODEL == 0 ? -(double)pair : -(double)pair + D * (double)qq;
         long long Ei = -(long long)pair + (MODEL == 0 ? 0 : Di * (long long)qq);
+        const bool lin = int_walk && (MODEL == 1 || (lin_sh <= 2 && ((Ei - start_i) & smask) == 0));
         int b_old = 0;
         bool old_ok = PS ? bin_int((long long)qq, b_old)
                          : (int_e ? ibin(Ei, b_old)
@@ -347,7 +355,86 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                 // and the running sums are added up from those records by
                 // all lanes after the chunk.
                 int err_v = -1;
-                if (lane == 0) {
+                if (lane == 0 && lin) {
+                    // Linear bins: the state is the bin alone and an attempt
+                    // proposes bin + (staged bin change).  The weight reads
+                    // of the next attempt may reach max_dbin bins past the
+                    // ends of the table (a proposal there is out of the
+                    // domain and its weight unused): the padding slots.
+                    if (b_cur != b_old && old_ok) {
+                        lw_cur = lw[b_old - wbase];
+                        b_cur = b_old;
+                    }
+                    const double* lwb = lw - wbase; // weight of bin b: lwb[b]
+                    double* lwbw = lw - wbase;
+                    int Eb = b_old;
+                    int kk = 0;
+                    if (!old_ok) {
+                        n_done = 0;
+                        err_v = st_site[buf][0];
+                        kk = cnt;
+                    }
+                    while (kk < cnt) {
+                        int stop = cnt;
+                        bool alive = true;
+                        int dB_c = st_dE[buf][kk];
+                        double luh_c = st_luh[buf][kk], lul_c = st_lul[buf][kk];
+                        double lw_pre = lwb[Eb + dB_c];
+#pragma unroll 2
+                        for (int k = kk; k < cnt; ++k) {
+                            const int dB_n = st_dE[buf][k + 1];
+                            const double luh_n = st_luh[buf][k + 1], lul_n = st_lul[buf][k + 1];
+                            const int bn = Eb + dB_c;
+                            const bool okn = (unsigned)bn < nbu;
+                            // a Wang-Landau walk has just lowered the current
+                            // bin's weight after k+1's was read
+                            const double lw_new = !WL ? lw_pre : (bn == Eb ? lw_cur : lw_pre);
+                            const double lr = lw_new - lw_cur;
+                            const bool acc_c = lr > luh_c;
+                            const bool go = alive && okn && (acc_c || lr < lul_c);
+                            const bool ok = go && acc_c;
+                            stop = alive && !go ? k : stop;
+                            alive = go;
+                            Eb = ok ? bn : Eb;
+                            lw_cur = ok ? lw_new : lw_cur;
+                            st_res[buf][k] = Eb | (ok ? (int)0x80000000 : 0);
+                            lw_pre = lwb[Eb + dB_n];
+                            if (WL) {
+                                // unchanged value when the attempt did not run
+                                lw_cur = go ? lw_cur - logf : lw_cur;
+                                lwbw[Eb] = lw_cur;
+                            }
+                            dB_c = dB_n;
+                            luh_c = luh_n;
+                            lul_c = lul_n;
+                        }
+                        if (stop == cnt)
+                            break;
+                        // the attempt that stopped the fast loop, exactly
+                        const int bb = Eb + st_dE[buf][stop];
+                        if (!((unsigned)bb < nbu)) {
+                            n_done = stop;
+                            err_v = st_site[buf][stop];
+                            break;
+                        }
+                        const double lw_new = bb == Eb ? lw_cur : lwb[bb];
+                        const double lr = lw_new - lw_cur;
+                        const bool ok = lr > 0 || st_u[buf][stop] < exp(lr);
+                        if (ok) {
+                            Eb = bb;
+                            lw_cur = lw_new;
+                        }
+                        st_res[buf][stop] = Eb | (ok ? (int)0x80000000 : 0);
+                        if (WL) {
+                            lw_cur = lw_cur - logf;
+                            lwbw[Eb] = lw_cur;
+                        }
+                        kk = stop + 1;
+                    }
+                    b_old = Eb;
+                    b_cur = Eb;
+                    Ei = start_i + ((long long)Eb << lin_sh);
+                } else if (lane == 0) {
                     // bin of an energy (32-bit): Ising floor-free div by 2^s
                     // rounding toward zero as div does; Blume-Capel E - start
                     auto bin32 = [&](int E) -> int {
@@ -616,7 +703,7 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                 st_luh[buf][lane] = t_luh;
                 st_lul[buf][lane] = t_lul;
             } else if (int_e) {
-                st_dE[buf][lane] = t_dE;
+                st_dE[buf][lane] = lin ? t_dE >> lin_sh : t_dE;
                 // lr > 0 || lr > log(u) + 1e-9  <=>  lr > min(0, log(u) + 1e-9)
                 st_luh[buf][lane] = t_luh < 0.0 ? t_luh : 0.0;
                 st_lul[buf][lane] = t_lul;

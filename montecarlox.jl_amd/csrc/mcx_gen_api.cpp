@@ -85,11 +85,11 @@ mcx_status check_group(mcx_gen* g, int32_t group, const char* who)
     return MCX_OK;
 }
 
-// dynamic LDS of one walker: weights (a window of win bins), optional
-// histogram (whole table), spins
-size_t gen_lds(int64_t win, int64_t nbins, int64_t nsites, bool hist_in_lds)
+// dynamic LDS of one walker: weights (a window of win bins between max_dbin
+// padding slots on each side), optional histogram (whole table), spins
+size_t gen_lds(int64_t win, int max_dbin, int64_t nbins, int64_t nsites, bool hist_in_lds)
 {
-    return 8 * (size_t)win + (hist_in_lds ? 4 * (size_t)nbins : 0)
+    return 8 * ((size_t)win + 2 * (size_t)max_dbin) + (hist_in_lds ? 4 * (size_t)nbins : 0)
          + (size_t)((nsites + 15) / 16 * 16);
 }
 
@@ -208,7 +208,7 @@ extern "C" mcx_status mcx_gen_create(mcx_batch* b, const mcx_gen_desc* d, mcx_ge
     }
     for (int k = 0; k < d->n_groups; ++k) {
         const GenWindow w = gen_window(g, k);
-        if (gen_lds(w.win, g->nbins[k], b->nsites[g->members[k][0]], false) + kGenStaticLds >
+        if (gen_lds(w.win, w.max_dbin, g->nbins[k], b->nsites[g->members[k][0]], false) + kGenStaticLds >
             b->ctx->lds_max) {
             delete g;
             return fail(MCX_EINVAL, "mcx_gen_create: group %d: the lattice (%lld sites) and a window "
@@ -416,7 +416,7 @@ mcx_status gen_run(mcx_gen* g, int64_t nsweeps, uint32_t flags, const std::vecto
         a.max_dbin = gw.max_dbin;
         // a windowed walker counts visits in HBM; a whole table may count in LDS
         a.hist_in_lds = gw.win == g->nbins[k] &&
-                        gen_lds(gw.win, g->nbins[k], ns, true) + kGenStaticLds <= b->ctx->lds_max ? 1 : 0;
+                        gen_lds(gw.win, gw.max_dbin, g->nbins[k], ns, true) + kGenStaticLds <= b->ctx->lds_max ? 1 : 0;
         // the uint32 LDS counts of one launch hold at most nsweeps * N visits per bin
         if ((double)nsweeps * (double)ns >= 4294967296.0)
             a.hist_in_lds = 0;
@@ -448,7 +448,7 @@ mcx_status gen_run(mcx_gen* g, int64_t nsweeps, uint32_t flags, const std::vecto
         a.nsweeps = nsweeps;
         a.rec_pos = b->rec_len;
         a.pair_beta = g->d_pbeta;
-        HIPCHK(launch_gen_sweeps(a, gen_lds(gw.win, g->nbins[k], ns, a.hist_in_lds), g->gs[oi % nstr]));
+        HIPCHK(launch_gen_sweeps(a, gen_lds(gw.win, gw.max_dbin, g->nbins[k], ns, a.hist_in_lds), g->gs[oi % nstr]));
     }
     for (int q = 0; q < nstr; ++q) {
         HIPCHK(hipEventRecord(g->join[q], g->gs[q]));

@@ -71,6 +71,18 @@ def test_ising_2d_walkers(mcx, orc, kind):
 
 
 @pytest.mark.parametrize("kind", ["muca", "wl"])
+@pytest.mark.parametrize("off,step", [(0, 1), (0, 2), (2, 4), (0, 8)])
+def test_ising_integer_walk_bin_layouts(mcx, orc, kind, off, step):
+    """Integer-energy walks whose bins follow the energy linearly (a step of
+    1 or 2 dividing E - start: the walk carries the bin alone) and whose bins
+    do not (step 4 with E - start = 2 mod 4; step 8)."""
+    N = 64
+    start = -2 * N - off
+    nbins = (4 * N + off) // step + 2
+    _check(mcx, orc, "ising", 2, 8, 2, kind, start, step, nbins, 23, 10)
+
+
+@pytest.mark.parametrize("kind", ["muca", "wl"])
 def test_ising_3d_walkers(mcx, orc, kind):
     N = 64
     _check(mcx, orc, "ising", 3, 4, 2, kind, -3 * N, 4, 3 * N // 2 + 1, 5, 6)
