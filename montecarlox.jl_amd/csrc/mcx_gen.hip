@@ -102,6 +102,12 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
     const int wid = __builtin_amdgcn_readfirstlane(tid >> 6); // 0 walks, 1 stages
     const int64_t nb = a.nbins;
     constexpr bool wl = WL;
+    // padding slots around the weight window: NaN, so that a linear walk's
+    // proposal past an end of the table fails both of its compares
+    for (int i = tid; i < a.max_dbin; i += kGenThreads) {
+        lw[-1 - i] = __builtin_nan("");
+        lw[wsz + i] = __builtin_nan("");
+    }
     if (tid < 2) {
         st_dE[tid][64] = 0;
         st_luh[tid][64] = 0.0;
@@ -223,7 +229,8 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
         // only when a move is accepted
         double Eo = MODEL == 0 ? -(double)pair : -(double)pair + D * (double)qq;
         long long Ei = -(long long)pair + (MODEL == 0 ? 0 : Di * (long long)qq);
-        const bool lin = int_walk && (MODEL == 1 || (lin_sh <= 2 && ((Ei - start_i) & smask) == 0));
+        const bool lin = int_walk && nb < (1 << 27) &&
+                         (MODEL == 1 || (lin_sh <= 2 && ((Ei - start_i) & smask) == 0));
         int b_old = 0;
         bool old_ok = PS ? bin_int((long long)qq, b_old)
                          : (int_e ? ibin(Ei, b_old)
@@ -356,18 +363,23 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                 // all lanes after the chunk.
                 int err_v = -1;
                 if (lane == 0 && lin) {
-                    // Linear bins: the state is the bin alone and an attempt
-                    // proposes bin + (staged bin change).  The weight reads
-                    // of the next attempt may reach max_dbin bins past the
-                    // ends of the table (a proposal there is out of the
-                    // domain and its weight unused): the padding slots.
+                    // Linear bins: the state is the byte offset Ea = 8 * bin of
+                    // the current bin's weight and an attempt proposes Ea plus
+                    // its staged offset change, so that the next weight's LDS
+                    // address is one add.  No domain test in the fast loop: a
+                    // chunk's proposals stay inside the window except past the
+                    // ends of the table, where they read the NaN padding slots
+                    // (lr = NaN fails both compares and stops the loop; the
+                    // exact step below raises the bounds error).
                     if (b_cur != b_old && old_ok) {
                         lw_cur = lw[b_old - wbase];
                         b_cur = b_old;
                     }
-                    const double* lwb = lw - wbase; // weight of bin b: lwb[b]
-                    double* lwbw = lw - wbase;
-                    int Eb = b_old;
+                    // weight at byte offset e: *(lwb + e)
+                    const char* lwb = reinterpret_cast<const char*>(lw - wbase);
+                    char* lwbw = reinterpret_cast<char*>(lw - wbase);
+                    auto wt = [&](int e) { return *reinterpret_cast<const double*>(lwb + e); };
+                    int Ea = 8 * b_old;
                     int kk = 0;
                     if (!old_ok) {
                         n_done = 0;
@@ -379,30 +391,29 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                         bool alive = true;
                         int dB_c = st_dE[buf][kk];
                         double luh_c = st_luh[buf][kk], lul_c = st_lul[buf][kk];
-                        double lw_pre = lwb[Eb + dB_c];
+                        double lw_pre = wt(Ea + dB_c);
 #pragma unroll 2
                         for (int k = kk; k < cnt; ++k) {
                             const int dB_n = st_dE[buf][k + 1];
                             const double luh_n = st_luh[buf][k + 1], lul_n = st_lul[buf][k + 1];
-                            const int bn = Eb + dB_c;
-                            const bool okn = (unsigned)bn < nbu;
+                            const int En = Ea + dB_c;
                             // a Wang-Landau walk has just lowered the current
                             // bin's weight after k+1's was read
-                            const double lw_new = !WL ? lw_pre : (bn == Eb ? lw_cur : lw_pre);
+                            const double lw_new = !WL ? lw_pre : (En == Ea ? lw_cur : lw_pre);
                             const double lr = lw_new - lw_cur;
                             const bool acc_c = lr > luh_c;
-                            const bool go = alive && okn && (acc_c || lr < lul_c);
+                            const bool go = alive && (acc_c || lr < lul_c);
                             const bool ok = go && acc_c;
                             stop = alive && !go ? k : stop;
                             alive = go;
-                            Eb = ok ? bn : Eb;
+                            Ea = ok ? En : Ea;
                             lw_cur = ok ? lw_new : lw_cur;
-                            st_res[buf][k] = Eb | (ok ? (int)0x80000000 : 0);
-                            lw_pre = lwb[Eb + dB_n];
+                            st_res[buf][k] = Ea | (ok ? (int)0x80000000 : 0);
+                            lw_pre = wt(Ea + dB_n);
                             if (WL) {
                                 // unchanged value when the attempt did not run
                                 lw_cur = go ? lw_cur - logf : lw_cur;
-                                lwbw[Eb] = lw_cur;
+                                *reinterpret_cast<double*>(lwbw + Ea) = lw_cur;
                             }
                             dB_c = dB_n;
                             luh_c = luh_n;
@@ -411,29 +422,29 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                         if (stop == cnt)
                             break;
                         // the attempt that stopped the fast loop, exactly
-                        const int bb = Eb + st_dE[buf][stop];
-                        if (!((unsigned)bb < nbu)) {
+                        const int En = Ea + st_dE[buf][stop];
+                        if (!((unsigned)En < 8u * nbu)) {
                             n_done = stop;
                             err_v = st_site[buf][stop];
                             break;
                         }
-                        const double lw_new = bb == Eb ? lw_cur : lwb[bb];
+                        const double lw_new = En == Ea ? lw_cur : wt(En);
                         const double lr = lw_new - lw_cur;
                         const bool ok = lr > 0 || st_u[buf][stop] < exp(lr);
                         if (ok) {
-                            Eb = bb;
+                            Ea = En;
                             lw_cur = lw_new;
                         }
-                        st_res[buf][stop] = Eb | (ok ? (int)0x80000000 : 0);
+                        st_res[buf][stop] = Ea | (ok ? (int)0x80000000 : 0);
                         if (WL) {
                             lw_cur = lw_cur - logf;
-                            lwbw[Eb] = lw_cur;
+                            *reinterpret_cast<double*>(lwbw + Ea) = lw_cur;
                         }
                         kk = stop + 1;
                     }
-                    b_old = Eb;
-                    b_cur = Eb;
-                    Ei = start_i + ((long long)Eb << lin_sh);
+                    b_old = Ea >> 3;
+                    b_cur = b_old;
+                    Ei = start_i + ((long long)b_old << lin_sh);
                 } else if (lane == 0) {
                     // bin of an energy (32-bit): Ising floor-free div by 2^s
                     // rounding toward zero as div does; Blume-Capel E - start
@@ -620,8 +631,9 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
             const int nd = __builtin_amdgcn_readfirstlane(sh_nd[pb]);
             int my_ok = 0, my_vis = 0;
             if (lane < nd) {
+                // (the linear walk records weight byte offsets, 8 * bin)
                 const int r = st_res[pb][lane];
-                my_vis = r & 0x7fffffff;
+                my_vis = (r & 0x7fffffff) >> (lin ? 3 : 0);
                 my_ok = (r >> 31) & 1;
             }
             const int s_ok = wave_sum(my_ok);
@@ -703,7 +715,7 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                 st_luh[buf][lane] = t_luh;
                 st_lul[buf][lane] = t_lul;
             } else if (int_e) {
-                st_dE[buf][lane] = lin ? t_dE >> lin_sh : t_dE;
+                st_dE[buf][lane] = lin ? 8 * (t_dE >> lin_sh) : t_dE;
                 // lr > 0 || lr > log(u) + 1e-9  <=>  lr > min(0, log(u) + 1e-9)
                 st_luh[buf][lane] = t_luh < 0.0 ? t_luh : 0.0;
                 st_lul[buf][lane] = t_lul;
