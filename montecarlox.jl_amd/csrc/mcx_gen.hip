@@ -388,39 +388,75 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                     }
                     while (kk < cnt) {
                         int stop = cnt;
-                        bool alive = true;
                         int dB_c = st_dE[buf][kk];
                         double luh_c = st_luh[buf][kk], lul_c = st_lul[buf][kk];
                         double lw_pre = wt(Ea + dB_c);
                         // (a walking wave issues about one instruction per 7
                         // cycles, so the loop's own counter and pointer updates
                         // count: unrolled by 16 they are ~1 of ~20 per attempt)
+                        if (!WL) {
+                            // Multicanonical: the weights do not change, so the
+                            // pass need not stop at an attempt in the band: it
+                            // notes that one occurred, and the first such
+                            // attempt is found afterwards from the records (the
+                            // walk's state before attempt k is the bin recorded
+                            // for k-1, and lw_cur is always that bin's weight)
+                            const int Ea0 = Ea;
+                            bool band = false;
 #pragma unroll 16
-                        for (int k = kk; k < cnt; ++k) {
-                            const int dB_n = st_dE[buf][k + 1];
-                            const double luh_n = st_luh[buf][k + 1], lul_n = st_lul[buf][k + 1];
-                            const int En = Ea + dB_c;
-                            // a Wang-Landau walk has just lowered the current
-                            // bin's weight after k+1's was read
-                            const double lw_new = !WL ? lw_pre : (En == Ea ? lw_cur : lw_pre);
-                            const double lr = lw_new - lw_cur;
-                            const bool acc_c = lr > luh_c;
-                            const bool go = alive && (acc_c || lr < lul_c);
-                            const bool ok = go && acc_c;
-                            stop = alive && !go ? k : stop;
-                            alive = go;
-                            Ea = ok ? En : Ea;
-                            lw_cur = ok ? lw_new : lw_cur;
-                            st_res[buf][k] = Ea | (ok ? (int)0x80000000 : 0);
-                            lw_pre = wt(Ea + dB_n);
-                            if (WL) {
+                            for (int k = kk; k < cnt; ++k) {
+                                const int dB_n = st_dE[buf][k + 1];
+                                const double luh_n = st_luh[buf][k + 1], lul_n = st_lul[buf][k + 1];
+                                const int En = Ea + dB_c;
+                                const double lr = lw_pre - lw_cur;
+                                const bool acc_c = lr > luh_c;
+                                band = band | !(acc_c | (lr < lul_c));
+                                Ea = acc_c ? En : Ea;
+                                lw_cur = acc_c ? lw_pre : lw_cur;
+                                st_res[buf][k] = Ea | (acc_c ? (int)0x80000000 : 0);
+                                lw_pre = wt(Ea + dB_n);
+                                dB_c = dB_n;
+                                luh_c = luh_n;
+                                lul_c = lul_n;
+                            }
+                            if (!band)
+                                break;
+                            int Es = Ea0;
+                            for (stop = kk; stop < cnt; ++stop) {
+                                const double lr = wt(Es + st_dE[buf][stop]) - wt(Es);
+                                if (!(lr > st_luh[buf][stop]) && !(lr < st_lul[buf][stop]))
+                                    break;
+                                Es = st_res[buf][stop] & 0x7fffffff;
+                            }
+                            Ea = Es;
+                            lw_cur = wt(Es);
+                        } else {
+                            bool alive = true;
+#pragma unroll 16
+                            for (int k = kk; k < cnt; ++k) {
+                                const int dB_n = st_dE[buf][k + 1];
+                                const double luh_n = st_luh[buf][k + 1], lul_n = st_lul[buf][k + 1];
+                                const int En = Ea + dB_c;
+                                // the current bin's weight has just been lowered
+                                // after k+1's was read
+                                const double lw_new = En == Ea ? lw_cur : lw_pre;
+                                const double lr = lw_new - lw_cur;
+                                const bool acc_c = lr > luh_c;
+                                const bool go = alive && (acc_c || lr < lul_c);
+                                const bool ok = go && acc_c;
+                                stop = alive && !go ? k : stop;
+                                alive = go;
+                                Ea = ok ? En : Ea;
+                                lw_cur = ok ? lw_new : lw_cur;
+                                st_res[buf][k] = Ea | (ok ? (int)0x80000000 : 0);
+                                lw_pre = wt(Ea + dB_n);
                                 // unchanged value when the attempt did not run
                                 lw_cur = go ? lw_cur - logf : lw_cur;
                                 *reinterpret_cast<double*>(lwbw + Ea) = lw_cur;
+                                dB_c = dB_n;
+                                luh_c = luh_n;
+                                lul_c = lul_n;
                             }
-                            dB_c = dB_n;
-                            luh_c = luh_n;
-                            lul_c = lul_n;
                         }
                         if (stop == cnt)
                             break;

@@ -34,9 +34,10 @@ def _walkers(mcx, model, dim, L, n, kind, start, step, nbins, seed, lw0=None, lo
 
 
 def _check(mcx, orc, model, dim, L, n, kind, start, step, nbins, seed, nsweeps, D=0.0, logf=1.0,
-           lw_scale=0.3, runs=None):
+           lw_scale=0.3, runs=None, lw0=None):
     rng = np.random.default_rng(seed)
-    lw0 = rng.normal(0, lw_scale, size=(n, nbins))
+    if lw0 is None:
+        lw0 = rng.normal(0, lw_scale, size=(n, nbins))
     sys, algs = _walkers(mcx, model, dim, L, n, kind, start, step, nbins, seed, lw0, logf, D,
                          cap=nsweeps)
     init = [sys.batch.get_spins(k) for k in range(n)]
@@ -55,7 +56,7 @@ def _check(mcx, orc, model, dim, L, n, kind, start, step, nbins, seed, nsweeps, 
         hg = e.histogram.values if kind == "muca" else e.device_histogram
         assert np.array_equal(s, sys.batch.get_spins(k)), f"walker {k} spins"
         assert np.array_equal(h, np.asarray(hg, np.int64)), f"walker {k} histogram"
-        assert np.array_equal(lw, e.logweight.values), f"walker {k} log-weights"
+        assert np.array_equal(lw, e.logweight.values, equal_nan=True), f"walker {k} log-weights"
         Eg, Mg, Qg = sys.batch.series(k)
         assert np.array_equal(E, Eg) and np.array_equal(M, Mg)
         if Qg is not None:
@@ -80,6 +81,34 @@ def test_ising_integer_walk_bin_layouts(mcx, orc, kind, off, step):
     start = -2 * N - off
     nbins = (4 * N + off) // step + 2
     _check(mcx, orc, "ising", 2, 8, 2, kind, start, step, nbins, 23, 10)
+
+
+@pytest.mark.parametrize("kind", ["muca", "wl"])
+@pytest.mark.parametrize("model", ["ising", "blume_capel"])
+def test_nan_weights_are_rejected(mcx, orc, kind, model):
+    """Bins whose log-weight is NaN: a proposal into one fails both of the
+    walk's fast compares (the multicanonical pass finds such an attempt
+    afterwards from its records) and is settled exactly -- rejected, since
+    lr > 0 || u < exp(lr) is false for lr = NaN -- and the walk goes on."""
+    n, seed = 2, 29
+    if model == "ising":
+        L, N = 8, 64
+        start, step, nbins, D = -2 * N, 4, N + 1, 0.0
+    else:
+        L, N = 6, 36
+        start, step, D = -2.0 * N, 1.0, 1.0
+        nbins = int(round(3 * N - start)) + 2
+    lw0 = np.random.default_rng(seed).normal(0, 0.3, size=(n, nbins))
+    lw0[:, 3::7] = np.nan
+    # the walkers' starting bins keep a number (the same random start as _walkers)
+    sys, _ = _walkers(mcx, model, 2, L, n, kind, start, step, nbins, seed, D=D)
+    for k in range(n):
+        s0 = sys.batch.get_spins(k).astype(np.int64).reshape(L, L)
+        E = -(s0 * np.roll(s0, 1, 0)).sum() - (s0 * np.roll(s0, 1, 1)).sum() + D * (s0 ** 2).sum()
+        b = int((E - start) // step)
+        lw0[k, max(0, b - 2):b + 3] = 0.0
+    sys.batch.close()
+    _check(mcx, orc, model, 2, L, n, kind, start, step, nbins, seed, 8, D=D, logf=0.5, lw0=lw0)
 
 
 @pytest.mark.parametrize("kind", ["muca", "wl"])
