@@ -24,7 +24,10 @@
 // load and no exp unless lr falls within 1e-9 of log(u).  In the integer walk
 // the chunk's spin writes, counters, running sums and visit counts are done
 // by wave 1 from the recorded outcomes while wave 0 walks the next chunk; the
-// other walks apply spins and visits with wave 0's lanes after the walk.
+// other walks apply spins and visits with wave 0's lanes after the walk.  When
+// the bins are linear in the energy the integer walk carries only the byte
+// offset of the current bin's weight (one add per proposal, no domain test:
+// NaN padding slots around the window stop a pass that leaves the table).
 // Histograms are uint32 in LDS when they fit, otherwise int64 atomics in HBM.
 //
 // KIND 2 (Blume-Capel): the CustomEnsemble of docs/src/examples/spin_systems/
