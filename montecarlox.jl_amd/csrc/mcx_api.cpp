@@ -218,15 +218,14 @@ size_t lattice_lds_bytes(int model, int dim, int L)
     // Blume-Capel: static class tables + kBCLdsHeaderWords-word header
     // (partials, E/M/Q ring) + occupancy and up planes per colour
     if (model == MCX_ISING)
-        return 768 + 2 * (size_t)rows * (size_t)nw * 4;
+        return 768 + 2 * (size_t)ising_plane_words(dim, L) * 4;
     return (size_t)kBCStaticLds + 4 * (size_t)kBCLdsHeaderWords + 4 * (size_t)rows * (size_t)nw * 4;
 }
 
-int lattice_block(int dim, int L)
+int lattice_block(int model, int dim, int L)
 {
-    const int half = L / 2;
-    const int nw = (half + 31) / 32;
-    const int64_t words = (dim == 3 ? (int64_t)L * L : L) * nw;
+    const int64_t words = model == MCX_ISING ? ising_plane_words(dim, L)
+                                             : (dim == 3 ? (int64_t)L * L : L) * ((L / 2 + 31) / 32);
     int64_t b = (words + 63) / 64 * 64;
     if (b < 64)
         b = 64;
@@ -266,7 +265,7 @@ mcx_status ensure_planes(mcx_batch* b)
         const int64_t nw = (L / 2 + 31) / 32;
         const int64_t rows = b->dim == 3 ? L * L : L;
         off[k] = total;
-        total += (b->model == MCX_ISING ? 2 : 4) * rows * nw;
+        total += b->model == MCX_ISING ? 2 * ising_plane_words(b->dim, (int)L) : 4 * rows * nw;
     }
     HIPCHK(hipMalloc((void**)&b->d_plane_off, sizeof(int64_t) * (size_t)b->n));
     hipError_t e = hipMalloc((void**)&b->d_planes, sizeof(uint32_t) * (size_t)std::max<int64_t>(total, 1));
@@ -392,20 +391,27 @@ extern "C" mcx_status mcx_batch_create(mcx_ctx* ctx, const mcx_batch_desc* d, mc
     }
     b->pool_bytes = off;
 
-    // launch plan: chains grouped by workgroup size, largest lattices first
+    // launch plan: chains grouped by workgroup size (and, for Ising, plane
+    // layout), largest lattices first
     std::vector<int32_t> order(b->n);
     for (int i = 0; i < b->n; ++i)
         order[i] = i;
+    auto flat_of = [&](int i) { return b->model == MCX_ISING && ising_flat(b->rec[i].L); };
     std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
-        const int ba = lattice_block(b->dim, b->rec[a].L), bc = lattice_block(b->dim, b->rec[c].L);
+        const int ba = lattice_block(b->model, b->dim, b->rec[a].L);
+        const int bc = lattice_block(b->model, b->dim, b->rec[c].L);
         if (ba != bc)
             return ba > bc;
+        if (flat_of(a) != flat_of(c))
+            return flat_of(c);
         return b->nsites[a] > b->nsites[c];
     });
     for (int k = 0; k < b->n;) {
-        const int blk = lattice_block(b->dim, b->rec[order[k]].L);
-        mcx_batch::Launch ln{k, 0, blk, 0};
-        while (k < b->n && lattice_block(b->dim, b->rec[order[k]].L) == blk) {
+        const int blk = lattice_block(b->model, b->dim, b->rec[order[k]].L);
+        const bool fl = flat_of(order[k]);
+        mcx_batch::Launch ln{k, 0, blk, 0, fl};
+        while (k < b->n && lattice_block(b->model, b->dim, b->rec[order[k]].L) == blk &&
+               flat_of(order[k]) == fl) {
             ln.lds = std::max(ln.lds, lattice_lds_bytes(b->model, b->dim, b->rec[order[k]].L));
             ++ln.count;
             ++k;
@@ -720,7 +726,7 @@ mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t
             const int cpg = sh.chains_per_group > 1 ? sh.chains_per_group : 1;
             const int grid = (ln.count + cpg - 1) / cpg;
             if (b->model == MCX_ISING)
-                HIPCHK(launch_ising_sweeps(b->dim, alg, a, block, grid, ln.lds, st));
+                HIPCHK(launch_ising_sweeps(b->dim, alg, ln.flat, a, block, grid, ln.lds, st));
             else
                 HIPCHK(launch_bc_sweeps(b->dim, alg, a, block, grid, ln.lds, st));
         }

@@ -34,6 +34,7 @@ struct mcx_batch {
     struct Launch {
         int first, count, block;
         size_t lds;
+        bool flat; // Ising: the chains' planes have the flat layout
     };
     mcx_ctx* ctx = nullptr;
     int model = 0, dim = 2, n = 0;
@@ -160,7 +161,7 @@ void fill_bc_metro_from_p(BCTable& t, int dim, const double* p);
 void fill_bc_hb_from_p(BCTable& t, int dim, const double* p);
 void fill_bc_gl_from_p(BCTable& t, int dim, const double* p);
 size_t lattice_lds_bytes(int model, int dim, int L);
-int lattice_block(int dim, int L);
+int lattice_block(int model, int dim, int L);
 mcx_status upload_chains(mcx_batch* b);
 mcx_status pull_chains(mcx_batch* b);
 mcx_status set_chain_tables(mcx_batch* b, const double* beta_per_table);

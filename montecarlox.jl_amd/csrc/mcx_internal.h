@@ -60,6 +60,22 @@ struct IsingTable {
 constexpr int kBCLdsHeaderWords = 300;
 constexpr int kBCStaticLds = 1088;
 
+// Ising lattices with L < 64 and L % 8 == 0 keep each colour's sites flat in
+// (z, y, x) order, 32 to a word with no partial words (the sweep stream's
+// group is 2 * word + colour); other sizes keep one or more words per row.
+// Such lattices always run in one workgroup (never on the tiled path).
+__host__ __device__ constexpr bool ising_flat(int L)
+{
+    return L < 64 && (L & 7) == 0;
+}
+
+// Ising bit-plane words per colour: L^dim / 64 flat, else rows * words per row.
+__host__ __device__ constexpr int64_t ising_plane_words(int dim, int L)
+{
+    return ising_flat(L) ? (int64_t)L * L * (dim == 3 ? L : 1) / 64
+                         : (int64_t)(dim == 3 ? L * L : L) * ((L / 2 + 31) / 32);
+}
+
 struct BCTable {
     uint32_t metro_thr[80];
     uint32_t metro_always[3];
@@ -149,8 +165,8 @@ struct TileArgs {
 };
 
 // Launchers (defined in the .hip translation units).
-hipError_t launch_ising_sweeps(int dim, int alg, const SweepArgs& a, int block, int grid,
-                               size_t lds_bytes, hipStream_t st);
+hipError_t launch_ising_sweeps(int dim, int alg, bool flat, const SweepArgs& a, int block,
+                               int grid, size_t lds_bytes, hipStream_t st);
 hipError_t launch_ising_plain(int dim, int alg, const PlainArgs& a, int n_sites_colour,
                               hipStream_t st);
 hipError_t launch_bc_sweeps(int dim, int alg, const SweepArgs& a, int block, int grid,

@@ -191,6 +191,133 @@ __device__ __forceinline__ void ising_unpack_word(int8_t* __restrict__ sp,
     }
 }
 
+// ---- flat layout (ising_flat(L)): bit i of word w of colour c is the colour-c
+// site with flat index f = 32w + i = (y + L z) * L/2 + j, x = 2j + ((c + y + z) & 1).
+// A z-plane is pw = L * L/2 / 32 whole words, so y-neighbours are the words
+// shifted by L/2 bits within the plane (cyclically), z-neighbours the words one
+// plane away, and the x-neighbours the other colour's same bit and its bit one
+// to the left or right within the row (cyclically within the row).
+
+// int8 lattice -> flat word w (0 .. 2 fw) of both colours' planes.
+template <int DIM>
+__device__ __forceinline__ void ising_pack_flat(const int8_t* __restrict__ sp,
+                                                uint32_t* __restrict__ P, int w, int L, int half,
+                                                int fw)
+{
+    const int c = w >= fw ? 1 : 0;
+    const int f = 32 * (w - c * fw);
+    int r = f / half, j = f - r * half;
+    int y = DIM == 3 ? r % L : r, z = DIM == 3 ? r / L : 0;
+    uint32_t bits = 0;
+    for (int i = 0; i < 32; ++i) {
+        const int x = 2 * j + ((c + y + z) & 1);
+        bits |= (uint32_t)(sp[(int64_t)r * L + x] > 0) << i;
+        if (++j == half) {
+            j = 0;
+            ++r;
+            if (++y == L) {
+                y = 0;
+                ++z;
+            }
+        }
+    }
+    P[w] = bits;
+}
+
+// flat word w (0 .. 2 fw) -> int8 lattice
+template <int DIM>
+__device__ __forceinline__ void ising_unpack_flat(int8_t* __restrict__ sp,
+                                                  const uint32_t* __restrict__ P, int w, int L,
+                                                  int half, int fw)
+{
+    const int c = w >= fw ? 1 : 0;
+    const int f = 32 * (w - c * fw);
+    int r = f / half, j = f - r * half;
+    int y = DIM == 3 ? r % L : r, z = DIM == 3 ? r / L : 0;
+    const uint32_t bits = P[w];
+    for (int i = 0; i < 32; ++i) {
+        const int x = 2 * j + ((c + y + z) & 1);
+        sp[(int64_t)r * L + x] = ((bits >> i) & 1u) ? 1 : -1;
+        if (++j == half) {
+            j = 0;
+            ++r;
+            if (++y == L) {
+                y = 0;
+                ++z;
+            }
+        }
+    }
+}
+
+// Per-lattice constants of the flat neighbour words.
+struct FlatGeo {
+    int half, pw, fw;  // bits per row, words per z-plane, words per colour
+    uint32_t starts;   // row starts of a word whose bit 0 starts a row
+    uint32_t odd;      // bits of that word in rows 1, 3, ... after bit 0's row
+    uint32_t mdiv;     // n / half = (n * mdiv) >> 16 for the n < 2^11 used here
+    uint32_t mplane;   // w / pw = (w * mplane) >> 20 for w < 2^14
+};
+
+__device__ __forceinline__ FlatGeo flat_geo(int dim, int L)
+{
+    FlatGeo g;
+    g.half = L >> 1;
+    g.pw = (L * g.half) >> 5;
+    g.fw = dim == 3 ? g.pw * L : g.pw;
+    g.starts = 0;
+    g.odd = 0;
+    for (int k = 0; k * g.half < 32; ++k) {
+        g.starts |= 1u << (k * g.half);
+        if (k & 1)
+            g.odd |= (0xffffffffu << (k * g.half))
+                   & ((k + 1) * g.half >= 32 ? 0xffffffffu : ((1u << ((k + 1) * g.half)) - 1u));
+    }
+    g.mdiv = (65536u + (uint32_t)g.half - 1u) / (uint32_t)g.half;
+    g.mplane = ((1u << 20) + (uint32_t)g.pw - 1u) / (uint32_t)g.pw;
+    return g;
+}
+
+// Neighbour words of flat word w of colour c (in the other colour's plane Po).
+template <int DIM>
+__device__ __forceinline__ void load_nbr_flat(const uint32_t* __restrict__ Po, const FlatGeo& g,
+                                              int L, int c, int w, NbrWords& n)
+{
+    const int zi = DIM == 3 ? (int)(__umul24((uint32_t)w, g.mplane) >> 20) : 0;
+    const int pb = DIM == 3 ? (int)__umul24((uint32_t)zi, (uint32_t)g.pw) : 0;
+    const int wl = w - pb;
+    const int wp = pb + (wl == 0 ? g.pw - 1 : wl - 1);
+    const int wn = pb + (wl == g.pw - 1 ? 0 : wl + 1);
+    const uint32_t a0 = Po[w], am = Po[wp], ap = Po[wn];
+    if (DIM == 3) {
+        n.Zu = Po[zi == 0 ? w + g.fw - g.pw : w - g.pw];
+        n.Zd = Po[zi == L - 1 ? w - g.fw + g.pw : w + g.pw];
+    } else {
+        n.Zu = 0;
+        n.Zd = 0;
+    }
+    const uint32_t h = (uint32_t)g.half;
+    // rows in this word: bit 0 is bit o of row y0; the next row starts at i0
+    const uint32_t f0 = 32u * (uint32_t)wl;
+    const uint32_t y0 = __umul24(f0, g.mdiv) >> 16;
+    const uint32_t o = f0 - y0 * h;
+    const uint32_t i0 = o == 0 ? 0u : h - o;
+    const uint32_t y1 = __umul24(f0 + 32u, g.mdiv) >> 16;
+    const uint32_t S = g.starts << i0;                                      // row starts
+    const uint32_t E = (S >> 1) | (f0 + 32u == y1 * h ? 0x80000000u : 0u); // row ends
+    // bits whose x-neighbour on the far side is to the right (p = 1)
+    const uint32_t X = (g.odd << i0) | ((1u << i0) - 1u);
+    const uint32_t yref = y0 + (i0 != 0 ? 1u : 0u);
+    const uint32_t Pm = X ^ (0u - (((uint32_t)c + yref + (uint32_t)zi) & 1u));
+    n.A = a0;
+    n.U = __builtin_amdgcn_alignbit(a0, am, 32u - h); // f - half
+    n.Dn = __builtin_amdgcn_alignbit(ap, a0, h);      // f + half
+    const uint32_t left = __builtin_amdgcn_bitop3_b32(
+        S, __builtin_amdgcn_alignbit(ap, a0, h - 1u), __builtin_amdgcn_alignbit(a0, am, 31u), 0xCA);
+    const uint32_t right = __builtin_amdgcn_bitop3_b32(
+        E, __builtin_amdgcn_alignbit(a0, am, 33u - h), __builtin_amdgcn_alignbit(ap, a0, 1u), 0xCA);
+    n.Bw = __builtin_amdgcn_bitop3_b32(Pm, right, left, 0xCA);
+}
+
 // One 32-site word of colour c: the new spins of the word from its old spins
 // s0 and its neighbour words (the per-site rule above, bit-sliced).  Shared by
 // the LDS-resident batched kernel and the multi-workgroup tile kernel.
@@ -286,7 +413,9 @@ __device__ __forceinline__ uint32_t ising_word(uint32_t s0, const NbrWords& n, u
     }
 }
 
-template <int DIM, int ALG>
+// FLAT: every item of the launch has the flat layout (ising_flat(L)), else
+// none has (the two word loops are separate instantiations).
+template <int DIM, int ALG, bool FLAT>
 __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
 {
     extern __shared__ uint32_t lds[];
@@ -313,7 +442,8 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
         const int half = L >> 1;
         const int nw = (half + 31) >> 5;
         const int rows = DIM == 3 ? L * L : L;
-        const int words = rows * nw;
+        constexpr bool flat = FLAT;
+        const int words = (int)ising_plane_words(DIM, L); // per colour
         const int64_t N = (int64_t)rows * L;
         int8_t* __restrict__ sp = a.spins + cr.spin_off;
         const IsingTable tb = a.itab[cr.table];
@@ -323,6 +453,9 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
             const uint32_t* src = a.planes + a.plane_off[ch];
             for (int w = tid; w < 2 * words; w += B)
                 P[w] = src[w];
+        } else if (flat) {
+            for (int w = tid; w < 2 * words; w += B)
+                ising_pack_flat<DIM>(sp, P, w, L, half, words);
         } else {
             for (int w = tid; w < words; w += B)
                 ising_pack_word<DIM>(sp, P, w, L, nw, half, words);
@@ -356,6 +489,29 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                 const uint32_t* __restrict__ Po = P + (c ^ 1) * words;
                 // (wave-uniform trip count: the chunk loop inside sees uniform
                 // control flow, so its per-chunk counter words stay scalar)
+                if (flat) {
+                    const FlatGeo g = flat_geo(DIM, L);
+                    for (int wb = tid - lane; wb < words; wb += B) {
+                        const int w = wb + lane;
+                        if (w >= words)
+                            continue;
+                        const uint32_t s0 = Pc[w];
+                        NbrWords n;
+                        load_nbr_flat<DIM>(Po, g, L, c, w, n);
+                        const uint32_t snew = ising_word<DIM, ALG>(
+                            s0, n, 0xffffffffu, tb, 2u * (uint32_t)w + (uint32_t)c, sw_lo, w2, c3,
+                            k0, cr.key);
+                        acc += (unsigned long long)__popc(snew ^ s0);
+                        Pc[w] = snew;
+                        if (c == 1 && obs) {
+                            na += __popc(snew ^ n.U) + __popc(snew ^ n.Dn) + __popc(snew ^ n.A)
+                                + __popc(snew ^ n.Bw);
+                            if (DIM == 3)
+                                na += __popc(snew ^ n.Zu) + __popc(snew ^ n.Zd);
+                            nu += __popc(snew) + __popc(n.A);
+                        }
+                    }
+                } else
                 for (int wb = tid - lane; wb < words; wb += B) {
                     const int w = wb + lane;
                     if (w >= words)
@@ -437,6 +593,9 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
             uint32_t* dst = a.planes + a.plane_off[ch];
             for (int w = tid; w < 2 * words; w += B)
                 dst[w] = P[w];
+        } else if (flat) {
+            for (int w = tid; w < 2 * words; w += B)
+                ising_unpack_flat<DIM>(sp, P, w, L, half, words);
         } else {
             for (int w = tid; w < words; w += B)
                 ising_unpack_word<DIM>(sp, P, w, L, nw, half, words);
@@ -518,8 +677,11 @@ __global__ void __launch_bounds__(256) k_ising_plain(PlainArgs a, int n_col)
         }
         const int8_t si = sp[rowb + x];
         const IsingTable& tb = a.itab[cr.table];
-        const uint32_t group = (uint32_t)((r * 2 + c) * nw + (j >> 5));
-        const int t = j & 31;
+        // the sweep stream's (group, bit) of the site: flat or row words
+        const int f = r * half + j;
+        const uint32_t group = ising_flat(L) ? (uint32_t)(2 * (f >> 5) + c)
+                                             : (uint32_t)((r * 2 + c) * nw + (j >> 5));
+        const int t = ising_flat(L) ? (f & 31) : (j & 31);
         const uint32_t sw_lo = (uint32_t)a.sweep;
         const uint32_t k0 = (uint32_t)cr.seed, c3 = (uint32_t)(cr.seed >> 32);
         int8_t snew = si;
@@ -820,18 +982,24 @@ hipError_t launch_ising_pack(int dim, int unpack, const TileArgs& a, int max_wor
     return hipGetLastError();
 }
 
-hipError_t launch_ising_sweeps(int dim, int alg, const SweepArgs& a, int block, int grid,
-                               size_t lds_bytes, hipStream_t st)
+hipError_t launch_ising_sweeps(int dim, int alg, bool flat, const SweepArgs& a, int block,
+                               int grid, size_t lds_bytes, hipStream_t st)
 {
     const dim3 g(grid), bl(block);
-    const int k = ising_alg(alg) + (dim == 3 ? 3 : 0);
+    const int k = ising_alg(alg) + (dim == 3 ? 3 : 0) + (flat ? 6 : 0);
     switch (k) {
-    case 0: hipLaunchKernelGGL((k_ising_sweeps<2, 0>), g, bl, lds_bytes, st, a); break;
-    case 1: hipLaunchKernelGGL((k_ising_sweeps<2, 1>), g, bl, lds_bytes, st, a); break;
-    case 2: hipLaunchKernelGGL((k_ising_sweeps<2, 2>), g, bl, lds_bytes, st, a); break;
-    case 3: hipLaunchKernelGGL((k_ising_sweeps<3, 0>), g, bl, lds_bytes, st, a); break;
-    case 4: hipLaunchKernelGGL((k_ising_sweeps<3, 1>), g, bl, lds_bytes, st, a); break;
-    default: hipLaunchKernelGGL((k_ising_sweeps<3, 2>), g, bl, lds_bytes, st, a); break;
+    case 0: hipLaunchKernelGGL((k_ising_sweeps<2, 0, false>), g, bl, lds_bytes, st, a); break;
+    case 1: hipLaunchKernelGGL((k_ising_sweeps<2, 1, false>), g, bl, lds_bytes, st, a); break;
+    case 2: hipLaunchKernelGGL((k_ising_sweeps<2, 2, false>), g, bl, lds_bytes, st, a); break;
+    case 3: hipLaunchKernelGGL((k_ising_sweeps<3, 0, false>), g, bl, lds_bytes, st, a); break;
+    case 4: hipLaunchKernelGGL((k_ising_sweeps<3, 1, false>), g, bl, lds_bytes, st, a); break;
+    case 5: hipLaunchKernelGGL((k_ising_sweeps<3, 2, false>), g, bl, lds_bytes, st, a); break;
+    case 6: hipLaunchKernelGGL((k_ising_sweeps<2, 0, true>), g, bl, lds_bytes, st, a); break;
+    case 7: hipLaunchKernelGGL((k_ising_sweeps<2, 1, true>), g, bl, lds_bytes, st, a); break;
+    case 8: hipLaunchKernelGGL((k_ising_sweeps<2, 2, true>), g, bl, lds_bytes, st, a); break;
+    case 9: hipLaunchKernelGGL((k_ising_sweeps<3, 0, true>), g, bl, lds_bytes, st, a); break;
+    case 10: hipLaunchKernelGGL((k_ising_sweeps<3, 1, true>), g, bl, lds_bytes, st, a); break;
+    default: hipLaunchKernelGGL((k_ising_sweeps<3, 2, true>), g, bl, lds_bytes, st, a); break;
     }
     return hipGetLastError();
 }

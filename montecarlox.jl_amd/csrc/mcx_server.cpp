@@ -169,10 +169,11 @@ private:
 
 struct ClassKey {
     int model, dim, alg, block, lpw;
+    bool flat; // Ising plane layout (ising_flat(L))
     bool operator<(const ClassKey& o) const
     {
-        return std::tie(model, dim, alg, block, lpw) <
-               std::tie(o.model, o.dim, o.alg, o.block, o.lpw);
+        return std::tie(model, dim, alg, block, lpw, flat) <
+               std::tie(o.model, o.dim, o.alg, o.block, o.lpw, o.flat);
     }
     bool operator==(const ClassKey& o) const { return !(*this < o) && !(o < *this); }
 };
@@ -187,7 +188,8 @@ using ShapeKey = std::tuple<int, int, int, int>; // model, dim, alg, L
 ClassKey class_of(const Job& j)
 {
     return ClassKey{j.d.model, j.d.dim, j.d.alg, j.block <= 256 ? 256 : 1024,
-                    j.d.model == MCX_ISING ? 1 : -1};
+                    j.d.model == MCX_ISING ? 1 : -1,
+                    j.d.model == MCX_ISING && ising_flat(j.d.L)};
 }
 
 } // namespace
@@ -260,6 +262,9 @@ int job_words(const mcx_job_desc& d)
 bool job_tiled(const mcx_server* s, const mcx_job_desc& d)
 {
     const int64_t t = s->tile_sites[d.model == MCX_ISING ? 0 : 1];
+    // (flat Ising lattices, L < 64, always fit one workgroup's LDS)
+    if (d.model == MCX_ISING && ising_flat(d.L))
+        return false;
     return t > 0 && job_sites(d) >= t;
 }
 
@@ -285,7 +290,8 @@ int64_t job_bytes(const mcx_server* s, const mcx_job_desc& d)
 void server_shape(int model, int dim, int L, int& block, int& lpw)
 {
     const int64_t nw = (L / 2 + 31) / 32;
-    const int64_t words = (dim == 3 ? (int64_t)L * L : L) * nw;
+    const int64_t words =
+        model == MCX_ISING ? ising_plane_words(dim, L) : (dim == 3 ? (int64_t)L * L : L) * nw;
     lpw = model == MCX_ISING ? 1 : 4;
     while (lpw > 1 && words * lpw > 1024)
         lpw /= 2;
@@ -643,7 +649,7 @@ hipError_t run_quantum(mcx_server* s, std::vector<int64_t>& ran, std::vector<int
         a.lanes_per_word = k.lpw;
         hipStream_t st = s->streams[unit_stream(unit0 + (int)c)];
         e = k.model == MCX_ISING
-                ? launch_ising_sweeps(k.dim, k.alg, a, k.block, a.n_items, lds, st)
+                ? launch_ising_sweeps(k.dim, k.alg, k.flat, a, k.block, a.n_items, lds, st)
                 : launch_bc_sweeps(k.dim, k.alg, a, k.block, a.n_items, lds, st);
     }
     for (int k = 1; k < ns && e == hipSuccess; ++k) {

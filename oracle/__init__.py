@@ -75,6 +75,8 @@ def _declare(L):
     L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
     L.orc_site_group.argtypes = [C.c_int, C.c_int, C.c_int64, C.POINTER(C.c_uint32),
                                  C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.orc_ising_group.argtypes = [C.c_int, C.c_int, C.c_int64, C.POINTER(C.c_uint32),
+                                  C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.orc_bitsliced_word.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_uint64, C.c_int]
     L.orc_bitsliced_word.restype = C.c_uint32
     L.orc_site_word.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_uint64, C.c_int]
@@ -143,6 +145,13 @@ def philox4x32_10(ctr, key):
 def site_group(dim, L, site):
     g, b, c = C.c_uint32(), C.c_int(), C.c_int()
     lib().orc_site_group(dim, L, site, C.byref(g), C.byref(b), C.byref(c))
+    return g.value, b.value, c.value
+
+
+def ising_group(dim, L, site):
+    """(group, bit, colour) of an Ising site's bit-sliced sweep stream."""
+    g, b, c = C.c_uint32(), C.c_int(), C.c_int()
+    lib().orc_ising_group(dim, L, site, C.byref(g), C.byref(b), C.byref(c))
     return g.value, b.value, c.value
 
 

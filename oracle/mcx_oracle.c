@@ -107,6 +107,27 @@ void orc_site_group(int dim, int L, int64_t site, uint32_t* group, int* bit, int
     *colour = c;
 }
 
+/* The bit-sliced sweep stream of an Ising site.  Lattices with L < 64 and
+ * L % 8 == 0 lay each colour's sites out flat, in the order f = (y + L z) L/2
+ * + j (x = 2j + ((c + y + z) & 1)), so that every 32-site word is full: word
+ * w = f / 32, bit f % 32, group 2w + c.  Other lattices use the row words of
+ * orc_site_group. */
+void orc_ising_group(int dim, int L, int64_t site, uint32_t* group, int* bit, int* colour)
+{
+    if (L < 64 && (L & 7) == 0) {
+        geo_t G = make_geo(dim, L);
+        int x, y, z;
+        coords(&G, site, &x, &y, &z);
+        int c = (x + y + z) & 1;
+        int64_t f = ((int64_t)y + (int64_t)L * z) * (L / 2) + (x >> 1);
+        *group = (uint32_t)(2 * (f >> 5) + c);
+        *bit = (int)(f & 31);
+        *colour = c;
+        return;
+    }
+    orc_site_group(dim, L, site, group, bit, colour);
+}
+
 /* sum of the 2*dim nearest-neighbour spins, periodic boundaries
  * (the neighbour set of IsingLatticeOptim, ising.jl:444-456, and of the
  * periodic Graphs.grid lattice used by Ising(dims)/BlumeCapel(dims)). */
@@ -321,7 +342,10 @@ void orc_canonical_sweeps(int model, int alg, int dim, int L, double beta, doubl
                     continue;
                 uint32_t g;
                 int t, c;
-                orc_site_group(dim, L, i, &g, &t, &c);
+                if (model == 0)
+                    orc_ising_group(dim, L, i, &g, &t, &c);
+                else
+                    orc_site_group(dim, L, i, &g, &t, &c);
                 int8_t si = spins[i];
                 int nb = nbr_sum(&G, spins, i);
                 if (model == 0 && alg == 0) {

@@ -37,6 +37,7 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 /* Site -> (group, bit) map of the checkerboard layout. */
 void orc_site_group(int dim, int L, int64_t site, uint32_t* group, int* bit, int* colour);
+void orc_ising_group(int dim, int L, int64_t site, uint32_t* group, int* bit, int* colour);
 
 /* 32-bit uniforms of the contract. */
 uint32_t orc_bitsliced_word(uint64_t seed, uint32_t chain_key, uint32_t group, int bit,

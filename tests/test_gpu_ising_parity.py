@@ -75,6 +75,16 @@ def test_3d_glauber(mcx, orc, L):
     _run_case(mcx, orc, 3, GLAUBER, [L] * 3, [0.15, 0.2216544, 0.35], 6)
 
 
+@pytest.mark.parametrize("dim,L", [(2, 8), (2, 24), (2, 40), (2, 56), (3, 24), (3, 32), (3, 40)])
+def test_flat_layout_sizes(mcx, orc, dim, L):
+    """L < 64 with L % 8 == 0: each colour's sites packed flat, 32 to a word
+    (rows split across words); every algorithm, batched and plain."""
+    betas = [0.3, 0.4406868, 0.7] if dim == 2 else [0.15, 0.2216544, 0.35]
+    for alg in (METRO, HB, GLAUBER):
+        _run_case(mcx, orc, dim, alg, [L] * 3, betas, 4)
+    _run_case(mcx, orc, dim, METRO, [L], betas[1:2], 2, plain=True)
+
+
 def test_glauber_extremes_plain_and_split(mcx, orc):
     # beta = 0: p = 1/2 everywhere; beta = 100: logistic rounds to 0 and 1
     _run_case(mcx, orc, 2, GLAUBER, [16] * 4, [0.0, 1e-12, 5.0, 100.0], 8)

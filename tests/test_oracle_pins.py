@@ -52,6 +52,29 @@ def test_site_group_is_a_bijection(orc, dim, L):
     assert len(seen) == N
 
 
+@pytest.mark.parametrize("dim,L", [(2, 8), (2, 24), (2, 56), (3, 8), (3, 16), (3, 24)])
+def test_ising_group_fills_whole_words(orc, dim, L):
+    """L < 64, L % 8 == 0: the Ising sweep stream lays each colour's sites out
+    flat -- every group's 32 bits are used (N/32 groups in all), a colour's
+    groups have its parity, and along x the next site of the colour takes the
+    next bit (or bit 0 of the colour's next group) except where its row wraps."""
+    N = L ** dim
+    at = {}
+    for i in range(N):
+        g, t, c = orc.ising_group(dim, L, i)
+        x, y, z = i % L, (i // L) % L, i // (L * L)
+        assert c == (x + y + (z if dim == 3 else 0)) % 2 and g % 2 == c and 0 <= t < 32
+        at[(g, t)] = (x, y, z)
+    assert len(at) == N and len({g for g, _ in at}) == N // 32
+    for (g, t), (x, y, z) in at.items():
+        nxt = (g, t + 1) if t < 31 else (g + 2, 0)
+        if x + 2 < L:
+            assert at[nxt] == (x + 2, y, z)
+    # the row words of other sizes are the site words' groups
+    for i in range(6 ** dim):
+        assert orc.ising_group(dim, 6, i) == orc.site_group(dim, 6, i)
+
+
 def test_bitsliced_word_is_built_from_bit_planes(orc):
     seed, key, g, sweep = (5 << 32) | 9, 77, 123, 4321
     for t in (0, 5, 31):
@@ -420,7 +443,7 @@ def test_glauber_acceptance_is_the_logistic_rule(orc):
     # colour 0 sites see only colour-1 neighbours, which are unchanged when colour 0 runs
     for i in range(L * L):
         x, y = i % L, i // L
-        g, t, colour = orc.site_group(2, L, i)
+        g, t, colour = orc.ising_group(2, L, i)
         if colour != 0:
             continue
         nb = (int(before[y * L + (x + 1) % L]) + int(before[y * L + (x - 1) % L])
