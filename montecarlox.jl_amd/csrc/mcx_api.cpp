@@ -726,7 +726,7 @@ mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t
             const int cpg = sh.chains_per_group > 1 ? sh.chains_per_group : 1;
             const int grid = (ln.count + cpg - 1) / cpg;
             if (b->model == MCX_ISING)
-                HIPCHK(launch_ising_sweeps(b->dim, alg, ln.flat, a, block, grid, ln.lds, st));
+                HIPCHK(launch_ising_sweeps(b->dim, alg, ln.flat ? 1 : 0, a, block, grid, ln.lds, st));
             else
                 HIPCHK(launch_bc_sweeps(b->dim, alg, a, block, grid, ln.lds, st));
         }

@@ -165,7 +165,8 @@ struct TileArgs {
 };
 
 // Launchers (defined in the .hip translation units).
-hipError_t launch_ising_sweeps(int dim, int alg, bool flat, const SweepArgs& a, int block,
+// layout: 0 row planes, 1 flat planes, 2 either per item (ising_flat(L))
+hipError_t launch_ising_sweeps(int dim, int alg, int layout, const SweepArgs& a, int block,
                                int grid, size_t lds_bytes, hipStream_t st);
 hipError_t launch_ising_plain(int dim, int alg, const PlainArgs& a, int n_sites_colour,
                               hipStream_t st);

@@ -18,7 +18,11 @@
 // p = (c + y + z) & 1, so one 32-bit VALU op updates 32 sites.  Each site's
 // uniform is bit-sliced: bit b (MSB first) of its 32-bit word w is bit j of
 // word (b & 3) of Philox block (group, sweep, chunk = b >> 2, purpose), group =
-// (2r + c)*nw + q.  u < p  <=>  w < ceil(p * 2^32) is decided bit-serially
+// (2r + c)*nw + q.  Lattices with L < 64 and L % 8 == 0 use a flat layout
+// instead (ising_flat): each colour's sites in (z, y, x) order, 32 to a word,
+// rows split across words, group 2*word + c -- no partial words, so a 3D L=48
+// lattice takes 1728 words per colour rather than 2304 of 24 sites.
+// u < p  <=>  w < ceil(p * 2^32) is decided bit-serially
 // MSB first, and a wavefront draws the next Philox block only while one of its
 // sites is still tied with its threshold -- usually three blocks for 32 sites.
 //
@@ -413,9 +417,11 @@ __device__ __forceinline__ uint32_t ising_word(uint32_t s0, const NbrWords& n, u
     }
 }
 
-// FLAT: every item of the launch has the flat layout (ising_flat(L)), else
-// none has (the two word loops are separate instantiations).
-template <int DIM, int ALG, bool FLAT>
+// LAYOUT 0: no item of the launch has the flat layout (ising_flat(L)); 1:
+// every item has it; 2: either, per item (the job server's mixed classes).
+// The word loops of 0 and 1 are separate instantiations: a loop for both
+// costs the row loop registers and ~3 % of its rate.
+template <int DIM, int ALG, int LAYOUT>
 __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
 {
     extern __shared__ uint32_t lds[];
@@ -442,7 +448,7 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
         const int half = L >> 1;
         const int nw = (half + 31) >> 5;
         const int rows = DIM == 3 ? L * L : L;
-        constexpr bool flat = FLAT;
+        const bool flat = LAYOUT == 2 ? ising_flat(L) : LAYOUT == 1;
         const int words = (int)ising_plane_words(DIM, L); // per colour
         const int64_t N = (int64_t)rows * L;
         int8_t* __restrict__ sp = a.spins + cr.spin_off;
@@ -982,25 +988,23 @@ hipError_t launch_ising_pack(int dim, int unpack, const TileArgs& a, int max_wor
     return hipGetLastError();
 }
 
-hipError_t launch_ising_sweeps(int dim, int alg, bool flat, const SweepArgs& a, int block,
+hipError_t launch_ising_sweeps(int dim, int alg, int layout, const SweepArgs& a, int block,
                                int grid, size_t lds_bytes, hipStream_t st)
 {
     const dim3 g(grid), bl(block);
-    const int k = ising_alg(alg) + (dim == 3 ? 3 : 0) + (flat ? 6 : 0);
+    const int k = ising_alg(alg) + (dim == 3 ? 3 : 0) + 6 * layout;
+#define MCX_IS(K, D, A, Y)                                                                  \
+    case K: hipLaunchKernelGGL((k_ising_sweeps<D, A, Y>), g, bl, lds_bytes, st, a); break;
     switch (k) {
-    case 0: hipLaunchKernelGGL((k_ising_sweeps<2, 0, false>), g, bl, lds_bytes, st, a); break;
-    case 1: hipLaunchKernelGGL((k_ising_sweeps<2, 1, false>), g, bl, lds_bytes, st, a); break;
-    case 2: hipLaunchKernelGGL((k_ising_sweeps<2, 2, false>), g, bl, lds_bytes, st, a); break;
-    case 3: hipLaunchKernelGGL((k_ising_sweeps<3, 0, false>), g, bl, lds_bytes, st, a); break;
-    case 4: hipLaunchKernelGGL((k_ising_sweeps<3, 1, false>), g, bl, lds_bytes, st, a); break;
-    case 5: hipLaunchKernelGGL((k_ising_sweeps<3, 2, false>), g, bl, lds_bytes, st, a); break;
-    case 6: hipLaunchKernelGGL((k_ising_sweeps<2, 0, true>), g, bl, lds_bytes, st, a); break;
-    case 7: hipLaunchKernelGGL((k_ising_sweeps<2, 1, true>), g, bl, lds_bytes, st, a); break;
-    case 8: hipLaunchKernelGGL((k_ising_sweeps<2, 2, true>), g, bl, lds_bytes, st, a); break;
-    case 9: hipLaunchKernelGGL((k_ising_sweeps<3, 0, true>), g, bl, lds_bytes, st, a); break;
-    case 10: hipLaunchKernelGGL((k_ising_sweeps<3, 1, true>), g, bl, lds_bytes, st, a); break;
-    default: hipLaunchKernelGGL((k_ising_sweeps<3, 2, true>), g, bl, lds_bytes, st, a); break;
+    MCX_IS(0, 2, 0, 0) MCX_IS(1, 2, 1, 0) MCX_IS(2, 2, 2, 0)
+    MCX_IS(3, 3, 0, 0) MCX_IS(4, 3, 1, 0) MCX_IS(5, 3, 2, 0)
+    MCX_IS(6, 2, 0, 1) MCX_IS(7, 2, 1, 1) MCX_IS(8, 2, 2, 1)
+    MCX_IS(9, 3, 0, 1) MCX_IS(10, 3, 1, 1) MCX_IS(11, 3, 2, 1)
+    MCX_IS(12, 2, 0, 2) MCX_IS(13, 2, 1, 2) MCX_IS(14, 2, 2, 2)
+    MCX_IS(15, 3, 0, 2) MCX_IS(16, 3, 1, 2)
+    default: hipLaunchKernelGGL((k_ising_sweeps<3, 2, 2>), g, bl, lds_bytes, st, a); break;
     }
+#undef MCX_IS
     return hipGetLastError();
 }
 

@@ -169,11 +169,10 @@ private:
 
 struct ClassKey {
     int model, dim, alg, block, lpw;
-    bool flat; // Ising plane layout (ising_flat(L))
     bool operator<(const ClassKey& o) const
     {
-        return std::tie(model, dim, alg, block, lpw, flat) <
-               std::tie(o.model, o.dim, o.alg, o.block, o.lpw, o.flat);
+        return std::tie(model, dim, alg, block, lpw) <
+               std::tie(o.model, o.dim, o.alg, o.block, o.lpw);
     }
     bool operator==(const ClassKey& o) const { return !(*this < o) && !(o < *this); }
 };
@@ -188,8 +187,7 @@ using ShapeKey = std::tuple<int, int, int, int>; // model, dim, alg, L
 ClassKey class_of(const Job& j)
 {
     return ClassKey{j.d.model, j.d.dim, j.d.alg, j.block <= 256 ? 256 : 1024,
-                    j.d.model == MCX_ISING ? 1 : -1,
-                    j.d.model == MCX_ISING && ising_flat(j.d.L)};
+                    j.d.model == MCX_ISING ? 1 : -1};
 }
 
 } // namespace
@@ -629,8 +627,13 @@ hipError_t run_quantum(mcx_server* s, std::vector<int64_t>& ran, std::vector<int
     for (size_t c = 0; c < cl.size() && e == hipSuccess; ++c) {
         const ClassKey& k = cl[c].first;
         size_t lds = 0;
-        for (int i : cl[c].second)
+        int n_flat = 0;
+        for (int i : cl[c].second) {
             lds = std::max(lds, lattice_lds_bytes(k.model, k.dim, s->active[i].d.L));
+            n_flat += k.model == MCX_ISING && ising_flat(s->active[i].d.L) ? 1 : 0;
+        }
+        // Ising plane layout of the launch: row, flat, or per item when mixed
+        const int layout = n_flat == 0 ? 0 : (n_flat == (int)cl[c].second.size() ? 1 : 2);
         SweepArgs a{};
         a.items = s->d_items + span[c].first;
         a.n_items = span[c].second;
@@ -649,7 +652,7 @@ hipError_t run_quantum(mcx_server* s, std::vector<int64_t>& ran, std::vector<int
         a.lanes_per_word = k.lpw;
         hipStream_t st = s->streams[unit_stream(unit0 + (int)c)];
         e = k.model == MCX_ISING
-                ? launch_ising_sweeps(k.dim, k.alg, k.flat, a, k.block, a.n_items, lds, st)
+                ? launch_ising_sweeps(k.dim, k.alg, layout, a, k.block, a.n_items, lds, st)
                 : launch_bc_sweeps(k.dim, k.alg, a, k.block, a.n_items, lds, st);
     }
     for (int k = 1; k < ns && e == hipSuccess; ++k) {

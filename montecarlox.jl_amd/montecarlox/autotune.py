@@ -79,6 +79,8 @@ def candidates(model, dim: int, Ls) -> List[dict]:
     Lmax = int(np.max(Ls))
     nw = (Lmax // 2 + 31) // 32
     words = (Lmax * Lmax if dim == 3 else Lmax) * nw
+    if _model_name(model) == "ising" and Lmax < 64 and Lmax % 8 == 0:
+        words = Lmax ** dim // 64  # flat planes: whole 32-site words
     base = min(256, max(64, (words + 63) // 64 * 64))
     threads = sorted({t for t in (64, 128, 256, 512, 1024) if base // 2 <= t <= 4 * base})
     out = []
