@@ -10,10 +10,10 @@ constexpr int kGenThreads = 128;
 
 // static LDS of the walker kernel: two chunks' stages (ints dE[65], dp, mq,
 // site, snew[64]; doubles luh[65], lul[65], u[64]), two chunks' walk results
-// (64 ints and a count each), two error slots and 64 bytes for the
-// compiler's alignment of the arrays
+// (64 ints and a count each), 64 pre-decrement weights, two error slots and
+// 64 bytes for the compiler's alignment of the arrays
 constexpr size_t kGenStaticLds =
-    2 * (4 * (65 + 4 * 64) + 8 * (65 + 65 + 64)) + 2 * (4 * 64 + 4) + 8 + 64;
+    2 * (4 * (65 + 4 * 64) + 8 * (65 + 65 + 64)) + 2 * (4 * 64 + 4) + 8 * 64 + 8 + 64;
 
 struct GenArgs {
     const int32_t* items;   // walkers of this launch (all from one binning group)

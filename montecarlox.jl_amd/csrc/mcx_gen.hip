@@ -92,6 +92,9 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
     // the walk's outcomes per attempt, and how many attempts it ran, per stage
     __shared__ int st_res[2][64], sh_nd[2];
     __shared__ double st_luh[2][65], st_lul[2][65], st_u[2][64];
+    // the linear Wang-Landau walk's weights of the visited bins before their
+    // decrements, per attempt of the pass (to undo a pass after a band attempt)
+    __shared__ double st_old[64];
     // the walk's error site, one slot per barrier parity (wave 0 writes the slot
     // of the barrier it is about to enter; both waves read it after that barrier)
     __shared__ int sh_err[2];
@@ -434,7 +437,14 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                             Ea = Es;
                             lw_cur = wt(Es);
                         } else {
-                            bool alive = true;
+                            // Wang-Landau: the pass likewise runs every attempt,
+                            // one in the band taken as a rejection, and keeps
+                            // each visited bin's weight from before its decrement;
+                            // after a band attempt the pass is undone (weights
+                            // restored last first) and replayed exactly up to it
+                            const int Ea0 = Ea;
+                            const double lw0 = lw_cur;
+                            bool band = false;
 #pragma unroll 16
                             for (int k = kk; k < cnt; ++k) {
                                 const int dB_n = st_dE[buf][k + 1];
@@ -445,20 +455,39 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                                 const double lw_new = En == Ea ? lw_cur : lw_pre;
                                 const double lr = lw_new - lw_cur;
                                 const bool acc_c = lr > luh_c;
-                                const bool go = alive && (acc_c || lr < lul_c);
-                                const bool ok = go && acc_c;
-                                stop = alive && !go ? k : stop;
-                                alive = go;
-                                Ea = ok ? En : Ea;
-                                lw_cur = ok ? lw_new : lw_cur;
-                                st_res[buf][k] = Ea | (ok ? (int)0x80000000 : 0);
+                                band = band | !(acc_c | (lr < lul_c));
+                                Ea = acc_c ? En : Ea;
+                                lw_cur = acc_c ? lw_new : lw_cur;
+                                st_res[buf][k] = Ea | (acc_c ? (int)0x80000000 : 0);
                                 lw_pre = wt(Ea + dB_n);
-                                // unchanged value when the attempt did not run
-                                lw_cur = go ? lw_cur - logf : lw_cur;
+                                st_old[k] = lw_cur;
+                                lw_cur = lw_cur - logf;
                                 *reinterpret_cast<double*>(lwbw + Ea) = lw_cur;
                                 dB_c = dB_n;
                                 luh_c = luh_n;
                                 lul_c = lul_n;
+                            }
+                            if (!band)
+                                break;
+                            for (int k = cnt - 1; k >= kk; --k)
+                                *reinterpret_cast<double*>(lwbw + (st_res[buf][k] & 0x7fffffff)) =
+                                    st_old[k];
+                            Ea = Ea0;
+                            lw_cur = lw0;
+                            for (stop = kk; stop < cnt; ++stop) {
+                                const int En = Ea + st_dE[buf][stop];
+                                const double lw_new = En == Ea ? lw_cur : wt(En);
+                                const double lr = lw_new - lw_cur;
+                                const bool acc_c = lr > st_luh[buf][stop];
+                                if (!acc_c && !(lr < st_lul[buf][stop]))
+                                    break;
+                                if (acc_c) {
+                                    Ea = En;
+                                    lw_cur = lw_new;
+                                }
+                                st_res[buf][stop] = Ea | (acc_c ? (int)0x80000000 : 0);
+                                lw_cur = lw_cur - logf;
+                                *reinterpret_cast<double*>(lwbw + Ea) = lw_cur;
                             }
                         }
                         if (stop == cnt)
