@@ -396,19 +396,19 @@ extern "C" mcx_status mcx_batch_create(mcx_ctx* ctx, const mcx_batch_desc* d, mc
     std::vector<int32_t> order(b->n);
     for (int i = 0; i < b->n; ++i)
         order[i] = i;
-    auto flat_of = [&](int i) { return b->model == MCX_ISING && ising_flat(b->rec[i].L); };
+    auto flat_of = [&](int i) { return b->model == MCX_ISING ? ising_layout(b->rec[i].L) : 0; };
     std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
         const int ba = lattice_block(b->model, b->dim, b->rec[a].L);
         const int bc = lattice_block(b->model, b->dim, b->rec[c].L);
         if (ba != bc)
             return ba > bc;
         if (flat_of(a) != flat_of(c))
-            return flat_of(c);
+            return flat_of(a) < flat_of(c);
         return b->nsites[a] > b->nsites[c];
     });
     for (int k = 0; k < b->n;) {
         const int blk = lattice_block(b->model, b->dim, b->rec[order[k]].L);
-        const bool fl = flat_of(order[k]);
+        const int fl = flat_of(order[k]);
         mcx_batch::Launch ln{k, 0, blk, 0, fl};
         while (k < b->n && lattice_block(b->model, b->dim, b->rec[order[k]].L) == blk &&
                flat_of(order[k]) == fl) {
@@ -726,7 +726,7 @@ mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t
             const int cpg = sh.chains_per_group > 1 ? sh.chains_per_group : 1;
             const int grid = (ln.count + cpg - 1) / cpg;
             if (b->model == MCX_ISING)
-                HIPCHK(launch_ising_sweeps(b->dim, alg, ln.flat ? 1 : 0, a, block, grid, ln.lds, st));
+                HIPCHK(launch_ising_sweeps(b->dim, alg, ln.layout, a, block, grid, ln.lds, st));
             else
                 HIPCHK(launch_bc_sweeps(b->dim, alg, a, block, grid, ln.lds, st));
         }

@@ -34,7 +34,7 @@ struct mcx_batch {
     struct Launch {
         int first, count, block;
         size_t lds;
-        bool flat; // Ising: the chains' planes have the flat layout
+        int layout; // Ising: the chains' plane layout (ising_layout)
     };
     mcx_ctx* ctx = nullptr;
     int model = 0, dim = 2, n = 0;

@@ -69,6 +69,12 @@ __host__ __device__ constexpr bool ising_flat(int L)
     return L < 64 && (L & 7) == 0;
 }
 
+// Launch layout of an Ising lattice alone (see launch_ising_sweeps).
+__host__ __device__ constexpr int ising_layout(int L)
+{
+    return ising_flat(L) ? 1 : (L == 64 ? 3 : 0);
+}
+
 // Ising bit-plane words per colour: L^dim / 64 flat, else rows * words per row.
 __host__ __device__ constexpr int64_t ising_plane_words(int dim, int L)
 {
@@ -165,7 +171,8 @@ struct TileArgs {
 };
 
 // Launchers (defined in the .hip translation units).
-// layout: 0 row planes, 1 flat planes, 2 either per item (ising_flat(L))
+// layout: 0 row planes, 1 flat planes, 2 either per item (ising_flat(L)),
+// 3 L = 64 (one row per word)
 hipError_t launch_ising_sweeps(int dim, int alg, int layout, const SweepArgs& a, int block,
                                int grid, size_t lds_bytes, hipStream_t st);
 hipError_t launch_ising_plain(int dim, int alg, const PlainArgs& a, int n_sites_colour,

@@ -282,10 +282,32 @@ __device__ __forceinline__ FlatGeo flat_geo(int dim, int L)
 }
 
 // Neighbour words of flat word w of colour c (in the other colour's plane Po).
-template <int DIM>
+// ROW32: L = 64, every word exactly one row (the flat and row layouts coincide).
+template <int DIM, bool ROW32 = false>
 __device__ __forceinline__ void load_nbr_flat(const uint32_t* __restrict__ Po, const FlatGeo& g,
                                               int L, int c, int w, NbrWords& n)
 {
+    if (ROW32) {
+        // word w is row w: y-neighbours the adjacent words of the plane, the
+        // far x-neighbour the other colour's word rotated by one bit
+        const int zi = DIM == 3 ? w >> 6 : 0;
+        const int pb = zi << 6;
+        const int wl = w - pb;
+        const uint32_t a0 = Po[w];
+        n.A = a0;
+        n.U = Po[pb + ((wl - 1) & 63)];
+        n.Dn = Po[pb + ((wl + 1) & 63)];
+        if (DIM == 3) {
+            n.Zu = Po[(w - 64) & (64 * 64 - 1)];
+            n.Zd = Po[(w + 64) & (64 * 64 - 1)];
+        } else {
+            n.Zu = 0;
+            n.Zd = 0;
+        }
+        const uint32_t sr = ((c + wl + zi) & 1) ? 1u : 31u;
+        n.Bw = __builtin_amdgcn_alignbit(a0, a0, sr);
+        return;
+    }
     const int zi = DIM == 3 ? (int)(__umul24((uint32_t)w, g.mplane) >> 20) : 0;
     const int pb = DIM == 3 ? (int)__umul24((uint32_t)zi, (uint32_t)g.pw) : 0;
     const int wl = w - pb;
@@ -418,9 +440,11 @@ __device__ __forceinline__ uint32_t ising_word(uint32_t s0, const NbrWords& n, u
 }
 
 // LAYOUT 0: no item of the launch has the flat layout (ising_flat(L)); 1:
-// every item has it; 2: either, per item (the job server's mixed classes).
-// The word loops of 0 and 1 are separate instantiations: a loop for both
-// costs the row loop registers and ~3 % of its rate.
+// every item has it; 2: either, per item (the job server's mixed classes);
+// 3: every item has L = 64, one row per word, run by the flat word loop with
+// the row neighbours (fewer registers than the general row loop).  The word
+// loops are separate instantiations: a loop for both layouts costs the row
+// loop registers and ~3 % of its rate.
 template <int DIM, int ALG, int LAYOUT>
 __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
 {
@@ -448,7 +472,7 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
         const int half = L >> 1;
         const int nw = (half + 31) >> 5;
         const int rows = DIM == 3 ? L * L : L;
-        const bool flat = LAYOUT == 2 ? ising_flat(L) : LAYOUT == 1;
+        const bool flat = LAYOUT == 2 ? ising_flat(L) : (LAYOUT == 1 || LAYOUT == 3);
         const int words = (int)ising_plane_words(DIM, L); // per colour
         const int64_t N = (int64_t)rows * L;
         int8_t* __restrict__ sp = a.spins + cr.spin_off;
@@ -459,7 +483,7 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
             const uint32_t* src = a.planes + a.plane_off[ch];
             for (int w = tid; w < 2 * words; w += B)
                 P[w] = src[w];
-        } else if (flat) {
+        } else if (flat && LAYOUT != 3) {
             for (int w = tid; w < 2 * words; w += B)
                 ising_pack_flat<DIM>(sp, P, w, L, half, words);
         } else {
@@ -503,7 +527,7 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                             continue;
                         const uint32_t s0 = Pc[w];
                         NbrWords n;
-                        load_nbr_flat<DIM>(Po, g, L, c, w, n);
+                        load_nbr_flat<DIM, LAYOUT == 3>(Po, g, L, c, w, n);
                         const uint32_t snew = ising_word<DIM, ALG>(
                             s0, n, 0xffffffffu, tb, 2u * (uint32_t)w + (uint32_t)c, sw_lo, w2, c3,
                             k0, cr.key);
@@ -599,7 +623,7 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
             uint32_t* dst = a.planes + a.plane_off[ch];
             for (int w = tid; w < 2 * words; w += B)
                 dst[w] = P[w];
-        } else if (flat) {
+        } else if (flat && LAYOUT != 3) {
             for (int w = tid; w < 2 * words; w += B)
                 ising_unpack_flat<DIM>(sp, P, w, L, half, words);
         } else {
@@ -1001,8 +1025,10 @@ hipError_t launch_ising_sweeps(int dim, int alg, int layout, const SweepArgs& a,
     MCX_IS(6, 2, 0, 1) MCX_IS(7, 2, 1, 1) MCX_IS(8, 2, 2, 1)
     MCX_IS(9, 3, 0, 1) MCX_IS(10, 3, 1, 1) MCX_IS(11, 3, 2, 1)
     MCX_IS(12, 2, 0, 2) MCX_IS(13, 2, 1, 2) MCX_IS(14, 2, 2, 2)
-    MCX_IS(15, 3, 0, 2) MCX_IS(16, 3, 1, 2)
-    default: hipLaunchKernelGGL((k_ising_sweeps<3, 2, 2>), g, bl, lds_bytes, st, a); break;
+    MCX_IS(15, 3, 0, 2) MCX_IS(16, 3, 1, 2) MCX_IS(17, 3, 2, 2)
+    MCX_IS(18, 2, 0, 3) MCX_IS(19, 2, 1, 3) MCX_IS(20, 2, 2, 3)
+    MCX_IS(21, 3, 0, 3) MCX_IS(22, 3, 1, 3)
+    default: hipLaunchKernelGGL((k_ising_sweeps<3, 2, 3>), g, bl, lds_bytes, st, a); break;
     }
 #undef MCX_IS
     return hipGetLastError();

@@ -627,13 +627,16 @@ hipError_t run_quantum(mcx_server* s, std::vector<int64_t>& ran, std::vector<int
     for (size_t c = 0; c < cl.size() && e == hipSuccess; ++c) {
         const ClassKey& k = cl[c].first;
         size_t lds = 0;
-        int n_flat = 0;
+        int n_flat = 0, n_64 = 0;
         for (int i : cl[c].second) {
             lds = std::max(lds, lattice_lds_bytes(k.model, k.dim, s->active[i].d.L));
             n_flat += k.model == MCX_ISING && ising_flat(s->active[i].d.L) ? 1 : 0;
+            n_64 += s->active[i].d.L == 64 ? 1 : 0;
         }
-        // Ising plane layout of the launch: row, flat, or per item when mixed
-        const int layout = n_flat == 0 ? 0 : (n_flat == (int)cl[c].second.size() ? 1 : 2);
+        // Ising plane layout of the launch: row, flat, L = 64 only, or per item
+        // when flat and row lattices mix
+        const int nc = (int)cl[c].second.size();
+        const int layout = n_flat == 0 ? (n_64 == nc ? 3 : 0) : (n_flat == nc ? 1 : 2);
         SweepArgs a{};
         a.items = s->d_items + span[c].first;
         a.n_items = span[c].second;
