@@ -96,20 +96,45 @@ __device__ __forceinline__ void load_nbr_words(const uint32_t* __restrict__ Po, 
 
 // Bit-serial "w < thr[class]" for the 32 sites of one word.  eq: sites still
 // undecided (tied so far); m[k]: sites of class k.  Returns the "less" mask.
-template <int NC>
+template <int NC, bool DRAW3>
 __device__ __forceinline__ uint32_t bitserial_less(uint32_t eq, const uint32_t (&m)[NC],
                                                    const uint32_t (&thr)[NC], uint32_t group,
                                                    uint32_t sw_lo, uint32_t w2, uint32_t c3,
                                                    uint32_t k0, uint32_t k1)
 {
     uint32_t lt = 0;
-    for (uint32_t chunk = 0; chunk < 8; ++chunk) {
-        if (!__any(eq != 0u))
-            break;
-        // (the chunk's counter word is wave-uniform: its Philox products with
-        // the uniform words go to the scalar unit)
-        const uint32_t cw = __builtin_amdgcn_readfirstlane(w2 | (chunk << 16));
-        const uint4 R = philox4x32_10(group, sw_lo, cw, c3, k0, k1);
+    if (!DRAW3) {
+        for (uint32_t chunk = 0; chunk < 8; ++chunk) {
+            if (!__any(eq != 0u))
+                break;
+            // (the chunk's counter word is wave-uniform: its Philox products
+            // with the uniform words go to the scalar unit)
+            const uint32_t cw = __builtin_amdgcn_readfirstlane(w2 | (chunk << 16));
+            const uint4 R = philox4x32_10(group, sw_lo, cw, c3, k0, k1);
+            const uint32_t Rb[4] = {R.x, R.y, R.z, R.w};
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint32_t sh = 31u - (4u * chunk + (uint32_t)b);
+                uint32_t Tb = 0;
+#pragma unroll
+                for (int k = 0; k < NC; ++k) {
+                    // all-ones where the class's threshold has bit sh (wave-
+                    // uniform); kept opaque so that the planes combine with
+                    // and / and-or ops rather than one select per class
+                    uint32_t tm = 0u - ((thr[k] >> sh) & 1u);
+                    asm volatile("" : "+s"(tm));
+                    Tb = k == 0 ? (m[k] & tm) : __builtin_amdgcn_bitop3_b32(m[k], tm, Tb, 0xEA);
+                }
+                // less where tied, the threshold bit is 1 and the random bit
+                // 0; still tied where the two bits agree
+                lt |= __builtin_amdgcn_bitop3_b32(eq, Tb, Rb[b], 0x40);
+                eq = __builtin_amdgcn_bitop3_b32(eq, Rb[b], Tb, 0x90);
+            }
+        }
+        return lt;
+    }
+    // the four bit-planes of chunk `chunk` (block R)
+    auto planes = [&](const uint4& R, uint32_t chunk) {
         const uint32_t Rb[4] = {R.x, R.y, R.z, R.w};
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
@@ -117,18 +142,38 @@ __device__ __forceinline__ uint32_t bitserial_less(uint32_t eq, const uint32_t (
             uint32_t Tb = 0;
 #pragma unroll
             for (int k = 0; k < NC; ++k) {
-                // all-ones where the class's threshold has bit sh (wave-uniform);
-                // kept opaque so that the planes combine with and / and-or ops
-                // rather than one select per class
                 uint32_t tm = 0u - ((thr[k] >> sh) & 1u);
                 asm volatile("" : "+s"(tm));
                 Tb = k == 0 ? (m[k] & tm) : __builtin_amdgcn_bitop3_b32(m[k], tm, Tb, 0xEA);
             }
-            // less where tied, the threshold bit is 1 and the random bit 0;
-            // still tied where the two bits agree
             lt |= __builtin_amdgcn_bitop3_b32(eq, Tb, Rb[b], 0x40);
             eq = __builtin_amdgcn_bitop3_b32(eq, Rb[b], Tb, 0x90);
         }
+    };
+    auto block = [&](uint32_t chunk) {
+        const uint32_t cw = __builtin_amdgcn_readfirstlane(w2 | (chunk << 16));
+        return philox4x32_10(group, sw_lo, cw, c3, k0, k1);
+    };
+    // DRAW3: a wavefront's 2048 sites nearly always need three chunks (a site
+    // stays tied past 8 bits with probability ~2^-8), so the first three
+    // blocks are drawn together -- three independent Philox chains the
+    // scheduler interleaves -- and only later chunks wait for the tie test.
+    // It costs registers, which pays where a launch has few waves per SIMD to
+    // hide latency with (one 64-lane wave per L=64 chain: +5.5 %) and not
+    // where occupancy does that (the flat layout's small lattices, or the
+    // many-workgroup tile and band kernels).
+    uint32_t chunk = 0;
+    if (__any(eq != 0u)) {
+        const uint4 R0 = block(0), R1 = block(1), R2 = block(2);
+        planes(R0, 0);
+        planes(R1, 1);
+        planes(R2, 2);
+        chunk = 3;
+    }
+    for (; chunk < 8; ++chunk) {
+        if (!__any(eq != 0u))
+            break;
+        planes(block(chunk), chunk);
     }
     return lt;
 }
@@ -347,7 +392,7 @@ __device__ __forceinline__ void load_nbr_flat(const uint32_t* __restrict__ Po, c
 // One 32-site word of colour c: the new spins of the word from its old spins
 // s0 and its neighbour words (the per-site rule above, bit-sliced).  Shared by
 // the LDS-resident batched kernel and the multi-workgroup tile kernel.
-template <int DIM, int ALG>
+template <int DIM, int ALG, bool DRAW3 = false>
 __device__ __forceinline__ uint32_t ising_word(uint32_t s0, const NbrWords& n, uint32_t valid,
                                                const IsingTable& tb, uint32_t group,
                                                uint32_t sw_lo, uint32_t w2, uint32_t c3,
@@ -377,7 +422,7 @@ __device__ __forceinline__ uint32_t ising_word(uint32_t s0, const NbrWords& n, u
             else if (!((tb.metro_never >> k) & 1u))
                 need |= m[k];
         }
-        const uint32_t lt = bitserial_less<NMC>(need, m, thr, group, sw_lo, w2, c3, k0, k1);
+        const uint32_t lt = bitserial_less<NMC, DRAW3>(need, m, thr, group, sw_lo, w2, c3, k0, k1);
         return s0 ^ ((freem | forced | lt) & valid);
     } else if (ALG == 2) {
         // Glauber: every class k (anti-aligned neighbours, 0..2 dim) has a threshold
@@ -406,7 +451,7 @@ __device__ __forceinline__ uint32_t ising_word(uint32_t s0, const NbrWords& n, u
             else if (!((tb.gl_never >> k) & 1u))
                 need |= m[k];
         }
-        const uint32_t lt = bitserial_less<NHC>(need, m, thr, group, sw_lo, w2, c3, k0, k1);
+        const uint32_t lt = bitserial_less<NHC, DRAW3>(need, m, thr, group, sw_lo, w2, c3, k0, k1);
         return s0 ^ ((forced | lt) & valid);
     } else {
         uint32_t b0, b1, b2;
@@ -434,7 +479,7 @@ __device__ __forceinline__ uint32_t ising_word(uint32_t s0, const NbrWords& n, u
             else if (!((tb.hb_never >> k) & 1u))
                 need |= m[k];
         }
-        const uint32_t lt = bitserial_less<NHC>(need, m, thr, group, sw_lo, w2, c3, k0, k1);
+        const uint32_t lt = bitserial_less<NHC, DRAW3>(need, m, thr, group, sw_lo, w2, c3, k0, k1);
         return (up | lt) & valid;
     }
 }
@@ -528,7 +573,7 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                         const uint32_t s0 = Pc[w];
                         NbrWords n;
                         load_nbr_flat<DIM, LAYOUT == 3>(Po, g, L, c, w, n);
-                        const uint32_t snew = ising_word<DIM, ALG>(
+                        const uint32_t snew = ising_word<DIM, ALG, LAYOUT != 1>(
                             s0, n, 0xffffffffu, tb, 2u * (uint32_t)w + (uint32_t)c, sw_lo, w2, c3,
                             k0, cr.key);
                         acc += (unsigned long long)__popc(snew ^ s0);
@@ -558,7 +603,7 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                     load_nbr_words<DIM, true>(Po, L, nw, half, r, q, y, z, p, W, valid, n);
                     const uint32_t group =
                         __umul24((uint32_t)(r * 2 + c), (uint32_t)nw) + (uint32_t)q;
-                    const uint32_t snew = ising_word<DIM, ALG>(s0, n, valid, tb, group, sw_lo,
+                    const uint32_t snew = ising_word<DIM, ALG, true>(s0, n, valid, tb, group, sw_lo,
                                                                w2, c3, k0, cr.key);
                     const uint32_t flip = snew ^ s0;
                     acc += (unsigned long long)__popc(flip);
