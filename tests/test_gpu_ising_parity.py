@@ -85,6 +85,15 @@ def test_flat_layout_sizes(mcx, orc, dim, L):
     _run_case(mcx, orc, dim, METRO, [L], betas[1:2], 2, plain=True)
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_one_word_rows_sizes(mcx, orc, dim):
+    """L = 64: rows of exactly one word, run by the flat word loop with row
+    neighbours (its own instantiation); every algorithm, 2D and 3D."""
+    betas = [0.3, 0.4406868] if dim == 2 else [0.2216544]
+    for alg in (METRO, HB, GLAUBER):
+        _run_case(mcx, orc, dim, alg, [64] * len(betas), betas, 3 if dim == 2 else 2)
+
+
 def test_glauber_extremes_plain_and_split(mcx, orc):
     # beta = 0: p = 1/2 everywhere; beta = 100: logistic rounds to 0 and 1
     _run_case(mcx, orc, 2, GLAUBER, [16] * 4, [0.0, 1e-12, 5.0, 100.0], 8)
