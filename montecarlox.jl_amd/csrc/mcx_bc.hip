@@ -682,7 +682,9 @@ __device__ __forceinline__ uint32_t bc_word(uint32_t& occ, uint32_t& up, uint32_
     return n_changed;
 }
 
-template <int DIM, int ALG>
+// ONE: one lane per word (lanes_per_word 0 or 1), its own instantiation without
+// the shared-word path's registers.
+template <int DIM, int ALG, bool ONE>
 __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
 {
     extern __shared__ uint32_t lds[];
@@ -720,8 +722,8 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
         // lanes per word: as asked, or (negative) the most, up to 4 (one prefix
         // block each), that keep words x lanes within the workgroup; uniform
         // over the workgroup
-        int lpw = a.lanes_per_word > 1 ? a.lanes_per_word : 1;
-        if (a.lanes_per_word < 0) {
+        int lpw = a.lanes_per_word > 1 && !ONE ? a.lanes_per_word : 1;
+        if (a.lanes_per_word < 0 && !ONE) {
             lpw = 4;
             while (lpw > 1 && words * lpw > B)
                 lpw >>= 1;
@@ -778,7 +780,7 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
                     uint32_t v0, v1, v2, v3;
                     bc_shifted_sum<DIM>(n, v0, v1, v2, v3);
                     const uint32_t group = __umul24((uint32_t)(r * 2 + c), (uint32_t)nw) + (uint32_t)q;
-                    acc += lpw > 1 ? bc_word<DIM, ALG, true>(occ, up, v0, v1, v2, v3, W, tabs,
+                    acc += !ONE && lpw > 1 ? bc_word<DIM, ALG, true>(occ, up, v0, v1, v2, v3, W, tabs,
                                                              group, sw_lo, w2, c3, k0, cr.key,
                                                              lpart, lpw)
                                    : bc_word<DIM, ALG, false>(occ, up, v0, v1, v2, v3, W, tabs,
@@ -1249,14 +1251,25 @@ hipError_t launch_bc_sweeps(int dim, int alg, const SweepArgs& a0, int block, in
     const bool hb = alg == MCX_HEATBATH;
     // lds_bytes is the whole footprint; the class tables are static
     lds_bytes = lds_bytes > (size_t)kBCStaticLds ? lds_bytes - (size_t)kBCStaticLds : 0;
+    const bool one = a.lanes_per_word == 0 || a.lanes_per_word == 1;
+#define MCX_BCS(D, A)                                                                        \
+    do {                                                                                     \
+        if (one)                                                                             \
+            hipLaunchKernelGGL((k_bc_sweeps<D, A, true>), dim3(grid), dim3(block), lds_bytes, \
+                               st, a);                                                       \
+        else                                                                                 \
+            hipLaunchKernelGGL((k_bc_sweeps<D, A, false>), dim3(grid), dim3(block), lds_bytes, \
+                               st, a);                                                       \
+    } while (0)
     if (dim == 2 && !hb)
-        hipLaunchKernelGGL((k_bc_sweeps<2, 0>), dim3(grid), dim3(block), lds_bytes, st, a);
+        MCX_BCS(2, 0);
     else if (dim == 2)
-        hipLaunchKernelGGL((k_bc_sweeps<2, 1>), dim3(grid), dim3(block), lds_bytes, st, a);
+        MCX_BCS(2, 1);
     else if (!hb)
-        hipLaunchKernelGGL((k_bc_sweeps<3, 0>), dim3(grid), dim3(block), lds_bytes, st, a);
+        MCX_BCS(3, 0);
     else
-        hipLaunchKernelGGL((k_bc_sweeps<3, 1>), dim3(grid), dim3(block), lds_bytes, st, a);
+        MCX_BCS(3, 1);
+#undef MCX_BCS
     return hipGetLastError();
 }
 
