@@ -704,15 +704,25 @@ __device__ __forceinline__ uint32_t bitserial_less_site(uint32_t thr, uint32_t g
                                                         uint32_t sw_lo, uint32_t w2, uint32_t c3,
                                                         uint32_t k0, uint32_t k1)
 {
-    for (uint32_t chunk = 0; chunk < 8; ++chunk) {
-        const uint4 R = philox4x32_10(group, sw_lo, w2 | (chunk << 16), c3, k0, k1);
-        const uint32_t Rb[4] = {R.x, R.y, R.z, R.w};
-        for (int b = 0; b < 4; ++b) {
+    // the 32 bits of chunks c, c + 1 against thr (MSB first); 2 = still tied
+    auto two = [&](const uint4& A, const uint4& Bk, uint32_t chunk) -> uint32_t {
+        const uint32_t Rb[8] = {A.x, A.y, A.z, A.w, Bk.x, Bk.y, Bk.z, Bk.w};
+        for (int b = 0; b < 8; ++b) {
             const uint32_t bit = (Rb[b] >> t) & 1u;
             const uint32_t tbit = (thr >> (31u - (4u * chunk + (uint32_t)b))) & 1u;
             if (bit != tbit)
                 return tbit; // w < thr iff the first differing bit is 0 in w
         }
+        return 2u;
+    };
+    // two chunks at a time: independent Philox chains the scheduler
+    // interleaves (a wavefront of 64 sites nearly always needs two)
+    for (uint32_t chunk = 0; chunk < 8; chunk += 2) {
+        const uint4 R0 = philox4x32_10(group, sw_lo, w2 | (chunk << 16), c3, k0, k1);
+        const uint4 R1 = philox4x32_10(group, sw_lo, w2 | ((chunk + 1) << 16), c3, k0, k1);
+        const uint32_t r = two(R0, R1, chunk);
+        if (r != 2u)
+            return r;
     }
     return 0u; // w == thr
 }
