@@ -75,6 +75,14 @@ __host__ __device__ constexpr int ising_layout(int L)
     return ising_flat(L) ? 1 : (L == 64 ? 3 : 0);
 }
 
+// Layout argument of a launch of `items` lattices of layout `layout` in
+// workgroups of `block` threads: flat launches with fewer waves than 4 per
+// SIMD of the device (1024 SIMDs) take the three-chunk draw.
+__host__ __device__ constexpr int ising_launch_layout(int layout, int64_t items, int block)
+{
+    return layout == 1 && items * ((block + 63) / 64) < 4096 ? 4 : layout;
+}
+
 // Ising bit-plane words per colour: L^dim / 64 flat, else rows * words per row.
 __host__ __device__ constexpr int64_t ising_plane_words(int dim, int L)
 {
@@ -172,7 +180,8 @@ struct TileArgs {
 
 // Launchers (defined in the .hip translation units).
 // layout: 0 row planes, 1 flat planes, 2 either per item (ising_flat(L)),
-// 3 L = 64 (one row per word)
+// 3 L = 64 (one row per word), 4 flat planes in a small launch
+// (ising_launch_layout)
 hipError_t launch_ising_sweeps(int dim, int alg, int layout, const SweepArgs& a, int block,
                                int grid, size_t lds_bytes, hipStream_t st);
 hipError_t launch_ising_plain(int dim, int alg, const PlainArgs& a, int n_sites_colour,

@@ -487,7 +487,9 @@ __device__ __forceinline__ uint32_t ising_word(uint32_t s0, const NbrWords& n, u
 // LAYOUT 0: no item of the launch has the flat layout (ising_flat(L)); 1:
 // every item has it; 2: either, per item (the job server's mixed classes);
 // 3: every item has L = 64, one row per word, run by the flat word loop with
-// the row neighbours (fewer registers than the general row loop).  The word
+// the row neighbours (fewer registers than the general row loop); 4: as 1,
+// for launches too small to fill the SIMDs, with the three-chunk Philox draw
+// (more registers, more independent work per wave).  The word
 // loops are separate instantiations: a loop for both layouts costs the row
 // loop registers and ~3 % of its rate.
 template <int DIM, int ALG, int LAYOUT>
@@ -517,7 +519,7 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
         const int half = L >> 1;
         const int nw = (half + 31) >> 5;
         const int rows = DIM == 3 ? L * L : L;
-        const bool flat = LAYOUT == 2 ? ising_flat(L) : (LAYOUT == 1 || LAYOUT == 3);
+        const bool flat = LAYOUT == 2 ? ising_flat(L) : (LAYOUT == 1 || LAYOUT == 3 || LAYOUT == 4);
         const int words = (int)ising_plane_words(DIM, L); // per colour
         const int64_t N = (int64_t)rows * L;
         int8_t* __restrict__ sp = a.spins + cr.spin_off;
@@ -1082,8 +1084,10 @@ hipError_t launch_ising_sweeps(int dim, int alg, int layout, const SweepArgs& a,
     MCX_IS(12, 2, 0, 2) MCX_IS(13, 2, 1, 2) MCX_IS(14, 2, 2, 2)
     MCX_IS(15, 3, 0, 2) MCX_IS(16, 3, 1, 2) MCX_IS(17, 3, 2, 2)
     MCX_IS(18, 2, 0, 3) MCX_IS(19, 2, 1, 3) MCX_IS(20, 2, 2, 3)
-    MCX_IS(21, 3, 0, 3) MCX_IS(22, 3, 1, 3)
-    default: hipLaunchKernelGGL((k_ising_sweeps<3, 2, 3>), g, bl, lds_bytes, st, a); break;
+    MCX_IS(21, 3, 0, 3) MCX_IS(22, 3, 1, 3) MCX_IS(23, 3, 2, 3)
+    MCX_IS(24, 2, 0, 4) MCX_IS(25, 2, 1, 4) MCX_IS(26, 2, 2, 4)
+    MCX_IS(27, 3, 0, 4) MCX_IS(28, 3, 1, 4)
+    default: hipLaunchKernelGGL((k_ising_sweeps<3, 2, 4>), g, bl, lds_bytes, st, a); break;
     }
 #undef MCX_IS
     return hipGetLastError();
