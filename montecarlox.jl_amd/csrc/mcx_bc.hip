@@ -805,9 +805,9 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
                     }
                 }
                 if (c == 1 && obs) {
-                    pr = wave_sum(pr);
-                    mm = wave_sum(mm);
-                    qq = wave_sum(qq);
+                    pr = wave_sum32(pr);
+                    mm = wave_sum32(mm);
+                    qq = wave_sum32(qq);
                     if (lane == 0) {
                         part[(s & 1) * 48 + wave * 3] = pr;
                         part[(s & 1) * 48 + wave * 3 + 1] = mm;

@@ -80,6 +80,27 @@ __device__ __forceinline__ T wave_sum(T v)
     return v;
 }
 
+// Sum of a 32-bit integer over the 64 lanes of a wavefront, through DPP
+// lane moves instead of LDS permutes: pairs and quads (quad_perm), halves of
+// a row and rows of 16 (half-row and row mirrors: every lane then holds its
+// row's sum), the last lane of rows 0 and 2 added into rows 1 and 3
+// (row_bcast:15, rows 1 and 3 only), and lanes 31 and 63 read out.  Every
+// lane of the wavefront must be active; the result is wave-uniform.
+__device__ __forceinline__ int32_t wave_sum32(int32_t v)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xf, 0xf, false);  // quad_perm [1,0,3,2]
+    v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xf, 0xf, false);  // quad_perm [2,3,0,1]
+    v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xf, 0xf, false); // row_half_mirror
+    v += __builtin_amdgcn_update_dpp(0, v, 0x140, 0xf, 0xf, false); // row_mirror
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false); // row_bcast:15
+    return __builtin_amdgcn_readlane(v, 31) + __builtin_amdgcn_readlane(v, 63);
+}
+
+__device__ __forceinline__ uint32_t wave_sum32(uint32_t v)
+{
+    return (uint32_t)wave_sum32((int32_t)v);
+}
+
 // Bit-sliced population count of 4 or 6 one-bit-per-site words:
 // count = b0 + 2*b1 + 4*b2 per bit position.
 __device__ __forceinline__ void count4(uint32_t x1, uint32_t x2, uint32_t x3, uint32_t x4,
@@ -128,10 +149,10 @@ __device__ __forceinline__ void tile_block_add_s(const TileArgs& a, int item, in
     __shared__ int32_t red_s[16][4];
     const int wave = threadIdx.x >> 6;
     const int nwaves = (int)(blockDim.x >> 6);
-    f0 = wave_sum(f0);
-    f1 = wave_sum(f1);
-    f2 = wave_sum(f2);
-    f3 = wave_sum(f3);
+    f0 = wave_sum32(f0);
+    f1 = wave_sum32(f1);
+    f2 = wave_sum32(f2);
+    f3 = wave_sum32(f3);
     if ((threadIdx.x & 63) == 0) {
         red_s[wave][0] = f0;
         red_s[wave][1] = f1;
@@ -159,10 +180,10 @@ __device__ __forceinline__ void tile_block_add(const TileArgs& a, int item, int3
 {
     __shared__ int32_t red[4][4];
     const int wave = threadIdx.x >> 6;
-    f0 = wave_sum(f0);
-    f1 = wave_sum(f1);
-    f2 = wave_sum(f2);
-    f3 = wave_sum(f3);
+    f0 = wave_sum32(f0);
+    f1 = wave_sum32(f1);
+    f2 = wave_sum32(f2);
+    f3 = wave_sum32(f3);
     if ((threadIdx.x & 63) == 0) {
         red[wave][0] = f0;
         red[wave][1] = f1;

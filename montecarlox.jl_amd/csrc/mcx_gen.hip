@@ -707,10 +707,10 @@ __global__ void __launch_bounds__(kGenThreads) k_gen_sweeps(GenArgs a)
                 my_vis = (r & 0x7fffffff) >> (lin ? 3 : 0);
                 my_ok = (r >> 31) & 1;
             }
-            const int s_ok = wave_sum(my_ok);
-            pair += wave_sum(my_ok ? st_dp[pb][lane] : 0);
-            mm += wave_sum(my_ok ? (st_mq[pb][lane] & 0xffff) - 8 : 0);
-            qq += wave_sum(my_ok ? (st_mq[pb][lane] >> 16) - 8 : 0);
+            const int s_ok = wave_sum32(my_ok);
+            pair += wave_sum32(my_ok ? st_dp[pb][lane] : 0);
+            mm += wave_sum32(my_ok ? (st_mq[pb][lane] & 0xffff) - 8 : 0);
+            qq += wave_sum32(my_ok ? (st_mq[pb][lane] >> 16) - 8 : 0);
             steps += (unsigned)nd;
             nacc += (unsigned)s_ok;
             if (lane < nd) {

@@ -620,8 +620,8 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                     }
                 }
                 if (c == 1 && obs) {
-                    nu = wave_sum(nu);
-                    na = wave_sum(na);
+                    nu = wave_sum32(nu);
+                    na = wave_sum32(na);
                     if (lane == 0) {
                         part[(s & 1) * 32 + wave * 2] = nu;
                         part[(s & 1) * 32 + wave * 2 + 1] = na;
