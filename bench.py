@@ -193,7 +193,11 @@ def headline(a, mcx, np, torch, out):
             "kernel": "k_ising_sweeps<2,0>", "kernel_ms": round(kern_ms, 3),
             "launch_shape": shape,
             "bytes_per_attempt": BYTES_PER_ATTEMPT,
-            "algorithmic_bytes_per_launch": BYTES_PER_ATTEMPT * attempts_per_step}
+            "algorithmic_bytes_per_launch": BYTES_PER_ATTEMPT * attempts_per_step,
+            "note": "achieved = the algorithmic 2 B/attempt (an int8 site read and written "
+                    "per sweep) over the launch time; the kernel keeps the lattice in LDS "
+                    "for all the sweeps of a launch, so its HBM traffic is the small "
+                    "'traffic' figure and frac can pass 1; the binding limit is 'valu'"}
     if traffic is not None and (n, L, S) == (4096, 64, 1000):
         roof["traffic_frac_of_algorithmic"] = round(traffic / roof["algorithmic_bytes_per_launch"], 5)
     if lane_ops is not None:
