@@ -808,7 +808,8 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
                     pr = wave_sum32(pr);
                     mm = wave_sum32(mm);
                     qq = wave_sum32(qq);
-                    if (lane == 0) {
+                    // (one wave: its sums stay in registers for the tail below)
+                    if (lane == 0 && nwaves > 1) {
                         part[(s & 1) * 48 + wave * 3] = pr;
                         part[(s & 1) * 48 + wave * 3 + 1] = mm;
                         part[(s & 1) * 48 + wave * 3 + 2] = qq;
@@ -817,11 +818,14 @@ __global__ void __launch_bounds__(1024) k_bc_sweeps(SweepArgs a)
                 __syncthreads();
             }
             if (obs && tid == 0) {
-                int64_t tp = 0, tm = 0, tq = 0;
-                for (int k = 0; k < nwaves; ++k) {
-                    tp += part[(s & 1) * 48 + k * 3];
-                    tm += part[(s & 1) * 48 + k * 3 + 1];
-                    tq += part[(s & 1) * 48 + k * 3 + 2];
+                int64_t tp = pr, tm = mm, tq = qq;
+                if (nwaves > 1) {
+                    tp = tm = tq = 0;
+                    for (int k = 0; k < nwaves; ++k) {
+                        tp += part[(s & 1) * 48 + k * 3];
+                        tm += part[(s & 1) * 48 + k * 3 + 1];
+                        tq += part[(s & 1) * 48 + k * 3 + 2];
+                    }
                 }
                 ring[s & 63] = (int32_t)(-tp);
                 ring[64 + (s & 63)] = (int32_t)tm;

@@ -622,7 +622,8 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                 if (c == 1 && obs) {
                     nu = wave_sum32(nu);
                     na = wave_sum32(na);
-                    if (lane == 0) {
+                    // (one wave: its sums stay in registers for the tail below)
+                    if (lane == 0 && nwaves > 1) {
                         part[(s & 1) * 32 + wave * 2] = nu;
                         part[(s & 1) * 32 + wave * 2 + 1] = na;
                     }
@@ -630,10 +631,14 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                 __syncthreads();
             }
             if (obs && tid == 0) {
-                uint32_t tu = 0, ta = 0;
-                for (int k = 0; k < nwaves; ++k) {
-                    tu += part[(s & 1) * 32 + k * 2];
-                    ta += part[(s & 1) * 32 + k * 2 + 1];
+                uint32_t tu = nu, ta = na;
+                if (nwaves > 1) {
+                    tu = 0;
+                    ta = 0;
+                    for (int k = 0; k < nwaves; ++k) {
+                        tu += part[(s & 1) * 32 + k * 2];
+                        ta += part[(s & 1) * 32 + k * 2 + 1];
+                    }
                 }
                 const int32_t E = (int32_t)(2 * (int64_t)ta - (int64_t)DIM * N);
                 const int32_t M = (int32_t)(2 * (int64_t)tu - N);
