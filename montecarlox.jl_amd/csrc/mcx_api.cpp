@@ -725,11 +725,13 @@ mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t
             const int block = sh.threads > 0 ? sh.threads : ln.block;
             const int cpg = sh.chains_per_group > 1 ? sh.chains_per_group : 1;
             const int grid = (ln.count + cpg - 1) / cpg;
-            if (b->model == MCX_ISING)
-                HIPCHK(launch_ising_sweeps(b->dim, alg, ising_launch_layout(ln.layout, ln.count, block),
-                                           a, block, grid, ln.lds, st));
-            else
+            if (b->model == MCX_ISING) {
+                const int lay = ising_launch_layout(ln.layout, ln.count, block);
+                HIPCHK(launch_ising_sweeps(b->dim, alg, lay, a, ising_launch_block(lay, block), grid,
+                                           ln.lds, st));
+            } else {
                 HIPCHK(launch_bc_sweeps(b->dim, alg, a, block, grid, ln.lds, st));
+            }
         }
     }
     for (int ch = 0; ch < b->n; ++ch) {

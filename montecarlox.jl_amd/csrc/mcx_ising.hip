@@ -389,14 +389,73 @@ __device__ __forceinline__ void load_nbr_flat(const uint32_t* __restrict__ Po, c
     n.Bw = __builtin_amdgcn_bitop3_b32(Pm, right, left, 0xCA);
 }
 
+// bitserial_less for a word shared by the four lanes of a quad (lane part p =
+// lane % 4): in each round lane p draws chunk base + p and reduces its four
+// bit-planes, from the sites still tied before the round, to (l, e) = (less
+// within the chunk, still tied after it); the quad then composes the four
+// pairs in chunk order, (l, e) then (l', e') -> (l | e & l', e & e'), with
+// two DPP steps (from lane p + 1, then from lane p + 2; the last lanes read
+// lane 3, which the composition leaves unchanged since l' and e' of one chunk
+// are disjoint), and lane 0's result -- chunks base .. base + 3 -- goes to the
+// whole quad.  The same bits and the same MSB-first comparison as
+// bitserial_less, in a quarter of the dependent Philox work per lane: for
+// launches with too few waves to hide latency.
+template <int CTRL>
+__device__ __forceinline__ uint32_t quad_from(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, false);
+}
+
+template <int NC>
+__device__ __forceinline__ uint32_t bitserial_less_quad(uint32_t eq, const uint32_t (&m)[NC],
+                                                        const uint32_t (&thr)[NC], uint32_t group,
+                                                        uint32_t sw_lo, uint32_t w2, uint32_t c3,
+                                                        uint32_t k0, uint32_t k1, uint32_t part)
+{
+    uint32_t lt = 0;
+#pragma unroll
+    for (uint32_t base = 0; base < 8; base += 4) {
+        if (!__any(eq != 0u))
+            break;
+        const uint32_t chunk = base + part;
+        const uint4 R = philox4x32_10(group, sw_lo, w2 | (chunk << 16), c3, k0, k1);
+        const uint32_t Rb[4] = {R.x, R.y, R.z, R.w};
+        uint32_t l = 0, e = eq;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const uint32_t sh = 31u - (4u * chunk + (uint32_t)b);
+            uint32_t Tb = 0;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                const uint32_t tm = 0u - ((thr[k] >> sh) & 1u);
+                Tb = k == 0 ? (m[k] & tm) : __builtin_amdgcn_bitop3_b32(m[k], tm, Tb, 0xEA);
+            }
+            l |= __builtin_amdgcn_bitop3_b32(e, Tb, Rb[b], 0x40);
+            e = __builtin_amdgcn_bitop3_b32(e, Rb[b], Tb, 0x90);
+        }
+        uint32_t ln = quad_from<0xF9>(l), en = quad_from<0xF9>(e); // quad_perm [1,2,3,3]
+        l |= e & ln;
+        e &= en;
+        ln = quad_from<0xFE>(l); // quad_perm [2,3,3,3]
+        en = quad_from<0xFE>(e);
+        l |= e & ln;
+        e &= en;
+        lt |= quad_from<0x00>(l); // quad_perm [0,0,0,0]
+        eq = quad_from<0x00>(e);
+    }
+    return lt;
+}
+
 // One 32-site word of colour c: the new spins of the word from its old spins
 // s0 and its neighbour words (the per-site rule above, bit-sliced).  Shared by
 // the LDS-resident batched kernel and the multi-workgroup tile kernel.
-template <int DIM, int ALG, bool DRAW3 = false>
+// QUAD: the word is shared by the four lanes of a quad (bitserial_less_quad,
+// lane part `part`); every lane of the quad returns the word's new spins.
+template <int DIM, int ALG, bool DRAW3 = false, bool QUAD = false>
 __device__ __forceinline__ uint32_t ising_word(uint32_t s0, const NbrWords& n, uint32_t valid,
                                                const IsingTable& tb, uint32_t group,
                                                uint32_t sw_lo, uint32_t w2, uint32_t c3,
-                                               uint32_t k0, uint32_t k1)
+                                               uint32_t k0, uint32_t k1, uint32_t part = 0)
 {
     constexpr int NMC = DIM;         // Metropolis uphill classes
     constexpr int NHC = 2 * DIM + 1; // heat-bath classes
@@ -422,7 +481,9 @@ __device__ __forceinline__ uint32_t ising_word(uint32_t s0, const NbrWords& n, u
             else if (!((tb.metro_never >> k) & 1u))
                 need |= m[k];
         }
-        const uint32_t lt = bitserial_less<NMC, DRAW3>(need, m, thr, group, sw_lo, w2, c3, k0, k1);
+        const uint32_t lt =
+            QUAD ? bitserial_less_quad<NMC>(need, m, thr, group, sw_lo, w2, c3, k0, k1, part)
+                 : bitserial_less<NMC, DRAW3>(need, m, thr, group, sw_lo, w2, c3, k0, k1);
         return s0 ^ ((freem | forced | lt) & valid);
     } else if (ALG == 2) {
         // Glauber: every class k (anti-aligned neighbours, 0..2 dim) has a threshold
@@ -451,7 +512,9 @@ __device__ __forceinline__ uint32_t ising_word(uint32_t s0, const NbrWords& n, u
             else if (!((tb.gl_never >> k) & 1u))
                 need |= m[k];
         }
-        const uint32_t lt = bitserial_less<NHC, DRAW3>(need, m, thr, group, sw_lo, w2, c3, k0, k1);
+        const uint32_t lt =
+            QUAD ? bitserial_less_quad<NHC>(need, m, thr, group, sw_lo, w2, c3, k0, k1, part)
+                 : bitserial_less<NHC, DRAW3>(need, m, thr, group, sw_lo, w2, c3, k0, k1);
         return s0 ^ ((forced | lt) & valid);
     } else {
         uint32_t b0, b1, b2;
@@ -479,7 +542,9 @@ __device__ __forceinline__ uint32_t ising_word(uint32_t s0, const NbrWords& n, u
             else if (!((tb.hb_never >> k) & 1u))
                 need |= m[k];
         }
-        const uint32_t lt = bitserial_less<NHC, DRAW3>(need, m, thr, group, sw_lo, w2, c3, k0, k1);
+        const uint32_t lt =
+            QUAD ? bitserial_less_quad<NHC>(need, m, thr, group, sw_lo, w2, c3, k0, k1, part)
+                 : bitserial_less<NHC, DRAW3>(need, m, thr, group, sw_lo, w2, c3, k0, k1);
         return (up | lt) & valid;
     }
 }
@@ -488,8 +553,8 @@ __device__ __forceinline__ uint32_t ising_word(uint32_t s0, const NbrWords& n, u
 // every item has it; 2: either, per item (the job server's mixed classes);
 // 3: every item has L = 64, one row per word, run by the flat word loop with
 // the row neighbours (fewer registers than the general row loop); 4: as 1,
-// for launches too small to fill the SIMDs, with the three-chunk Philox draw
-// (more registers, more independent work per wave).  The word
+// for launches too small to fill the SIMDs: each word shared by a quad of
+// lanes (bitserial_less_quad), launched with four times the threads.  The word
 // loops are separate instantiations: a loop for both layouts costs the row
 // loop registers and ~3 % of its rate.
 template <int DIM, int ALG, int LAYOUT>
@@ -505,6 +570,8 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
     const int wave = tid >> 6;
     const int nwaves = (B + 63) >> 6;
     constexpr uint32_t purpose = ALG == 1 ? P_HB : P_METRO;
+    constexpr int QPW = LAYOUT == 4 ? 4 : 1; // lanes per flat word
+    const uint32_t qpart = (uint32_t)(lane & (QPW - 1));
 
     for (int item = blockIdx.x; item < a.n_items; item += gridDim.x) {
         const int ch = a.items[item];
@@ -568,16 +635,19 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                 // control flow, so its per-chunk counter words stay scalar)
                 if (flat) {
                     const FlatGeo g = flat_geo(DIM, L);
-                    for (int wb = tid - lane; wb < words; wb += B) {
-                        const int w = wb + lane;
+                    // LAYOUT 4: a quad of lanes per word (QPW = 4), part 0 writes it
+                    for (int wb = (tid - lane) / QPW; wb < words; wb += B / QPW) {
+                        const int w = wb + lane / QPW;
                         if (w >= words)
                             continue;
                         const uint32_t s0 = Pc[w];
                         NbrWords n;
                         load_nbr_flat<DIM, LAYOUT == 3>(Po, g, L, c, w, n);
-                        const uint32_t snew = ising_word<DIM, ALG, LAYOUT != 1>(
+                        const uint32_t snew = ising_word<DIM, ALG, LAYOUT == 3, (QPW > 1)>(
                             s0, n, 0xffffffffu, tb, 2u * (uint32_t)w + (uint32_t)c, sw_lo, w2, c3,
-                            k0, cr.key);
+                            k0, cr.key, qpart);
+                        if (QPW > 1 && qpart != 0)
+                            continue;
                         acc += (unsigned long long)__popc(snew ^ s0);
                         Pc[w] = snew;
                         if (c == 1 && obs) {

@@ -656,7 +656,10 @@ hipError_t run_quantum(mcx_server* s, std::vector<int64_t>& ran, std::vector<int
         hipStream_t st = s->streams[unit_stream(unit0 + (int)c)];
         e = k.model == MCX_ISING
                 ? launch_ising_sweeps(k.dim, k.alg, ising_launch_layout(layout, a.n_items, k.block),
-                                      a, k.block, a.n_items, lds, st)
+                                      a, ising_launch_block(ising_launch_layout(layout, a.n_items,
+                                                                                k.block),
+                                                            k.block),
+                                      a.n_items, lds, st)
                 : launch_bc_sweeps(k.dim, k.alg, a, k.block, a.n_items, lds, st);
     }
     for (int k = 1; k < ns && e == hipSuccess; ++k) {
