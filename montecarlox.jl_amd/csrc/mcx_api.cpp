@@ -409,10 +409,12 @@ extern "C" mcx_status mcx_batch_create(mcx_ctx* ctx, const mcx_batch_desc* d, mc
     for (int k = 0; k < b->n;) {
         const int blk = lattice_block(b->model, b->dim, b->rec[order[k]].L);
         const int fl = flat_of(order[k]);
-        mcx_batch::Launch ln{k, 0, blk, 0, fl};
+        mcx_batch::Launch ln{k, 0, blk, 0, fl, 0};
         while (k < b->n && lattice_block(b->model, b->dim, b->rec[order[k]].L) == blk &&
                flat_of(order[k]) == fl) {
             ln.lds = std::max(ln.lds, lattice_lds_bytes(b->model, b->dim, b->rec[order[k]].L));
+            if (b->model == MCX_ISING)
+                ln.words = std::max(ln.words, ising_plane_words(b->dim, b->rec[order[k]].L));
             ++ln.count;
             ++k;
         }
@@ -727,8 +729,9 @@ mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t
             const int grid = (ln.count + cpg - 1) / cpg;
             if (b->model == MCX_ISING) {
                 const int lay = ising_launch_layout(ln.layout, ln.count, block);
-                HIPCHK(launch_ising_sweeps(b->dim, alg, lay, a, ising_launch_block(lay, block), grid,
-                                           ln.lds, st));
+                HIPCHK(launch_ising_sweeps(b->dim, alg, lay, a,
+                                           ising_launch_block(lay, block, ln.words), grid, ln.lds,
+                                           st));
             } else {
                 HIPCHK(launch_bc_sweeps(b->dim, alg, a, block, grid, ln.lds, st));
             }

@@ -34,7 +34,8 @@ struct mcx_batch {
     struct Launch {
         int first, count, block;
         size_t lds;
-        int layout; // Ising: the chains' plane layout (ising_layout)
+        int layout;    // Ising: the chains' plane layout (ising_layout)
+        int64_t words; // Ising: largest plane words per colour of the chains
     };
     mcx_ctx* ctx = nullptr;
     int model = 0, dim = 2, n = 0;

@@ -78,17 +78,19 @@ __host__ __device__ constexpr int ising_layout(int L)
 // Layout argument of a launch of `items` lattices of layout `layout` in
 // workgroups of `block` threads: flat launches with fewer waves than the
 // device has SIMDs (1024) and workgroups of at most 256 threads share each
-// word among a quad of lanes (layout 4), in workgroups of 4 * block threads
-// (ising_launch_block): the same words per pass, a quarter of the dependent
-// work per lane.
+// word among a quad of lanes (layout 4), in workgroups of ising_launch_block
+// threads: a quad per word of the launch's largest lattice (`words` per
+// colour), at most 4 * block -- the same words per pass, a quarter of the
+// dependent work per lane.
 __host__ __device__ constexpr int ising_launch_layout(int layout, int64_t items, int block)
 {
     return layout == 1 && block <= 256 && items * ((block + 63) / 64) < 1024 ? 4 : layout;
 }
 
-__host__ __device__ constexpr int ising_launch_block(int launch_layout, int block)
+__host__ __device__ constexpr int ising_launch_block(int launch_layout, int block, int64_t words)
 {
-    return launch_layout == 4 ? 4 * block : block;
+    return launch_layout != 4 ? block
+         : (4 * words + 63) / 64 * 64 < 4 * block ? (int)((4 * words + 63) / 64 * 64) : 4 * block;
 }
 
 // Ising bit-plane words per colour: L^dim / 64 flat, else rows * words per row.

@@ -589,6 +589,9 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
         const bool flat = LAYOUT == 2 ? ising_flat(L) : (LAYOUT == 1 || LAYOUT == 3 || LAYOUT == 4);
         const int words = (int)ising_plane_words(DIM, L); // per colour
         const int64_t N = (int64_t)rows * L;
+        // (once per item: its divisions and row-mask loop are scalar work that
+        // a one-wave chain would otherwise repeat every half-sweep)
+        const FlatGeo g = flat_geo(DIM, flat ? L : 8);
         int8_t* __restrict__ sp = a.spins + cr.spin_off;
         const IsingTable tb = a.itab[cr.table];
 
@@ -634,7 +637,6 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                 // (wave-uniform trip count: the chunk loop inside sees uniform
                 // control flow, so its per-chunk counter words stay scalar)
                 if (flat) {
-                    const FlatGeo g = flat_geo(DIM, L);
                     // LAYOUT 4: a quad of lanes per word (QPW = 4), part 0 writes it
                     for (int wb = (tid - lane) / QPW; wb < words; wb += B / QPW) {
                         const int w = wb + lane / QPW;

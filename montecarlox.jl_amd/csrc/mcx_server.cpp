@@ -628,8 +628,11 @@ hipError_t run_quantum(mcx_server* s, std::vector<int64_t>& ran, std::vector<int
         const ClassKey& k = cl[c].first;
         size_t lds = 0;
         int n_flat = 0, n_64 = 0;
+        int64_t words = 0; // Ising: largest plane words per colour
         for (int i : cl[c].second) {
             lds = std::max(lds, lattice_lds_bytes(k.model, k.dim, s->active[i].d.L));
+            if (k.model == MCX_ISING)
+                words = std::max(words, ising_plane_words(k.dim, s->active[i].d.L));
             n_flat += k.model == MCX_ISING && ising_flat(s->active[i].d.L) ? 1 : 0;
             n_64 += s->active[i].d.L == 64 ? 1 : 0;
         }
@@ -654,11 +657,9 @@ hipError_t run_quantum(mcx_server* s, std::vector<int64_t>& ran, std::vector<int
         a.item_ticks = s->d_ticks + span[c].first;
         a.lanes_per_word = k.lpw;
         hipStream_t st = s->streams[unit_stream(unit0 + (int)c)];
+        const int lay = ising_launch_layout(layout, a.n_items, k.block);
         e = k.model == MCX_ISING
-                ? launch_ising_sweeps(k.dim, k.alg, ising_launch_layout(layout, a.n_items, k.block),
-                                      a, ising_launch_block(ising_launch_layout(layout, a.n_items,
-                                                                                k.block),
-                                                            k.block),
+                ? launch_ising_sweeps(k.dim, k.alg, lay, a, ising_launch_block(lay, k.block, words),
                                       a.n_items, lds, st)
                 : launch_bc_sweeps(k.dim, k.alg, a, k.block, a.n_items, lds, st);
     }
