@@ -75,21 +75,21 @@ __host__ __device__ constexpr int ising_layout(int L)
     return ising_flat(L) ? 1 : (L == 64 ? 3 : 0);
 }
 
-// Layout argument of a launch of `items` lattices of layout `layout` in
-// workgroups of `block` threads: flat launches with fewer waves than the
-// device has SIMDs (1024) and workgroups of at most 256 threads share each
-// word among a quad of lanes (layout 4), in workgroups of ising_launch_block
-// threads: a quad per word of the launch's largest lattice (`words` per
-// colour), at most 4 * block -- the same words per pass, a quarter of the
-// dependent work per lane.
+// Layout argument of a launch of `items` lattices of layout `layout` (0..3)
+// in workgroups of `block` threads: launches with fewer waves than the device
+// has SIMDs (1024) and workgroups of at most 256 threads share each word among
+// a quad of lanes (layout + 4), in workgroups of ising_launch_block threads: a
+// quad per word of the launch's largest lattice (`words` per colour), at most
+// 4 * block -- the same words per pass, a quarter of the dependent work per
+// lane.
 __host__ __device__ constexpr int ising_launch_layout(int layout, int64_t items, int block)
 {
-    return layout == 1 && block <= 256 && items * ((block + 63) / 64) < 1024 ? 4 : layout;
+    return block <= 256 && items * ((block + 63) / 64) < 1024 ? layout + 4 : layout;
 }
 
 __host__ __device__ constexpr int ising_launch_block(int launch_layout, int block, int64_t words)
 {
-    return launch_layout != 4 ? block
+    return launch_layout < 4 ? block
          : (4 * words + 63) / 64 * 64 < 4 * block ? (int)((4 * words + 63) / 64 * 64) : 4 * block;
 }
 
@@ -190,7 +190,7 @@ struct TileArgs {
 
 // Launchers (defined in the .hip translation units).
 // layout: 0 row planes, 1 flat planes, 2 either per item (ising_flat(L)),
-// 3 L = 64 (one row per word), 4 flat planes in a small launch
+// 3 L = 64 (one row per word); + 4: a small launch, a quad of lanes per word
 // (ising_launch_layout)
 hipError_t launch_ising_sweeps(int dim, int alg, int layout, const SweepArgs& a, int block,
                                int grid, size_t lds_bytes, hipStream_t st);

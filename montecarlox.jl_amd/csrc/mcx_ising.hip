@@ -549,17 +549,19 @@ __device__ __forceinline__ uint32_t ising_word(uint32_t s0, const NbrWords& n, u
     }
 }
 
-// LAYOUT 0: no item of the launch has the flat layout (ising_flat(L)); 1:
-// every item has it; 2: either, per item (the job server's mixed classes);
+// LAYOUT & 3 = 0: no item of the launch has the flat layout (ising_flat(L));
+// 1: every item has it; 2: either, per item (the job server's mixed classes);
 // 3: every item has L = 64, one row per word, run by the flat word loop with
-// the row neighbours (fewer registers than the general row loop); 4: as 1,
-// for launches too small to fill the SIMDs: each word shared by a quad of
-// lanes (bitserial_less_quad), launched with four times the threads.  The word
-// loops are separate instantiations: a loop for both layouts costs the row
-// loop registers and ~3 % of its rate.
+// the row neighbours (fewer registers than the general row loop).  LAYOUT >= 4:
+// the same for launches too small to fill the SIMDs, each word shared by a
+// quad of lanes (bitserial_less_quad), in workgroups of up to four times the
+// threads.  The word loops are separate instantiations: a loop for both
+// layouts costs the row loop registers and ~3 % of its rate.
 template <int DIM, int ALG, int LAYOUT>
 __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
 {
+    constexpr int BASE = LAYOUT & 3;
+    constexpr bool QUAD = LAYOUT >= 4;
     extern __shared__ uint32_t lds[];
     uint32_t* part = lds;                                // [2 parity][16 waves][2]
     int32_t* ring = reinterpret_cast<int32_t*>(lds + 64); // E[64], M[64] of the last sweeps
@@ -570,7 +572,7 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
     const int wave = tid >> 6;
     const int nwaves = (B + 63) >> 6;
     constexpr uint32_t purpose = ALG == 1 ? P_HB : P_METRO;
-    constexpr int QPW = LAYOUT == 4 ? 4 : 1; // lanes per flat word
+    constexpr int QPW = QUAD ? 4 : 1; // lanes per word
     const uint32_t qpart = (uint32_t)(lane & (QPW - 1));
 
     for (int item = blockIdx.x; item < a.n_items; item += gridDim.x) {
@@ -586,7 +588,7 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
         const int half = L >> 1;
         const int nw = (half + 31) >> 5;
         const int rows = DIM == 3 ? L * L : L;
-        const bool flat = LAYOUT == 2 ? ising_flat(L) : (LAYOUT == 1 || LAYOUT == 3 || LAYOUT == 4);
+        const bool flat = BASE == 2 ? ising_flat(L) : (BASE == 1 || BASE == 3);
         const int words = (int)ising_plane_words(DIM, L); // per colour
         const int64_t N = (int64_t)rows * L;
         // (once per item: its divisions and row-mask loop are scalar work that
@@ -600,7 +602,7 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
             const uint32_t* src = a.planes + a.plane_off[ch];
             for (int w = tid; w < 2 * words; w += B)
                 P[w] = src[w];
-        } else if (flat && LAYOUT != 3) {
+        } else if (flat && BASE != 3) {
             for (int w = tid; w < 2 * words; w += B)
                 ising_pack_flat<DIM>(sp, P, w, L, half, words);
         } else {
@@ -637,15 +639,15 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                 // (wave-uniform trip count: the chunk loop inside sees uniform
                 // control flow, so its per-chunk counter words stay scalar)
                 if (flat) {
-                    // LAYOUT 4: a quad of lanes per word (QPW = 4), part 0 writes it
+                    // QUAD: a quad of lanes per word (QPW = 4), part 0 writes it
                     for (int wb = (tid - lane) / QPW; wb < words; wb += B / QPW) {
                         const int w = wb + lane / QPW;
                         if (w >= words)
                             continue;
                         const uint32_t s0 = Pc[w];
                         NbrWords n;
-                        load_nbr_flat<DIM, LAYOUT == 3>(Po, g, L, c, w, n);
-                        const uint32_t snew = ising_word<DIM, ALG, LAYOUT == 3, (QPW > 1)>(
+                        load_nbr_flat<DIM, BASE == 3>(Po, g, L, c, w, n);
+                        const uint32_t snew = ising_word<DIM, ALG, BASE != 1 && !QUAD, QUAD>(
                             s0, n, 0xffffffffu, tb, 2u * (uint32_t)w + (uint32_t)c, sw_lo, w2, c3,
                             k0, cr.key, qpart);
                         if (QPW > 1 && qpart != 0)
@@ -661,8 +663,8 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                         }
                     }
                 } else
-                for (int wb = tid - lane; wb < words; wb += B) {
-                    const int w = wb + lane;
+                for (int wb = (tid - lane) / QPW; wb < words; wb += B / QPW) {
+                    const int w = wb + lane / QPW;
                     if (w >= words)
                         continue;
                     const int r = nw == 1 ? w : w / nw;
@@ -677,8 +679,10 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
                     load_nbr_words<DIM, true>(Po, L, nw, half, r, q, y, z, p, W, valid, n);
                     const uint32_t group =
                         __umul24((uint32_t)(r * 2 + c), (uint32_t)nw) + (uint32_t)q;
-                    const uint32_t snew = ising_word<DIM, ALG, true>(s0, n, valid, tb, group, sw_lo,
-                                                               w2, c3, k0, cr.key);
+                    const uint32_t snew = ising_word<DIM, ALG, !QUAD, QUAD>(
+                        s0, n, valid, tb, group, sw_lo, w2, c3, k0, cr.key, qpart);
+                    if (QUAD && qpart != 0)
+                        continue;
                     const uint32_t flip = snew ^ s0;
                     acc += (unsigned long long)__popc(flip);
                     Pc[w] = snew;
@@ -747,7 +751,7 @@ __global__ void __launch_bounds__(1024) k_ising_sweeps(SweepArgs a)
             uint32_t* dst = a.planes + a.plane_off[ch];
             for (int w = tid; w < 2 * words; w += B)
                 dst[w] = P[w];
-        } else if (flat && LAYOUT != 3) {
+        } else if (flat && BASE != 3) {
             for (int w = tid; w < 2 * words; w += B)
                 ising_unpack_flat<DIM>(sp, P, w, L, half, words);
         } else {
@@ -1163,8 +1167,14 @@ hipError_t launch_ising_sweeps(int dim, int alg, int layout, const SweepArgs& a,
     MCX_IS(18, 2, 0, 3) MCX_IS(19, 2, 1, 3) MCX_IS(20, 2, 2, 3)
     MCX_IS(21, 3, 0, 3) MCX_IS(22, 3, 1, 3) MCX_IS(23, 3, 2, 3)
     MCX_IS(24, 2, 0, 4) MCX_IS(25, 2, 1, 4) MCX_IS(26, 2, 2, 4)
-    MCX_IS(27, 3, 0, 4) MCX_IS(28, 3, 1, 4)
-    default: hipLaunchKernelGGL((k_ising_sweeps<3, 2, 4>), g, bl, lds_bytes, st, a); break;
+    MCX_IS(27, 3, 0, 4) MCX_IS(28, 3, 1, 4) MCX_IS(29, 3, 2, 4)
+    MCX_IS(30, 2, 0, 5) MCX_IS(31, 2, 1, 5) MCX_IS(32, 2, 2, 5)
+    MCX_IS(33, 3, 0, 5) MCX_IS(34, 3, 1, 5) MCX_IS(35, 3, 2, 5)
+    MCX_IS(36, 2, 0, 6) MCX_IS(37, 2, 1, 6) MCX_IS(38, 2, 2, 6)
+    MCX_IS(39, 3, 0, 6) MCX_IS(40, 3, 1, 6) MCX_IS(41, 3, 2, 6)
+    MCX_IS(42, 2, 0, 7) MCX_IS(43, 2, 1, 7) MCX_IS(44, 2, 2, 7)
+    MCX_IS(45, 3, 0, 7) MCX_IS(46, 3, 1, 7)
+    default: hipLaunchKernelGGL((k_ising_sweeps<3, 2, 7>), g, bl, lds_bytes, st, a); break;
     }
 #undef MCX_IS
     return hipGetLastError();
