@@ -94,6 +94,17 @@ def test_one_word_rows_sizes(mcx, orc, dim):
         _run_case(mcx, orc, dim, alg, [64] * len(betas), betas, 3 if dim == 2 else 2)
 
 
+@pytest.mark.parametrize("dim,L,alg", [(2, 10, METRO), (2, 10, HB), (2, 16, GLAUBER),
+                                       (3, 6, METRO)])
+def test_launches_past_the_small_launch_threshold(mcx, orc, dim, L, alg):
+    """1100 one-wave chains: at least as many waves as the device has SIMDs,
+    so the launch runs one lane per word (a smaller one shares each word among
+    a quad of lanes); row planes (L = 10, 3D L = 6) and flat planes (L = 16)."""
+    rng = np.random.default_rng(3)
+    betas = rng.uniform(0.1, 1.0, 1100) if dim == 2 else rng.uniform(0.1, 0.5, 1100)
+    _run_case(mcx, orc, dim, alg, [L] * 1100, betas, 3)
+
+
 def test_glauber_extremes_plain_and_split(mcx, orc):
     # beta = 0: p = 1/2 everywhere; beta = 100: logistic rounds to 0 and 1
     _run_case(mcx, orc, 2, GLAUBER, [16] * 4, [0.0, 1e-12, 5.0, 100.0], 8)

@@ -74,6 +74,27 @@ def test_server_matches_oracle_alone_and_in_mix(mcx, orc):
             _same(alone, r)
 
 
+def test_large_mixed_layout_class(mcx, orc):
+    """1100 jobs in one class, flat (L = 8) and row (L = 10) planes mixed: a
+    launch of at least as many waves as the device has SIMDs, one lane per
+    word (a smaller class shares each word among a quad of lanes)."""
+    # 10^4 sweeps each: the jobs submitted first are still running when the
+    # last arrive, so later quanta launch the whole class at once (earlier,
+    # smaller ones share words among quads: a job crosses both kinds)
+    J = mcx.Job
+    jobs = [J("ising", 8 if k % 2 else 10, 0.2 + 0.0005 * k, 10000, "metropolis", seed=11,
+              key=k) for k in range(1100)]
+    print("serving: one class of", len(jobs), "jobs", flush=True)
+    with mcx.Server(hbm_budget=1 << 24, max_jobs=2048) as srv:
+        res = srv.run(jobs, timeout_ms=120000)
+    assert all(r["status"] == 0 and r["nsweeps"] == 10000 for r in res)
+    for k in list(range(0, 1100, 10)) + [1097, 1098, 1099]:
+        E, M, Q, acc, mom = _oracle(orc, jobs[k])
+        r = res[k]
+        assert (r["E_pair"], r["M"], r["accepted"]) == (E, M, acc), jobs[k]
+        assert np.allclose(r["moments"], mom, rtol=1e-12, atol=0), jobs[k]
+
+
 @pytest.mark.parametrize("band_sweeps", [0, 3, -1])
 def test_tiled_path_equals_lds_path_and_oracle(mcx, orc, band_sweeps):
     """Every lattice on the multi-workgroup path (tile_sites=16: band kernels
