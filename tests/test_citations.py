@@ -10,16 +10,23 @@ import re
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CITE = re.compile(r"([\w/.-]+\.(?:jl|csv))(:\d+(?:-\d+)?(?:,\d+(?:-\d+)?)*)")
-# the build's own inputs written by the driver / survey, not this repository's text
-SKIP = {"SURVEY.md", "VERDICT.md", "ADVICE.md", "BASELINE.md"}
+
+
+def _top_docs():
+    """The repository's own top-level documents: README.md and the documents it
+    links; other top-level markdown files are inputs to the build, not its text."""
+    readme = os.path.join(ROOT, "README.md")
+    linked = set(re.findall(r"([\w-]+\.md)", open(readme).read()))
+    return {"README.md"} | linked
 
 
 def _files():
+    docs = _top_docs()
     for p in glob.glob(os.path.join(ROOT, "**", "*"), recursive=True):
         rel = os.path.relpath(p, ROOT)
         if rel.startswith(("_work", ".git", "gpurun_out")) or not os.path.isfile(p):
             continue
-        if os.path.basename(p) in SKIP:
+        if rel.endswith(".md") and os.sep not in rel and rel not in docs:
             continue
         if p.endswith((".py", ".h", ".hip", ".cpp", ".c", ".md", ".txt")):
             yield rel, p
