@@ -811,7 +811,7 @@ __device__ __forceinline__ uint32_t bitserial_less_site(uint32_t thr, uint32_t g
 }
 
 template <int DIM, int ALG>
-__global__ void __launch_bounds__(256) k_ising_plain(PlainArgs a, int n_col)
+__global__ void __launch_bounds__(1024) k_ising_plain(PlainArgs a, int n_col)
 {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     const ChainRec cr = a.chains[a.chain];
@@ -1182,7 +1182,9 @@ hipError_t launch_ising_sweeps(int dim, int alg, int layout, const SweepArgs& a,
 
 hipError_t launch_ising_plain(int dim, int alg, const PlainArgs& a, int n_col, hipStream_t st)
 {
-    const dim3 g((n_col + 255) / 256), bl(256);
+    // one workgroup for a colour of up to 1024 sites (a single dispatch), else 256-thread ones
+    const int blk = n_col <= 1024 ? (n_col + 63) / 64 * 64 : 256;
+    const dim3 g((n_col + blk - 1) / blk), bl(blk);
     const int k = ising_alg(alg) + (dim == 3 ? 3 : 0);
     switch (k) {
     case 0: hipLaunchKernelGGL((k_ising_plain<2, 0>), g, bl, 0, st, a, n_col); break;
