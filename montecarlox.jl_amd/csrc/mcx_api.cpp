@@ -674,6 +674,8 @@ mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t
                 const ChainRec& r = b->rec[ch];
                 a.chain = ch;
                 a.sweep = r.sweep + (uint64_t)s;
+                a.spin_off = r.spin_off;
+                a.L = r.L;
                 const int n_col = (int)(b->nsites[ch] / 2);
                 for (int c = 0; c < 2; ++c) {
                     a.colour = c;

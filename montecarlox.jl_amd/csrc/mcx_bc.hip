@@ -891,13 +891,13 @@ __global__ void __launch_bounds__(256) k_bc_plain(PlainArgs a, int n_col)
 {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     const ChainRec cr = a.chains[a.chain];
-    const int L = cr.L;
+    const int L = a.L;
     const int half = L >> 1;
     const int nw = (half + 31) >> 5;
     const int c = a.colour;
     constexpr int NS = 4 * DIM + 1;
     int e_part = 0, m_part = 0, q_part = 0;
-    unsigned long long flips = 0;
+    uint32_t flips = 0;
     if (idx < n_col) {
         const int r = idx / half;
         const int j = idx - r * half;
@@ -905,7 +905,7 @@ __global__ void __launch_bounds__(256) k_bc_plain(PlainArgs a, int n_col)
         const int z = DIM == 3 ? r / L : 0;
         const int p = (c + y + z) & 1;
         const int x = 2 * j + p;
-        int8_t* sp = a.spins + cr.spin_off;
+        int8_t* sp = a.spins + a.spin_off;
         const int64_t rowb = (int64_t)L * r;
         const int xl = x == 0 ? L - 1 : x - 1;
         const int xr = x == L - 1 ? 0 : x + 1;
@@ -960,13 +960,13 @@ __global__ void __launch_bounds__(256) k_bc_plain(PlainArgs a, int n_col)
             q_part = s_new * s_new + partner * partner;
         }
     }
-    flips = wave_sum(flips);
+    flips = wave_sum32(flips);
     if ((threadIdx.x & 63) == 0 && flips)
-        atomicAdd(&a.accepted[a.chain], flips);
+        atomicAdd(&a.accepted[a.chain], (unsigned long long)flips);
     if (c == 1 && a.rec_index >= 0) {
-        e_part = wave_sum(e_part);
-        m_part = wave_sum(m_part);
-        q_part = wave_sum(q_part);
+        e_part = wave_sum32(e_part);
+        m_part = wave_sum32(m_part);
+        q_part = wave_sum32(q_part);
         if ((threadIdx.x & 63) == 0) {
             atomicAdd(&a.ser_E[cr.series_off + a.rec_index], e_part);
             atomicAdd(&a.ser_M[cr.series_off + a.rec_index], m_part);

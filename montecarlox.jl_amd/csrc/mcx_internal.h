@@ -152,6 +152,9 @@ struct PlainArgs {
     unsigned long long* accepted;
     int64_t rec_index;  // series index of this sweep (< 0: not recording)
     uint64_t sweep;     // sweep counter value for this sweep
+    int64_t spin_off;   // the chain's lattice in `spins` and its side (the chain
+    int32_t L;          // record's spin_off and L, passed so that the spin loads
+                        // need not wait for the record's)
     int32_t chain;
     int32_t colour;
     int32_t glauber;    // Blume-Capel: Glauber thresholds instead of Metropolis ones
