@@ -1,8 +1,10 @@
 // mcx_pt.hip -- replica-exchange (parallel tempering) rounds on the device.
 //
-// One thread per PT set runs update!(rx::ReplicaExchange{ThreadsBackend}, xs)
+// One workgroup per PT set runs update!(rx::ReplicaExchange{ThreadsBackend}, xs)
 // (src/algorithms/replica_exchange.jl:158-178): pairs (pair_id, pair_id + 1) of
-// ladder indices, starting at 1 on even stages and 2 on odd ones; the shared
+// ladder indices, starting at 1 on even stages and 2 on odd ones -- every
+// other position, so a stage's pairs are disjoint and each thread decides its
+// own pairs, with the same words and the same result as in order; the shared
 // uniform comes from the stream of the replica holding the lower ladder index
 // (rand(algorithm(rx, ri).rng), :168); exchange_log_ratio (:110-113) and
 // _accept_exchange (:115) decide; on acceptance the ensembles (here: the
@@ -15,9 +17,7 @@ namespace mcx {
 
 __global__ void k_pt_exchange(PTArgs a)
 {
-    const int set = blockIdx.x * blockDim.x + threadIdx.x;
-    if (set >= a.n_sets)
-        return;
+    const int set = blockIdx.x;
     const int R = a.R;
     int32_t* idx = a.idx + (int64_t)set * R;
     int32_t* pos = a.pos + (int64_t)set * R;
@@ -27,7 +27,7 @@ __global__ void k_pt_exchange(PTArgs a)
     const int stage = a.stage[set];
     const uint64_t round = a.round[set];
     const int first = (stage % 2 == 0) ? 1 : 2;
-    for (int pid = first; pid <= R - 1; pid += 2) {
+    for (int pid = first + 2 * (int)threadIdx.x; pid <= R - 1; pid += 2 * (int)blockDim.x) {
         const int ri = pos[pid - 1];
         const int rj = pos[pid];
         const int ci = set * R + ri;
@@ -57,14 +57,17 @@ __global__ void k_pt_exchange(PTArgs a)
             a.chains[cj].table = set * R + idx[rj] - 1;
         }
     }
-    a.stage[set] = 1 - stage;
-    a.round[set] = round + 1;
+    // every thread has read the set's stage and round
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a.stage[set] = 1 - stage;
+        a.round[set] = round + 1;
+    }
 }
 
 hipError_t launch_pt_exchange(const PTArgs& a, hipStream_t st)
 {
-    const int block = 64;
-    hipLaunchKernelGGL(k_pt_exchange, dim3((a.n_sets + block - 1) / block), dim3(block), 0, st, a);
+    hipLaunchKernelGGL(k_pt_exchange, dim3(a.n_sets), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 
