@@ -24,9 +24,7 @@
  *   - The library owns all device memory.  Host buffers are caller-owned and
  *     copied synchronously before the call returns.
  *   - A handle may be used from one host thread at a time; all device work of
- *     a context is ordered on that context's stream (a batch whose chains
- *     form several launch groups runs the smaller groups on a second stream
- *     forked from that stream and joined back into it within the call).
+ *     a context is ordered on that context's stream.
  *   - Spins are int8, sites numbered x-fastest: i = x + L*y (+ L*L*z), 0-based,
  *     periodic boundaries (IsingLatticeOptim layout, SpinSystems/src/ising.jl:444-456).
  */
