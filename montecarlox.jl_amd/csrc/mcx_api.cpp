@@ -250,12 +250,6 @@ void batch_free(mcx_batch* b)
     hipFree(b->d_last);
     hipFree(b->d_planes);
     hipFree(b->d_plane_off);
-    if (b->side)
-        hipStreamDestroy(b->side);
-    if (b->ev_fork)
-        hipEventDestroy(b->ev_fork);
-    if (b->ev_join)
-        hipEventDestroy(b->ev_join);
 }
 
 // Bit-plane storage of every chain (2 planes per colour word for Ising, 4 for
@@ -729,24 +723,7 @@ mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t
         // the tuned launch shape of this algorithm, if any (results do not depend on it)
         const mcx_launch_shape& sh = b->shape[alg];
         a.lanes_per_word = sh.lanes_per_word;
-        // several launch groups (chains of different workgroup sizes or plane
-        // layouts): the largest runs on the context's stream, the others on a
-        // second stream forked from it, so that they fill the CUs the largest
-        // leaves idle as it drains; the join makes the call one unit of work
-        // on the context's stream again (the chains are independent)
-        const bool fork = b->launches.size() > 1;
-        if (fork) {
-            if (!b->side) {
-                HIPCHK(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
-                HIPCHK(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
-                HIPCHK(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
-            }
-            HIPCHK(hipEventRecord(b->ev_fork, st));
-            HIPCHK(hipStreamWaitEvent(b->side, b->ev_fork, 0));
-        }
-        for (size_t g = 0; g < b->launches.size(); ++g) {
-            const auto& ln = b->launches[g];
-            const hipStream_t gs = g == 0 ? st : b->side;
+        for (const auto& ln : b->launches) {
             a.items = b->d_items + ln.first;
             a.n_items = ln.count;
             const int block = sh.threads > 0 ? sh.threads : ln.block;
@@ -756,14 +733,10 @@ mcx_status run_sweeps_async(mcx_batch* b, int32_t alg, int64_t nsweeps, uint32_t
                 const int lay = ising_launch_layout(ln.layout, ln.count, block);
                 HIPCHK(launch_ising_sweeps(b->dim, alg, lay, a,
                                            ising_launch_block(lay, block, ln.words), grid, ln.lds,
-                                           gs));
+                                           st));
             } else {
-                HIPCHK(launch_bc_sweeps(b->dim, alg, a, block, grid, ln.lds, gs));
+                HIPCHK(launch_bc_sweeps(b->dim, alg, a, block, grid, ln.lds, st));
             }
-        }
-        if (fork) {
-            HIPCHK(hipEventRecord(b->ev_join, b->side));
-            HIPCHK(hipStreamWaitEvent(st, b->ev_join, 0));
         }
     }
     for (int ch = 0; ch < b->n; ++ch) {

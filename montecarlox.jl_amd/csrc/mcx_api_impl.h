@@ -68,10 +68,6 @@ struct mcx_batch {
     mcx_pt* pt = nullptr;      // replica-exchange object driving the tables, if any
     mcx_launch_shape shape[5] = {}; // per algorithm (MCX_METROPOLIS .. MCX_GLAUBER): tuned launch
     int n_gen = 0;             // multicanonical / Wang-Landau objects built on this batch
-    // a second stream for the smaller launch groups of a call (forked from and
-    // joined back into the context's stream), created on first use
-    hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace mcx {
